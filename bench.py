@@ -1,0 +1,154 @@
+"""Benchmark of the mean-field DMD force path on one MI355X.
+
+A step is one full FP64 residual evaluation (phase-1 site sums, phase-2
+gradient gather, deterministic totals) of the 54,000-site Mg-H workload:
+hcp Mg 20 x 15 x 15 orthorhombic cells (18,000 Mg) plus 36,000 tetrahedral H
+sites with occupancy x ~ U[0, 0.1], fully periodic, reference Mg-H ADP tables,
+3rd-order Stroud quadrature.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line: throughput in pair-quadrature evaluations per second
+(PQE/s; 50 PQE per active ordered neighbour pair per residual), the roofline
+figure of the dominant kernel (FP64 VALU) and the CPU baseline (the factorised
+FP64 oracle on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bt-am_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+# Algorithmic FP64 operations per active ordered pair (FMA = 2, others 1,
+# sqrt / reciprocal = 1), counted from the formulas of each kernel.  25 PQE per
+# pair in each phase.
+FLOP_PER_PAIR = {"phase1_site_sums": 1814, "phase2_gradient": 4755}
+PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
+HBM_PEAK = 8.0e12          # B/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phase2_gradient_mgh_20x15x15_hbm.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload():
+    import dmdgpu.lattice as lat
+    return lat.hcp_mgh(20, 15, 15, x_h=(0.0, 0.1), seed=2024)
+
+
+def traffic_per_launch():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
+    """The factorised FP64 oracle timed on the host cores (bounded sample:
+    whole residuals of the same workload until ~budget_s)."""
+    from oracle.oracle import Oracle
+    import dmdgpu
+    O = Oracle(dmdgpu.TABLE_DIR)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    s = O.system(sysd, nnb, nbr)
+    O.fact_residual(s, threads)  # warm
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        O.fact_residual(s, threads)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 50:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": pqe_per_res * reps / dt, "unit": "PQE/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} full FP64 residuals of the same 54,000-site workload "
+                      f"({dt:.1f} s, factorised oracle, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--lanes", type=str, default="4,4")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import dmdgpu
+    sysd = workload()
+    log(f"workload: {sysd['n']} sites")
+    ctx = dmdgpu.Context(0)
+    ctx.set_state(sysd)
+    ctx.neighbors()
+    k1, k2 = (int(v) for v in args.lanes.split(","))
+    ctx.set_lanes(k1, k2)
+    pairs, pqe = ctx.work_counts()
+    log(f"active ordered pairs {pairs}, PQE per residual {pqe}")
+
+    for _ in range(args.warmup):
+        ctx.residual_resident()
+    ctx.sync()
+
+    ctx.reset_kernel_times()
+    ctx.set_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.residual_resident()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    kt = ctx.kernel_times()
+    dt = t1 - t0
+    log(f"timed {args.steps} steps in {dt:.4f} s; kernels: {kt}")
+
+    dom = max(FLOP_PER_PAIR, key=lambda k: kt[k][1])
+    launches, ms = kt[dom]
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = FLOP_PER_PAIR[dom] * pairs / avg_s
+    traffic = traffic_per_launch()
+    roof = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
+            "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
+            "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
+            "flop_per_launch": FLOP_PER_PAIR[dom] * pairs}
+
+    cpu = None
+    if not args.no_cpu:
+        nn, lst = ctx.get_neighbors()
+        cpu = cpu_baseline(sysd, nn, lst, pqe)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "pair-quad evals/s", "value": pqe * args.steps / dt, "unit": "PQE/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "Mg-H hcp 20x15x15 periodic: 18,000 Mg + 36,000 tetrahedral H "
+                               "(x~U[0,0.1]), reference ADP tables, 3rd-order Stroud rule, "
+                               "full FP64 residual per step",
+                   "sites": int(sysd["n"]), "active_pairs": int(pairs),
+                   "pqe_per_step": int(pqe), "lanes_per_site": [k1, k2],
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,85 @@
+// dmd_kernels.h -- device-side data layout and kernel launchers (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dmd {
+
+// Species slots: 0 = Mg, 1 = H.  Pair-table slots: 0 = MgMg, 1 = HH, 2 = MgH.
+constexpr int kCap = 256;          // neighbour row stride (reference maxneigh = 250)
+constexpr int kMaxNeigh = 250;     // src/Macros.hpp:862
+constexpr int kRec = 80;           // doubles per phase-1 record
+constexpr int kRecLA = 0;          // 6 x 9 : (1/9) M G_ls     (i-shift sums)
+constexpr int kRecLU = 54;         // 9     : (1/9) M U        (j-shift sums)
+constexpr int kRecLG0 = 63;        // 9     : (1/9) M G0       (unshifted sums)
+constexpr int kRecFP = 72;         // F'(rho_bar)
+constexpr int kRecDVDN = 73;       // dV0_i / dx_i (own occupancy)
+constexpr int kRecRHO = 74;        // rho_bar
+
+// Spline tables resident in HBM (L2-resident in practice: ~1.5 MB for the
+// reference tables).  All r-tables share one (nr, dx) grid.  Coefficient rows:
+//   uw[t][m] : u: a b c d db dc dd 0 | w: a b c d db dc dd 0   (16 doubles)
+//   ph[t][m] : phi: a b c d db dc dd 0                           (8 doubles)
+//   rh[s][m] : rho: a b c d db dc dd 0
+//   em[s][m] : embedding F: a b c d db dc dd 0 (own grid per species)
+struct Tables {
+  const double* uw;
+  const double* ph;
+  const double* rh;
+  const double* em;
+  int nr;
+  double dxr, inv_dxr;
+  int ne[2];
+  int ne_stride;
+  double dxe[2], inv_dxe[2];
+  double mass[2];
+};
+
+struct Box {
+  double lo[3], L[3], invL[3];
+  int per[3];
+};
+
+struct State {
+  int n;
+  const double4* qs;   // (x, y, z, sigma)
+  const double* ne;    // effective occupancy: x if active (x >= 5e-4) else 0
+  const double* xr;    // raw occupancy
+  const int* spc;      // species slot
+  const double* beta;
+  const double* gamma;
+  const int* nnb;
+  const int* nbr;      // n x kCap
+};
+
+struct Outputs {
+  double* rec;         // n x kRec
+  double* V0;
+  double* rho;
+  double* Rq;          // n x 3
+  double* Rs;
+  double* Rx;
+  double* S0;
+  double* partials;    // per-block [S0, V0] partial sums
+  double* totals;      // [sum S0, sum V0]
+};
+
+// Neighbour build (cell list).  Returns via flags[0] = overflow, flags[1] = empty row.
+struct CellGrid {
+  int nc[3];
+  double lo[3], cs[3], inv_cs[3];
+};
+
+hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, double rc, int n,
+                            const double4* qs, int* cell_of, int* cell_count, int* cell_start,
+                            int* cell_cursor, int* cell_atoms, int* nnb, int* nbr, int* flags);
+
+hipError_t launch_phase1(hipStream_t st, int K, const State& s, const Box& box, const Tables& T,
+                         const Outputs& o);
+hipError_t launch_phase2(hipStream_t st, int K, const State& s, const Box& box, const Tables& T,
+                         const Outputs& o);
+hipError_t launch_totals(hipStream_t st, int n, const Outputs& o);
+hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_pairs_per_block,
+                               int nblocks);
+
+}  // namespace dmd

@@ -1,0 +1,210 @@
+"""Python host binding of libdmdgpu.so (the C ABI in include/dmdgpu.h).
+
+The operator surface mirrors the reference's per-site evaluators and its SNES
+callback (src/ADP/MgHx-mf-V-bulk.cpp, src/Mechanical-eqs/
+Mechanical-Relaxation-bulk.cpp:441); every call runs the HIP kernels of
+``csrc/dmd_kernels.hip``.  There is no CPU fallback: constructing a
+``Context`` without the built library or without a GPU raises.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libdmdgpu.so")
+TABLE_DIR = os.path.join(PKG, "potentials")
+R_CUTOFF = 6.2934034034  # r_cutoff_ADP_MgHx, src/Macros.hpp:857
+MAXNEIGH = 250           # src/Macros.hpp:862
+
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_IO", 4: "ERR_SPECIES", 5: "ERR_MAXNEIGH",
+          6: "ERR_NONEIGH", 7: "ERR_BOX", 8: "ERR_STATE", 9: "ERR_TABLES"}
+
+# Every symbol include/dmdgpu.h declares.
+EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_stream", "dmdgpu_sync",
+           "dmdgpu_init_adp_MgHx", "dmdgpu_set_cutoff", "dmdgpu_set_state",
+           "dmdgpu_update_state", "dmdgpu_neighbors", "dmdgpu_get_neighbors", "dmdgpu_mf_rho",
+           "dmdgpu_site_energies", "dmdgpu_residual", "dmdgpu_residual_resident",
+           "dmdgpu_set_lanes", "dmdgpu_set_timing", "dmdgpu_kernel_count", "dmdgpu_kernel_name",
+           "dmdgpu_kernel_times", "dmdgpu_reset_kernel_times", "dmdgpu_work_counts"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+
+
+class DmdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def build():
+    """Compile libdmdgpu.so for gfx950 in place (make -C bt-am_amd)."""
+    subprocess.run(["make", "-s", "-C", PKG], check=True)
+
+
+_lib = None
+
+
+def load_library():
+    """Load libdmdgpu.so and declare its prototypes (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run dmdgpu.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.dmdgpu_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.dmdgpu_destroy.argtypes = [vp]
+    L.dmdgpu_last_error.restype = C.c_char_p
+    L.dmdgpu_stream.argtypes = [vp]
+    L.dmdgpu_stream.restype = vp
+    L.dmdgpu_sync.argtypes = [vp]
+    L.dmdgpu_init_adp_MgHx.argtypes = [vp, C.c_int, C.c_char_p]
+    L.dmdgpu_set_cutoff.argtypes = [vp, C.c_double]
+    L.dmdgpu_set_state.argtypes = [vp, C.c_int, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _ip]
+    L.dmdgpu_update_state.argtypes = [vp, _dp, _dp, _dp]
+    L.dmdgpu_neighbors.argtypes = [vp]
+    L.dmdgpu_get_neighbors.argtypes = [vp, _ip, _ip, C.c_int]
+    L.dmdgpu_mf_rho.argtypes = [vp, _dp]
+    L.dmdgpu_site_energies.argtypes = [vp, _dp, _dp]
+    L.dmdgpu_residual.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.dmdgpu_residual_resident.argtypes = [vp]
+    L.dmdgpu_set_lanes.argtypes = [vp, C.c_int, C.c_int]
+    L.dmdgpu_set_timing.argtypes = [vp, C.c_int]
+    L.dmdgpu_kernel_count.restype = C.c_int
+    L.dmdgpu_kernel_name.argtypes = [C.c_int]
+    L.dmdgpu_kernel_name.restype = C.c_char_p
+    L.dmdgpu_kernel_times.argtypes = [vp, _lp, _dp]
+    L.dmdgpu_reset_kernel_times.argtypes = [vp]
+    L.dmdgpu_work_counts.argtypes = [vp, _lp, _lp]
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _dn(a):
+    return None if a is None else _d(a)
+
+
+class Context:
+    """One device-resident swarm + tables + neighbour lists on one stream."""
+
+    def __init__(self, device=0, table_dir=TABLE_DIR, rc=R_CUTOFF):
+        self.L = load_library()
+        h = C.c_void_p()
+        self._chk(self.L.dmdgpu_create(device, C.byref(h)))
+        self.h = h
+        for comb in (0, 1, 2):
+            self._chk(self.L.dmdgpu_init_adp_MgHx(self.h, comb, table_dir.encode()))
+        self._chk(self.L.dmdgpu_set_cutoff(self.h, rc))
+        self.n = 0
+
+    def _chk(self, code):
+        if code != 0:
+            raise DmdError(code, self.L.dmdgpu_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dmdgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ------------------------------------------------------------
+    def set_state(self, sysd):
+        n = int(sysd["n"])
+        self._keep = {
+            "q": np.ascontiguousarray(sysd["q"], dtype=np.float64),
+            "sigma": np.ascontiguousarray(sysd["sigma"], dtype=np.float64),
+            "x": np.ascontiguousarray(sysd["x"], dtype=np.float64),
+            "spc": np.ascontiguousarray(sysd["species"], dtype=np.int32),
+            "beta": np.ascontiguousarray(sysd["beta"], dtype=np.float64),
+            "gamma": np.ascontiguousarray(sysd["gamma"], dtype=np.float64),
+            "box": np.ascontiguousarray(np.concatenate([sysd["lo"], sysd["hi"]]), dtype=np.float64),
+            "per": np.ascontiguousarray(sysd["periodic"], dtype=np.int32),
+        }
+        k = self._keep
+        self._chk(self.L.dmdgpu_set_state(self.h, n, _d(k["q"]), _d(k["sigma"]), _d(k["x"]),
+                                          _i(k["spc"]), _d(k["beta"]), _d(k["gamma"]),
+                                          _d(k["box"]), _i(k["per"])))
+        self.n = n
+
+    def update_state(self, q=None, sigma=None, x=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (q, sigma, x)]
+        self._chk(self.L.dmdgpu_update_state(self.h, *[_dn(a) for a in arrs]))
+
+    def set_lanes(self, k1, k2):
+        self._chk(self.L.dmdgpu_set_lanes(self.h, k1, k2))
+
+    # -- neighbours ---------------------------------------------------------
+    def neighbors(self):
+        self._chk(self.L.dmdgpu_neighbors(self.h))
+
+    def get_neighbors(self, cap=MAXNEIGH):
+        nn = np.zeros(self.n, dtype=np.int32)
+        lst = np.zeros((self.n, cap), dtype=np.int32)
+        self._chk(self.L.dmdgpu_get_neighbors(self.h, _i(nn), _i(lst), cap))
+        return nn, lst
+
+    # -- evaluation ---------------------------------------------------------
+    def mf_rho(self):
+        out = np.zeros(self.n)
+        self._chk(self.L.dmdgpu_mf_rho(self.h, _d(out)))
+        return out
+
+    def site_energies(self):
+        V0 = np.zeros(self.n)
+        S0 = np.zeros(self.n)
+        self._chk(self.L.dmdgpu_site_energies(self.h, _d(V0), _d(S0)))
+        return V0, S0
+
+    def residual(self):
+        Rq = np.zeros((self.n, 3))
+        Rs = np.zeros(self.n)
+        Rx = np.zeros(self.n)
+        tot = np.zeros(2)
+        self._chk(self.L.dmdgpu_residual(self.h, _d(Rq), _d(Rs), _d(Rx), _d(tot)))
+        return {"Rq": Rq, "Rs": Rs, "Rx": Rx, "S0_total": tot[0], "V0_total": tot[1]}
+
+    def residual_resident(self):
+        self._chk(self.L.dmdgpu_residual_resident(self.h))
+
+    def sync(self):
+        self._chk(self.L.dmdgpu_sync(self.h))
+
+    # -- instrumentation ----------------------------------------------------
+    def set_timing(self, on):
+        self._chk(self.L.dmdgpu_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        k = self.L.dmdgpu_kernel_count()
+        la = np.zeros(k, dtype=np.int64)
+        ms = np.zeros(k)
+        self._chk(self.L.dmdgpu_kernel_times(self.h, la.ctypes.data_as(_lp), _d(ms)))
+        return {self.L.dmdgpu_kernel_name(i).decode(): (int(la[i]), float(ms[i])) for i in range(k)}
+
+    def reset_kernel_times(self):
+        self._chk(self.L.dmdgpu_reset_kernel_times(self.h))
+
+    def work_counts(self):
+        p = C.c_int64()
+        q = C.c_int64()
+        self._chk(self.L.dmdgpu_work_counts(self.h, C.byref(p), C.byref(q)))
+        return p.value, q.value

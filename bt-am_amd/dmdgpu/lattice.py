@@ -1,0 +1,107 @@
+"""Seeded Mg-H geometries and the reference dump reader.
+
+A *system* is a dict of DMSwarm-equivalent SoA arrays (the fields registered in
+src/Atoms/Atom.cpp:149-203): ``q`` (n x 3 mean positions, Angstrom), ``sigma``
+(n widths), ``x`` (n occupancies / molar fractions), ``species`` (AtomicSpecie
+codes, H = 1, Mg = 12), ``beta`` (1 / k_B T) and ``gamma`` (chemical
+multiplier), plus the box ``lo``/``hi`` and the ``periodic`` flag of each axis.
+"""
+import numpy as np
+
+K_B = 8.617332478e-5  # eV/K, src/Macros.hpp:849
+MG, H = 12, 1
+
+# Orthorhombic hcp cell (a, sqrt(3) a, c) with four atoms (fractional units).
+HCP_BASIS = np.array([[0.0, 0.0, 0.0],
+                      [0.5, 0.5, 0.0],
+                      [0.5, 5.0 / 6.0, 0.5],
+                      [0.0, 1.0 / 3.0, 0.5]])
+
+# Width rule of this build: the reference stores sigma only in the dump.  The
+# harmonic estimate sigma^2 = k_B T / k with k ~ 1.8 eV/A^2 for Mg at 300 K
+# gives ~0.12 A; H sits in a softer, lighter well and starts wider.
+SIGMA0 = {MG: 0.12, H: 0.15}
+
+
+def _finish(q, spc, x, lo, hi, periodic, T, rng, sigma_jitter):
+    n = len(spc)
+    sig = np.array([SIGMA0[s] for s in spc], dtype=np.float64)
+    if sigma_jitter > 0:
+        sig *= 1.0 + sigma_jitter * (2.0 * rng.random(n) - 1.0)
+    beta = np.full(n, 1.0 / (K_B * T))
+    return {"n": n, "q": np.ascontiguousarray(q, dtype=np.float64), "sigma": sig,
+            "x": np.ascontiguousarray(x, dtype=np.float64),
+            "species": np.ascontiguousarray(spc, dtype=np.int32), "beta": beta,
+            "gamma": np.zeros(n), "lo": np.asarray(lo, dtype=np.float64),
+            "hi": np.asarray(hi, dtype=np.float64),
+            "periodic": np.asarray(periodic, dtype=np.int32)}
+
+
+def hcp_mgh(nx, ny, nz, a=3.209, c_over_a=1.624, with_h=True, x_h=(0.0, 0.1), seed=0,
+            pos_noise=0.01, sigma_jitter=0.05, T=300.0, periodic=(1, 1, 1)):
+    """hcp Mg replicated nx x ny x nz orthorhombic cells (4 atoms each).
+
+    With ``with_h`` every Mg carries two tetrahedral H sites at +/- 3c/8 along c
+    (2 per Mg), with occupancy x ~ U[x_h].  Mean positions get seeded
+    N(0, pos_noise^2) noise per coordinate."""
+    rng = np.random.default_rng(seed)
+    c = a * c_over_a
+    cell = np.array([a, np.sqrt(3.0) * a, c])
+    idx = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"),
+                   axis=-1).reshape(-1, 3)
+    mg = ((idx[:, None, :] + HCP_BASIS[None, :, :]) * cell).reshape(-1, 3)
+    pos = [mg]
+    spc = [np.full(len(mg), MG)]
+    x = [np.ones(len(mg))]
+    if with_h:
+        up = mg + np.array([0.0, 0.0, 3.0 * c / 8.0])
+        dn = mg - np.array([0.0, 0.0, 3.0 * c / 8.0])
+        hpos = np.stack([up, dn], axis=1).reshape(-1, 3)
+        pos.append(hpos)
+        spc.append(np.full(len(hpos), H))
+        x.append(rng.uniform(x_h[0], x_h[1], len(hpos)))
+    q = np.concatenate(pos)
+    spc = np.concatenate(spc)
+    x = np.concatenate(x)
+    q = q + rng.normal(0.0, pos_noise, q.shape)
+    lo = np.zeros(3)
+    hi = cell * np.array([nx, ny, nz])
+    # wrap into the box on periodic axes
+    for ax in range(3):
+        if periodic[ax]:
+            q[:, ax] = lo[ax] + np.mod(q[:, ax] - lo[ax], hi[ax] - lo[ax])
+    return _finish(q, spc, x, lo, hi, periodic, T, rng, sigma_jitter)
+
+
+def cluster(nx, ny, nz, **kw):
+    """Free (non-periodic) hcp block: every axis open."""
+    kw.setdefault("periodic", (0, 0, 0))
+    return hcp_mgh(nx, ny, nz, **kw)
+
+
+def read_dump(path):
+    """Reader for the reference dump layout (src/IO/dump-input.cpp:51-215):
+    TIMESTEP, NUMBER OF ATOMS, BOX BOUNDS with pp/ff flags, then rows of
+    ``type x y z stdvq molarfraction chemicalmultp thermalmultp ...``."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[3].split()[0])
+    flags = lines[4].split()[3:6]
+    periodic = [1 if fl == "pp" else 0 for fl in flags]
+    for fl in flags:
+        if fl not in ("pp", "ff"):
+            raise ValueError(f"unrecognised boundary flag {fl}")
+    lo, hi = [], []
+    for k in range(3):
+        a, b = lines[5 + k].split()[:2]
+        lo.append(float(a))
+        hi.append(float(b))
+    rows = np.array([[float(v) for v in ln.split()] for ln in lines[9:9 + n]])
+    spc = rows[:, 0].astype(np.int32)
+    bad = set(spc.tolist()) - {MG, H}
+    if bad:
+        raise ValueError(f"species {sorted(bad)} not supported (Mg-H only)")
+    return {"n": n, "q": np.ascontiguousarray(rows[:, 1:4]), "sigma": rows[:, 4].copy(),
+            "x": rows[:, 5].copy(), "species": spc, "beta": rows[:, 7].copy(),
+            "gamma": rows[:, 6].copy(), "lo": np.array(lo), "hi": np.array(hi),
+            "periodic": np.array(periodic, dtype=np.int32)}

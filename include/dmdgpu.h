@@ -1,0 +1,152 @@
+/*
+ * dmdgpu.h -- C ABI of the MI355X-native mean-field DMD force path.
+ *
+ * The library replaces the per-site CPU evaluators of BT-AM
+ * (src/ADP/MgHx-mf-V-bulk.cpp, reached through the dmd_equations table of
+ * src/Macros.hpp:540-798) and the O(N^2) neighbour search
+ * (src/Atoms/Neighbors.cpp:39-134) with HIP kernels for gfx950.  A PETSc SNES
+ * FormFunction (the reference's evaluate_RHS,
+ * src/Mechanical-eqs/Mechanical-Relaxation-bulk.cpp:441) calls
+ * dmdgpu_residual() in place of its OpenMP site loops.
+ *
+ * Conventions
+ *  - Every function returns a dmdgpu_status; 0 is success (PetscErrorCode
+ *    style, src/Macros.hpp:28-38).  dmdgpu_last_error() gives a message.
+ *  - Plain pointers and sizes only.  Pointers named h_* are host memory; the
+ *    caller keeps ownership.  Device buffers belong to the context until
+ *    dmdgpu_destroy().
+ *  - Arrays are structure-of-arrays over n sites.  Positions are n x 3
+ *    row-major (x, y, z per site).  Species codes follow enum AtomicSpecie
+ *    (src/Macros.hpp:56-175): H = 1, Mg = 12; other species are rejected.
+ *  - One context per stream; calls on one context are serialised by the
+ *    caller.  Distinct contexts may be used from distinct threads.
+ *  - All arithmetic of this interface is FP64 unless the precision option
+ *    selects the split FP32 evaluation (dmdgpu_set_precision).
+ */
+#ifndef DMDGPU_H
+#define DMDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DMDGPU_OK = 0,
+  DMDGPU_ERR_ARG = 1,          /* bad argument / size                          */
+  DMDGPU_ERR_HIP = 2,          /* HIP runtime error (no device, OOM, launch)    */
+  DMDGPU_ERR_IO = 3,           /* table file missing or malformed              */
+  DMDGPU_ERR_SPECIES = 4,      /* species other than H / Mg, or Mg with x<5e-4  */
+  DMDGPU_ERR_MAXNEIGH = 5,     /* a site has more than 250 neighbours
+                                  (Neighbors.cpp:99-107)                        */
+  DMDGPU_ERR_NONEIGH = 6,      /* a site has no neighbours (Neighbors.cpp:110-116) */
+  DMDGPU_ERR_BOX = 7,          /* periodic box edge < 2 r_c (Atom.cpp:230-255) */
+  DMDGPU_ERR_STATE = 8,        /* call order: tables / state / neighbours unset */
+  DMDGPU_ERR_TABLES = 9        /* r-tables do not share one (n, dx) grid        */
+} dmdgpu_status;
+
+typedef struct dmdgpu_ctx dmdgpu_ctx;
+
+/* Which table file init_adp_MgHx reads (src/ADP/MgHx-ADP.cpp:366-501). */
+typedef enum { DMDGPU_MgMg = 0, DMDGPU_HH = 1, DMDGPU_MgH = 2 } dmdgpu_species_comb;
+
+/* Context ------------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device` with its own stream.  No
+ * counterpart in the reference (the swarm and the tables are globals there). */
+dmdgpu_status dmdgpu_create(int device, dmdgpu_ctx** out);
+dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* ctx);
+const char* dmdgpu_last_error(void);
+
+/* The stream all kernels of this context are launched on (a hipStream_t). */
+void* dmdgpu_stream(dmdgpu_ctx* ctx);
+dmdgpu_status dmdgpu_sync(dmdgpu_ctx* ctx);
+
+/* Tables --------------------------------------------------------------------
+ * Replaces init_adp_MgHx(adpPotential*, species_comb_MgH, const char* folder)
+ * (src/ADP/MgHx-ADP.cpp:366-501): reads adpmg.dat / adph.dat / adpmgh.dat in
+ * the reference layout from `folder` and uploads them.  All three files must be
+ * loaded before dmdgpu_set_state. */
+dmdgpu_status dmdgpu_init_adp_MgHx(dmdgpu_ctx* ctx, int species_comb, const char* folder);
+
+/* Neighbour cut-off on mean distances; default r_cutoff_ADP_MgHx =
+ * 6.2934034034 (src/Macros.hpp:857). */
+dmdgpu_status dmdgpu_set_cutoff(dmdgpu_ctx* ctx, double rc);
+
+/* State ----------------------------------------------------------------------
+ * Copies the DMSwarm-equivalent fields (src/Atoms/Atom.cpp:149-203:
+ * coor, stdv-q, molar-fraction, specie, thermalmultp = beta,
+ * chemicalmultp = gamma) and the box (dump-input.cpp:76-125) to device SoA
+ * buffers.  box = {xlo, ylo, zlo, xhi, yhi, zhi}; periodic[3] = 1 for "pp". */
+dmdgpu_status dmdgpu_set_state(dmdgpu_ctx* ctx, int n, const double* h_mean_q,
+                               const double* h_stdv_q, const double* h_xi,
+                               const int32_t* h_specie, const double* h_beta,
+                               const double* h_gamma, const double* h_box,
+                               const int32_t* h_periodic);
+
+/* Replaces the unknowns only (positions, widths, occupancies), keeping the
+ * neighbour lists: the SNES iterate update.  NULL leaves a field unchanged. */
+dmdgpu_status dmdgpu_update_state(dmdgpu_ctx* ctx, const double* h_mean_q,
+                                  const double* h_stdv_q, const double* h_xi);
+
+/* Neighbours -----------------------------------------------------------------
+ * Replaces neighbors(DMD*) (src/Atoms/Neighbors.cpp:39-134): every site j != i
+ * with |q_i - q_j| <= r_c at mean positions, minimum image on periodic axes,
+ * built on the GPU with a cell list.  Rows are sorted by site index, as the
+ * reference's O(N^2) scan produces them. */
+dmdgpu_status dmdgpu_neighbors(dmdgpu_ctx* ctx);
+
+/* Copies the lists out: h_numneigh[n], h_list[n * cap] (row stride cap >= 250).
+ * Mirrors AtomTopology {numneigh, mech_neighs_ptr} (src/Macros.hpp:238-244). */
+dmdgpu_status dmdgpu_get_neighbors(dmdgpu_ctx* ctx, int32_t* h_numneigh, int32_t* h_list,
+                                   int cap);
+
+/* Evaluation -----------------------------------------------------------------
+ * Mean-field density of every site, the loop of main-BT-AM.cpp:197-207 with
+ * evaluate_mf_rho_i_adp_MgHx (MgHx-mf-V-bulk.cpp:708-787) as the body. */
+dmdgpu_status dmdgpu_mf_rho(dmdgpu_ctx* ctx, double* h_mf_rho);
+
+/* Per-site V0_i and S0_i of every site: evaluate_V0_i_adp_MgHx (:791-953) and
+ * evaluate_S0_i_adp_MgHx (:957-1142) with mf_rho from the same call. */
+dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* ctx, double* h_V0, double* h_S0);
+
+/* The SNES FormFunction body.  Gradient of the free-entropy functional
+ * sum_i S0_i (MgHx-mf-V-bulk.cpp:957-1142) with respect to the mean positions,
+ * widths and occupancies of every site:
+ *   R_q(u)     = sum_{i in {u} U N(u)} beta_i dV0_i/dq_u         (n x 3)
+ *   R_sigma(u) = sum_i beta_i dV0_i/dsigma_u - 3 / sigma_u        (n)
+ *   R_x(u)     = sum_i beta_i dV0_i/dx_u + d(-log Z0_u)/dx_u - gamma_u [H] (n)
+ * The per-(u, i) terms are evaluate_DV0_i_Dq_u_MgHx (:1823-2098) and
+ * evaluate_DV0_i_Dxi_u_adp_MgHx (:2102-2325).  Inactive sites (x < 5e-4) get
+ * zeros.  totals[2] = {sum_i S0_i, sum_i V0_i}.  Any output may be NULL. */
+dmdgpu_status dmdgpu_residual(dmdgpu_ctx* ctx, double* h_Rq, double* h_Rsigma, double* h_Rx,
+                              double* h_totals);
+
+/* Same evaluation on the device-resident state, leaving the results in device
+ * buffers (no host copies); used by resident solvers and for timing. */
+dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
+
+/* Lanes of a wavefront that share one site in the phase-1 and phase-2
+ * kernels (1, 2, 4 or 8; the tuner picks them per workload).  No counterpart
+ * in the reference. */
+dmdgpu_status dmdgpu_set_lanes(dmdgpu_ctx* ctx, int k_phase1, int k_phase2);
+
+/* Instrumentation: when on, every kernel launch is bracketed by HIP events on
+ * the context's stream.  dmdgpu_kernel_times returns, per kernel id (see
+ * dmdgpu_kernel_name), the launch count and the summed milliseconds. */
+dmdgpu_status dmdgpu_set_timing(dmdgpu_ctx* ctx, int on);
+int dmdgpu_kernel_count(void);
+const char* dmdgpu_kernel_name(int id);
+dmdgpu_status dmdgpu_kernel_times(dmdgpu_ctx* ctx, int64_t* h_launches, double* h_ms);
+dmdgpu_status dmdgpu_reset_kernel_times(dmdgpu_ctx* ctx);
+
+/* Counts for the throughput metric: active ordered pairs of the current
+ * lists (P), and pair-quadrature evaluations per residual (50 P). */
+dmdgpu_status dmdgpu_work_counts(dmdgpu_ctx* ctx, int64_t* pairs, int64_t* pqe_per_residual);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

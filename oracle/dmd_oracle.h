@@ -1,0 +1,105 @@
+/*
+ * dmd_oracle.h -- CPU FP64 restatement of the BT-AM mean-field DMD force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it.  The product library (libdmdgpu.so) never links or
+ * calls it.
+ *
+ * Two independent restatements of the same mathematics are provided:
+ *
+ *  - "reference order" (or_ref_*): the per-site evaluators of
+ *    src/ADP/MgHx-mf-V-bulk.cpp restated loop for loop.  They use the generic
+ *    Gaussian measure of src/Numerical/Quadrature-Measure.cpp:30-110, the 3rd
+ *    order Stroud rule of src/Numerical/Quadrature-Hermitian-3th.cpp:36-464 and
+ *    the pair / triplet ADP functions of src/ADP/MgHx-ADP.cpp.  The cost is
+ *    O(nb^2) per site and O(nb^3) per residual entry, so this form is used only
+ *    on small systems.
+ *
+ *  - "factorised" (or_fact_*): the O(nb) two-phase form that the GPU kernels
+ *    implement (per-atom dipole / quadrupole sums, then a gather of adjoints).
+ *    It is the CPU baseline that is timed.  Tests pin it against the
+ *    reference-order form and against finite differences of the energy.
+ *
+ * Species codes follow enum AtomicSpecie (src/Macros.hpp:56-175): H = 1, Mg = 12.
+ */
+#ifndef DMD_ORACLE_H
+#define DMD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXNEIGH 250          /* src/Macros.hpp:862 */
+#define OR_MIN_OCCUPANCY 5e-4    /* src/Macros.hpp:871 */
+
+/* Cubic spline with n segments of width dx (src/Macros.hpp:286-329). */
+typedef struct {
+  int n;
+  double dx;
+  double *a, *b, *c, *d, *db, *dc, *dd, *ddc, *ddd;
+} or_spline;
+
+/* ADP tables for the Mg-H system (src/Macros.hpp:486-533).
+ * Species slot 0 = Mg, 1 = H.  Pair slot 0 = MgMg, 1 = HH, 2 = MgH. */
+typedef struct {
+  or_spline rho[2], embed[2];
+  or_spline pair[3], u[3], w[3];
+  double mass[2];
+} or_tables;
+
+/* Atom arrays (DMSwarm-equivalent SoA) plus the neighbour lists. */
+typedef struct {
+  int n;
+  const double* q;      /* n x 3 mean positions            */
+  const double* sig;    /* n     widths (stdv-q)             */
+  const double* x;      /* n     occupancies (molar-fraction) */
+  const int* spc;       /* n     AtomicSpecie codes          */
+  const double* beta;   /* n     1 / (k_B T)                 */
+  const double* gamma;  /* n     chemical multiplier         */
+  double lo[3], hi[3];
+  int periodic[3];
+  int cap;              /* row stride of nbr                 */
+  const int* nnb;       /* n                                 */
+  const int* nbr;       /* n x cap                           */
+} or_system;
+
+/* Tables ------------------------------------------------------------------ */
+int or_load_tables(or_tables** out, const char* folder);
+void or_free_tables(or_tables* t);
+/* which: 0 rho, 1 embed, 2 pair, 3 u, 4 w; slot as above; deriv 0/1/2. */
+double or_spline_eval(const or_tables* t, int which, int slot, int deriv, double x);
+int or_spline_size(const or_tables* t, int which, int slot, double* dx);
+
+/* Neighbours (brute force, src/Atoms/Neighbors.cpp:39-134) ----------------- */
+int or_neighbors(int n, const double* q, const double* lo, const double* hi,
+                 const int* periodic, double rc, int cap, int* nnb, int* nbr);
+
+/* Reference order (src/ADP/MgHx-mf-V-bulk.cpp) ------------------------------ */
+double or_ref_rho0_i(const or_system* s, const or_tables* t, int i);
+double or_ref_mf_rho_i(const or_system* s, const or_tables* t, int i);
+double or_ref_V0_i(const or_system* s, const or_tables* t, int i, const double* mf_rho);
+double or_ref_S0_i(const or_system* s, const or_tables* t, int i, const double* mf_rho);
+void or_ref_DV0_i_Dq_u(const or_system* s, const or_tables* t, int u, int i,
+                       const double* mf_rho, double out[4]);
+double or_ref_DV0_i_Dxi_u(const or_system* s, const or_tables* t, int u, int i,
+                          const double* mf_rho);
+/* Residual of sum_i S0_i for the listed sites u (all sites when sites==NULL). */
+void or_ref_residual(const or_system* s, const or_tables* t, int nsites, const int* sites,
+                     const double* mf_rho, double* Rq, double* Rs, double* Rx);
+
+/* Factorised O(nb) form ------------------------------------------------------ */
+/* Phase 1 per site: mf_rho, V0 and the per-atom adjoint data (stride 80). */
+#define OR_P1_STRIDE 80
+void or_fact_phase1(const or_system* s, const or_tables* t, int nthreads,
+                    double* mf_rho, double* V0, double* p1);
+/* Full residual.  Outputs Rq (n x 3), Rs, Rx, S0, V0, mf_rho per site and the
+ * totals [sum S0, sum V0] over active sites. */
+void or_fact_residual(const or_system* s, const or_tables* t, int nthreads,
+                      double* Rq, double* Rs, double* Rx, double* S0, double* V0,
+                      double* mf_rho, double* totals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

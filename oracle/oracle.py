@@ -1,0 +1,180 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the parity checker and the CPU baseline.  Only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module; the product path (``dmdgpu``) never does.
+
+The C sources restate the reference's mean-field DMD path in two forms
+(reference order and factorised); see ``dmd_oracle.h`` for the files each
+function follows.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build():
+    """Compile liboracle.so in place (gcc; see oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class _System(C.Structure):
+    _fields_ = [("n", C.c_int), ("q", _dp), ("sig", _dp), ("x", _dp), ("spc", _ip),
+                ("beta", _dp), ("gamma", _dp), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("periodic", C.c_int * 3), ("cap", C.c_int), ("nnb", _ip), ("nbr", _ip)]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Oracle:
+    """FP64 CPU restatement bound to one set of ADP tables."""
+
+    MAXNEIGH = 250
+
+    def __init__(self, table_dir):
+        if not os.path.exists(LIB):
+            build()
+        self.lib = C.CDLL(LIB)
+        L = self.lib
+        L.or_load_tables.argtypes = [C.POINTER(C.c_void_p), C.c_char_p]
+        L.or_free_tables.argtypes = [C.c_void_p]
+        L.or_spline_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.or_spline_eval.restype = C.c_double
+        L.or_spline_size.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.or_neighbors.argtypes = [C.c_int, _dp, _dp, _dp, _ip, C.c_double, C.c_int, _ip, _ip]
+        sp = C.POINTER(_System)
+        for name in ("or_ref_rho0_i", "or_ref_mf_rho_i"):
+            getattr(L, name).argtypes = [sp, C.c_void_p, C.c_int]
+            getattr(L, name).restype = C.c_double
+        for name in ("or_ref_V0_i", "or_ref_S0_i"):
+            getattr(L, name).argtypes = [sp, C.c_void_p, C.c_int, _dp]
+            getattr(L, name).restype = C.c_double
+        L.or_ref_DV0_i_Dq_u.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.or_ref_DV0_i_Dxi_u.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp]
+        L.or_ref_DV0_i_Dxi_u.restype = C.c_double
+        L.or_ref_residual.argtypes = [sp, C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, _dp]
+        L.or_fact_phase1.argtypes = [sp, C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.or_fact_residual.argtypes = [sp, C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        h = C.c_void_p()
+        rc = L.or_load_tables(C.byref(h), table_dir.encode())
+        if rc != 0:
+            raise RuntimeError(f"oracle: cannot load ADP tables from {table_dir}")
+        self.tab = h
+
+    def __del__(self):
+        try:
+            self.lib.or_free_tables(self.tab)
+        except Exception:
+            pass
+
+    # -- tables -------------------------------------------------------------
+    def spline(self, which, slot, deriv, x):
+        return self.lib.or_spline_eval(self.tab, which, slot, deriv, float(x))
+
+    def spline_size(self, which, slot):
+        dx = C.c_double()
+        n = self.lib.or_spline_size(self.tab, which, slot, C.byref(dx))
+        return n, dx.value
+
+    # -- neighbours ---------------------------------------------------------
+    def neighbors(self, sysd, rc):
+        n = sysd["n"]
+        q = np.ascontiguousarray(sysd["q"], dtype=np.float64)
+        lo = np.ascontiguousarray(sysd["lo"], dtype=np.float64)
+        hi = np.ascontiguousarray(sysd["hi"], dtype=np.float64)
+        per = np.ascontiguousarray(sysd["periodic"], dtype=np.int32)
+        nnb = np.zeros(n, dtype=np.int32)
+        nbr = np.zeros((n, self.MAXNEIGH), dtype=np.int32)
+        st = self.lib.or_neighbors(n, _d(q), _d(lo), _d(hi), _i(per), rc, self.MAXNEIGH,
+                                   _i(nnb), _i(nbr))
+        return st, nnb, nbr
+
+    def system(self, sysd, nnb, nbr):
+        """Pack a system dict (see dmdgpu.lattice) into the C struct.  The
+        returned object keeps the arrays alive."""
+        keep = {}
+        for k in ("q", "sigma", "x", "beta", "gamma"):
+            keep[k] = np.ascontiguousarray(sysd[k], dtype=np.float64)
+        keep["spc"] = np.ascontiguousarray(sysd["species"], dtype=np.int32)
+        keep["nnb"] = np.ascontiguousarray(nnb, dtype=np.int32)
+        keep["nbr"] = np.ascontiguousarray(nbr, dtype=np.int32)
+        s = _System()
+        s.n = int(sysd["n"])
+        s.q, s.sig, s.x = _d(keep["q"]), _d(keep["sigma"]), _d(keep["x"])
+        s.spc = _i(keep["spc"])
+        s.beta, s.gamma = _d(keep["beta"]), _d(keep["gamma"])
+        for a in range(3):
+            s.lo[a] = float(sysd["lo"][a])
+            s.hi[a] = float(sysd["hi"][a])
+            s.periodic[a] = int(sysd["periodic"][a])
+        s.cap = keep["nbr"].shape[1]
+        s.nnb, s.nbr = _i(keep["nnb"]), _i(keep["nbr"])
+        s._keep = keep
+        return s
+
+    # -- reference order ----------------------------------------------------
+    def ref_mf_rho(self, s):
+        return np.array([self.lib.or_ref_mf_rho_i(C.byref(s), self.tab, i) for i in range(s.n)])
+
+    def ref_rho0(self, s):
+        return np.array([self.lib.or_ref_rho0_i(C.byref(s), self.tab, i) for i in range(s.n)])
+
+    def ref_V0(self, s, mf_rho, sites=None):
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        sites = range(s.n) if sites is None else sites
+        return np.array([self.lib.or_ref_V0_i(C.byref(s), self.tab, i, _d(mf_rho)) for i in sites])
+
+    def ref_S0(self, s, mf_rho, sites=None):
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        sites = range(s.n) if sites is None else sites
+        return np.array([self.lib.or_ref_S0_i(C.byref(s), self.tab, i, _d(mf_rho)) for i in sites])
+
+    def ref_DV0_Dq(self, s, u, i, mf_rho):
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        out = np.zeros(4)
+        self.lib.or_ref_DV0_i_Dq_u(C.byref(s), self.tab, u, i, _d(mf_rho), _d(out))
+        return out
+
+    def ref_DV0_Dxi(self, s, u, i, mf_rho):
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        return self.lib.or_ref_DV0_i_Dxi_u(C.byref(s), self.tab, u, i, _d(mf_rho))
+
+    def ref_residual(self, s, mf_rho, sites=None):
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        if sites is None:
+            cnt, sp = s.n, None
+            sarr = None
+        else:
+            sarr = np.ascontiguousarray(sites, dtype=np.int32)
+            cnt, sp = len(sarr), _i(sarr)
+        Rq = np.zeros((cnt, 3))
+        Rs = np.zeros(cnt)
+        Rx = np.zeros(cnt)
+        self.lib.or_ref_residual(C.byref(s), self.tab, cnt, sp, _d(mf_rho), _d(Rq), _d(Rs), _d(Rx))
+        return Rq, Rs, Rx
+
+    # -- factorised ----------------------------------------------------------
+    def fact_residual(self, s, nthreads=0):
+        n = s.n
+        out = {k: np.zeros(n) for k in ("Rs", "Rx", "S0", "V0", "mf_rho")}
+        out["Rq"] = np.zeros((n, 3))
+        tot = np.zeros(2)
+        self.lib.or_fact_residual(C.byref(s), self.tab, nthreads, _d(out["Rq"]), _d(out["Rs"]),
+                                  _d(out["Rx"]), _d(out["S0"]), _d(out["V0"]), _d(out["mf_rho"]),
+                                  _d(tot))
+        out["S0_total"], out["V0_total"] = tot[0], tot[1]
+        return out

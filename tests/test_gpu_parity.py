@@ -1,0 +1,169 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: neighbour lists bit-exact; FP64 residual, energies and densities within
+1e-10 relative (max-norm of the difference over max-norm of the oracle) -- the
+FP64 force-error bar of this path.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, RC
+
+import dmdgpu
+import dmdgpu.lattice as lat
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "cluster_mgh": lambda: lat.cluster(3, 2, 2, x_h=(0.0, 0.4), seed=3),
+    "hcp_mgh_5x3x3": lambda: lat.hcp_mgh(5, 3, 3, x_h=(0.0, 0.3), seed=9),
+    "mg_5x5x5_c1": lambda: lat.hcp_mgh(5, 5, 5, with_h=False, seed=1),
+    "dump_x5": lambda: lat.read_dump(os.path.join(GOLDEN, "Mg-hcp-cube-x5-x5-x5.dump")),
+}
+
+
+def run_both(gpu, oracle, sysd, k1=4, k2=4):
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    assert st == 0
+    gpu.set_state(sysd)
+    gpu.set_lanes(k1, k2)
+    gpu.neighbors()
+    gn, gl = gpu.get_neighbors()
+    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr))
+    return (nnb, nbr), (gn, gl), ref
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_neighbors_bit_exact(gpu, oracle, case):
+    say(f"[gpu] neighbors {case}")
+    sysd = CASES[case]()
+    (nnb, nbr), (gn, gl), _ = run_both(gpu, oracle, sysd)
+    assert np.array_equal(gn, nnb)
+    for i in range(sysd["n"]):
+        assert np.array_equal(gl[i, :gn[i]], nbr[i, :nnb[i]])
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_residual_parity_fp64(gpu, oracle, case):
+    say(f"[gpu] residual {case}")
+    sysd = CASES[case]()
+    _, _, ref = run_both(gpu, oracle, sysd)
+    out = gpu.residual()
+    assert rel(out["Rq"], ref["Rq"]) < TOL
+    assert rel(out["Rs"], ref["Rs"]) < TOL
+    assert rel(out["Rx"], ref["Rx"]) < TOL
+    assert out["S0_total"] == pytest.approx(ref["S0_total"], rel=TOL)
+    assert out["V0_total"] == pytest.approx(ref["V0_total"], rel=TOL)
+    assert rel(gpu.mf_rho(), ref["mf_rho"]) < TOL
+    V0, S0 = gpu.site_energies()
+    assert rel(V0, ref["V0"]) < TOL
+    assert rel(S0, ref["S0"]) < TOL
+
+
+@pytest.mark.parametrize("k1,k2", [(1, 1), (2, 8), (8, 2)])
+def test_lane_counts_agree(gpu, oracle, k1, k2):
+    say(f"[gpu] lanes {k1} {k2}")
+    sysd = CASES["hcp_mgh_5x3x3"]()
+    _, _, ref = run_both(gpu, oracle, sysd, k1, k2)
+    out = gpu.residual()
+    assert rel(out["Rq"], ref["Rq"]) < TOL
+    assert rel(out["Rs"], ref["Rs"]) < TOL
+    assert rel(out["Rx"], ref["Rx"]) < TOL
+    gpu.set_lanes(4, 4)
+
+
+def test_reference_order_spot_check(gpu, oracle):
+    """GPU against the reference-order restatement directly (small cluster)."""
+    say("[gpu] reference-order spot check")
+    sysd = lat.cluster(2, 1, 1, x_h=(0.0, 0.4), seed=3)
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    s = oracle.system(sysd, nnb, nbr)
+    rho = oracle.ref_mf_rho(s)
+    Rq, Rs, Rx = oracle.ref_residual(s, rho)
+    gpu.set_state(sysd)
+    gpu.neighbors()
+    out = gpu.residual()
+    assert rel(gpu.mf_rho(), rho) < TOL
+    assert rel(out["Rq"], Rq) < TOL
+    assert rel(out["Rs"], Rs) < TOL
+    h = sysd["species"] == 1
+    assert rel(out["Rx"][h], Rx[h]) < TOL
+
+
+def test_update_state_and_determinism(gpu, oracle):
+    say("[gpu] update_state + determinism")
+    sysd = CASES["hcp_mgh_5x3x3"]()
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    gpu.set_state(sysd)
+    gpu.neighbors()
+    rng = np.random.default_rng(5)
+    sysd["q"] = sysd["q"] + rng.normal(0, 0.02, sysd["q"].shape)
+    sysd["sigma"] = sysd["sigma"] * (1 + 0.05 * rng.random(sysd["n"]))
+    hs = sysd["species"] == 1
+    sysd["x"][hs] = rng.uniform(0, 0.3, hs.sum())
+    gpu.update_state(sysd["q"], sysd["sigma"], sysd["x"])
+    a = gpu.residual()
+    b = gpu.residual()
+    for k in ("Rq", "Rs", "Rx"):
+        assert np.array_equal(a[k], b[k])
+    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr))
+    assert rel(a["Rq"], ref["Rq"]) < TOL
+    assert rel(a["Rx"], ref["Rx"]) < TOL
+
+
+def test_full_size_mgh_20x15x15(gpu, oracle):
+    """The 54,000-site Mg-H workload (18k Mg + 36k tetrahedral H):
+    neighbour lists bit-exact and every residual entry within 1e-10."""
+    say("[gpu] full-size 20x15x15")
+    sysd = lat.hcp_mgh(20, 15, 15, x_h=(0.0, 0.1), seed=2024)
+    assert sysd["n"] == 54000
+    (nnb, nbr), (gn, gl), ref = run_both(gpu, oracle, sysd)
+    assert np.array_equal(gn, nnb)
+    assert np.array_equal(np.where(np.arange(250)[None, :] < gn[:, None], gl, -1),
+                          np.where(np.arange(250)[None, :] < nnb[:, None], nbr, -1))
+    out = gpu.residual()
+    assert rel(out["Rq"], ref["Rq"]) < TOL
+    assert rel(out["Rs"], ref["Rs"]) < TOL
+    assert rel(out["Rx"], ref["Rx"]) < TOL
+    assert out["S0_total"] == pytest.approx(ref["S0_total"], rel=TOL)
+    # size-independent property: translation invariance of the periodic sum
+    assert np.abs(out["Rq"].sum(0)).max() < 1e-9 * np.abs(out["Rq"]).max() * sysd["n"]
+
+
+def test_error_codes(gpu):
+    say("[gpu] error codes")
+    small = lat.hcp_mgh(3, 2, 2, seed=0)  # periodic edges < 2 r_c
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.set_state(small)
+    assert e.value.code == 7
+    bad = lat.cluster(2, 1, 1, seed=0)
+    bad["species"][0] = 13
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.set_state(bad)
+    assert e.value.code == 4
+    iso = lat.cluster(2, 1, 1, seed=0)
+    iso["q"][0] += 100.0
+    gpu.set_state(iso)
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.neighbors()
+    assert e.value.code == 6
+    dense = lat.cluster(3, 2, 2, a=1.2, seed=0)  # ~1000 sites within r_c
+    gpu.set_state(dense)
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.neighbors()
+    assert e.value.code == 5

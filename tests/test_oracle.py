@@ -1,0 +1,372 @@
+"""CPU tests that pin the oracle (no GPU).
+
+The reference has no tests or golden vectors for this path, so the oracle is
+pinned by: the reference's own table data and its closed-form coefficient
+relations; closed forms of the quadrature rule on polynomial tables; an
+independent numpy enumeration of the reference's quadrature points; agreement of
+the two oracle forms (reference order vs factorised); finite differences of the
+free-entropy total; and symmetry (translation invariance, zero forces on a
+perfect hcp crystal).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, RC, TABLE_DIR
+
+import dmdgpu.lattice as lat
+import dmdgpu.tables as tabs
+from oracle.oracle import Oracle
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# Tables: src/ADP/MgHx-ADP.cpp:366-501, src/Numerical/cubic-spline.cpp:84-128
+# ---------------------------------------------------------------------------
+
+def _raw_blocks(path, alloy):
+    vals = open(path).read().split()
+    pos = 1 if alloy else 3
+    blocks = []
+    while pos < len(vals):
+        n, dx = int(vals[pos]), float(vals[pos + 1])
+        rows = np.array(vals[pos + 2:pos + 2 + 9 * n], dtype=np.float64).reshape(n, 9)
+        blocks.append((n, dx, rows))
+        pos += 2 + 9 * n
+    return blocks
+
+
+@pytest.mark.parametrize("fname,alloy", [("adpmg.dat", False), ("adph.dat", False),
+                                         ("adpmgh.dat", True)])
+def test_reference_table_columns(fname, alloy):
+    """Derived columns satisfy db=b, dc=2c, dd=3d, ddc=2c, ddd=6d."""
+    blocks = _raw_blocks(os.path.join(TABLE_DIR, fname), alloy)
+    assert len(blocks) == (3 if alloy else 5)
+    for n, dx, r in blocks:
+        a, b, c, d, db, dc, dd, ddc, ddd = r.T
+        scale = np.abs(r).max()
+        for lhs, rhs in ((db, b), (dc, 2 * c), (dd, 3 * d), (ddc, 2 * c), (ddd, 6 * d)):
+            assert np.abs(lhs - rhs).max() <= 1e-9 * scale
+
+
+def test_reference_table_shapes(oracle):
+    # inputs/adpmg.dat:1-2,1197 -- 1194 r-segments of 0.00527527..., 967 embedding segments
+    assert oracle.spline_size(0, 0) == (1194, pytest.approx(0.00527527527528))
+    assert oracle.spline_size(1, 0) == (967, pytest.approx(0.000982083660807))
+    for which in (2, 3, 4):
+        for slot in (0, 1, 2):
+            assert oracle.spline_size(which, slot)[0] == 1194
+
+
+def test_spline_clamp_and_horner(oracle):
+    """Value/derivative use the Horner forms; past n dx the last segment's end
+    value and end slope are returned (cubic-spline.cpp:89-93)."""
+    blocks = _raw_blocks(os.path.join(TABLE_DIR, "adpmg.dat"), False)
+    n, dx, rows = blocks[3]  # u
+    for x in (0.7, 2.3456, 5.0):
+        m = int(x / dx)
+        p = x - m * dx
+        a, b, c, d, db, dc, dd, ddc, ddd = rows[m]
+        assert oracle.spline(3, 0, 0, x) == pytest.approx(a + (b + (c + d * p) * p) * p, rel=1e-14)
+        assert oracle.spline(3, 0, 1, x) == pytest.approx(db + (dc + dd * p) * p, rel=1e-14)
+    a, b, c, d, db, dc, dd, ddc, ddd = rows[-1]
+    end_v = a + (b + (c + d * dx) * dx) * dx
+    end_d = db + (dc + dd * dx) * dx
+    for x in (n * dx + 1e-9, n * dx + 0.3, 7.5):
+        assert oracle.spline(3, 0, 0, x) == pytest.approx(end_v, rel=1e-14, abs=1e-300)
+        assert oracle.spline(3, 0, 1, x) == pytest.approx(end_d, rel=1e-14, abs=1e-300)
+
+
+# ---------------------------------------------------------------------------
+# Neighbours: src/Atoms/Neighbors.cpp:39-134
+# ---------------------------------------------------------------------------
+
+def numpy_neighbors(sysd, rc):
+    q = sysd["q"]
+    L = sysd["hi"] - sysd["lo"]
+    d = q[:, None, :] - q[None, :, :]
+    for a in range(3):
+        if sysd["periodic"][a]:
+            d[..., a] -= L[a] * np.rint(d[..., a] / L[a])
+    r = np.sqrt((d[..., 0] ** 2 + d[..., 1] ** 2) + d[..., 2] ** 2)
+    np.fill_diagonal(r, np.inf)
+    return [np.nonzero(r[i] <= rc)[0] for i in range(len(q))]
+
+
+@pytest.mark.parametrize("maker", [lambda: lat.hcp_mgh(5, 3, 3, seed=1),
+                                   lambda: lat.cluster(3, 2, 2, seed=2),
+                                   lambda: lat.read_dump(os.path.join(GOLDEN, "Mg-hcp-cube-x5-x5-x5.dump"))])
+def test_neighbors_match_numpy(oracle, maker):
+    sysd = maker()
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    assert st == 0
+    ref = numpy_neighbors(sysd, RC)
+    for i in range(sysd["n"]):
+        assert list(nbr[i, :nnb[i]]) == list(ref[i])
+
+
+def test_dump_fixture_reader():
+    d = lat.read_dump(os.path.join(GOLDEN, "Mg-hcp-cube-x5-x5-x5.dump"))
+    assert d["n"] == 500 and (d["species"] == 12).all() and list(d["periodic"]) == [1, 1, 1]
+    assert d["beta"][0] == pytest.approx(38.6817305917)
+    assert d["sigma"][0] == pytest.approx(0.01)
+
+
+def test_neighbor_errors(oracle):
+    sysd = lat.cluster(2, 1, 1, seed=0)
+    sysd["q"][0] += 100.0  # isolated site
+    st, _, _ = oracle.neighbors(sysd, RC)
+    assert st == 2
+
+
+# ---------------------------------------------------------------------------
+# Quadrature rule closed forms on polynomial tables
+# ---------------------------------------------------------------------------
+
+def _poly_tables(tmp, rho, F, phi, u, w):
+    fn = {"rho_Mg": tabs.poly(*rho), "rho_H": tabs.poly(*rho), "F_Mg": tabs.poly(*F),
+          "F_H": tabs.poly(*F)}
+    for pp in ("MgMg", "HH", "MgH"):
+        fn["phi_" + pp] = tabs.poly(*phi)
+        fn["u_" + pp] = tabs.poly(*u)
+        fn["w_" + pp] = tabs.poly(*w)
+    tabs.write_tables(str(tmp), fn, nr=2400, dxr=0.005, ne=2000, dxe=0.05)
+    return Oracle(str(tmp))
+
+
+def _small_cluster(seed=4, with_h=True):
+    s = lat.cluster(2, 1, 1, with_h=with_h, x_h=(0.2, 0.9), seed=seed, sigma_jitter=0.3)
+    return s
+
+
+def test_quadrature_closed_form_density_and_dipole(tmp_path):
+    """rho = 0.01 r^2, F = rho_bar, u = 1, phi = w = 0:
+    rho_bar_i = 0.01 sum_j n_j (|r_ij|^2 + 3 (s_i^2 + s_j^2))   (6D rule exact to degree 3)
+    V0_i = n_i rho_bar_i + 1/2 n_i^2 [ |sum_j n_j r_ij|^2 + 3 s_i^2 (sum_j n_j)^2
+                                       + 3 sum_j n_j^2 s_j^2 ]"""
+    O = _poly_tables(tmp_path, rho=(0, 0, 0.01), F=(0, 1), phi=(0,), u=(1,), w=(0,))
+    sysd = _small_cluster()
+    st, nnb, nbr = O.neighbors(sysd, RC)
+    assert st == 0
+    s = O.system(sysd, nnb, nbr)
+    f = O.fact_residual(s)
+    rho_ref = O.ref_mf_rho(s)
+    V0_ref = O.ref_V0(s, rho_ref)
+    q, sg, x = sysd["q"], sysd["sigma"], sysd["x"]
+    rho_cf = np.zeros(sysd["n"])
+    V0_cf = np.zeros(sysd["n"])
+    for i in range(sysd["n"]):
+        js = nbr[i, :nnb[i]]
+        r = q[i] - q[js]
+        nj = x[js]
+        rho_cf[i] = 0.01 * np.sum(nj * ((r ** 2).sum(1) + 3 * (sg[i] ** 2 + sg[js] ** 2)))
+        G = (nj[:, None] * r).sum(0)
+        V0_cf[i] = x[i] * rho_cf[i] + 0.5 * x[i] ** 2 * (
+            G @ G + 3 * sg[i] ** 2 * nj.sum() ** 2 + 3 * np.sum(nj ** 2 * sg[js] ** 2))
+    assert rel(rho_ref, rho_cf) < 1e-12
+    assert rel(f["mf_rho"], rho_cf) < 1e-12
+    assert rel(V0_ref, V0_cf) < 1e-12
+    assert rel(f["V0"], V0_cf) < 1e-12
+
+
+def numpy_stroud_V0(sysd, nnb, nbr, rho, F, phi, u, w):
+    """Independent enumeration of the reference's rule for V0_i: pair integrals
+    with sqrt(6) sigma shifts (weight 1/12), distinct triplets with 3 sigma
+    shifts of one site (weight 1/18, factor 2 for j1 < j2), coincident triplets
+    with i or j (both copies) shifted by sqrt(6) sigma (weight 1/12).
+    rho/F/phi/u/w are plain numpy callables (polynomials)."""
+    q, sg, x = sysd["q"], sysd["sigma"], sysd["x"]
+    E = np.eye(3)
+
+    def dip_quad(qi, q1, q2, ni, n1, n2):
+        r1, r2 = qi - q1, qi - q2
+        l1, l2 = np.linalg.norm(r1), np.linalg.norm(r2)
+        a1, a2 = ni * n1 * u(l1), ni * n2 * u(l2)
+        b1, b2 = ni * n1 * w(l1), ni * n2 * w(l2)
+        dot = r1 @ r2
+        return 0.5 * a1 * a2 * dot + 0.5 * b1 * b2 * dot ** 2 - b1 * b2 * (r1 @ r1) * (r2 @ r2) / 6
+
+    V0 = np.zeros(sysd["n"])
+    for i in range(sysd["n"]):
+        js = nbr[i, :nnb[i]]
+        rb, vp, vt = 0.0, 0.0, 0.0
+        for j in js:
+            acc_r, acc_p = 0.0, 0.0
+            for who in (0, 1):
+                for l in range(3):
+                    for s in (1, -1):
+                        d = s * np.sqrt(6) * (sg[i] if who == 0 else sg[j]) * E[l]
+                        qi = q[i] + (d if who == 0 else 0)
+                        qj = q[j] + (d if who == 1 else 0)
+                        rr = np.linalg.norm(qi - qj)
+                        acc_r += x[j] * rho(rr) / 12
+                        acc_p += 0.5 * x[i] * x[j] * phi(rr) / 12
+            rb += acc_r
+            vp += acc_p
+        for a, j1 in enumerate(js):
+            for j2 in js[a:]:
+                if j1 == j2:
+                    acc = 0.0
+                    for who in (0, 1):
+                        for l in range(3):
+                            for s in (1, -1):
+                                d = s * np.sqrt(6) * (sg[i] if who == 0 else sg[j1]) * E[l]
+                                qi = q[i] + (d if who == 0 else 0)
+                                qj = q[j1] + (d if who == 1 else 0)
+                                acc += dip_quad(qi, qj, qj, x[i], x[j1], x[j1]) / 12
+                    vt += acc
+                else:
+                    acc = 0.0
+                    for who in range(3):
+                        site = (i, j1, j2)[who]
+                        for l in range(3):
+                            for s in (1, -1):
+                                pos = [q[i].copy(), q[j1].copy(), q[j2].copy()]
+                                pos[who] = pos[who] + s * 3.0 * sg[site] * E[l]
+                                acc += dip_quad(*pos, x[i], x[j1], x[j2]) / 18
+                    vt += 2 * acc
+        V0[i] = x[i] * F(rb) + vp + vt
+    return V0
+
+
+def test_numpy_enumeration_of_stroud_points(tmp_path):
+    """Both oracle forms equal an independent enumeration of the reference's
+    quadrature points for polynomial u, w, phi, rho, F (all terms on)."""
+    c = dict(rho=(0.02, -0.001, 0.002), F=(0.1, -0.5, 0.3), phi=(0.4, -0.1, 0.004, 0.0002),
+             u=(0.3, -0.02, 0.001), w=(0.1, 0.01, -0.001))
+    O = _poly_tables(tmp_path, **c)
+    sysd = _small_cluster(seed=7)
+    st, nnb, nbr = O.neighbors(sysd, RC)
+    s = O.system(sysd, nnb, nbr)
+    f = O.fact_residual(s)
+    V0_ref = O.ref_V0(s, O.ref_mf_rho(s))
+    P = {k: np.polynomial.polynomial.Polynomial(v) for k, v in c.items()}
+    V0_np = numpy_stroud_V0(sysd, nnb, nbr, **P)
+    assert rel(V0_ref, V0_np) < 1e-12
+    assert rel(f["V0"], V0_np) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Reference order vs factorised, with the reference tables
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("maker", [lambda: lat.cluster(2, 1, 1, x_h=(0.0, 0.4), seed=3),
+                                   lambda: lat.hcp_mgh(4, 3, 3, x_h=(0.0, 0.3), seed=9)])
+def test_reference_order_equals_factorised(oracle, maker):
+    sysd = maker()
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    assert st == 0
+    s = oracle.system(sysd, nnb, nbr)
+    f = oracle.fact_residual(s)
+    rho_ref = oracle.ref_mf_rho(s)
+    assert rel(f["mf_rho"], rho_ref) < 1e-13
+    n = sysd["n"]
+    sites = np.arange(n) if n < 100 else np.random.default_rng(0).choice(n, 12, replace=False)
+    assert rel(f["V0"][sites], oracle.ref_V0(s, rho_ref, sites)) < 1e-13
+    assert rel(f["S0"][sites], oracle.ref_S0(s, rho_ref, sites)) < 1e-13
+    Rq, Rs, Rx = oracle.ref_residual(s, rho_ref, sites)
+    assert rel(f["Rq"][sites], Rq) < 1e-12
+    assert rel(f["Rs"][sites], Rs) < 1e-12
+    h = sysd["species"][sites] == 1
+    # Mg occupancy is not an unknown; the reference's Mg embedding term for
+    # i == u omits F(rho_bar) (MgHx-mf-V-bulk.cpp:2311-2315), so compare H rows.
+    assert rel(f["Rx"][sites][h], Rx[h]) < 1e-12
+
+
+def test_masked_sites_and_zero_occupancy(oracle):
+    sysd = lat.cluster(2, 1, 1, x_h=(0.0, 0.3), seed=11)
+    hs = np.nonzero(sysd["species"] == 1)[0]
+    sysd["x"][hs[:4]] = [0.0, 1e-4, 4.99e-4, 5.01e-4]
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    s = oracle.system(sysd, nnb, nbr)
+    f = oracle.fact_residual(s)
+    for k in range(3):
+        u = hs[k]
+        assert f["V0"][u] == 0 and f["S0"][u] == 0 and (f["Rq"][u] == 0).all()
+    rho_ref = oracle.ref_mf_rho(s)
+    assert rel(f["mf_rho"], rho_ref) < 1e-13
+    Rq, Rs, _ = oracle.ref_residual(s, rho_ref)
+    assert rel(f["Rq"], Rq) < 1e-12 and rel(f["Rs"], Rs) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Finite differences and symmetry
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def synth(tmp_path_factory):
+    d = tmp_path_factory.mktemp("synth")
+    tabs.synthetic_mgh(str(d), seed=3, nr=3000, ne=3000)
+    return Oracle(str(d))
+
+
+def test_finite_differences_of_free_entropy(synth):
+    """R = d(sum_i S0_i)/d(q, sigma, x) on smooth tables (consistent clamp)."""
+    sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21)
+    st, nnb, nbr = synth.neighbors(sysd, 6.5)
+    assert st == 0
+    base = synth.fact_residual(synth.system(sysd, nnb, nbr))
+
+    def tot(key, u, a, h):
+        sd = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sysd.items()}
+        if key == "q":
+            sd["q"][u, a] += h
+        else:
+            sd[key][u] += h
+        return synth.fact_residual(synth.system(sd, nnb, nbr))["S0_total"]
+
+    rng = np.random.default_rng(1)
+    for u in rng.choice(sysd["n"], 4, replace=False):
+        for a in range(3):
+            h = 1e-4
+            fd = (tot("q", u, a, h) - tot("q", u, a, -h)) / (2 * h)
+            assert fd == pytest.approx(base["Rq"][u, a], rel=1e-5, abs=1e-5)
+        h = 1e-5
+        fd = (tot("sigma", u, 0, h) - tot("sigma", u, 0, -h)) / (2 * h)
+        assert fd == pytest.approx(base["Rs"][u], rel=1e-6)
+        if sysd["species"][u] == 1:
+            h = 1e-6
+            fd = (tot("x", u, 0, h) - tot("x", u, 0, -h)) / (2 * h)
+            assert fd == pytest.approx(base["Rx"][u], rel=1e-6)
+
+
+def test_translation_invariance_free_cluster(oracle):
+    sysd = lat.cluster(3, 2, 2, x_h=(0.0, 0.3), seed=5)
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    f = oracle.fact_residual(oracle.system(sysd, nnb, nbr))
+    total = f["Rq"].sum(0)
+    assert np.abs(total).max() < 1e-10 * np.abs(f["Rq"]).max() * sysd["n"]
+
+
+def test_perfect_hcp_symmetry(oracle):
+    """The Stroud points lie on the Cartesian axes, so of the hcp site symmetry
+    (-6m2) the rule keeps the mirrors x -> -x and z -> -z: on a perfect crystal
+    the x and z force components vanish, the y component is the same for every
+    site up to sign, and every width residual is equal."""
+    sysd = lat.hcp_mgh(5, 3, 3, with_h=False, pos_noise=0.0, sigma_jitter=0.0, seed=0)
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    f = oracle.fact_residual(oracle.system(sysd, nnb, nbr))
+    scale = np.abs(f["Rq"]).max()
+    assert np.abs(f["Rq"][:, [0, 2]]).max() < 1e-9 * max(scale, 1.0)
+    assert np.ptp(np.abs(f["Rq"][:, 1])) < 1e-9 * max(scale, 1.0)
+    assert abs(f["Rq"][:, 1].sum()) < 1e-9 * max(scale, 1.0) * sysd["n"]
+    assert np.ptp(f["Rs"]) < 1e-9 * np.abs(f["Rs"]).max()
+
+
+def test_zero_width_limit_is_zero_temperature_density(oracle):
+    """sigma -> 0: the mean-field density tends to the zero-T density
+    (src/ADP/density.cpp:44-104) at O(sigma^2)."""
+    sysd = lat.read_dump(os.path.join(GOLDEN, "Mg-hcp-cube-x5-x5-x5.dump"))
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    errs = []
+    for sgv in (1e-2, 1e-3):
+        sysd["sigma"][:] = sgv
+        s = oracle.system(sysd, nnb, nbr)
+        f = oracle.fact_residual(s)
+        errs.append(rel(f["mf_rho"], oracle.ref_rho0(s)))
+    assert errs[1] < 2e-2 * errs[0] + 1e-13
