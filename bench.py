@@ -29,7 +29,7 @@ import numpy as np  # noqa: E402
 FLOP_PER_PAIR = {"phase1_site_sums": 1814, "phase2_gradient": 4755}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12          # B/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phase2_gradient_mgh_20x15x15_hbm.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_mgh_20x15x15_fp64.md")
 
 
 def log(*a):

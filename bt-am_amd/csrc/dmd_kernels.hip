@@ -467,7 +467,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Box box, Tables T, Outp
             const double cs = (cphi * cder(cp, pp) + cra * cder(ck, pp) + crb * cder(cu, pp)) * ir;
             g3[0] += cs * p[0]; g3[1] += cs * p[1]; g3[2] += cs * p[2];
             q0 += g3[0]; q1 += g3[1]; q2 += g3[2];
-            if (!wsel) sacc += sh * g3[l];
+            if (!wsel) sacc += (sg ? -kSqrt6 : kSqrt6) * g3[l];
             xacc += xang * dotM(ps, ps) + xrho * cval(cu, pp) + xphi * cval(cp, pp);
           }
         }

@@ -162,7 +162,7 @@ def test_error_codes(gpu):
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.neighbors()
     assert e.value.code == 6
-    dense = lat.cluster(3, 2, 2, a=1.2, seed=0)  # ~1000 sites within r_c
+    dense = lat.cluster(6, 4, 4, a=1.2, seed=0)  # 1152 sites, most within r_c
     gpu.set_state(dense)
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.neighbors()
