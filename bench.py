@@ -75,13 +75,43 @@ def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
                       f"({dt:.1f} s, factorised oracle, OpenMP)"}
 
 
+def split_leg(ctx, args, pqe):
+    """The split-precision evaluation (FP32 points, FP64 sums/totals) of the
+    same workload: its throughput and its error against the FP64 result."""
+    from dmdgpu.tune import autotune
+    ref = ctx.residual()
+    ctx.set_precision("split32")
+    cfg, _ = autotune(ctx)
+    out = ctx.residual()
+    for _ in range(args.warmup):
+        ctx.residual_resident()
+    ctx.sync()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.residual_resident()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.set_timing(False)
+    ctx.reset_kernel_times()
+    ctx.set_precision("fp64")
+    err = {k: float(np.abs(out[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in ("Rq", "Rs", "Rx")}
+    err["S0_total"] = float(abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"]))
+    return {"value": pqe * args.steps / dt, "unit": "PQE/s", "ms_per_step": dt * 1e3 / args.steps,
+            "tuning_k1_k2_unroll": list(cfg), "max_rel_err_vs_fp64": err,
+            "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--lanes", type=str, default="4,4")
+    ap.add_argument("--tuning", type=str, default="auto",
+                    help="auto, or k1,k2,unroll2 (lanes per site in phase 1/2, unroll)")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -94,8 +124,13 @@ def main():
     ctx = dmdgpu.Context(0)
     ctx.set_state(sysd)
     ctx.neighbors()
-    k1, k2 = (int(v) for v in args.lanes.split(","))
-    ctx.set_lanes(k1, k2)
+    from dmdgpu.tune import autotune
+    if args.tuning == "auto":
+        (k1, k2, un), table = autotune(ctx)
+        log(f"autotune fp64: {(k1, k2, un)} {table}")
+    else:
+        k1, k2, un = (int(v) for v in args.tuning.split(","))
+        ctx.set_tuning(k1, k2, un)
     pairs, pqe = ctx.work_counts()
     log(f"active ordered pairs {pairs}, PQE per residual {pqe}")
 
@@ -126,6 +161,11 @@ def main():
             "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
             "flop_per_launch": FLOP_PER_PAIR[dom] * pairs}
 
+    split = None
+    if not args.no_split:
+        split = split_leg(ctx, args, pqe)
+        log(f"split32: {split}")
+
     cpu = None
     if not args.no_cpu:
         nn, lst = ctx.get_neighbors()
@@ -141,9 +181,10 @@ def main():
                                "(x~U[0,0.1]), reference ADP tables, 3rd-order Stroud rule, "
                                "full FP64 residual per step",
                    "sites": int(sysd["n"]), "active_pairs": int(pairs),
-                   "pqe_per_step": int(pqe), "lanes_per_site": [k1, k2],
+                   "pqe_per_step": int(pqe), "tuning_k1_k2_unroll": [k1, k2, un],
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "split32": split,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -16,19 +16,24 @@ constexpr int kRecFP = 72;         // F'(rho_bar)
 constexpr int kRecDVDN = 73;       // dV0_i / dx_i (own occupancy)
 constexpr int kRecRHO = 74;        // rho_bar
 
-// Spline tables resident in HBM (L2-resident in practice: ~1.5 MB for the
-// reference tables).  All r-tables share one (nr, dx) grid.  Coefficient rows:
-//   uw[t][m] : u: a b c d db dc dd 0 | w: a b c d db dc dd 0   (16 doubles)
-//   ph[t][m] : phi: a b c d db dc dd 0                           (8 doubles)
-//   rh[s][m] : rho: a b c d db dc dd 0
-//   em[s][m] : embedding F: a b c d db dc dd 0 (own grid per species)
+// Spline tables resident in HBM (L2-resident in practice: ~0.8 MB for the
+// reference tables).  All r-tables share one (nr, dx) grid.  Coefficient rows
+// hold the value coefficients only (derivatives are b + (2c + 3d p) p):
+//   uw[t][m] : u: a b c d | w: a b c d   (8 doubles, one 64-B line segment)
+//   ph[t][m] : phi: a b c d               (4 doubles)
+//   rh[s][m] : rho: a b c d
+//   em[s][m] : embedding F: a b c d (own grid per species)
 struct Tables {
   const double* uw;
   const double* ph;
   const double* rh;
   const double* em;
+  const float* uw32;   // FP32 copies of uw, ph, rh (split-precision path)
+  const float* ph32;
+  const float* rh32;
   int nr;
   double dxr, inv_dxr;
+  float dxr32, inv_dxr32;
   int ne[2];
   int ne_stride;
   double dxe[2], inv_dxe[2];
@@ -74,10 +79,20 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
                             const double4* qs, int* cell_of, int* cell_count, int* cell_start,
                             int* cell_cursor, int* cell_atoms, int* nnb, int* nbr, int* flags);
 
-hipError_t launch_phase1(hipStream_t st, int K, const State& s, const Box& box, const Tables& T,
-                         const Outputs& o);
-hipError_t launch_phase2(hipStream_t st, int K, const State& s, const Box& box, const Tables& T,
-                         const Outputs& o);
+// Precision of the quadrature-point evaluation: FP64 throughout, or the split
+// form (FP32 points and splines, FP64 distances, sums, records and totals).
+enum Precision { kFP64 = 0, kSplit32 = 1 };
+
+struct PhaseConfig {
+  int precision;   // Precision
+  int k1, k2;      // lanes per site in phase 1 / phase 2 (1, 2, 4, 8)
+  int unroll2;     // 1: fully unrolled point loops in phase 2, 0: rolled
+};
+
+hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+                         const Tables& T, const Outputs& o);
+hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+                         const Tables& T, const Outputs& o);
 hipError_t launch_totals(hipStream_t st, int n, const Outputs& o);
 hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_pairs_per_block,
                                int nblocks);

@@ -1,0 +1,470 @@
+// dmd_phase.hip -- phase-1 / phase-2 kernels of the mean-field DMD residual.
+//
+// Work decomposition: K consecutive lanes of a wavefront own one site; each
+// lane walks every K-th neighbour of the site and the K partial sums are
+// combined with a butterfly of cross-lane shuffles.  All reductions run in a
+// fixed order, so a site's result does not depend on what else is in flight.
+//
+// The per-point quantities are the reference's Gaussian-averaged ADP terms
+//   rho_ij  = n_j rho(r)                 (src/ADP/MgHx-ADP.cpp:526-553)
+//   V_pair  = 1/2 n_i n_j phi(r)         (:869-900)
+//   V_dip   = 1/2 a_j1 . a_j2, a = n_i n_j u(r) r                 (:1233-1281)
+//   V_quad  = 1/2 B_j1:B_j2 - 1/6 trB_j1 trB_j2, B = n_i n_j w r r^T (:1781-1831)
+// under the 3rd-order Stroud rule (src/Numerical/Quadrature-Hermitian-3th.cpp):
+// sqrt(6) sigma shifts with weight 1/12 for pairs and coincident triplets,
+// 3 sigma shifts with weight 1/18 for distinct triplets.  The reference sums
+// the triplets over ordered pairs j1 <= j2 (MgHx-mf-V-bulk.cpp:879-936); the
+// angular terms are bilinear in psi_j(r) = n_j (u r, w r r^T), and the 9D rule
+// moves one site at a time, so with
+//   G0 = sum_j psi_j(r_j), G_ls = sum_j psi_j(r_j + 3 s sigma_i e_l),
+//   T_j = sum_ls psi_j(r_j + 3 s sigma_j e_l), U = sum_j T_j
+// the distinct-triplet sum is
+//   1/18 [ sum_ls (<G_ls,G_ls> - sum_j <psi_j^ls,psi_j^ls>) + 2<U,G0>
+//          - 2 sum_j <T_j, psi_j(r_j)> ]
+// -- O(nb) per site instead of O(nb^2).  Phase 1 forms these sums; phase 2
+// forms the gradient from their adjoints.
+//
+// Precision: P is the type of the quadrature-point evaluation (splines,
+// feature vectors, adjoint combination, gradients).  Per-site sums, records,
+// the embedding, entropy terms and totals are FP64 for both P.
+#include "dmd_device.h"
+
+namespace dmd {
+
+// ---------------------------------------------------------------------------
+// Phase 1: per-site sums.  For site i:
+//   G0 = sum_j n_j psi(r_ij),  G_ls = sum_j n_j psi(r_ij + 3 s sigma_i e_l),
+//   U = sum_j n_j T_j with T_j = sum_ls psi(r_ij + 3 s sigma_j e_l),
+//   rho_bar, pair sum, coincident and self terms on the sqrt(6) sigma points.
+// Writes V0_i, rho_bar_i and the adjoint record of site i.
+// ---------------------------------------------------------------------------
+template <typename P, int K>
+__global__ __launch_bounds__(256) void k_phase1(State s, Box box, Tables T, Outputs o) {
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int gt = blk * blockDim.x + threadIdx.x;
+  const int i = gt / K;
+  const int g = gt % K;
+  if (i >= s.n) return;
+  double* rec = o.rec + (size_t)i * kRec;
+  const double ni = s.ne[i];
+  if (ni == 0.0) {
+    for (int c = g; c < kRec; c += K) rec[c] = 0.0;
+    if (g == 0) { o.V0[i] = 0.0; o.rho[i] = 0.0; }
+    return;
+  }
+  const P* uwt = tab_uw<P>(T);
+  const P* pht = tab_ph<P>(T);
+  const P* rht = tab_rh<P>(T);
+  const int nr = T.nr;
+  const double4 qi = s.qs[i];
+  const P si = (P)qi.w;
+  const int spi = s.spc[i];
+  double rho = 0, Ep = 0, Cc = 0, selfA = 0, TPsi = 0;
+  double G0[9], U[9], GA[6][9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    G0[c] = 0; U[c] = 0;
+#pragma unroll
+    for (int ls = 0; ls < 6; ls++) GA[ls][c] = 0;
+  }
+  const int nn = s.nnb[i];
+  const int* row = s.nbr + (size_t)i * kCap;
+  for (int kk = g; kk < nn; kk += K) {
+    const int j = row[kk];
+    const double nj = s.ne[j];
+    if (nj == 0.0) continue;
+    const double4 qj = s.qs[j];
+    const P sj = (P)qj.w;
+    const int spj = s.spc[j];
+    const int t = pair_slot(spi, spj);
+    const P* uwr = uwt + ((size_t)t * nr << 3);
+    double rd[3];
+    min_image(box, qi, qj, rd);
+    PairGeo<P> geo;
+    geo.init(rd);
+    P p[3], off, ir, psi0[9], ps[9], Tj[9];
+    int m;
+    geo.point(T, 0, P(0), p, m, off, ir);
+    psi_from(cval(uwr + ((size_t)m << 3), off), cval(uwr + ((size_t)m << 3) + 4, off), p, psi0);
+#pragma unroll
+    for (int c = 0; c < 9; c++) { G0[c] += nj * (double)psi0[c]; Tj[c] = P(0); }
+    const double nj2 = nj * nj;
+    // i-shift points: G_ls accumulators need compile-time indices (unrolled)
+    P selfp = 0;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+#pragma unroll
+      for (int sg = 0; sg < 2; sg++) {
+        const P sh = sg ? P(-3) : P(3);
+        geo.point(T, l, sh * si, p, m, off, ir);
+        const P* c = uwr + ((size_t)m << 3);
+        psi_from(cval(c, off), cval(c + 4, off), p, ps);
+#pragma unroll
+        for (int e = 0; e < 9; e++) GA[2 * l + sg][e] += nj * (double)ps[e];
+        selfp += dotM(ps, ps);
+      }
+    }
+    selfA += nj2 * (double)selfp;
+    // j-shift points
+#pragma unroll 1
+    for (int pt = 0; pt < 6; pt++) {
+      const int l = pt >> 1;
+      geo.point(T, l, (pt & 1 ? P(-3) : P(3)) * sj, p, m, off, ir);
+      const P* cb = uwr + ((size_t)m << 3);
+      psi_from(cval(cb, off), cval(cb + 4, off), p, ps);
+#pragma unroll
+      for (int e = 0; e < 9; e++) Tj[e] += ps[e];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
+    TPsi += nj2 * (double)dotM(Tj, psi0);
+    P Phi = 0, R = 0, C6 = 0;
+    const P* phr = pht + ((size_t)t * nr << 2);
+    const P* rhr = rht + ((size_t)spj * nr << 2);
+    // sqrt(6) sigma points
+#pragma unroll 1
+    for (int pt = 0; pt < 12; pt++) {
+      const int wsel = pt / 6, l = (pt >> 1) % 3;
+      const P sd = P(kSqrt6) * (wsel ? sj : si);
+      geo.point(T, l, pt & 1 ? -sd : sd, p, m, off, ir);
+      const P* c = uwr + ((size_t)m << 3);
+      psi_from(cval(c, off), cval(c + 4, off), p, ps);
+      C6 += dotM(ps, ps);
+      Phi += cval(phr + ((size_t)m << 2), off);
+      R += cval(rhr + ((size_t)m << 2), off);
+    }
+    Cc += nj2 * (double)C6;
+    rho += nj * (double)R * (1.0 / 12.0);
+    Ep += nj * (double)Phi * (1.0 / 12.0);
+  }
+  if (K > 1) {
+    rho = group_sum<K>(rho); Ep = group_sum<K>(Ep); Cc = group_sum<K>(Cc);
+    selfA = group_sum<K>(selfA); TPsi = group_sum<K>(TPsi);
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      G0[c] = group_sum<K>(G0[c]);
+      U[c] = group_sum<K>(U[c]);
+#pragma unroll
+      for (int ls = 0; ls < 6; ls++) GA[ls][c] = group_sum<K>(GA[ls][c]);
+    }
+  }
+  double GG = 0;
+#pragma unroll
+  for (int ls = 0; ls < 6; ls++) GG += dotM(GA[ls], GA[ls]);
+  const double At = (GG - selfA + 2.0 * dotM(U, G0) - 2.0 * TPsi) * (1.0 / 18.0) + Cc * (1.0 / 12.0);
+  double pe;
+  const int me = seg_index(rho, T.dxe[spi], T.inv_dxe[spi], T.ne[spi], pe);
+  const double* ce = T.em + ((size_t)spi * T.ne_stride + me) * 4;
+  const double F = cval(ce, pe), Fp = cder(ce, pe);
+  double tmp[9];
+#pragma unroll
+  for (int ls = 0; ls < 6; ls++) {
+    applyM(1.0 / 9.0, GA[ls], tmp);
+#pragma unroll
+    for (int c = 0; c < 9; c++)
+      if (((ls * 9 + c) % K) == g) rec[kRecLA + ls * 9 + c] = tmp[c];
+  }
+  applyM(1.0 / 9.0, U, tmp);
+#pragma unroll
+  for (int c = 0; c < 9; c++)
+    if (((kRecLU + c) % K) == g) rec[kRecLU + c] = tmp[c];
+  applyM(1.0 / 9.0, G0, tmp);
+#pragma unroll
+  for (int c = 0; c < 9; c++)
+    if (((kRecLG0 + c) % K) == g) rec[kRecLG0 + c] = tmp[c];
+  if (g == 0) {
+    rec[kRecFP] = Fp;
+    rec[kRecDVDN] = F + 0.5 * Ep + 2.0 * ni * At;
+    rec[kRecRHO] = rho;
+    o.V0[i] = ni * F + 0.5 * ni * Ep + ni * ni * At;
+    o.rho[i] = rho;
+  }
+}
+
+// Loads 9 record values as P, negating the dipole part when flip is set.
+template <typename P>
+DEV void load9(const double* r, bool flip, P* o) {
+  const P sgn = flip ? P(-1) : P(1);
+  o[0] = sgn * (P)r[0]; o[1] = sgn * (P)r[1]; o[2] = sgn * (P)r[2];
+#pragma unroll
+  for (int c = 3; c < 9; c++) o[c] = (P)r[c];
+}
+
+// ---------------------------------------------------------------------------
+// Phase 2: gradient of sum_c beta_c V0_c with respect to (q_u, sigma_u, x_u).
+// Role (a): V0_u (u is the centre); role (b): V0_k of each neighbour k (u is a
+// neighbour).  Both roles are evaluated on the same 25 points
+// p = r_uk + shift; their adjoints are combined before one gradient per point.
+// psi(-p) = flip(psi(p)) (the dipole part is odd, the quadrupole part even).
+// ---------------------------------------------------------------------------
+template <typename P, int K, bool UNR>
+__global__ __launch_bounds__(256) void k_phase2(State s, Box box, Tables T, Outputs o) {
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int gt = blk * blockDim.x + threadIdx.x;
+  const int u = gt / K;
+  const int g = gt % K;
+  if (u >= s.n) return;
+  const double nu = s.ne[u];
+  if (nu == 0.0) {
+    if (g == 0) {
+      o.Rq[3 * u] = 0.0; o.Rq[3 * u + 1] = 0.0; o.Rq[3 * u + 2] = 0.0;
+      o.Rs[u] = 0.0; o.Rx[u] = 0.0; o.S0[u] = 0.0;
+    }
+    return;
+  }
+  const P* uwt = tab_uw<P>(T);
+  const P* pht = tab_ph<P>(T);
+  const P* rht = tab_rh<P>(T);
+  const int nr = T.nr;
+  const double4 qu = s.qs[u];
+  const double su = qu.w;
+  const int spu = s.spc[u];
+  const double bu = s.beta[u];
+  const double* Ru = o.rec + (size_t)u * kRec;
+  const double FPu = Ru[kRecFP];
+  double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
+  const int nn = s.nnb[u];
+  const int* row = s.nbr + (size_t)u * kCap;
+  for (int kk = g; kk < nn; kk += K) {
+    const int k = row[kk];
+    const double nk = s.ne[k];
+    if (nk == 0.0) continue;
+    const double4 qk = s.qs[k];
+    const double sk = qk.w;
+    const int spk = s.spc[k];
+    const double bk = s.beta[k];
+    const double* Rk = o.rec + (size_t)k * kRec;
+    const double wa = bu * nu * nu * nk, wb = bk * nk * nk * nu;
+    const double xb = bk * nk * nk;
+    const P waP = (P)wa, wbP = (P)wb;
+    const P cnk = P(-nk * (1.0 / 9.0)), cnu = P(-nu * (1.0 / 9.0));
+    const int t = pair_slot(spu, spk);
+    const P* uwr = uwt + ((size_t)t * nr << 3);
+    double rd[3];
+    min_image(box, qu, qk, rd);
+    PairGeo<P> geo;
+    geo.init(rd);
+    P p0[3], ir0, off, psi0[9], Tuk[9], SA[9], tmp[9];
+    int m;
+    geo.point(T, 0, P(0), p0, m, off, ir0);
+    const P* c0 = uwr + ((size_t)m << 3);
+    const P u0 = cval(c0, off), du0 = cder(c0, off), w0 = cval(c0 + 4, off), dw0 = cder(c0 + 4, off);
+    psi_from(u0, w0, p0, psi0);
+    // Role-(a) adjoint at the j-shift points (constant over them) and role-(b)
+    // adjoint at the i-shift points (constant over them).
+    P laB[9], lbA[9];
+    load9(Ru + kRecLG0, false, laB);
+    applyM(cnk, psi0, tmp);
+#pragma unroll
+    for (int c = 0; c < 9; c++) laB[c] = waP * (laB[c] + tmp[c]);
+    load9(Rk + kRecLG0, true, lbA);
+    applyM(cnu, psi0, tmp);
+#pragma unroll
+    for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; Tuk[c] = P(0); SA[c] = P(0); }
+    const P skP = (P)sk, suP = (P)su;
+
+    // j-shift points of centre u (shift sigma_k) == i-shift points (l,-s) of k
+    auto bpt = [&](int pt) {
+      const int l = pt >> 1, sg = pt & 1;
+      const P sh = sg ? P(-3) : P(3);
+      P p[3], ir, o2, ps[9], lb[9], L[9], g3[3];
+      int mm;
+      geo.point(T, l, sh * skP, p, mm, o2, ir);
+      const P* c = uwr + ((size_t)mm << 3);
+      const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
+      psi_from(uu, ww, p, ps);
+#pragma unroll
+      for (int e = 0; e < 9; e++) Tuk[e] += ps[e];
+      load9(Rk + kRecLA + 9 * (2 * l + (1 - sg)), true, lb);
+      P mps[9];
+      applyM(cnu, ps, mps);
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        lb[e] += mps[e];
+        L[e] = laB[e] + wbP * lb[e];
+      }
+      grad_psi(L, p, ir, uu, du, ww, dw, g3);
+      q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
+      xacc += xb * (double)dot9(lb, ps);
+    };
+    // i-shift points of centre u (shift sigma_u) == j-shift points of k
+    auto apt = [&](int pt) {
+      const int l = pt >> 1, sg = pt & 1;
+      const P sh = sg ? P(-3) : P(3);
+      P p[3], ir, o2, ps[9], la[9], L[9], g3[3];
+      int mm;
+      geo.point(T, l, sh * suP, p, mm, o2, ir);
+      const P* c = uwr + ((size_t)mm << 3);
+      const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
+      psi_from(uu, ww, p, ps);
+#pragma unroll
+      for (int e = 0; e < 9; e++) SA[e] += ps[e];
+      load9(Ru + kRecLA + 9 * (2 * l + sg), false, la);
+      P mps[9];
+      applyM(cnk, ps, mps);
+#pragma unroll
+      for (int e = 0; e < 9; e++) L[e] = waP * (la[e] + mps[e]) + wbP * lbA[e];
+      grad_psi(L, p, ir, uu, du, ww, dw, g3);
+      q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
+      const P gl = l == 0 ? g3[0] : (l == 1 ? g3[1] : g3[2]);
+      sacc += (double)(sh * gl);
+      xacc += xb * (double)dot9(lbA, ps);
+    };
+    if (UNR) {
+#pragma unroll
+      for (int pt = 0; pt < 6; pt++) bpt(pt);
+#pragma unroll
+      for (int pt = 0; pt < 6; pt++) apt(pt);
+    } else {
+#pragma unroll 1
+      for (int pt = 0; pt < 6; pt++) bpt(pt);
+#pragma unroll 1
+      for (int pt = 0; pt < 6; pt++) apt(pt);
+    }
+    // unshifted point
+    {
+      P la[9], lb[9], L[9], g3[3];
+      load9(Ru + kRecLU, false, la);
+      applyM(cnk, Tuk, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) la[c] += tmp[c];
+      load9(Rk + kRecLU, true, lb);
+      applyM(cnu, SA, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) {
+        lb[c] += tmp[c];
+        L[c] = waP * la[c] + wbP * lb[c];
+      }
+      grad_psi(L, p0, ir0, u0, du0, w0, dw0, g3);
+      q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
+      xacc += xb * (double)dot9(lb, psi0);
+    }
+    // sqrt(6) sigma points: coincident angular term, pair and density
+    {
+      const double FPk = Rk[kRecFP];
+      const P cphi = (P)(0.5 * nu * nk * (bu + bk) * (1.0 / 12.0));
+      const P cra = (P)(bu * nu * FPu * nk * (1.0 / 12.0));
+      const P crb = (P)(bk * nk * FPk * nu * (1.0 / 12.0));
+      const P cang = (P)((wa * nk + wb * nu) * (1.0 / 6.0));
+      const double xang = xb * nu * (1.0 / 6.0);
+      const P xrho = (P)(bk * nk * FPk * (1.0 / 12.0));
+      const P xphi = (P)(bk * 0.5 * nk * (1.0 / 12.0));
+      const P* phr = pht + ((size_t)t * nr << 2);
+      const P* rkr = rht + ((size_t)spk * nr << 2);
+      const P* rur = rht + ((size_t)spu * nr << 2);
+      auto cpt = [&](int pt) {
+        const int wsel = pt / 6, l = (pt >> 1) % 3, sg = pt & 1;
+        const P sd = P(kSqrt6) * (wsel ? skP : suP);
+        P p[3], ir, o2, ps[9], L[9], g3[3];
+        int mm;
+        geo.point(T, l, sg ? -sd : sd, p, mm, o2, ir);
+        const P* c = uwr + ((size_t)mm << 3);
+        const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
+        const P* cp = phr + ((size_t)mm << 2);
+        const P* ck = rkr + ((size_t)mm << 2);
+        const P* cu = rur + ((size_t)mm << 2);
+        psi_from(uu, ww, p, ps);
+        applyM(cang, ps, L);
+        grad_psi(L, p, ir, uu, du, ww, dw, g3);
+        const P cs = (cphi * cder(cp, o2) + cra * cder(ck, o2) + crb * cder(cu, o2)) * ir;
+        g3[0] += cs * p[0]; g3[1] += cs * p[1]; g3[2] += cs * p[2];
+        q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
+        if (!wsel) {
+          const P gl = l == 0 ? g3[0] : (l == 1 ? g3[1] : g3[2]);
+          sacc += (double)((sg ? P(-kSqrt6) : P(kSqrt6)) * gl);
+        }
+        xacc += xang * (double)dotM(ps, ps) + (double)(xrho * cval(cu, o2) + xphi * cval(cp, o2));
+      };
+      if (UNR) {
+#pragma unroll
+        for (int pt = 0; pt < 12; pt++) cpt(pt);
+      } else {
+#pragma unroll 1
+        for (int pt = 0; pt < 12; pt++) cpt(pt);
+      }
+    }
+  }
+  if (K > 1) {
+    q0 = group_sum<K>(q0); q1 = group_sum<K>(q1); q2 = group_sum<K>(q2);
+    sacc = group_sum<K>(sacc); xacc = group_sum<K>(xacc);
+  }
+  if (g == 0) {
+    const double xu = nu;
+    const double mi = kUma * xu * T.mass[spu];
+    double logZ = 3.0 * log((su * sqrt(mi / bu)) / kHbar);
+    double dx = -1.5 / xu;
+    if (xu < 1.0) {
+      logZ += log(1.0 / (1.0 - xu));
+      dx -= 1.0 / (1.0 - xu);
+    }
+    double gx = 0.0;
+    if (spu == 1) {
+      gx = s.gamma[u] * (xu > 0.9999 ? 0.9999 : xu);
+      if (xu < 0.9999) dx -= s.gamma[u];
+    }
+    const double V0 = o.V0[u];
+    o.S0[u] = -logZ + bu * (1.0 / (2.0 * bu) + V0) - gx;
+    o.Rq[3 * u] = q0; o.Rq[3 * u + 1] = q1; o.Rq[3 * u + 2] = q2;
+    o.Rs[u] = sacc - 3.0 / su;
+    o.Rx[u] = bu * Ru[kRecDVDN] + xacc + dx;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <typename P, int K>
+static void p1(hipStream_t st, const State& s, const Box& b, const Tables& T, const Outputs& o) {
+  const long long threads = (long long)s.n * K;
+  const int nb = (int)((threads + 255) / 256);
+  k_phase1<P, K><<<nb, 256, 0, st>>>(s, b, T, o);
+}
+template <typename P, int K, bool UNR>
+static void p2(hipStream_t st, const State& s, const Box& b, const Tables& T, const Outputs& o) {
+  const long long threads = (long long)s.n * K;
+  const int nb = (int)((threads + 255) / 256);
+  k_phase2<P, K, UNR><<<nb, 256, 0, st>>>(s, b, T, o);
+}
+
+template <typename P>
+static hipError_t phase1_p(hipStream_t st, int K, const State& s, const Box& b, const Tables& T,
+                           const Outputs& o) {
+  switch (K) {
+    case 1: p1<P, 1>(st, s, b, T, o); break;
+    case 2: p1<P, 2>(st, s, b, T, o); break;
+    case 4: p1<P, 4>(st, s, b, T, o); break;
+    case 8: p1<P, 8>(st, s, b, T, o); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename P, bool UNR>
+static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Box& b, const Tables& T,
+                            const Outputs& o) {
+  switch (K) {
+    case 1: p2<P, 1, UNR>(st, s, b, T, o); break;
+    case 2: p2<P, 2, UNR>(st, s, b, T, o); break;
+    case 4: p2<P, 4, UNR>(st, s, b, T, o); break;
+    case 8: p2<P, 8, UNR>(st, s, b, T, o); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+                         const Tables& T, const Outputs& o) {
+  if (cfg.precision == kSplit32) return phase1_p<float>(st, cfg.k1, s, box, T, o);
+  return phase1_p<double>(st, cfg.k1, s, box, T, o);
+}
+
+hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+                         const Tables& T, const Outputs& o) {
+  if (cfg.precision == kSplit32)
+    return cfg.unroll2 ? phase2_pu<float, true>(st, cfg.k2, s, box, T, o)
+                       : phase2_pu<float, false>(st, cfg.k2, s, box, T, o);
+  return cfg.unroll2 ? phase2_pu<double, true>(st, cfg.k2, s, box, T, o)
+                     : phase2_pu<double, false>(st, cfg.k2, s, box, T, o);
+}
+
+}  // namespace dmd

@@ -71,18 +71,21 @@ struct dmdgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   double rc = kDefaultRc;
-  int K1 = 4, K2 = 4;
+  dmd::PhaseConfig cfg{dmd::kFP64, 4, 4, 1};
   // tables
   bool have[3] = {false, false, false};
   Spline rho[2], embed[2], pair[3], u[3], w[3];
   double mass[2] = {0.0, 0.0};
   bool tables_dirty = true;
   DevBuf<double> d_uw, d_ph, d_rh, d_em;
+  DevBuf<float> d_uw32, d_ph32, d_rh32;
   dmd::Tables T{};
   // state
   int n = 0;
   dmd::Box box{};
-  std::vector<double> h_q;
+  std::vector<double> h_q;      // positions, caller's order
+  std::vector<int> perm, inv;   // internal site i holds caller's site perm[i]
+  std::vector<int> h_slot;      // species slot, internal order
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
   DevBuf<int> d_spc;
@@ -155,11 +158,10 @@ bool read_block(FILE* f, Spline& s) {
   return true;
 }
 
-// Packs an r-grid spline into rows of 8 doubles: a b c d db dc dd 0.
-void pack8(const Spline& s, double* out, int stride_rows) {
+// Packs a spline into rows of 4 doubles: a b c d.
+void pack4(const Spline& s, double* out) {
   for (int m = 0; m < s.n; m++)
-    for (int k = 0; k < 7; k++) out[(size_t)m * 8 + k] = s.c[(size_t)m * 9 + k];
-  (void)stride_rows;
+    for (int k = 0; k < 4; k++) out[(size_t)m * 4 + k] = s.c[(size_t)m * 9 + k];
 }
 
 dmdgpu_status upload_tables(dmdgpu_ctx* c) {
@@ -173,22 +175,22 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   for (const Spline* s : rt)
     if (s->n != nr || s->dx != dx)
       return fail(DMDGPU_ERR_TABLES, "rho/phi/u/w tables must share one (n, dx) grid");
-  std::vector<double> uw((size_t)3 * nr * 16, 0.0), ph((size_t)3 * nr * 8, 0.0),
-      rh((size_t)2 * nr * 8, 0.0);
+  std::vector<double> uw((size_t)3 * nr * 8, 0.0), ph((size_t)3 * nr * 4, 0.0),
+      rh((size_t)2 * nr * 4, 0.0);
   for (int t = 0; t < 3; t++) {
     for (int m = 0; m < nr; m++) {
-      double* row = &uw[((size_t)t * nr + m) * 16];
-      for (int k = 0; k < 7; k++) {
+      double* row = &uw[((size_t)t * nr + m) * 8];
+      for (int k = 0; k < 4; k++) {
         row[k] = c->u[t].c[(size_t)m * 9 + k];
-        row[8 + k] = c->w[t].c[(size_t)m * 9 + k];
+        row[4 + k] = c->w[t].c[(size_t)m * 9 + k];
       }
     }
-    pack8(c->pair[t], &ph[(size_t)t * nr * 8], nr);
+    pack4(c->pair[t], &ph[(size_t)t * nr * 4]);
   }
-  for (int s = 0; s < 2; s++) pack8(c->rho[s], &rh[(size_t)s * nr * 8], nr);
+  for (int s = 0; s < 2; s++) pack4(c->rho[s], &rh[(size_t)s * nr * 4]);
   const int ne_stride = std::max(c->embed[0].n, c->embed[1].n);
-  std::vector<double> em((size_t)2 * ne_stride * 8, 0.0);
-  for (int s = 0; s < 2; s++) pack8(c->embed[s], &em[(size_t)s * ne_stride * 8], ne_stride);
+  std::vector<double> em((size_t)2 * ne_stride * 4, 0.0);
+  for (int s = 0; s < 2; s++) pack4(c->embed[s], &em[(size_t)s * ne_stride * 4]);
   HIPCHK(c->d_uw.ensure(uw.size()));
   HIPCHK(c->d_ph.ensure(ph.size()));
   HIPCHK(c->d_rh.ensure(rh.size()));
@@ -197,14 +199,27 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   HIPCHK(hipMemcpy(c->d_ph.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_rh.p, rh.data(), rh.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_em.p, em.data(), em.size() * 8, hipMemcpyHostToDevice));
+  // FP32 copies for the split-precision evaluation
+  std::vector<float> uw32(uw.begin(), uw.end()), ph32(ph.begin(), ph.end()), rh32(rh.begin(), rh.end());
+  HIPCHK(c->d_uw32.ensure(uw32.size()));
+  HIPCHK(c->d_ph32.ensure(ph32.size()));
+  HIPCHK(c->d_rh32.ensure(rh32.size()));
+  HIPCHK(hipMemcpy(c->d_uw32.p, uw32.data(), uw32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_ph32.p, ph32.data(), ph32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rh32.p, rh32.data(), rh32.size() * 4, hipMemcpyHostToDevice));
   dmd::Tables& T = c->T;
   T.uw = c->d_uw.p;
   T.ph = c->d_ph.p;
   T.rh = c->d_rh.p;
   T.em = c->d_em.p;
+  T.uw32 = c->d_uw32.p;
+  T.ph32 = c->d_ph32.p;
+  T.rh32 = c->d_rh32.p;
   T.nr = nr;
   T.dxr = dx;
   T.inv_dxr = 1.0 / dx;
+  T.dxr32 = (float)dx;
+  T.inv_dxr32 = (float)(1.0 / dx);
   T.ne_stride = ne_stride;
   for (int s = 0; s < 2; s++) {
     T.ne[s] = c->embed[s].n;
@@ -255,7 +270,7 @@ dmdgpu_status run_phase1(dmdgpu_ctx* c) {
   hipEvent_t a{};
   dmdgpu_status st = begin_timed(c, KID_PHASE1, &a);
   if (st) return st;
-  HIPCHK(dmd::launch_phase1(c->stream, c->K1, make_state(c), c->box, c->T, make_outputs(c)));
+  HIPCHK(dmd::launch_phase1(c->stream, c->cfg, make_state(c), c->box, c->T, make_outputs(c)));
   return end_timed(c, KID_PHASE1, a);
 }
 
@@ -264,7 +279,7 @@ dmdgpu_status run_residual(dmdgpu_ctx* c) {
   if (st) return st;
   hipEvent_t a{};
   if ((st = begin_timed(c, KID_PHASE2, &a))) return st;
-  HIPCHK(dmd::launch_phase2(c->stream, c->K2, make_state(c), c->box, c->T, make_outputs(c)));
+  HIPCHK(dmd::launch_phase2(c->stream, c->cfg, make_state(c), c->box, c->T, make_outputs(c)));
   if ((st = end_timed(c, KID_PHASE2, a))) return st;
   if ((st = begin_timed(c, KID_TOTALS, &a))) return st;
   HIPCHK(dmd::launch_totals(c->stream, c->n, make_outputs(c)));
@@ -275,6 +290,74 @@ dmdgpu_status d2h(dmdgpu_ctx* c, double* h, const double* d, size_t cnt) {
   if (!h) return DMDGPU_OK;
   HIPCHK(hipMemcpyAsync(h, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return DMDGPU_OK;
+}
+
+// Copies a per-site array of `width` doubles per site back to the caller's
+// site order.
+dmdgpu_status d2h_sites(dmdgpu_ctx* c, double* h, const double* d, int width) {
+  if (!h) return DMDGPU_OK;
+  const int n = c->n;
+  std::vector<double> tmp((size_t)n * width);
+  HIPCHK(hipMemcpyAsync(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < width; k++) h[(size_t)c->perm[i] * width + k] = tmp[(size_t)i * width + k];
+  return DMDGPU_OK;
+}
+
+// Cell grid with edges >= r_c over the box (periodic axes) or the bounding box
+// of the mean positions (free axes).
+dmd::CellGrid make_grid(const dmdgpu_ctx* c) {
+  dmd::CellGrid g{};
+  const int n = c->n;
+  for (int a = 0; a < 3; a++) {
+    double lo, ext;
+    if (c->box.per[a]) {
+      lo = c->box.lo[a];
+      ext = c->box.L[a];
+    } else {
+      double mn = 1e300, mx = -1e300;
+      for (int i = 0; i < n; i++) {
+        mn = std::min(mn, c->h_q[3 * i + a]);
+        mx = std::max(mx, c->h_q[3 * i + a]);
+      }
+      lo = mn;
+      ext = mx - mn;
+    }
+    int nc = (int)std::floor(ext / c->rc);
+    nc = std::max(1, std::min(nc, 1024));
+    g.nc[a] = nc;
+    g.lo[a] = lo;
+    g.cs[a] = ext > 0 ? ext / nc : 1.0;
+    g.inv_cs[a] = 1.0 / g.cs[a];
+  }
+  return g;
+}
+
+// Internal site order: sites sorted by cell (x fastest), ties by the caller's
+// index, so that a contiguous range of sites is a compact region of space and
+// the neighbour records phase 2 gathers stay in one L2.
+void make_order(dmdgpu_ctx* c) {
+  const int n = c->n;
+  const dmd::CellGrid g = make_grid(c);
+  std::vector<long long> key(n);
+  for (int i = 0; i < n; i++) {
+    int cc[3];
+    for (int a = 0; a < 3; a++) {
+      double v = c->h_q[3 * i + a] - g.lo[a];
+      if (c->box.per[a]) v = v - c->box.L[a] * std::floor(v / c->box.L[a]);
+      int k = (int)std::floor(v * g.inv_cs[a]);
+      cc[a] = k < 0 ? 0 : (k > g.nc[a] - 1 ? g.nc[a] - 1 : k);
+    }
+    key[i] = ((long long)cc[2] * g.nc[1] + cc[1]) * g.nc[0] + cc[0];
+  }
+  c->perm.resize(n);
+  for (int i = 0; i < n; i++) c->perm[i] = i;
+  std::stable_sort(c->perm.begin(), c->perm.end(),
+                   [&](int a, int b) { return key[a] < key[b]; });
+  c->inv.resize(n);
+  for (int i = 0; i < n; i++) c->inv[c->perm[i]] = i;
 }
 
 }  // namespace
@@ -309,6 +392,9 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
+  c->d_uw32.release();
+  c->d_ph32.release();
+  c->d_rh32.release();
   DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph, &c->d_rh, &c->d_em, &c->d_ne, &c->d_xr,
                            &c->d_beta, &c->d_gamma, &c->d_rec, &c->d_V0, &c->d_rho, &c->d_Rq,
                            &c->d_Rs, &c->d_Rx, &c->d_S0, &c->d_partials, &c->d_totals};
@@ -364,49 +450,54 @@ dmdgpu_status dmdgpu_set_cutoff(dmdgpu_ctx* c, double rc) {
   return DMDGPU_OK;
 }
 
-dmdgpu_status dmdgpu_set_lanes(dmdgpu_ctx* c, int k1, int k2) {
+dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
   auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8; };
   if (!c || !ok(k1) || !ok(k2)) return fail(DMDGPU_ERR_ARG, "lanes per site must be 1, 2, 4 or 8");
-  c->K1 = k1;
-  c->K2 = k2;
+  c->cfg.k1 = k1;
+  c->cfg.k2 = k2;
+  c->cfg.unroll2 = unroll2 ? 1 : 0;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* c, int precision) {
+  if (!c || (precision != DMDGPU_FP64 && precision != DMDGPU_SPLIT32))
+    return fail(DMDGPU_ERR_ARG, "precision must be DMDGPU_FP64 or DMDGPU_SPLIT32");
+  c->cfg.precision = precision == DMDGPU_SPLIT32 ? dmd::kSplit32 : dmd::kFP64;
   return DMDGPU_OK;
 }
 
 static dmdgpu_status upload_unknowns(dmdgpu_ctx* c, const double* q, const double* sg,
                                      const double* x, const int32_t* spc, bool first) {
   const int n = c->n;
+  (void)spc;
   if (q) c->h_q.assign(q, q + 3 * (size_t)n);
-  static thread_local std::vector<double4> qs;
-  static thread_local std::vector<double> ne, xr;
+  std::vector<double4> qs;
   if (q || sg) {
     qs.resize(n);
     if (!first) HIPCHK(hipMemcpy(qs.data(), c->d_qs.p, sizeof(double4) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) {
+      const int o = c->perm[i];
       if (q) {
-        qs[i].x = q[3 * i];
-        qs[i].y = q[3 * i + 1];
-        qs[i].z = q[3 * i + 2];
+        qs[i].x = q[3 * o];
+        qs[i].y = q[3 * o + 1];
+        qs[i].z = q[3 * o + 2];
       }
       if (sg) {
-        if (!(sg[i] > 0)) return fail(DMDGPU_ERR_ARG, "widths must be positive");
-        qs[i].w = sg[i];
+        if (!(sg[o] > 0)) return fail(DMDGPU_ERR_ARG, "widths must be positive");
+        qs[i].w = sg[o];
       }
     }
     HIPCHK(hipMemcpy(c->d_qs.p, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
   }
   if (x) {
-    ne.resize(n);
-    xr.assign(x, x + n);
-    std::vector<int> sp(n);
-    if (!spc) {
-      HIPCHK(hipMemcpy(sp.data(), c->d_spc.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    }
+    std::vector<double> ne(n), xr(n);
     for (int i = 0; i < n; i++) {
-      int slot = spc ? (spc[i] == 1 ? 1 : 0) : sp[i];
-      if (slot == 0 && x[i] < kMinOcc)
+      const int o = c->perm[i];
+      if (c->h_slot[i] == 0 && x[o] < kMinOcc)
         return fail(DMDGPU_ERR_SPECIES, "Mg site with occupancy below 5e-4 (site " +
-                                            std::to_string(i) + ")");
-      ne[i] = x[i] >= kMinOcc ? x[i] : 0.0;
+                                            std::to_string(o) + ")");
+      xr[i] = x[o];
+      ne[i] = x[o] >= kMinOcc ? x[o] : 0.0;
     }
     HIPCHK(hipMemcpy(c->d_ne.p, ne.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_xr.p, xr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -458,11 +549,19 @@ dmdgpu_status dmdgpu_set_state(dmdgpu_ctx* c, int n, const double* q, const doub
   HIPCHK(c->d_partials.ensure((size_t)2 * ((n + 255) / 256)));
   HIPCHK(c->d_totals.ensure(2));
   HIPCHK(c->d_pairs.ensure((n + 255) / 256));
-  std::vector<int> slot(n);
-  for (int i = 0; i < n; i++) slot[i] = spc[i] == 1 ? 1 : 0;
-  HIPCHK(hipMemcpy(c->d_spc.p, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_beta.p, beta, sizeof(double) * n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_gamma.p, gamma, sizeof(double) * n, hipMemcpyHostToDevice));
+  c->h_q.assign(q, q + 3 * (size_t)n);
+  make_order(c);
+  c->h_slot.resize(n);
+  std::vector<double> bp(n), gp(n);
+  for (int i = 0; i < n; i++) {
+    const int o = c->perm[i];
+    c->h_slot[i] = spc[o] == 1 ? 1 : 0;
+    bp[i] = beta[o];
+    gp[i] = gamma[o];
+  }
+  HIPCHK(hipMemcpy(c->d_spc.p, c->h_slot.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_beta.p, bp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_gamma.p, gp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   st = upload_unknowns(c, q, sg, x, spc, true);
   if (st) return st;
   c->have_state = true;
@@ -480,29 +579,8 @@ dmdgpu_status dmdgpu_update_state(dmdgpu_ctx* c, const double* q, const double* 
 dmdgpu_status dmdgpu_neighbors(dmdgpu_ctx* c) {
   if (!c || !c->have_state) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
   HIPCHK(hipSetDevice(c->device));
-  dmd::CellGrid g{};
+  const dmd::CellGrid g = make_grid(c);
   const int n = c->n;
-  for (int a = 0; a < 3; a++) {
-    double lo, ext;
-    if (c->box.per[a]) {
-      lo = c->box.lo[a];
-      ext = c->box.L[a];
-    } else {
-      double mn = 1e300, mx = -1e300;
-      for (int i = 0; i < n; i++) {
-        mn = std::min(mn, c->h_q[3 * i + a]);
-        mx = std::max(mx, c->h_q[3 * i + a]);
-      }
-      lo = mn;
-      ext = mx - mn;
-    }
-    int nc = (int)std::floor(ext / c->rc);
-    nc = std::max(1, std::min(nc, 1024));
-    g.nc[a] = nc;
-    g.lo[a] = lo;
-    g.cs[a] = ext > 0 ? ext / nc : 1.0;
-    g.inv_cs[a] = 1.0 / g.cs[a];
-  }
   const size_t ncell = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
   HIPCHK(c->d_cell_count.ensure(ncell));
   HIPCHK(c->d_cell_start.ensure(ncell + 1));
@@ -542,10 +620,15 @@ dmdgpu_status dmdgpu_get_neighbors(dmdgpu_ctx* c, int32_t* h_nn, int32_t* h_list
   HIPCHK(hipMemcpyAsync(rows.data(), c->d_nbr.p, sizeof(int) * rows.size(), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < n; i++) {
-    if (h_nn) h_nn[i] = nn[i];
-    if (h_list)
-      for (int k = 0; k < cap; k++) h_list[(size_t)i * cap + k] = k < nn[i] ? rows[(size_t)i * dmd::kCap + k] : -1;
+  std::vector<int> row;
+  for (int o = 0; o < n; o++) {
+    const int i = c->inv[o];
+    if (h_nn) h_nn[o] = nn[i];
+    if (!h_list) continue;
+    row.assign(rows.begin() + (size_t)i * dmd::kCap, rows.begin() + (size_t)i * dmd::kCap + nn[i]);
+    for (int& v : row) v = c->perm[v];
+    std::sort(row.begin(), row.end());
+    for (int k = 0; k < cap; k++) h_list[(size_t)o * cap + k] = k < nn[i] ? row[k] : -1;
   }
   return DMDGPU_OK;
 }
@@ -555,7 +638,7 @@ dmdgpu_status dmdgpu_mf_rho(dmdgpu_ctx* c, double* h_rho) {
   if (st) return st;
   HIPCHK(hipSetDevice(c->device));
   if ((st = run_phase1(c))) return st;
-  if ((st = d2h(c, h_rho, c->d_rho.p, c->n))) return st;
+  if ((st = d2h_sites(c, h_rho, c->d_rho.p, 1))) return st;
   HIPCHK(hipStreamSynchronize(c->stream));
   return DMDGPU_OK;
 }
@@ -565,8 +648,8 @@ dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* c, double* h_V0, double* h_S0) {
   if (st) return st;
   HIPCHK(hipSetDevice(c->device));
   if ((st = run_residual(c))) return st;
-  if ((st = d2h(c, h_V0, c->d_V0.p, c->n))) return st;
-  if ((st = d2h(c, h_S0, c->d_S0.p, c->n))) return st;
+  if ((st = d2h_sites(c, h_V0, c->d_V0.p, 1))) return st;
+  if ((st = d2h_sites(c, h_S0, c->d_S0.p, 1))) return st;
   HIPCHK(hipStreamSynchronize(c->stream));
   return DMDGPU_OK;
 }
@@ -577,9 +660,9 @@ dmdgpu_status dmdgpu_residual(dmdgpu_ctx* c, double* h_Rq, double* h_Rs, double*
   if (st) return st;
   HIPCHK(hipSetDevice(c->device));
   if ((st = run_residual(c))) return st;
-  if ((st = d2h(c, h_Rq, c->d_Rq.p, (size_t)3 * c->n))) return st;
-  if ((st = d2h(c, h_Rs, c->d_Rs.p, c->n))) return st;
-  if ((st = d2h(c, h_Rx, c->d_Rx.p, c->n))) return st;
+  if ((st = d2h_sites(c, h_Rq, c->d_Rq.p, 3))) return st;
+  if ((st = d2h_sites(c, h_Rs, c->d_Rs.p, 1))) return st;
+  if ((st = d2h_sites(c, h_Rx, c->d_Rx.p, 1))) return st;
   if ((st = d2h(c, h_tot, c->d_totals.p, 2))) return st;
   HIPCHK(hipStreamSynchronize(c->stream));
   return DMDGPU_OK;

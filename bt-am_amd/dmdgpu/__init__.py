@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libdmdgpu.so")
 TABLE_DIR = os.path.join(PKG, "potentials")
 R_CUTOFF = 6.2934034034  # r_cutoff_ADP_MgHx, src/Macros.hpp:857
 MAXNEIGH = 250           # src/Macros.hpp:862
+FP64, SPLIT32 = 0, 1     # dmdgpu_set_precision codes
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_IO", 4: "ERR_SPECIES", 5: "ERR_MAXNEIGH",
           6: "ERR_NONEIGH", 7: "ERR_BOX", 8: "ERR_STATE", 9: "ERR_TABLES"}
@@ -27,7 +28,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_init_adp_MgHx", "dmdgpu_set_cutoff", "dmdgpu_set_state",
            "dmdgpu_update_state", "dmdgpu_neighbors", "dmdgpu_get_neighbors", "dmdgpu_mf_rho",
            "dmdgpu_site_energies", "dmdgpu_residual", "dmdgpu_residual_resident",
-           "dmdgpu_set_lanes", "dmdgpu_set_timing", "dmdgpu_kernel_count", "dmdgpu_kernel_name",
+           "dmdgpu_set_tuning", "dmdgpu_set_precision", "dmdgpu_set_timing", "dmdgpu_kernel_count", "dmdgpu_kernel_name",
            "dmdgpu_kernel_times", "dmdgpu_reset_kernel_times", "dmdgpu_work_counts"]
 
 _dp = C.POINTER(C.c_double)
@@ -74,7 +75,8 @@ def load_library():
     L.dmdgpu_site_energies.argtypes = [vp, _dp, _dp]
     L.dmdgpu_residual.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.dmdgpu_residual_resident.argtypes = [vp]
-    L.dmdgpu_set_lanes.argtypes = [vp, C.c_int, C.c_int]
+    L.dmdgpu_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.dmdgpu_set_precision.argtypes = [vp, C.c_int]
     L.dmdgpu_set_timing.argtypes = [vp, C.c_int]
     L.dmdgpu_kernel_count.restype = C.c_int
     L.dmdgpu_kernel_name.argtypes = [C.c_int]
@@ -150,8 +152,13 @@ class Context:
                 for a in (q, sigma, x)]
         self._chk(self.L.dmdgpu_update_state(self.h, *[_dn(a) for a in arrs]))
 
-    def set_lanes(self, k1, k2):
-        self._chk(self.L.dmdgpu_set_lanes(self.h, k1, k2))
+    def set_tuning(self, k1=4, k2=4, unroll2=1):
+        self._chk(self.L.dmdgpu_set_tuning(self.h, k1, k2, unroll2))
+
+    def set_precision(self, precision):
+        """'fp64' or 'split32' (FP32 points, FP64 sums and totals)."""
+        code = {"fp64": FP64, "split32": SPLIT32}[precision]
+        self._chk(self.L.dmdgpu_set_precision(self.h, code))
 
     # -- neighbours ---------------------------------------------------------
     def neighbors(self):

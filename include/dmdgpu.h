@@ -128,10 +128,18 @@ dmdgpu_status dmdgpu_residual(dmdgpu_ctx* ctx, double* h_Rq, double* h_Rsigma, d
  * buffers (no host copies); used by resident solvers and for timing. */
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
 
-/* Lanes of a wavefront that share one site in the phase-1 and phase-2
- * kernels (1, 2, 4 or 8; the tuner picks them per workload).  No counterpart
- * in the reference. */
-dmdgpu_status dmdgpu_set_lanes(dmdgpu_ctx* ctx, int k_phase1, int k_phase2);
+/* Kernel tuning: lanes of a wavefront that share one site in the phase-1 and
+ * phase-2 kernels (1, 2, 4 or 8) and whether phase 2 unrolls its point loops.
+ * The tuner picks them per workload.  No counterpart in the reference. */
+dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* ctx, int k_phase1, int k_phase2, int unroll2);
+
+/* Precision of the quadrature-point evaluation.  DMDGPU_FP64: everything FP64
+ * (the reference's arithmetic).  DMDGPU_SPLIT32: splines, feature vectors and
+ * point gradients in FP32, with pair distances, spline offsets, per-site sums,
+ * records, embedding, entropy terms and totals in FP64.  No counterpart in the
+ * reference. */
+enum { DMDGPU_FP64 = 0, DMDGPU_SPLIT32 = 1 };
+dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* ctx, int precision);
 
 /* Instrumentation: when on, every kernel launch is bracketed by HIP events on
  * the context's stream.  dmdgpu_kernel_times returns, per kernel id (see

@@ -37,11 +37,12 @@ CASES = {
 }
 
 
-def run_both(gpu, oracle, sysd, k1=4, k2=4):
+def run_both(gpu, oracle, sysd, k1=4, k2=4, unroll2=1, precision="fp64"):
     st, nnb, nbr = oracle.neighbors(sysd, RC)
     assert st == 0
     gpu.set_state(sysd)
-    gpu.set_lanes(k1, k2)
+    gpu.set_tuning(k1, k2, unroll2)
+    gpu.set_precision(precision)
     gpu.neighbors()
     gn, gl = gpu.get_neighbors()
     ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr))
@@ -75,16 +76,39 @@ def test_residual_parity_fp64(gpu, oracle, case):
     assert rel(S0, ref["S0"]) < TOL
 
 
-@pytest.mark.parametrize("k1,k2", [(1, 1), (2, 8), (8, 2)])
-def test_lane_counts_agree(gpu, oracle, k1, k2):
-    say(f"[gpu] lanes {k1} {k2}")
+@pytest.mark.parametrize("k1,k2,un", [(1, 1, 1), (2, 8, 0), (8, 2, 1), (4, 4, 0)])
+def test_tuning_variants_agree(gpu, oracle, k1, k2, un):
+    say(f"[gpu] tuning {k1} {k2} {un}")
     sysd = CASES["hcp_mgh_5x3x3"]()
-    _, _, ref = run_both(gpu, oracle, sysd, k1, k2)
+    _, _, ref = run_both(gpu, oracle, sysd, k1, k2, un)
     out = gpu.residual()
     assert rel(out["Rq"], ref["Rq"]) < TOL
     assert rel(out["Rs"], ref["Rs"]) < TOL
     assert rel(out["Rx"], ref["Rx"]) < TOL
-    gpu.set_lanes(4, 4)
+    gpu.set_tuning(4, 4, 1)
+
+
+# Split precision (FP32 point evaluation, FP64 distances/sums/totals) against
+# the FP64 oracle.  Stated per quantity: forces R_q and width residuals R_sigma
+# within 1e-6 relative (the accuracy bar of the split path), occupancy
+# residuals and densities within 1e-7, totals within 1e-8.
+SPLIT_TOL = {"Rq": 1e-6, "Rs": 1e-6, "Rx": 1e-7, "mf_rho": 1e-7, "S0_total": 1e-8}
+
+
+@pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "mg_5x5x5_c1", "cluster_mgh"])
+@pytest.mark.parametrize("un", [0, 1])
+def test_split32_against_fp64(gpu, oracle, case, un):
+    say(f"[gpu] split32 {case} unroll={un}")
+    sysd = CASES[case]()
+    _, _, ref = run_both(gpu, oracle, sysd, 4, 4, un, "split32")
+    out = gpu.residual()
+    err = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
+    err["mf_rho"] = rel(gpu.mf_rho(), ref["mf_rho"])
+    err["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"])
+    say(f"    split32 errors {err}")
+    gpu.set_precision("fp64")
+    for k, tol in SPLIT_TOL.items():
+        assert err[k] < tol, (k, err[k])
 
 
 def test_reference_order_spot_check(gpu, oracle):
