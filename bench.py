@@ -103,6 +103,71 @@ def split_leg(ctx, args, pqe):
             "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
 
 
+def solve_instances():
+    """The batched relaxation workload: Mg-H nanowires along c (periodic along
+    the axis, R = 15..30 A, 10 cells long) and free spheres (R = 12..25 A),
+    H occupancy x ~ U[0, 0.3], seeded."""
+    import dmdgpu.lattice as lat
+    wires = [lat.nanowire(R, seed=100 + R) for R in (15, 20, 25, 30)]
+    balls = [lat.sphere(R, seed=200 + R) for R in (12, 16, 20, 25)]
+    return wires + balls
+
+
+def solves_leg(tuning):
+    """Solves per hour: the plain path (one instance at a time, FP64 Hessian
+    probes) against the batched path (all instances resident, one launch per
+    tick, split-precision probes, FP64 residuals and stopping test).  Both
+    relax Mg positions and widths to |R|_2 <= 1e-8, each with the inner
+    tolerance that minimises its own residual evaluations (CG eta 0.05 for
+    FP64 probes, 0.1 for split-precision probes)."""
+    import dmdgpu
+    insts = solve_instances()
+    k1, k2, un = tuning
+    out = {"instances": len(insts), "sites": [int(s["n"]) for s in insts], "atol": 1e-8}
+    ctx = dmdgpu.Context(0)
+    ctx.set_tuning(k1, k2, un)
+    plain_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, mixed=False)
+    t_plain = 0.0
+    plain = []
+    for sd in insts:
+        ctx.set_state(sd)
+        ctx.neighbors()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.relax(plain_opts)
+        ctx.sync()
+        t_plain += time.perf_counter() - t0
+        st = ctx.relax_status(0)
+        q, sg, _ = ctx.instance_state(0)
+        plain.append((st, q, sg))
+    ctx.remove_instance(0)
+    ids = [ctx.add_instance(sd) for sd in insts]
+    mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.1, mixed=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.relax(mixed_opts)
+    ctx.sync()
+    t_batch = time.perf_counter() - t0
+    batch = [ctx.relax_status(i) for i in ids]
+    dq = 0.0
+    for k, i in enumerate(ids):
+        q, sg, _ = ctx.instance_state(i)
+        dq = max(dq, float(np.abs(q - plain[k][1]).max()))
+    ctx.close()
+    n = len(insts)
+    out["plain_fp64_per_instance"] = {
+        "solves_per_hour": n * 3600.0 / t_plain, "seconds": t_plain,
+        "converged": sum(p[0]["state"] == "converged" for p in plain),
+        "residual_evals": sum(p[0]["evals"] for p in plain)}
+    out["batched_split32_probes"] = {
+        "solves_per_hour": n * 3600.0 / t_batch, "seconds": t_batch,
+        "converged": sum(s["state"] == "converged" for s in batch),
+        "residual_evals": sum(s["evals"] for s in batch)}
+    out["speedup"] = t_plain / t_batch
+    out["max_position_diff_A"] = dq
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +177,7 @@ def main():
     ap.add_argument("--tuning", type=str, default="auto",
                     help="auto, or k1,k2,unroll2 (lanes per site in phase 1/2, unroll)")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
+    ap.add_argument("--no-solves", action="store_true", help="skip the solves/hr leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -161,6 +227,11 @@ def main():
             "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
             "flop_per_launch": FLOP_PER_PAIR[dom] * pairs}
 
+    solves = None
+    if not args.no_solves:
+        solves = solves_leg((k1, k2, un))
+        log(f"solves: {solves}")
+
     split = None
     if not args.no_split:
         split = split_leg(ctx, args, pqe)
@@ -185,6 +256,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "split32": split,
+        "solves": solves,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

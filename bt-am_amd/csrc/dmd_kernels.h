@@ -45,8 +45,16 @@ struct Box {
   int per[3];
 };
 
+// Sites of all resident instances live in one arena.  Every instance starts
+// at a multiple of kChunk sites, so a kChunk-aligned block of sites belongs to
+// a single instance.
+constexpr int kChunk = 256;
+
 struct State {
-  int n;
+  int n;               // arena extent (sites)
+  const int* inst;     // instance of each site, -1 for a free slot
+  const Box* boxes;    // per instance
+  const int* prec;     // per instance: Precision to evaluate with, -1 = skip
   const double4* qs;   // (x, y, z, sigma)
   const double* ne;    // effective occupancy: x if active (x >= 5e-4) else 0
   const double* xr;    // raw occupancy
@@ -65,8 +73,8 @@ struct Outputs {
   double* Rs;
   double* Rx;
   double* S0;
-  double* partials;    // per-block [S0, V0] partial sums
-  double* totals;      // [sum S0, sum V0]
+  double* partials;    // per-chunk [S0, V0] partial sums
+  double* totals;      // per instance [sum S0, sum V0]
 };
 
 // Neighbour build (cell list).  Returns via flags[0] = overflow, flags[1] = empty row.
@@ -75,9 +83,12 @@ struct CellGrid {
   double lo[3], cs[3], inv_cs[3];
 };
 
-hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, double rc, int n,
-                            const double4* qs, int* cell_of, int* cell_count, int* cell_start,
-                            int* cell_cursor, int* cell_atoms, int* nnb, int* nbr, int* flags);
+// Builds the lists of one instance whose sites are qs[base, base + n); the
+// lists hold arena indices.
+hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, double rc, int base,
+                            int n, const double4* qs, int* cell_of, int* cell_count,
+                            int* cell_start, int* cell_cursor, int* cell_atoms, int* nnb, int* nbr,
+                            int* flags);
 
 // Precision of the quadrature-point evaluation: FP64 throughout, or the split
 // form (FP32 points and splines, FP64 distances, sums, records and totals).
@@ -89,12 +100,16 @@ struct PhaseConfig {
   int unroll2;     // 1: fully unrolled point loops in phase 2, 0: rolled
 };
 
-hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+// Evaluates every instance whose prec[] equals the given precision (both
+// precisions are launched when instances of both kinds are present).
+hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,
                          const Tables& T, const Outputs& o);
-hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,
                          const Tables& T, const Outputs& o);
-hipError_t launch_totals(hipStream_t st, int n, const Outputs& o);
-hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_pairs_per_block,
-                               int nblocks);
+// Per-instance totals: chunk sums, then per instance over its chunk range
+// [chunk_lo[k], chunk_hi[k]) in a fixed order.
+hipError_t launch_totals(hipStream_t st, const State& s, const Outputs& o, int ninst,
+                         const int* chunk_lo, const int* chunk_hi);
+hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_per_chunk);
 
 }  // namespace dmd

@@ -6,12 +6,15 @@ namespace dmd {
 // ---------------------------------------------------------------------------
 // Deterministic totals: per-block sums in a fixed tree, then one block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_block_sums(int n, const double* S0, const double* V0,
+__global__ __launch_bounds__(256) void k_block_sums(int n, const int* inst, const int* prec,
+                                                   const double* S0, const double* V0,
                                                    double* partials) {
   __shared__ double sa[256], sb[256];
-  int i = blockIdx.x * 256 + threadIdx.x;
-  sa[threadIdx.x] = i < n ? S0[i] : 0.0;
-  sb[threadIdx.x] = i < n ? V0[i] : 0.0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int k = inst[blockIdx.x * 256];  // one instance per 256-site chunk
+  const bool on = i < n && k >= 0 && prec[k] >= 0;
+  sa[threadIdx.x] = on ? S0[i] : 0.0;
+  sb[threadIdx.x] = on ? V0[i] : 0.0;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
@@ -26,12 +29,17 @@ __global__ __launch_bounds__(256) void k_block_sums(int n, const double* S0, con
   }
 }
 
-__global__ __launch_bounds__(256) void k_final_sums(int nb, const double* partials, double* totals) {
+// One block per instance: sums its chunk partials in a fixed order.
+__global__ __launch_bounds__(256) void k_inst_sums(const int* prec, const int* chunk_lo,
+                                                  const int* chunk_hi, const double* partials,
+                                                  double* totals) {
   __shared__ double sa[256], sb[256];
+  const int k = blockIdx.x;
+  if (prec[k] < 0) return;
   double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < nb; k += 256) {
-    a += partials[2 * k];
-    b += partials[2 * k + 1];
+  for (int c = chunk_lo[k] + threadIdx.x; c < chunk_hi[k]; c += 256) {
+    a += partials[2 * c];
+    b += partials[2 * c + 1];
   }
   sa[threadIdx.x] = a;
   sb[threadIdx.x] = b;
@@ -44,17 +52,17 @@ __global__ __launch_bounds__(256) void k_final_sums(int nb, const double* partia
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    totals[0] = sa[0];
-    totals[1] = sb[0];
+    totals[2 * k] = sa[0];
+    totals[2 * k + 1] = sb[0];
   }
 }
 
-// Active ordered pairs per block (for the work count).
+// Active ordered pairs per 256-site chunk (for the work count).
 __global__ __launch_bounds__(256) void k_active_pairs(State s, long long* out) {
   __shared__ long long sh[256];
   int i = blockIdx.x * 256 + threadIdx.x;
   long long c = 0;
-  if (i < s.n && s.ne[i] != 0.0) {
+  if (i < s.n && s.inst[i] >= 0 && s.ne[i] != 0.0) {
     const int* row = s.nbr + (size_t)i * kCap;
     for (int k = 0; k < s.nnb[i]; k++) c += s.ne[row[k]] != 0.0;
   }
@@ -144,9 +152,10 @@ DEV int axis_cells(int c, int nc, int per, int* out) {
   return k;
 }
 
-__global__ __launch_bounds__(256) void k_build(int n, Box box, CellGrid g, double rc, const double4* qs,
-                                              const int* cell_of, const int* start, const int* atoms,
-                                              int* nnb, int* nbr, int* flags) {
+__global__ __launch_bounds__(256) void k_build(int base, int n, Box box, CellGrid g, double rc,
+                                              const double4* qs, const int* cell_of,
+                                              const int* start, const int* atoms, int* nnb,
+                                              int* nbr, int* flags) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -180,7 +189,7 @@ __global__ __launch_bounds__(256) void k_build(int n, Box box, CellGrid g, doubl
           double s2 = v[2] * v[2];
           double rr = sqrt((s0 + s1) + s2);
           if (rr <= rc) {
-            if (cnt < kMaxNeigh) row[cnt] = j;
+            if (cnt < kMaxNeigh) row[cnt] = base + j;
             else over = true;
             cnt++;
           }
@@ -217,9 +226,13 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n, const int* nnb, int* n
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, double rc, int n,
-                            const double4* qs, int* cell_of, int* cell_count, int* cell_start,
-                            int* cell_cursor, int* cell_atoms, int* nnb, int* nbr, int* flags) {
+hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, double rc, int base,
+                            int n, const double4* qs_all, int* cell_of, int* cell_count,
+                            int* cell_start, int* cell_cursor, int* cell_atoms, int* nnb_all,
+                            int* nbr_all, int* flags) {
+  const double4* qs = qs_all + base;
+  int* nnb = nnb_all + base;
+  int* nbr = nbr_all + (size_t)base * kCap;
   int nc = g.nc[0] * g.nc[1] * g.nc[2];
   hipError_t e = hipMemsetAsync(cell_count, 0, sizeof(int) * nc, st);
   if (e != hipSuccess) return e;
@@ -229,20 +242,22 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
   k_cell_assign<<<nb, 256, 0, st>>>(n, box, g, qs, cell_of, cell_count);
   k_scan_cells<<<1, 1024, 0, st>>>(nc, cell_count, cell_start, cell_cursor);
   k_cell_fill<<<nb, 256, 0, st>>>(n, cell_of, cell_cursor, cell_atoms);
-  k_build<<<nb, 256, 0, st>>>(n, box, g, rc, qs, cell_of, cell_start, cell_atoms, nnb, nbr, flags);
+  k_build<<<nb, 256, 0, st>>>(base, n, box, g, rc, qs, cell_of, cell_start, cell_atoms, nnb, nbr,
+                              flags);
   k_sort_rows<<<n, kCap, 0, st>>>(n, nnb, nbr);
   return hipGetLastError();
 }
 
-hipError_t launch_totals(hipStream_t st, int n, const Outputs& o) {
-  int nb = (n + 255) / 256;
-  k_block_sums<<<nb, 256, 0, st>>>(n, o.S0, o.V0, o.partials);
-  k_final_sums<<<1, 256, 0, st>>>(nb, o.partials, o.totals);
+hipError_t launch_totals(hipStream_t st, const State& s, const Outputs& o, int ninst,
+                         const int* chunk_lo, const int* chunk_hi) {
+  const int nb = (s.n + 255) / 256;
+  k_block_sums<<<nb, 256, 0, st>>>(s.n, s.inst, s.prec, o.S0, o.V0, o.partials);
+  k_inst_sums<<<ninst, 256, 0, st>>>(s.prec, chunk_lo, chunk_hi, o.partials, o.totals);
   return hipGetLastError();
 }
 
-hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out, int nblocks) {
-  k_active_pairs<<<nblocks, 256, 0, st>>>(s, out);
+hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out) {
+  k_active_pairs<<<(s.n + 255) / 256, 256, 0, st>>>(s, out);
   return hipGetLastError();
 }
 

@@ -38,13 +38,20 @@ namespace dmd {
 //   rho_bar, pair sum, coincident and self terms on the sqrt(6) sigma points.
 // Writes V0_i, rho_bar_i and the adjoint record of site i.
 // ---------------------------------------------------------------------------
+template <typename P> struct PrecCode;
+template <> struct PrecCode<double> { static constexpr int v = kFP64; };
+template <> struct PrecCode<float> { static constexpr int v = kSplit32; };
+
 template <typename P, int K>
-__global__ __launch_bounds__(256) void k_phase1(State s, Box box, Tables T, Outputs o) {
+__global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int gt = blk * blockDim.x + threadIdx.x;
   const int i = gt / K;
   const int g = gt % K;
   if (i >= s.n) return;
+  const int inst = s.inst[i];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  const Box box = s.boxes[inst];
   double* rec = o.rec + (size_t)i * kRec;
   const double ni = s.ne[i];
   if (ni == 0.0) {
@@ -198,12 +205,15 @@ DEV void load9(const double* r, bool flip, P* o) {
 // psi(-p) = flip(psi(p)) (the dipole part is odd, the quadrupole part even).
 // ---------------------------------------------------------------------------
 template <typename P, int K, bool UNR>
-__global__ __launch_bounds__(256) void k_phase2(State s, Box box, Tables T, Outputs o) {
+__global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int gt = blk * blockDim.x + threadIdx.x;
   const int u = gt / K;
   const int g = gt % K;
   if (u >= s.n) return;
+  const int inst = s.inst[u];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  const Box box = s.boxes[inst];
   const double nu = s.ne[u];
   if (nu == 0.0) {
     if (g == 0) {
@@ -414,57 +424,59 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Box box, Tables T, Outp
 // Launchers
 // ---------------------------------------------------------------------------
 template <typename P, int K>
-static void p1(hipStream_t st, const State& s, const Box& b, const Tables& T, const Outputs& o) {
+static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
-  k_phase1<P, K><<<nb, 256, 0, st>>>(s, b, T, o);
+  k_phase1<P, K><<<nb, 256, 0, st>>>(s, T, o);
 }
 template <typename P, int K, bool UNR>
-static void p2(hipStream_t st, const State& s, const Box& b, const Tables& T, const Outputs& o) {
+static void p2(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
-  k_phase2<P, K, UNR><<<nb, 256, 0, st>>>(s, b, T, o);
+  k_phase2<P, K, UNR><<<nb, 256, 0, st>>>(s, T, o);
 }
 
 template <typename P>
-static hipError_t phase1_p(hipStream_t st, int K, const State& s, const Box& b, const Tables& T,
+static hipError_t phase1_p(hipStream_t st, int K, const State& s, const Tables& T,
                            const Outputs& o) {
   switch (K) {
-    case 1: p1<P, 1>(st, s, b, T, o); break;
-    case 2: p1<P, 2>(st, s, b, T, o); break;
-    case 4: p1<P, 4>(st, s, b, T, o); break;
-    case 8: p1<P, 8>(st, s, b, T, o); break;
+    case 1: p1<P, 1>(st, s, T, o); break;
+    case 2: p1<P, 2>(st, s, T, o); break;
+    case 4: p1<P, 4>(st, s, T, o); break;
+    case 8: p1<P, 8>(st, s, T, o); break;
+    case 16: p1<P, 16>(st, s, T, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <typename P, bool UNR>
-static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Box& b, const Tables& T,
+static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Tables& T,
                             const Outputs& o) {
   switch (K) {
-    case 1: p2<P, 1, UNR>(st, s, b, T, o); break;
-    case 2: p2<P, 2, UNR>(st, s, b, T, o); break;
-    case 4: p2<P, 4, UNR>(st, s, b, T, o); break;
-    case 8: p2<P, 8, UNR>(st, s, b, T, o); break;
+    case 1: p2<P, 1, UNR>(st, s, T, o); break;
+    case 2: p2<P, 2, UNR>(st, s, T, o); break;
+    case 4: p2<P, 4, UNR>(st, s, T, o); break;
+    case 8: p2<P, 8, UNR>(st, s, T, o); break;
+    case 16: p2<P, 16, UNR>(st, s, T, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,
                          const Tables& T, const Outputs& o) {
-  if (cfg.precision == kSplit32) return phase1_p<float>(st, cfg.k1, s, box, T, o);
-  return phase1_p<double>(st, cfg.k1, s, box, T, o);
+  if (precision == kSplit32) return phase1_p<float>(st, cfg.k1, s, T, o);
+  return phase1_p<double>(st, cfg.k1, s, T, o);
 }
 
-hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, const State& s, const Box& box,
+hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,
                          const Tables& T, const Outputs& o) {
-  if (cfg.precision == kSplit32)
-    return cfg.unroll2 ? phase2_pu<float, true>(st, cfg.k2, s, box, T, o)
-                       : phase2_pu<float, false>(st, cfg.k2, s, box, T, o);
-  return cfg.unroll2 ? phase2_pu<double, true>(st, cfg.k2, s, box, T, o)
-                     : phase2_pu<double, false>(st, cfg.k2, s, box, T, o);
+  if (precision == kSplit32)
+    return cfg.unroll2 ? phase2_pu<float, true>(st, cfg.k2, s, T, o)
+                       : phase2_pu<float, false>(st, cfg.k2, s, T, o);
+  return cfg.unroll2 ? phase2_pu<double, true>(st, cfg.k2, s, T, o)
+                     : phase2_pu<double, false>(st, cfg.k2, s, T, o);
 }
 
 }  // namespace dmd

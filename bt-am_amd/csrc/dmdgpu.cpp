@@ -1,8 +1,13 @@
 // dmdgpu.cpp -- host side of the C ABI declared in include/dmdgpu.h.
 //
-// Owns the device-resident swarm (SoA), the spline tables, the neighbour lists
-// and the output buffers of one context, and sequences the kernels of
-// dmd_kernels.hip on the context's stream.
+// A context owns the spline tables and one device arena of sites in which any
+// number of independent instances (systems) are resident at once.  Each
+// instance occupies a 256-site-aligned slot range with its own box, site
+// order and neighbour lists; one launch of the phase kernels evaluates every
+// instance.  On top of the arena runs a batched inexact Newton-Krylov
+// relaxation: every instance advances its own state machine, one residual
+// evaluation per instance per tick, with the tick's device work optionally
+// captured once as a hipGraph and replayed.
 #include "../../include/dmdgpu.h"
 
 #include <hip/hip_runtime.h>
@@ -15,6 +20,7 @@
 #include <vector>
 
 #include "dmd_kernels.h"
+#include "dmd_solver.h"
 
 namespace {
 
@@ -31,9 +37,18 @@ dmdgpu_status fail(dmdgpu_status st, const std::string& msg) {
     if (e_ != hipSuccess)                                                                  \
       return fail(DMDGPU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
   } while (0)
+#define CHK(expr)                   \
+  do {                              \
+    dmdgpu_status s_ = (expr);      \
+    if (s_ != DMDGPU_OK) return s_; \
+  } while (0)
 
 constexpr double kMinOcc = 5e-4;            // src/Macros.hpp:871
 constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
+constexpr int kChunk = dmd::kChunk;
+// Device bytes per arena site (state, lists, records, outputs, solver vectors).
+constexpr size_t kBytesPerSite =
+    32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 + 6 * 32;
 
 struct Spline {
   int n = 0;
@@ -49,14 +64,20 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
-  hipError_t ensure(size_t cnt) {
+  // Grows to at least cnt elements, keeping the first `keep` elements.
+  hipError_t ensure(size_t cnt, size_t keep = 0) {
     if (cnt <= n && p) return hipSuccess;
+    T* q = nullptr;
+    hipError_t e = hipMalloc((void**)&q, sizeof(T) * std::max<size_t>(cnt, 1));
+    if (e != hipSuccess) return e;
+    if (p && keep) {
+      e = hipMemcpy(q, p, sizeof(T) * std::min(keep, n), hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) return e;
+    }
     if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-    hipError_t e = hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(cnt, 1));
-    if (e == hipSuccess) n = cnt;
-    return e;
+    p = q;
+    n = cnt;
+    return hipSuccess;
   }
   void release() {
     if (p) (void)hipFree(p);
@@ -65,13 +86,39 @@ struct DevBuf {
   }
 };
 
+enum SolveStage { ST_IDLE = 0, ST_START, ST_BASE32, ST_CG, ST_LSPREP, ST_LS, ST_DONE };
+
+struct Solve {
+  int stage = ST_IDLE;
+  int result = 0;  // DMDGPU_RELAX_*
+  dmdgpu_relax_opts o{};
+  int newton = 0, cg = 0, ls = 0, evals = 0;
+  double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
+  int ndof = 0;
+  dmd::SolveOp pending{};  // updates to apply at the start of the next tick
+};
+
+struct Instance {
+  bool used = false;
+  int base = 0, n = 0, cap = 0;
+  dmd::Box box{};
+  std::vector<int> perm, inv;  // internal site i holds the caller's site perm[i]
+  std::vector<int> h_slot;     // species slot, internal order
+  std::vector<char> h_act;     // site active (x >= 5e-4), internal order
+  std::vector<double> h_q;     // positions, caller's order
+  bool have_neigh = false;
+  long long pairs = 0;
+  int nactive = 0;
+  Solve sv;
+};
+
 }  // namespace
 
 struct dmdgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   double rc = kDefaultRc;
-  dmd::PhaseConfig cfg{dmd::kFP64, 4, 4, 1};
+  dmd::PhaseConfig cfg{dmd::kFP64, 8, 8, 0};
   // tables
   bool have[3] = {false, false, false};
   Spline rho[2], embed[2], pair[3], u[3], w[3];
@@ -80,24 +127,34 @@ struct dmdgpu_ctx {
   DevBuf<double> d_uw, d_ph, d_rh, d_em;
   DevBuf<float> d_uw32, d_ph32, d_rh32;
   dmd::Tables T{};
-  // state
-  int n = 0;
-  dmd::Box box{};
-  std::vector<double> h_q;      // positions, caller's order
-  std::vector<int> perm, inv;   // internal site i holds caller's site perm[i]
-  std::vector<int> h_slot;      // species slot, internal order
+  // arena
+  int extent = 0;     // sites in use, multiple of kChunk
+  size_t budget = 0;  // device bytes allowed for the arena (0 = unlimited)
+  std::vector<Instance> inst;
+  int primary = -1;         // instance of the one-instance API
+  std::vector<int> h_inst;  // arena site -> instance
+  DevBuf<int> d_inst, d_prec, d_chunk_lo, d_chunk_hi;
+  DevBuf<dmd::Box> d_boxes;
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
-  DevBuf<int> d_spc;
-  bool have_state = false;
-  // neighbours
-  DevBuf<int> d_nnb, d_nbr, d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms,
-      d_flags;
-  bool have_neigh = false;
-  long long pairs = 0;
-  // outputs
+  DevBuf<int> d_spc, d_nnb, d_nbr;
+  DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals;
   DevBuf<long long> d_pairs;
+  std::vector<int> h_prec;
+  bool meta_dirty = true;
+  // solver
+  DevBuf<double4> d_X, d_R0, d_R0f, d_D, d_Rr, d_Pp;
+  DevBuf<dmd::SolveOp> d_ops;
+  DevBuf<double> d_sdots, d_spart;
+  std::vector<dmd::SolveOp> h_ops;
+  std::vector<double> h_sdots;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_extent = -1, graph_ninst = -1;
+  dmd::PhaseConfig graph_cfg{};
+  long long ticks = 0;
   // timing
   bool timing = false;
   struct Pending {
@@ -111,11 +168,11 @@ struct dmdgpu_ctx {
 
 namespace {
 
-dmdgpu_status begin_timed(dmdgpu_ctx* c, int id, hipEvent_t* a) {
+// ---- instrumentation --------------------------------------------------------
+dmdgpu_status begin_timed(dmdgpu_ctx* c, hipEvent_t* a) {
   if (!c->timing) return DMDGPU_OK;
   HIPCHK(hipEventCreate(a));
   HIPCHK(hipEventRecord(*a, c->stream));
-  (void)id;
   return DMDGPU_OK;
 }
 dmdgpu_status end_timed(dmdgpu_ctx* c, int id, hipEvent_t a) {
@@ -141,7 +198,7 @@ dmdgpu_status drain_timing(dmdgpu_ctx* c) {
   return DMDGPU_OK;
 }
 
-// ---- table files (src/ADP/MgHx-ADP.cpp:366-501 layout) ----------------------
+// ---- tables (src/ADP/MgHx-ADP.cpp:366-501 layout) ---------------------------
 bool read_block(FILE* f, Spline& s) {
   int n;
   double dx;
@@ -200,7 +257,8 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   HIPCHK(hipMemcpy(c->d_rh.p, rh.data(), rh.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_em.p, em.data(), em.size() * 8, hipMemcpyHostToDevice));
   // FP32 copies for the split-precision evaluation
-  std::vector<float> uw32(uw.begin(), uw.end()), ph32(ph.begin(), ph.end()), rh32(rh.begin(), rh.end());
+  std::vector<float> uw32(uw.begin(), uw.end()), ph32(ph.begin(), ph.end()),
+      rh32(rh.begin(), rh.end());
   HIPCHK(c->d_uw32.ensure(uw32.size()));
   HIPCHK(c->d_ph32.ensure(ph32.size()));
   HIPCHK(c->d_rh32.ensure(rh32.size()));
@@ -231,9 +289,13 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   return DMDGPU_OK;
 }
 
+// ---- arena ------------------------------------------------------------------
 dmd::State make_state(dmdgpu_ctx* c) {
   dmd::State s;
-  s.n = c->n;
+  s.n = c->extent;
+  s.inst = c->d_inst.p;
+  s.boxes = c->d_boxes.p;
+  s.prec = c->d_prec.p;
   s.qs = c->d_qs.p;
   s.ne = c->d_ne.p;
   s.xr = c->d_xr.p;
@@ -259,68 +321,108 @@ dmd::Outputs make_outputs(dmdgpu_ctx* c) {
   return o;
 }
 
-dmdgpu_status check_ready(dmdgpu_ctx* c) {
-  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
-  if (!c->have_state) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
-  if (!c->have_neigh) return fail(DMDGPU_ERR_STATE, "dmdgpu_neighbors has not been called");
+dmd::SolveVecs make_vecs(dmdgpu_ctx* c) {
+  return dmd::SolveVecs{c->d_X.p, c->d_R0.p, c->d_R0f.p, c->d_D.p, c->d_Rr.p, c->d_Pp.p};
+}
+
+void drop_graph(dmdgpu_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_extent = -1;
+}
+
+// Makes every per-site array hold at least `sites` sites (keeping contents).
+dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
+  const size_t keep = c->extent;
+  if (sites <= c->d_qs.n && c->d_qs.p) return DMDGPU_OK;
+  size_t cap = std::max(sites, (size_t)(c->d_qs.n * 3 / 2));
+  cap = (cap + kChunk - 1) / kChunk * kChunk;
+  drop_graph(c);
+  HIPCHK(c->d_qs.ensure(cap, keep));
+  HIPCHK(c->d_ne.ensure(cap, keep));
+  HIPCHK(c->d_xr.ensure(cap, keep));
+  HIPCHK(c->d_beta.ensure(cap, keep));
+  HIPCHK(c->d_gamma.ensure(cap, keep));
+  HIPCHK(c->d_spc.ensure(cap, keep));
+  HIPCHK(c->d_inst.ensure(cap, keep));
+  HIPCHK(c->d_nnb.ensure(cap, keep));
+  HIPCHK(c->d_nbr.ensure(cap * dmd::kCap, keep * dmd::kCap));
+  HIPCHK(c->d_cell_of.ensure(cap));
+  HIPCHK(c->d_cell_atoms.ensure(cap));
+  HIPCHK(c->d_rec.ensure(cap * dmd::kRec, keep * dmd::kRec));
+  HIPCHK(c->d_V0.ensure(cap, keep));
+  HIPCHK(c->d_rho.ensure(cap, keep));
+  HIPCHK(c->d_Rq.ensure(cap * 3, keep * 3));
+  HIPCHK(c->d_Rs.ensure(cap, keep));
+  HIPCHK(c->d_Rx.ensure(cap, keep));
+  HIPCHK(c->d_S0.ensure(cap, keep));
+  HIPCHK(c->d_partials.ensure(cap / kChunk * 2));
+  HIPCHK(c->d_pairs.ensure(cap / kChunk));
+  HIPCHK(c->d_spart.ensure(cap / kChunk * 5));
+  HIPCHK(c->d_X.ensure(cap, keep));
+  HIPCHK(c->d_R0.ensure(cap, keep));
+  HIPCHK(c->d_R0f.ensure(cap, keep));
+  HIPCHK(c->d_D.ensure(cap, keep));
+  HIPCHK(c->d_Rr.ensure(cap, keep));
+  HIPCHK(c->d_Pp.ensure(cap, keep));
+  HIPCHK(c->d_flags.ensure(2));
   return DMDGPU_OK;
 }
 
-dmdgpu_status run_phase1(dmdgpu_ctx* c) {
-  hipEvent_t a{};
-  dmdgpu_status st = begin_timed(c, KID_PHASE1, &a);
-  if (st) return st;
-  HIPCHK(dmd::launch_phase1(c->stream, c->cfg, make_state(c), c->box, c->T, make_outputs(c)));
-  return end_timed(c, KID_PHASE1, a);
-}
-
-dmdgpu_status run_residual(dmdgpu_ctx* c) {
-  dmdgpu_status st = run_phase1(c);
-  if (st) return st;
-  hipEvent_t a{};
-  if ((st = begin_timed(c, KID_PHASE2, &a))) return st;
-  HIPCHK(dmd::launch_phase2(c->stream, c->cfg, make_state(c), c->box, c->T, make_outputs(c)));
-  if ((st = end_timed(c, KID_PHASE2, a))) return st;
-  if ((st = begin_timed(c, KID_TOTALS, &a))) return st;
-  HIPCHK(dmd::launch_totals(c->stream, c->n, make_outputs(c)));
-  return end_timed(c, KID_TOTALS, a);
-}
-
-dmdgpu_status d2h(dmdgpu_ctx* c, double* h, const double* d, size_t cnt) {
-  if (!h) return DMDGPU_OK;
-  HIPCHK(hipMemcpyAsync(h, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+// Per-instance metadata (boxes, chunk ranges) to the device.
+dmdgpu_status upload_meta(dmdgpu_ctx* c) {
+  if (!c->meta_dirty) return DMDGPU_OK;
+  const int ni = (int)c->inst.size();
+  const size_t cap = std::max(ni, 1);
+  if (c->d_boxes.n < cap) drop_graph(c);
+  HIPCHK(c->d_boxes.ensure(cap));
+  HIPCHK(c->d_prec.ensure(cap));
+  HIPCHK(c->d_chunk_lo.ensure(cap));
+  HIPCHK(c->d_chunk_hi.ensure(cap));
+  HIPCHK(c->d_totals.ensure(cap * 2));
+  HIPCHK(c->d_ops.ensure(cap));
+  HIPCHK(c->d_sdots.ensure(cap * 5));
+  std::vector<dmd::Box> bx(cap);
+  std::vector<int> lo(cap, 0), hi(cap, 0);
+  if (c->h_prec.size() < cap) c->h_prec.resize(cap, -1);
+  if (c->h_ops.size() < cap) c->h_ops.resize(cap);
+  for (int k = 0; k < ni; k++) {
+    const Instance& I = c->inst[k];
+    if (!I.used) continue;
+    bx[k] = I.box;
+    lo[k] = I.base / kChunk;
+    hi[k] = (I.base + I.cap) / kChunk;
+  }
+  HIPCHK(hipMemcpy(c->d_boxes.p, bx.data(), sizeof(dmd::Box) * cap, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_chunk_lo.p, lo.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_chunk_hi.p, hi.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  c->meta_dirty = false;
   return DMDGPU_OK;
 }
 
-// Copies a per-site array of `width` doubles per site back to the caller's
-// site order.
-dmdgpu_status d2h_sites(dmdgpu_ctx* c, double* h, const double* d, int width) {
-  if (!h) return DMDGPU_OK;
-  const int n = c->n;
-  std::vector<double> tmp((size_t)n * width);
-  HIPCHK(hipMemcpyAsync(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < n; i++)
-    for (int k = 0; k < width; k++) h[(size_t)c->perm[i] * width + k] = tmp[(size_t)i * width + k];
+dmdgpu_status upload_prec(dmdgpu_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->d_prec.p, c->h_prec.data(), sizeof(int) * c->inst.size(),
+                        hipMemcpyHostToDevice, c->stream));
   return DMDGPU_OK;
 }
 
 // Cell grid with edges >= r_c over the box (periodic axes) or the bounding box
 // of the mean positions (free axes).
-dmd::CellGrid make_grid(const dmdgpu_ctx* c) {
+dmd::CellGrid make_grid(const dmdgpu_ctx* c, const Instance& I) {
   dmd::CellGrid g{};
-  const int n = c->n;
+  const int n = I.n;
   for (int a = 0; a < 3; a++) {
     double lo, ext;
-    if (c->box.per[a]) {
-      lo = c->box.lo[a];
-      ext = c->box.L[a];
+    if (I.box.per[a]) {
+      lo = I.box.lo[a];
+      ext = I.box.L[a];
     } else {
       double mn = 1e300, mx = -1e300;
       for (int i = 0; i < n; i++) {
-        mn = std::min(mn, c->h_q[3 * i + a]);
-        mx = std::max(mx, c->h_q[3 * i + a]);
+        mn = std::min(mn, I.h_q[3 * i + a]);
+        mx = std::max(mx, I.h_q[3 * i + a]);
       }
       lo = mn;
       ext = mx - mn;
@@ -338,26 +440,568 @@ dmd::CellGrid make_grid(const dmdgpu_ctx* c) {
 // Internal site order: sites sorted by cell (x fastest), ties by the caller's
 // index, so that a contiguous range of sites is a compact region of space and
 // the neighbour records phase 2 gathers stay in one L2.
-void make_order(dmdgpu_ctx* c) {
-  const int n = c->n;
-  const dmd::CellGrid g = make_grid(c);
+void make_order(const dmdgpu_ctx* c, Instance& I) {
+  const int n = I.n;
+  const dmd::CellGrid g = make_grid(c, I);
   std::vector<long long> key(n);
   for (int i = 0; i < n; i++) {
     int cc[3];
     for (int a = 0; a < 3; a++) {
-      double v = c->h_q[3 * i + a] - g.lo[a];
-      if (c->box.per[a]) v = v - c->box.L[a] * std::floor(v / c->box.L[a]);
+      double v = I.h_q[3 * i + a] - g.lo[a];
+      if (I.box.per[a]) v = v - I.box.L[a] * std::floor(v / I.box.L[a]);
       int k = (int)std::floor(v * g.inv_cs[a]);
       cc[a] = k < 0 ? 0 : (k > g.nc[a] - 1 ? g.nc[a] - 1 : k);
     }
     key[i] = ((long long)cc[2] * g.nc[1] + cc[1]) * g.nc[0] + cc[0];
   }
-  c->perm.resize(n);
-  for (int i = 0; i < n; i++) c->perm[i] = i;
-  std::stable_sort(c->perm.begin(), c->perm.end(),
-                   [&](int a, int b) { return key[a] < key[b]; });
-  c->inv.resize(n);
-  for (int i = 0; i < n; i++) c->inv[c->perm[i]] = i;
+  I.perm.resize(n);
+  for (int i = 0; i < n; i++) I.perm[i] = i;
+  std::stable_sort(I.perm.begin(), I.perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  I.inv.resize(n);
+  for (int i = 0; i < n; i++) I.inv[I.perm[i]] = i;
+}
+
+// Copies the caller's unknowns of instance I into its arena range (internal
+// order).  NULL leaves a field unchanged.
+dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const double* sg,
+                              const double* x, bool first) {
+  const int n = I.n;
+  if (q) I.h_q.assign(q, q + 3 * (size_t)n);
+  if (q || sg) {
+    std::vector<double4> qs(n);
+    if (!first)
+      HIPCHK(hipMemcpy(qs.data(), c->d_qs.p + I.base, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+      const int o = I.perm[i];
+      if (q) {
+        qs[i].x = q[3 * o];
+        qs[i].y = q[3 * o + 1];
+        qs[i].z = q[3 * o + 2];
+      }
+      if (sg) {
+        if (!(sg[o] > 0)) return fail(DMDGPU_ERR_ARG, "widths must be positive");
+        qs[i].w = sg[o];
+      }
+    }
+    HIPCHK(hipMemcpy(c->d_qs.p + I.base, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_X.p + I.base, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+  }
+  if (x) {
+    std::vector<double> ne(n), xr(n);
+    int na = 0;
+    for (int i = 0; i < n; i++) {
+      const int o = I.perm[i];
+      if (I.h_slot[i] == 0 && x[o] < kMinOcc)
+        return fail(DMDGPU_ERR_SPECIES, "Mg site with occupancy below 5e-4 (site " +
+                                            std::to_string(o) + ")");
+      xr[i] = x[o];
+      ne[i] = x[o] >= kMinOcc ? x[o] : 0.0;
+      na += ne[i] != 0.0;
+    }
+    I.nactive = na;
+    I.h_act.resize(n);
+    for (int i = 0; i < n; i++) I.h_act[i] = ne[i] != 0.0;
+    HIPCHK(hipMemcpy(c->d_ne.p + I.base, ne.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_xr.p + I.base, xr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  return DMDGPU_OK;
+}
+
+// First-fit placement of `cap` sites in the arena.
+int place(const dmdgpu_ctx* c, int cap) {
+  std::vector<std::pair<int, int>> ranges;
+  for (const Instance& I : c->inst)
+    if (I.used) ranges.push_back({I.base, I.base + I.cap});
+  std::sort(ranges.begin(), ranges.end());
+  int at = 0;
+  for (auto& r : ranges) {
+    if (r.first - at >= cap) return at;
+    at = std::max(at, r.second);
+  }
+  return at;
+}
+
+void recompute_extent(dmdgpu_ctx* c) {
+  int e = 0;
+  for (const Instance& I : c->inst)
+    if (I.used) e = std::max(e, I.base + I.cap);
+  if (e != c->extent) drop_graph(c);
+  c->extent = e;
+}
+
+dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
+  const dmd::CellGrid g = make_grid(c, I);
+  const size_t ncell = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
+  HIPCHK(c->d_cell_count.ensure(ncell));
+  HIPCHK(c->d_cell_start.ensure(ncell + 1));
+  HIPCHK(c->d_cell_cursor.ensure(ncell));
+  hipEvent_t ev{};
+  CHK(begin_timed(c, &ev));
+  HIPCHK(dmd::launch_neighbors(c->stream, I.box, g, c->rc, I.base, I.n, c->d_qs.p, c->d_cell_of.p,
+                               c->d_cell_count.p, c->d_cell_start.p, c->d_cell_cursor.p,
+                               c->d_cell_atoms.p, c->d_nnb.p, c->d_nbr.p, c->d_flags.p));
+  CHK(end_timed(c, KID_NEIGH, ev));
+  int flags[2];
+  HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (flags[0]) return fail(DMDGPU_ERR_MAXNEIGH, "a site has more than 250 neighbours");
+  if (flags[1]) return fail(DMDGPU_ERR_NONEIGH, "a site has no neighbours");
+  // active ordered pairs of this instance
+  CHK(upload_meta(c));
+  HIPCHK(dmd::launch_active_pairs(c->stream, make_state(c), c->d_pairs.p));
+  const int c0 = I.base / kChunk, nch = I.cap / kChunk;
+  std::vector<long long> pp(nch);
+  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p + c0, sizeof(long long) * nch,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  I.pairs = 0;
+  for (long long v : pp) I.pairs += v;
+  I.have_neigh = true;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status remove_instance(dmdgpu_ctx* c, int id) {
+  if (id < 0 || id >= (int)c->inst.size() || !c->inst[id].used)
+    return fail(DMDGPU_ERR_ARG, "no such instance");
+  Instance& I = c->inst[id];
+  for (int i = 0; i < I.cap; i++) c->h_inst[I.base + i] = -1;
+  HIPCHK(hipMemcpy(c->d_inst.p + I.base, c->h_inst.data() + I.base, sizeof(int) * I.cap,
+                   hipMemcpyHostToDevice));
+  I = Instance();
+  c->h_prec[id] = -1;
+  if (c->primary == id) c->primary = -1;
+  recompute_extent(c);
+  c->meta_dirty = true;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* sg, const double* x,
+                           const int32_t* spc, const double* beta, const double* gamma,
+                           const double* box, const int32_t* periodic, int* out_id) {
+  if (n <= 0 || !q || !sg || !x || !spc || !beta || !gamma || !box || !periodic)
+    return fail(DMDGPU_ERR_ARG, "null argument or n <= 0");
+  HIPCHK(hipSetDevice(c->device));
+  for (int i = 0; i < n; i++)
+    if (spc[i] != 1 && spc[i] != 12)
+      return fail(DMDGPU_ERR_SPECIES,
+                  "species other than H (1) or Mg (12) at site " + std::to_string(i));
+  Instance I;
+  for (int a = 0; a < 3; a++) {
+    I.box.lo[a] = box[a];
+    I.box.L[a] = box[3 + a] - box[a];
+    I.box.per[a] = periodic[a] ? 1 : 0;
+    if (I.box.per[a] && !(I.box.L[a] >= 2.0 * c->rc))
+      return fail(DMDGPU_ERR_BOX, "periodic box edge shorter than 2 r_c");
+    I.box.invL[a] = I.box.L[a] > 0 ? 1.0 / I.box.L[a] : 0.0;
+  }
+  CHK(upload_tables(c));
+  I.n = n;
+  I.cap = (n + kChunk - 1) / kChunk * kChunk;
+  I.base = place(c, I.cap);
+  const size_t need = (size_t)I.base + I.cap;
+  if (c->budget && need * kBytesPerSite > c->budget)
+    return fail(DMDGPU_ERR_BUDGET, "instance does not fit the device-memory budget");
+  CHK(ensure_sites(c, need));
+  I.used = true;
+  I.h_q.assign(q, q + 3 * (size_t)n);
+  make_order(c, I);
+  I.h_slot.resize(n);
+  std::vector<double> bp(I.cap, 0.0), gp(I.cap, 0.0), z(I.cap, 0.0);
+  std::vector<int> sl(I.cap, 0), zi(I.cap, 0);
+  for (int i = 0; i < n; i++) {
+    const int o = I.perm[i];
+    I.h_slot[i] = spc[o] == 1 ? 1 : 0;
+    sl[i] = I.h_slot[i];
+    bp[i] = beta[o];
+    gp[i] = gamma[o];
+  }
+  int id = -1;
+  for (int k = 0; k < (int)c->inst.size(); k++)
+    if (!c->inst[k].used) {
+      id = k;
+      break;
+    }
+  if (id < 0) {
+    id = (int)c->inst.size();
+    c->inst.emplace_back();
+  }
+  const int b = I.base, cp = I.cap;
+  HIPCHK(hipMemcpy(c->d_spc.p + b, sl.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_beta.p + b, bp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_gamma.p + b, gp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  // padding sites of the range: inactive (x = 0), no neighbours, unit width
+  HIPCHK(hipMemcpy(c->d_ne.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_xr.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_nnb.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  {
+    std::vector<double4> q0(cp, make_double4(0.0, 0.0, 0.0, 1.0));
+    std::vector<double4> z4(cp, make_double4(0.0, 0.0, 0.0, 0.0));
+    HIPCHK(hipMemcpy(c->d_qs.p + b, q0.data(), sizeof(double4) * cp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_X.p + b, q0.data(), sizeof(double4) * cp, hipMemcpyHostToDevice));
+    double4* vz[] = {c->d_R0.p, c->d_R0f.p, c->d_D.p, c->d_Rr.p, c->d_Pp.p};
+    for (double4* v : vz)
+      HIPCHK(hipMemcpy(v + b, z4.data(), sizeof(double4) * cp, hipMemcpyHostToDevice));
+  }
+  c->inst[id] = std::move(I);
+  Instance& J = c->inst[id];
+  dmdgpu_status st = upload_unknowns(c, J, q, sg, x, true);
+  if (st) {
+    J = Instance();
+    return st;
+  }
+  if ((int)c->h_inst.size() < J.base + J.cap) c->h_inst.resize(J.base + J.cap, -1);
+  for (int i = 0; i < J.cap; i++) c->h_inst[J.base + i] = id;
+  HIPCHK(hipMemcpy(c->d_inst.p + J.base, c->h_inst.data() + J.base, sizeof(int) * J.cap,
+                   hipMemcpyHostToDevice));
+  recompute_extent(c);
+  c->meta_dirty = true;
+  if (c->h_prec.size() < c->inst.size()) c->h_prec.resize(c->inst.size(), -1);
+  if (c->h_ops.size() < c->inst.size()) c->h_ops.resize(c->inst.size());
+  c->h_prec[id] = -1;
+  *out_id = id;
+  return DMDGPU_OK;
+}
+
+Instance* get_inst(dmdgpu_ctx* c, int id) {
+  if (!c || id < 0 || id >= (int)c->inst.size() || !c->inst[id].used) return nullptr;
+  return &c->inst[id];
+}
+
+// One residual evaluation of every instance whose h_prec >= 0.
+dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
+  CHK(upload_meta(c));
+  CHK(upload_prec(c));
+  bool any[2] = {false, false};
+  for (int k = 0; k < (int)c->inst.size(); k++)
+    if (c->inst[k].used && c->h_prec[k] >= 0) any[c->h_prec[k]] = true;
+  const dmd::State s = make_state(c);
+  const dmd::Outputs o = make_outputs(c);
+  hipEvent_t a{};
+  CHK(begin_timed(c, &a));
+  for (int p = 0; p < 2; p++)
+    if (any[p]) HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
+  CHK(end_timed(c, KID_PHASE1, a));
+  if (!phase2) return DMDGPU_OK;
+  CHK(begin_timed(c, &a));
+  for (int p = 0; p < 2; p++)
+    if (any[p]) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
+  CHK(end_timed(c, KID_PHASE2, a));
+  CHK(begin_timed(c, &a));
+  HIPCHK(dmd::launch_totals(c->stream, s, o, (int)c->inst.size(), c->d_chunk_lo.p,
+                            c->d_chunk_hi.p));
+  CHK(end_timed(c, KID_TOTALS, a));
+  return DMDGPU_OK;
+}
+
+// Evaluates one instance in the context's default precision.
+dmdgpu_status eval_one(dmdgpu_ctx* c, int id, bool phase2 = true) {
+  std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+  c->h_prec[id] = c->cfg.precision;
+  return run_eval(c, phase2);
+}
+
+dmdgpu_status eval_all(dmdgpu_ctx* c) {
+  for (int k = 0; k < (int)c->inst.size(); k++)
+    c->h_prec[k] = c->inst[k].used && c->inst[k].have_neigh ? c->cfg.precision : -1;
+  return run_eval(c, true);
+}
+
+// Copies `width` doubles per site of instance I back to the caller's order.
+dmdgpu_status d2h_sites(dmdgpu_ctx* c, const Instance& I, double* h, const double* d, int width) {
+  if (!h) return DMDGPU_OK;
+  const int n = I.n;
+  std::vector<double> tmp((size_t)n * width);
+  HIPCHK(hipMemcpyAsync(tmp.data(), d + (size_t)I.base * width, tmp.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < width; k++) h[(size_t)I.perm[i] * width + k] = tmp[(size_t)i * width + k];
+  return DMDGPU_OK;
+}
+
+dmdgpu_status copy_residual(dmdgpu_ctx* c, int id, double* Rq, double* Rs, double* Rx,
+                            double* tot) {
+  const Instance& I = c->inst[id];
+  CHK(d2h_sites(c, I, Rq, c->d_Rq.p, 3));
+  CHK(d2h_sites(c, I, Rs, c->d_Rs.p, 1));
+  CHK(d2h_sites(c, I, Rx, c->d_Rx.p, 1));
+  if (tot) {
+    HIPCHK(hipMemcpyAsync(tot, c->d_totals.p + 2 * id, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return DMDGPU_OK;
+}
+
+dmdgpu_status check_primary(dmdgpu_ctx* c) {
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  if (c->primary < 0) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
+  if (!c->inst[c->primary].have_neigh)
+    return fail(DMDGPU_ERR_STATE, "dmdgpu_neighbors has not been called");
+  return DMDGPU_OK;
+}
+
+// ---- batched Newton-Krylov relaxation ----------------------------------------
+//
+// Per instance (unknowns: q and sigma of its active sites; occupancies fixed):
+//   START  : FP64 residual at X; stop when |R|_2 <= atol.
+//   BASE32 : (mixed) split-precision residual at X, the base of FP32 probes.
+//   CG     : conjugate gradients on H d = -R with H v = (R(X + eps v) - R(X))/eps;
+//            stop at |r| <= eta |R|, max_cg iterations or negative curvature.
+//   LSPREP : step cap so that max |dq| <= 0.2 A and sigma stays >= sigma/2.
+//   LS     : backtracking on |R(X + a d)| (sufficient decrease), FP64.
+// Every stage that needs a residual gets exactly one per tick.
+constexpr double kProbe64 = 1e-7;  // RMS displacement of an FP64 probe (A)
+constexpr double kProbe32 = 3e-4;  // RMS displacement of a split-precision probe (A)
+constexpr double kMaxStep = 0.2;   // max |dq| of a Newton step (A)
+
+void plan_tick(dmdgpu_ctx* c) {
+  const int ni = (int)c->inst.size();
+  for (int k = 0; k < ni; k++) {
+    Instance& I = c->inst[k];
+    dmd::SolveOp op = I.used ? I.sv.pending : dmd::SolveOp{};
+    op.make = dmd::kMakeNone;
+    op.dot = dmd::kDotNone;
+    op.mixed = I.sv.o.mixed;
+    op.dof = I.sv.o.dof_mask;
+    c->h_prec[k] = -1;
+    if (I.used) {
+      Solve& S = I.sv;
+      switch (S.stage) {
+        case ST_START:
+          op.make = dmd::kMakeX;
+          op.dot = dmd::kDotNorm;
+          c->h_prec[k] = dmd::kFP64;
+          break;
+        case ST_BASE32:
+          op.make = dmd::kMakeX;
+          c->h_prec[k] = dmd::kSplit32;
+          break;
+        case ST_CG:
+          S.eps = (S.o.mixed ? kProbe32 : kProbe64) *
+                  std::sqrt((double)S.ndof / std::max(S.pp, 1e-300));
+          op.eps = S.eps;
+          op.make = dmd::kMakeXP;
+          op.dot = dmd::kDotCg;
+          c->h_prec[k] = S.o.mixed ? dmd::kSplit32 : dmd::kFP64;
+          break;
+        case ST_LSPREP:
+          op.dot = dmd::kDotLs;
+          break;
+        case ST_LS:
+          op.make = dmd::kMakeXD;
+          op.alpha_ls = S.alpha_ls;
+          op.dot = dmd::kDotNorm;
+          c->h_prec[k] = dmd::kFP64;
+          break;
+        default:
+          break;
+      }
+      if (c->h_prec[k] >= 0) S.evals++;
+      S.pending = dmd::SolveOp{};
+    }
+    c->h_ops[k] = op;
+  }
+}
+
+void finish(Solve& S, int result) {
+  S.stage = ST_DONE;
+  S.result = result;
+}
+
+// Consumes the tick's dots: advances each instance's state machine and queues
+// the vector updates for the start of the next tick.
+void digest_tick(dmdgpu_ctx* c) {
+  const int ni = (int)c->inst.size();
+  for (int k = 0; k < ni; k++) {
+    Instance& I = c->inst[k];
+    if (!I.used) continue;
+    Solve& S = I.sv;
+    const double* d = &c->h_sdots[5 * k];
+    dmd::SolveOp& nx = S.pending;
+    switch (S.stage) {
+      case ST_START: {
+        S.rnorm = std::sqrt(d[0]);
+        S.rnorm0 = S.rnorm;
+        nx.upd |= dmd::kUpdR0FromRv;
+        if (S.rnorm <= S.o.atol) {
+          finish(S, DMDGPU_RELAX_CONVERGED);
+          break;
+        }
+        if (S.o.mixed) {
+          S.stage = ST_BASE32;
+          break;
+        }
+        nx.upd |= dmd::kUpdCgInit;
+        S.rr = S.pp = S.rnorm * S.rnorm;
+        S.cg = 0;
+        S.stage = ST_CG;
+        break;
+      }
+      case ST_BASE32: {
+        nx.upd |= dmd::kUpdR0fFromRv | dmd::kUpdCgInit;
+        S.rr = S.pp = S.rnorm * S.rnorm;
+        S.cg = 0;
+        S.stage = ST_CG;
+        break;
+      }
+      case ST_CG: {
+        const double pAp = d[0], rAp = d[1], ApAp = d[2];
+        if (!(pAp > 0.0)) {
+          // negative curvature: keep the current step, or a short steepest step
+          if (S.cg == 0) {
+            nx.upd |= dmd::kUpdSdDir;
+            nx.c_sd = kMaxStep / std::sqrt(std::max(S.pp, 1e-300) / std::max(S.ndof, 1));
+          }
+          S.stage = ST_LSPREP;
+          break;
+        }
+        const double a = S.rr / pAp;
+        double rr1 = S.rr - 2.0 * a * rAp + a * a * ApAp;
+        rr1 = std::max(rr1, 0.0);
+        const double b = rr1 / S.rr;
+        nx.upd |= dmd::kUpdCgStep;
+        nx.eps_prev = S.eps;
+        nx.alpha = a;
+        nx.beta = b;
+        S.pp = rr1 + b * b * S.pp;
+        S.rr = rr1;
+        S.cg++;
+        if (std::sqrt(rr1) <= S.o.eta * S.rnorm || S.cg >= S.o.max_cg || !(rr1 > 0.0))
+          S.stage = ST_LSPREP;
+        break;
+      }
+      case ST_LSPREP: {
+        const double dq = d[3], ds = d[4];
+        double a = 1.0;
+        if (dq > kMaxStep) a = std::min(a, kMaxStep / dq);
+        if (ds > 0.5) a = std::min(a, 0.5 / ds);
+        S.alpha_ls = a;
+        S.ls = 0;
+        S.stage = ST_LS;
+        break;
+      }
+      case ST_LS: {
+        const double rn = std::sqrt(d[0]);
+        if (std::isfinite(rn) && rn <= (1.0 - 1e-4 * S.alpha_ls) * S.rnorm) {
+          nx.upd |= dmd::kUpdAcceptY | dmd::kUpdR0FromRv;
+          S.rnorm = rn;
+          S.newton++;
+          if (rn <= S.o.atol) {
+            finish(S, DMDGPU_RELAX_CONVERGED);
+            break;
+          }
+          if (S.newton >= S.o.max_newton) {
+            finish(S, DMDGPU_RELAX_MAXITER);
+            break;
+          }
+          if (S.o.mixed) {
+            S.stage = ST_BASE32;
+            break;
+          }
+          nx.upd |= dmd::kUpdCgInit;
+          S.rr = S.pp = rn * rn;
+          S.cg = 0;
+          S.stage = ST_CG;
+        } else {
+          S.alpha_ls *= 0.5;
+          if (++S.ls > 12) finish(S, DMDGPU_RELAX_STALLED);
+        }
+        break;
+      }
+      default:
+        break;
+    }
+    if (S.stage != ST_DONE && S.stage != ST_IDLE && S.evals >= S.o.max_evals)
+      finish(S, DMDGPU_RELAX_MAXITER);
+  }
+}
+
+// The device work of one tick: vector updates + evaluation points, residual
+// of every instance that needs one (both precisions), totals, dots.
+dmdgpu_status tick_device(dmdgpu_ctx* c) {
+  const dmd::State s = make_state(c);
+  const dmd::Outputs o = make_outputs(c);
+  const int ni = (int)c->inst.size();
+  HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
+                                 c->d_ops.p, make_vecs(c), c->d_qs.p, o));
+  for (int p = 0; p < 2; p++) HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
+  for (int p = 0; p < 2; p++) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
+  HIPCHK(dmd::launch_totals(c->stream, s, o, ni, c->d_chunk_lo.p, c->d_chunk_hi.p));
+  HIPCHK(dmd::launch_solver_dots(c->stream, c->extent, ni, c->d_inst.p, c->d_spc.p, c->d_ne.p,
+                                 c->d_ops.p, make_vecs(c), o, c->d_spart.p, c->d_chunk_lo.p,
+                                 c->d_chunk_hi.p, c->d_sdots.p));
+  return DMDGPU_OK;
+}
+
+dmdgpu_status run_tick(dmdgpu_ctx* c) {
+  CHK(upload_meta(c));
+  plan_tick(c);
+  const int ni = (int)c->inst.size();
+  HIPCHK(hipMemcpyAsync(c->d_ops.p, c->h_ops.data(), sizeof(dmd::SolveOp) * ni,
+                        hipMemcpyHostToDevice, c->stream));
+  CHK(upload_prec(c));
+  const bool same_cfg = c->graph_cfg.k1 == c->cfg.k1 && c->graph_cfg.k2 == c->cfg.k2 &&
+                        c->graph_cfg.unroll2 == c->cfg.unroll2;
+  if (c->use_graph) {
+    if (!c->gexec || c->graph_extent != c->extent || c->graph_ninst != ni || !same_cfg) {
+      drop_graph(c);
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      dmdgpu_status st = tick_device(c);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (st) return st;
+      if (e != hipSuccess)
+        return fail(DMDGPU_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      c->graph = g;
+      HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      c->graph_extent = c->extent;
+      c->graph_ninst = ni;
+      c->graph_cfg = c->cfg;
+    }
+    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    CHK(tick_device(c));
+  }
+  c->h_sdots.resize((size_t)5 * ni);
+  HIPCHK(hipMemcpyAsync(c->h_sdots.data(), c->d_sdots.p, sizeof(double) * 5 * ni,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<int> before(ni);
+  for (int k = 0; k < ni; k++) before[k] = c->inst[k].sv.stage;
+  digest_tick(c);
+  c->ticks++;
+  // Instances that finished this tick: apply their last updates (accepting the
+  // final step) now, and leave their evaluation point at the final iterate.
+  bool flush = false;
+  for (int k = 0; k < ni; k++) {
+    Instance& I = c->inst[k];
+    dmd::SolveOp op{};
+    if (I.used && before[k] != ST_DONE && I.sv.stage == ST_DONE) {
+      op = I.sv.pending;
+      op.make = dmd::kMakeX;
+      op.mixed = I.sv.o.mixed;
+      op.dof = I.sv.o.dof_mask;
+      I.sv.pending = dmd::SolveOp{};
+      flush = true;
+    }
+    c->h_ops[k] = op;
+  }
+  if (flush) {
+    HIPCHK(hipMemcpyAsync(c->d_ops.p, c->h_ops.data(), sizeof(dmd::SolveOp) * ni,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
+                                   c->d_ops.p, make_vecs(c), c->d_qs.p, make_outputs(c)));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return DMDGPU_OK;
+}
+
+int running_count(const dmdgpu_ctx* c) {
+  int run = 0;
+  for (const Instance& I : c->inst)
+    if (I.used && I.sv.stage != ST_IDLE && I.sv.stage != ST_DONE) run++;
+  return run;
 }
 
 }  // namespace
@@ -392,17 +1036,24 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
+  drop_graph(c);
   c->d_uw32.release();
   c->d_ph32.release();
   c->d_rh32.release();
-  DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph, &c->d_rh, &c->d_em, &c->d_ne, &c->d_xr,
-                           &c->d_beta, &c->d_gamma, &c->d_rec, &c->d_V0, &c->d_rho, &c->d_Rq,
-                           &c->d_Rs, &c->d_Rx, &c->d_S0, &c->d_partials, &c->d_totals};
+  DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph,    &c->d_rh,  &c->d_em,       &c->d_ne,
+                           &c->d_xr,  &c->d_beta,  &c->d_gamma, &c->d_rec,    &c->d_V0,
+                           &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
+                           &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart};
   for (auto* b : dbs) b->release();
-  DevBuf<int>* ibs[] = {&c->d_spc, &c->d_nnb, &c->d_nbr, &c->d_cell_of, &c->d_cell_count,
-                        &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms, &c->d_flags};
+  DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
+                        &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
+                        &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_chunk_lo,
+                        &c->d_chunk_hi};
   for (auto* b : ibs) b->release();
-  c->d_qs.release();
+  DevBuf<double4>* vbs[] = {&c->d_qs, &c->d_X, &c->d_R0, &c->d_R0f, &c->d_D, &c->d_Rr, &c->d_Pp};
+  for (auto* b : vbs) b->release();
+  c->d_boxes.release();
+  c->d_ops.release();
   c->d_pairs.release();
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -446,13 +1097,14 @@ dmdgpu_status dmdgpu_init_adp_MgHx(dmdgpu_ctx* c, int comb, const char* folder) 
 dmdgpu_status dmdgpu_set_cutoff(dmdgpu_ctx* c, double rc) {
   if (!c || !(rc > 0)) return fail(DMDGPU_ERR_ARG, "cut-off must be positive");
   c->rc = rc;
-  c->have_neigh = false;
+  for (Instance& I : c->inst) I.have_neigh = false;
   return DMDGPU_OK;
 }
 
 dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
-  auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8; };
-  if (!c || !ok(k1) || !ok(k2)) return fail(DMDGPU_ERR_ARG, "lanes per site must be 1, 2, 4 or 8");
+  auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; };
+  if (!c || !ok(k1) || !ok(k2))
+    return fail(DMDGPU_ERR_ARG, "lanes per site must be 1, 2, 4, 8 or 16");
   c->cfg.k1 = k1;
   c->cfg.k2 = k2;
   c->cfg.unroll2 = unroll2 ? 1 : 0;
@@ -466,167 +1118,49 @@ dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* c, int precision) {
   return DMDGPU_OK;
 }
 
-static dmdgpu_status upload_unknowns(dmdgpu_ctx* c, const double* q, const double* sg,
-                                     const double* x, const int32_t* spc, bool first) {
-  const int n = c->n;
-  (void)spc;
-  if (q) c->h_q.assign(q, q + 3 * (size_t)n);
-  std::vector<double4> qs;
-  if (q || sg) {
-    qs.resize(n);
-    if (!first) HIPCHK(hipMemcpy(qs.data(), c->d_qs.p, sizeof(double4) * n, hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; i++) {
-      const int o = c->perm[i];
-      if (q) {
-        qs[i].x = q[3 * o];
-        qs[i].y = q[3 * o + 1];
-        qs[i].z = q[3 * o + 2];
-      }
-      if (sg) {
-        if (!(sg[o] > 0)) return fail(DMDGPU_ERR_ARG, "widths must be positive");
-        qs[i].w = sg[o];
-      }
-    }
-    HIPCHK(hipMemcpy(c->d_qs.p, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-  }
-  if (x) {
-    std::vector<double> ne(n), xr(n);
-    for (int i = 0; i < n; i++) {
-      const int o = c->perm[i];
-      if (c->h_slot[i] == 0 && x[o] < kMinOcc)
-        return fail(DMDGPU_ERR_SPECIES, "Mg site with occupancy below 5e-4 (site " +
-                                            std::to_string(o) + ")");
-      xr[i] = x[o];
-      ne[i] = x[o] >= kMinOcc ? x[o] : 0.0;
-    }
-    HIPCHK(hipMemcpy(c->d_ne.p, ne.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_xr.p, xr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  }
-  return DMDGPU_OK;
-}
-
 dmdgpu_status dmdgpu_set_state(dmdgpu_ctx* c, int n, const double* q, const double* sg,
                                const double* x, const int32_t* spc, const double* beta,
                                const double* gamma, const double* box, const int32_t* periodic) {
-  if (!c || n <= 0 || !q || !sg || !x || !spc || !beta || !gamma || !box || !periodic)
-    return fail(DMDGPU_ERR_ARG, "set_state: null argument or n <= 0");
-  HIPCHK(hipSetDevice(c->device));
-  for (int i = 0; i < n; i++)
-    if (spc[i] != 1 && spc[i] != 12)
-      return fail(DMDGPU_ERR_SPECIES, "species other than H (1) or Mg (12) at site " +
-                                          std::to_string(i));
-  dmd::Box b{};
-  for (int a = 0; a < 3; a++) {
-    b.lo[a] = box[a];
-    b.L[a] = box[3 + a] - box[a];
-    b.per[a] = periodic[a] ? 1 : 0;
-    if (b.per[a] && !(b.L[a] >= 2.0 * c->rc))
-      return fail(DMDGPU_ERR_BOX, "periodic box edge shorter than 2 r_c");
-    b.invL[a] = b.L[a] > 0 ? 1.0 / b.L[a] : 0.0;
-  }
-  dmdgpu_status st = upload_tables(c);
-  if (st) return st;
-  c->n = n;
-  c->box = b;
-  HIPCHK(c->d_qs.ensure(n));
-  HIPCHK(c->d_ne.ensure(n));
-  HIPCHK(c->d_xr.ensure(n));
-  HIPCHK(c->d_beta.ensure(n));
-  HIPCHK(c->d_gamma.ensure(n));
-  HIPCHK(c->d_spc.ensure(n));
-  HIPCHK(c->d_nnb.ensure(n));
-  HIPCHK(c->d_nbr.ensure((size_t)n * dmd::kCap));
-  HIPCHK(c->d_cell_of.ensure(n));
-  HIPCHK(c->d_cell_atoms.ensure(n));
-  HIPCHK(c->d_flags.ensure(2));
-  HIPCHK(c->d_rec.ensure((size_t)n * dmd::kRec));
-  HIPCHK(c->d_V0.ensure(n));
-  HIPCHK(c->d_rho.ensure(n));
-  HIPCHK(c->d_Rq.ensure((size_t)3 * n));
-  HIPCHK(c->d_Rs.ensure(n));
-  HIPCHK(c->d_Rx.ensure(n));
-  HIPCHK(c->d_S0.ensure(n));
-  HIPCHK(c->d_partials.ensure((size_t)2 * ((n + 255) / 256)));
-  HIPCHK(c->d_totals.ensure(2));
-  HIPCHK(c->d_pairs.ensure((n + 255) / 256));
-  c->h_q.assign(q, q + 3 * (size_t)n);
-  make_order(c);
-  c->h_slot.resize(n);
-  std::vector<double> bp(n), gp(n);
-  for (int i = 0; i < n; i++) {
-    const int o = c->perm[i];
-    c->h_slot[i] = spc[o] == 1 ? 1 : 0;
-    bp[i] = beta[o];
-    gp[i] = gamma[o];
-  }
-  HIPCHK(hipMemcpy(c->d_spc.p, c->h_slot.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_beta.p, bp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_gamma.p, gp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  st = upload_unknowns(c, q, sg, x, spc, true);
-  if (st) return st;
-  c->have_state = true;
-  c->have_neigh = false;
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  for (int k = 0; k < (int)c->inst.size(); k++)
+    if (c->inst[k].used) CHK(remove_instance(c, k));
+  int id = -1;
+  CHK(add_instance(c, n, q, sg, x, spc, beta, gamma, box, periodic, &id));
+  c->primary = id;
   return DMDGPU_OK;
 }
 
 dmdgpu_status dmdgpu_update_state(dmdgpu_ctx* c, const double* q, const double* sg,
                                   const double* x) {
-  if (!c || !c->have_state) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
+  if (!c || c->primary < 0) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
   HIPCHK(hipSetDevice(c->device));
-  return upload_unknowns(c, q, sg, x, nullptr, false);
+  return upload_unknowns(c, c->inst[c->primary], q, sg, x, false);
 }
 
 dmdgpu_status dmdgpu_neighbors(dmdgpu_ctx* c) {
-  if (!c || !c->have_state) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
+  if (!c || c->primary < 0) return fail(DMDGPU_ERR_STATE, "dmdgpu_set_state has not been called");
   HIPCHK(hipSetDevice(c->device));
-  const dmd::CellGrid g = make_grid(c);
-  const int n = c->n;
-  const size_t ncell = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
-  HIPCHK(c->d_cell_count.ensure(ncell));
-  HIPCHK(c->d_cell_start.ensure(ncell + 1));
-  HIPCHK(c->d_cell_cursor.ensure(ncell));
-  hipEvent_t ev{};
-  dmdgpu_status st = begin_timed(c, KID_NEIGH, &ev);
-  if (st) return st;
-  HIPCHK(dmd::launch_neighbors(c->stream, c->box, g, c->rc, n, c->d_qs.p, c->d_cell_of.p,
-                               c->d_cell_count.p, c->d_cell_start.p, c->d_cell_cursor.p,
-                               c->d_cell_atoms.p, c->d_nnb.p, c->d_nbr.p, c->d_flags.p));
-  if ((st = end_timed(c, KID_NEIGH, ev))) return st;
-  int flags[2];
-  HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (flags[0]) return fail(DMDGPU_ERR_MAXNEIGH, "a site has more than 250 neighbours");
-  if (flags[1]) return fail(DMDGPU_ERR_NONEIGH, "a site has no neighbours");
-  // active ordered pairs
-  const int nb = (n + 255) / 256;
-  HIPCHK(dmd::launch_active_pairs(c->stream, make_state(c), c->d_pairs.p, nb));
-  std::vector<long long> pp(nb);
-  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p, sizeof(long long) * nb, hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  c->pairs = 0;
-  for (long long v : pp) c->pairs += v;
-  c->have_neigh = true;
-  return DMDGPU_OK;
+  return build_neighbors(c, c->inst[c->primary]);
 }
 
 dmdgpu_status dmdgpu_get_neighbors(dmdgpu_ctx* c, int32_t* h_nn, int32_t* h_list, int cap) {
-  dmdgpu_status st = check_ready(c);
-  if (st) return st;
+  CHK(check_primary(c));
   if (cap < dmd::kMaxNeigh) return fail(DMDGPU_ERR_ARG, "cap must be >= 250");
-  const int n = c->n;
+  const Instance& I = c->inst[c->primary];
+  const int n = I.n;
   std::vector<int> nn(n), rows((size_t)n * dmd::kCap);
-  HIPCHK(hipMemcpyAsync(nn.data(), c->d_nnb.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(rows.data(), c->d_nbr.p, sizeof(int) * rows.size(), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(nn.data(), c->d_nnb.p + I.base, sizeof(int) * n, hipMemcpyDeviceToHost,
                         c->stream));
+  HIPCHK(hipMemcpyAsync(rows.data(), c->d_nbr.p + (size_t)I.base * dmd::kCap,
+                        sizeof(int) * rows.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<int> row;
   for (int o = 0; o < n; o++) {
-    const int i = c->inv[o];
+    const int i = I.inv[o];
     if (h_nn) h_nn[o] = nn[i];
     if (!h_list) continue;
     row.assign(rows.begin() + (size_t)i * dmd::kCap, rows.begin() + (size_t)i * dmd::kCap + nn[i]);
-    for (int& v : row) v = c->perm[v];
+    for (int& v : row) v = I.perm[v - I.base];
     std::sort(row.begin(), row.end());
     for (int k = 0; k < cap; k++) h_list[(size_t)o * cap + k] = k < nn[i] ? row[k] : -1;
   }
@@ -634,45 +1168,187 @@ dmdgpu_status dmdgpu_get_neighbors(dmdgpu_ctx* c, int32_t* h_nn, int32_t* h_list
 }
 
 dmdgpu_status dmdgpu_mf_rho(dmdgpu_ctx* c, double* h_rho) {
-  dmdgpu_status st = check_ready(c);
-  if (st) return st;
+  CHK(check_primary(c));
   HIPCHK(hipSetDevice(c->device));
-  if ((st = run_phase1(c))) return st;
-  if ((st = d2h_sites(c, h_rho, c->d_rho.p, 1))) return st;
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return DMDGPU_OK;
+  CHK(eval_one(c, c->primary, false));
+  return d2h_sites(c, c->inst[c->primary], h_rho, c->d_rho.p, 1);
 }
 
 dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* c, double* h_V0, double* h_S0) {
-  dmdgpu_status st = check_ready(c);
-  if (st) return st;
+  CHK(check_primary(c));
   HIPCHK(hipSetDevice(c->device));
-  if ((st = run_residual(c))) return st;
-  if ((st = d2h_sites(c, h_V0, c->d_V0.p, 1))) return st;
-  if ((st = d2h_sites(c, h_S0, c->d_S0.p, 1))) return st;
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return DMDGPU_OK;
+  CHK(eval_one(c, c->primary));
+  CHK(d2h_sites(c, c->inst[c->primary], h_V0, c->d_V0.p, 1));
+  return d2h_sites(c, c->inst[c->primary], h_S0, c->d_S0.p, 1);
 }
 
 dmdgpu_status dmdgpu_residual(dmdgpu_ctx* c, double* h_Rq, double* h_Rs, double* h_Rx,
                               double* h_tot) {
-  dmdgpu_status st = check_ready(c);
-  if (st) return st;
+  CHK(check_primary(c));
   HIPCHK(hipSetDevice(c->device));
-  if ((st = run_residual(c))) return st;
-  if ((st = d2h_sites(c, h_Rq, c->d_Rq.p, 3))) return st;
-  if ((st = d2h_sites(c, h_Rs, c->d_Rs.p, 1))) return st;
-  if ((st = d2h_sites(c, h_Rx, c->d_Rx.p, 1))) return st;
-  if ((st = d2h(c, h_tot, c->d_totals.p, 2))) return st;
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return DMDGPU_OK;
+  CHK(eval_one(c, c->primary));
+  return copy_residual(c, c->primary, h_Rq, h_Rs, h_Rx, h_tot);
 }
 
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* c) {
-  dmdgpu_status st = check_ready(c);
-  if (st) return st;
-  return run_residual(c);
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  return eval_all(c);
 }
+
+// ---- instances ----------------------------------------------------------------
+
+dmdgpu_status dmdgpu_add_instance(dmdgpu_ctx* c, int n, const double* q, const double* sg,
+                                  const double* x, const int32_t* spc, const double* beta,
+                                  const double* gamma, const double* box,
+                                  const int32_t* periodic, int* id) {
+  if (!c || !id) return fail(DMDGPU_ERR_ARG, "null argument");
+  CHK(add_instance(c, n, q, sg, x, spc, beta, gamma, box, periodic, id));
+  dmdgpu_status st = build_neighbors(c, c->inst[*id]);
+  if (st) {
+    std::string msg = g_err;
+    (void)remove_instance(c, *id);
+    *id = -1;
+    return fail(st, msg);
+  }
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_remove_instance(dmdgpu_ctx* c, int id) {
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  return remove_instance(c, id);
+}
+
+dmdgpu_status dmdgpu_instance_info(dmdgpu_ctx* c, int id, int* n, int64_t* pairs,
+                                   int64_t* arena_sites) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (n) *n = I->n;
+  if (pairs) *pairs = I->pairs;
+  if (arena_sites) *arena_sites = c->extent;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_set_budget(dmdgpu_ctx* c, int64_t bytes) {
+  if (!c || bytes < 0) return fail(DMDGPU_ERR_ARG, "bad budget");
+  c->budget = (size_t)bytes;
+  return DMDGPU_OK;
+}
+
+int64_t dmdgpu_bytes_per_site(void) { return (int64_t)kBytesPerSite; }
+
+dmdgpu_status dmdgpu_evaluate(dmdgpu_ctx* c) {
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  return eval_all(c);
+}
+
+dmdgpu_status dmdgpu_instance_residual(dmdgpu_ctx* c, int id, double* Rq, double* Rs, double* Rx,
+                                       double* tot) {
+  if (!get_inst(c, id)) return fail(DMDGPU_ERR_ARG, "no such instance");
+  return copy_residual(c, id, Rq, Rs, Rx, tot);
+}
+
+dmdgpu_status dmdgpu_instance_state(dmdgpu_ctx* c, int id, double* q, double* sg, double* x) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  const int n = I->n;
+  std::vector<double4> X(n);
+  HIPCHK(hipMemcpyAsync(X.data(), c->d_X.p + I->base, sizeof(double4) * n, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) {
+    const int o = I->perm[i];
+    if (q) {
+      q[3 * o] = X[i].x;
+      q[3 * o + 1] = X[i].y;
+      q[3 * o + 2] = X[i].z;
+    }
+    if (sg) sg[o] = X[i].w;
+  }
+  if (x) CHK(d2h_sites(c, *I, x, c->d_xr.p, 1));
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_instance_set_unknowns(dmdgpu_ctx* c, int id, const double* q,
+                                           const double* sg, const double* x) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  return upload_unknowns(c, *I, q, sg, x, false);
+}
+
+// ---- relaxation ----------------------------------------------------------------
+
+dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts* o) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
+  Solve S;
+  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0};
+  if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
+      S.o.dof_mask < 0 || S.o.dof_mask > 15)
+    return fail(DMDGPU_ERR_ARG, "bad relaxation options");
+  if (S.o.max_evals <= 0) S.o.max_evals = 100000;
+  if (S.o.dof_mask == 0) S.o.dof_mask = DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA;
+  S.stage = ST_START;
+  int nd = 0;
+  for (int i = 0; i < I->n; i++) {
+    if (!I->h_act[i]) continue;
+    const bool mg = I->h_slot[i] == 0;
+    nd += (S.o.dof_mask & (mg ? DMDGPU_DOF_MG_Q : DMDGPU_DOF_H_Q)) ? 3 : 0;
+    nd += (S.o.dof_mask & (mg ? DMDGPU_DOF_MG_SIGMA : DMDGPU_DOF_H_SIGMA)) ? 1 : 0;
+  }
+  if (nd == 0) return fail(DMDGPU_ERR_ARG, "no degree of freedom selected");
+  S.ndof = nd;
+  I->sv = S;
+  // the iterate starts from the current evaluation state of the instance
+  HIPCHK(hipMemcpyAsync(c->d_X.p + I->base, c->d_qs.p + I->base, sizeof(double4) * I->cap,
+                        hipMemcpyDeviceToDevice, c->stream));
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_relax_tick(dmdgpu_ctx* c, int* running) {
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  if (running_count(c) > 0) CHK(run_tick(c));
+  if (running) *running = running_count(c);
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* c, int id, int* state, double* rnorm, int* newton,
+                                  int* evals) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  const Solve& S = I->sv;
+  if (state)
+    *state = S.stage == ST_IDLE ? DMDGPU_RELAX_IDLE
+             : S.stage == ST_DONE ? S.result
+                                  : DMDGPU_RELAX_RUNNING;
+  if (rnorm) *rnorm = S.rnorm;
+  if (newton) *newton = S.newton;
+  if (evals) *evals = S.evals;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_relax(dmdgpu_ctx* c, const dmdgpu_relax_opts* o, int max_ticks) {
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  for (int k = 0; k < (int)c->inst.size(); k++)
+    if (c->inst[k].used && c->inst[k].have_neigh) CHK(dmdgpu_relax_start(c, k, o));
+  for (int t = 0; t < max_ticks; t++) {
+    int run = 0;
+    CHK(dmdgpu_relax_tick(c, &run));
+    if (run == 0) break;
+  }
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_set_graph(dmdgpu_ctx* c, int on) {
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  c->use_graph = on != 0;
+  if (!c->use_graph) drop_graph(c);
+  return DMDGPU_OK;
+}
+
+// ---- instrumentation -------------------------------------------------------------
 
 dmdgpu_status dmdgpu_set_timing(dmdgpu_ctx* c, int on) {
   if (!c) return fail(DMDGPU_ERR_ARG, "null context");
@@ -688,8 +1364,7 @@ const char* dmdgpu_kernel_name(int id) {
 
 dmdgpu_status dmdgpu_kernel_times(dmdgpu_ctx* c, int64_t* h_launches, double* h_ms) {
   if (!c) return fail(DMDGPU_ERR_ARG, "null context");
-  dmdgpu_status st = drain_timing(c);
-  if (st) return st;
+  CHK(drain_timing(c));
   for (int k = 0; k < KID_COUNT; k++) {
     if (h_launches) h_launches[k] = c->launches[k];
     if (h_ms) h_ms[k] = c->ms[k];
@@ -699,8 +1374,7 @@ dmdgpu_status dmdgpu_kernel_times(dmdgpu_ctx* c, int64_t* h_launches, double* h_
 
 dmdgpu_status dmdgpu_reset_kernel_times(dmdgpu_ctx* c) {
   if (!c) return fail(DMDGPU_ERR_ARG, "null context");
-  dmdgpu_status st = drain_timing(c);
-  if (st) return st;
+  CHK(drain_timing(c));
   for (int k = 0; k < KID_COUNT; k++) {
     c->launches[k] = 0;
     c->ms[k] = 0.0;
@@ -709,10 +1383,12 @@ dmdgpu_status dmdgpu_reset_kernel_times(dmdgpu_ctx* c) {
 }
 
 dmdgpu_status dmdgpu_work_counts(dmdgpu_ctx* c, int64_t* pairs, int64_t* pqe) {
-  dmdgpu_status st = check_ready(c);
-  if (st) return st;
-  if (pairs) *pairs = c->pairs;
-  if (pqe) *pqe = 50 * c->pairs;
+  if (!c) return fail(DMDGPU_ERR_ARG, "null context");
+  long long p = 0;
+  for (const Instance& I : c->inst)
+    if (I.used) p += I.pairs;
+  if (pairs) *pairs = p;
+  if (pqe) *pqe = 50 * p;
   return DMDGPU_OK;
 }
 

@@ -21,7 +21,8 @@ MAXNEIGH = 250           # src/Macros.hpp:862
 FP64, SPLIT32 = 0, 1     # dmdgpu_set_precision codes
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_IO", 4: "ERR_SPECIES", 5: "ERR_MAXNEIGH",
-          6: "ERR_NONEIGH", 7: "ERR_BOX", 8: "ERR_STATE", 9: "ERR_TABLES"}
+          6: "ERR_NONEIGH", 7: "ERR_BOX", 8: "ERR_STATE", 9: "ERR_TABLES", 10: "ERR_BUDGET"}
+RELAX_STATE = {0: "idle", 1: "running", 2: "converged", 3: "maxiter", 4: "stalled"}
 
 # Every symbol include/dmdgpu.h declares.
 EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_stream", "dmdgpu_sync",
@@ -29,11 +30,31 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_update_state", "dmdgpu_neighbors", "dmdgpu_get_neighbors", "dmdgpu_mf_rho",
            "dmdgpu_site_energies", "dmdgpu_residual", "dmdgpu_residual_resident",
            "dmdgpu_set_tuning", "dmdgpu_set_precision", "dmdgpu_set_timing", "dmdgpu_kernel_count", "dmdgpu_kernel_name",
-           "dmdgpu_kernel_times", "dmdgpu_reset_kernel_times", "dmdgpu_work_counts"]
+           "dmdgpu_kernel_times", "dmdgpu_reset_kernel_times", "dmdgpu_work_counts",
+           "dmdgpu_add_instance", "dmdgpu_remove_instance", "dmdgpu_instance_info",
+           "dmdgpu_set_budget", "dmdgpu_bytes_per_site", "dmdgpu_evaluate",
+           "dmdgpu_instance_residual", "dmdgpu_instance_state", "dmdgpu_instance_set_unknowns",
+           "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
+           "dmdgpu_set_graph"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lp = C.POINTER(C.c_int64)
+
+
+class RelaxOpts(C.Structure):
+    """dmdgpu_relax_opts."""
+    _fields_ = [("atol", C.c_double), ("max_newton", C.c_int), ("max_cg", C.c_int),
+                ("eta", C.c_double), ("mixed", C.c_int), ("max_evals", C.c_int),
+                ("dof_mask", C.c_int)]
+
+
+DOF_MG_Q, DOF_MG_SIGMA, DOF_H_Q, DOF_H_SIGMA = 1, 2, 4, 8
+
+
+def relax_opts(atol=1e-8, max_newton=50, max_cg=60, eta=0.05, mixed=False, max_evals=0,
+               dof_mask=DOF_MG_Q | DOF_MG_SIGMA):
+    return RelaxOpts(atol, max_newton, max_cg, eta, 1 if mixed else 0, max_evals, dof_mask)
 
 
 class DmdError(RuntimeError):
@@ -84,6 +105,22 @@ def load_library():
     L.dmdgpu_kernel_times.argtypes = [vp, _lp, _dp]
     L.dmdgpu_reset_kernel_times.argtypes = [vp]
     L.dmdgpu_work_counts.argtypes = [vp, _lp, _lp]
+    L.dmdgpu_add_instance.argtypes = [vp, C.c_int, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _ip,
+                                      C.POINTER(C.c_int)]
+    L.dmdgpu_remove_instance.argtypes = [vp, C.c_int]
+    L.dmdgpu_instance_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _lp, _lp]
+    L.dmdgpu_set_budget.argtypes = [vp, C.c_int64]
+    L.dmdgpu_bytes_per_site.restype = C.c_int64
+    L.dmdgpu_evaluate.argtypes = [vp]
+    L.dmdgpu_instance_residual.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp]
+    L.dmdgpu_instance_state.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    L.dmdgpu_instance_set_unknowns.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    L.dmdgpu_relax_start.argtypes = [vp, C.c_int, C.POINTER(RelaxOpts)]
+    L.dmdgpu_relax_tick.argtypes = [vp, C.POINTER(C.c_int)]
+    L.dmdgpu_relax_status.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
+    L.dmdgpu_relax.argtypes = [vp, C.POINTER(RelaxOpts), C.c_int]
+    L.dmdgpu_set_graph.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -215,3 +252,91 @@ class Context:
         q = C.c_int64()
         self._chk(self.L.dmdgpu_work_counts(self.h, C.byref(p), C.byref(q)))
         return p.value, q.value
+
+    # -- resident instances ----------------------------------------------------
+    def add_instance(self, sysd):
+        """Copies a system in, builds its neighbour lists; returns its id."""
+        n = int(sysd["n"])
+        k = {
+            "q": np.ascontiguousarray(sysd["q"], dtype=np.float64),
+            "sigma": np.ascontiguousarray(sysd["sigma"], dtype=np.float64),
+            "x": np.ascontiguousarray(sysd["x"], dtype=np.float64),
+            "spc": np.ascontiguousarray(sysd["species"], dtype=np.int32),
+            "beta": np.ascontiguousarray(sysd["beta"], dtype=np.float64),
+            "gamma": np.ascontiguousarray(sysd["gamma"], dtype=np.float64),
+            "box": np.ascontiguousarray(np.concatenate([sysd["lo"], sysd["hi"]]), dtype=np.float64),
+            "per": np.ascontiguousarray(sysd["periodic"], dtype=np.int32),
+        }
+        iid = C.c_int(-1)
+        self._chk(self.L.dmdgpu_add_instance(self.h, n, _d(k["q"]), _d(k["sigma"]), _d(k["x"]),
+                                             _i(k["spc"]), _d(k["beta"]), _d(k["gamma"]),
+                                             _d(k["box"]), _i(k["per"]), C.byref(iid)))
+        return iid.value
+
+    def remove_instance(self, iid):
+        self._chk(self.L.dmdgpu_remove_instance(self.h, iid))
+
+    def instance_info(self, iid):
+        n = C.c_int()
+        p = C.c_int64()
+        e = C.c_int64()
+        self._chk(self.L.dmdgpu_instance_info(self.h, iid, C.byref(n), C.byref(p), C.byref(e)))
+        return {"n": n.value, "pairs": p.value, "arena_sites": e.value}
+
+    def set_budget(self, nbytes):
+        self._chk(self.L.dmdgpu_set_budget(self.h, int(nbytes)))
+
+    def bytes_per_site(self):
+        return int(self.L.dmdgpu_bytes_per_site())
+
+    def evaluate(self):
+        self._chk(self.L.dmdgpu_evaluate(self.h))
+
+    def instance_residual(self, iid):
+        n = self.instance_info(iid)["n"]
+        Rq = np.zeros((n, 3))
+        Rs = np.zeros(n)
+        Rx = np.zeros(n)
+        tot = np.zeros(2)
+        self._chk(self.L.dmdgpu_instance_residual(self.h, iid, _d(Rq), _d(Rs), _d(Rx), _d(tot)))
+        return {"Rq": Rq, "Rs": Rs, "Rx": Rx, "S0_total": tot[0], "V0_total": tot[1]}
+
+    def instance_state(self, iid):
+        n = self.instance_info(iid)["n"]
+        q = np.zeros((n, 3))
+        sg = np.zeros(n)
+        x = np.zeros(n)
+        self._chk(self.L.dmdgpu_instance_state(self.h, iid, _d(q), _d(sg), _d(x)))
+        return q, sg, x
+
+    def instance_set_unknowns(self, iid, q=None, sigma=None, x=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (q, sigma, x)]
+        self._chk(self.L.dmdgpu_instance_set_unknowns(self.h, iid, *[_dn(a) for a in arrs]))
+
+    # -- relaxation ------------------------------------------------------------
+    def relax_start(self, iid, opts=None):
+        o = opts if opts is not None else relax_opts()
+        self._chk(self.L.dmdgpu_relax_start(self.h, iid, C.byref(o)))
+
+    def relax_tick(self):
+        r = C.c_int()
+        self._chk(self.L.dmdgpu_relax_tick(self.h, C.byref(r)))
+        return r.value
+
+    def relax_status(self, iid):
+        st = C.c_int()
+        rn = C.c_double()
+        nw = C.c_int()
+        ev = C.c_int()
+        self._chk(self.L.dmdgpu_relax_status(self.h, iid, C.byref(st), C.byref(rn), C.byref(nw),
+                                             C.byref(ev)))
+        return {"state": RELAX_STATE[st.value], "rnorm": rn.value, "newton": nw.value,
+                "evals": ev.value}
+
+    def relax(self, opts=None, max_ticks=100000):
+        o = opts if opts is not None else relax_opts()
+        self._chk(self.L.dmdgpu_relax(self.h, C.byref(o), max_ticks))
+
+    def set_graph(self, on):
+        self._chk(self.L.dmdgpu_set_graph(self.h, 1 if on else 0))

@@ -105,3 +105,46 @@ def read_dump(path):
             "x": rows[:, 5].copy(), "species": spc, "beta": rows[:, 7].copy(),
             "gamma": rows[:, 6].copy(), "lo": np.array(lo), "hi": np.array(hi),
             "periodic": np.array(periodic, dtype=np.int32)}
+
+
+def _hcp_region(keep, nx, ny, nz, a, c_over_a, x_h, seed, pos_noise, sigma_jitter, T, periodic):
+    """hcp Mg (+ tetrahedral H) of an nx x ny x nz block, keeping the sites for
+    which keep(pos) is true; the box spans the block."""
+    full = hcp_mgh(nx, ny, nz, a=a, c_over_a=c_over_a, with_h=True, x_h=x_h, seed=seed,
+                   pos_noise=0.0, sigma_jitter=0.0, T=T, periodic=periodic)
+    rng = np.random.default_rng(seed + 7919)
+    m = keep(full["q"])
+    q = full["q"][m] + rng.normal(0.0, pos_noise, (int(m.sum()), 3))
+    spc = full["species"][m]
+    x = full["x"][m]
+    for ax in range(3):
+        if periodic[ax]:
+            L = full["hi"][ax] - full["lo"][ax]
+            q[:, ax] = full["lo"][ax] + np.mod(q[:, ax] - full["lo"][ax], L)
+    return _finish(q, spc, x, full["lo"], full["hi"], periodic, T, rng, sigma_jitter)
+
+
+def nanowire(radius, ncells_c=10, a=3.209, c_over_a=1.624, x_h=(0.0, 0.3), seed=0,
+             pos_noise=0.01, sigma_jitter=0.05, T=300.0):
+    """Circular hcp Mg-H nanowire along c of the given radius (A), ncells_c
+    cells long and periodic along its axis only (free surfaces elsewhere)."""
+    c = a * c_over_a
+    nx = int(np.ceil(2 * radius / a)) + 2
+    ny = int(np.ceil(2 * radius / (np.sqrt(3) * a))) + 2
+    cx, cy = nx * a / 2, ny * np.sqrt(3) * a / 2
+    keep = lambda q: (q[:, 0] - cx) ** 2 + (q[:, 1] - cy) ** 2 <= radius ** 2
+    return _hcp_region(keep, nx, ny, ncells_c, a, c_over_a, x_h, seed, pos_noise, sigma_jitter, T,
+                       (0, 0, 1))
+
+
+def sphere(radius, a=3.209, c_over_a=1.624, x_h=(0.0, 0.3), seed=0, pos_noise=0.01,
+           sigma_jitter=0.05, T=300.0):
+    """Free hcp Mg-H sphere of the given radius (A)."""
+    c = a * c_over_a
+    nx = int(np.ceil(2 * radius / a)) + 2
+    ny = int(np.ceil(2 * radius / (np.sqrt(3) * a))) + 2
+    nz = int(np.ceil(2 * radius / c)) + 2
+    ctr = np.array([nx * a, ny * np.sqrt(3) * a, nz * c]) / 2
+    keep = lambda q: ((q - ctr) ** 2).sum(1) <= radius ** 2
+    return _hcp_region(keep, nx, ny, nz, a, c_over_a, x_h, seed, pos_noise, sigma_jitter, T,
+                       (0, 0, 0))

@@ -44,7 +44,8 @@ typedef enum {
   DMDGPU_ERR_NONEIGH = 6,      /* a site has no neighbours (Neighbors.cpp:110-116) */
   DMDGPU_ERR_BOX = 7,          /* periodic box edge < 2 r_c (Atom.cpp:230-255) */
   DMDGPU_ERR_STATE = 8,        /* call order: tables / state / neighbours unset */
-  DMDGPU_ERR_TABLES = 9        /* r-tables do not share one (n, dx) grid        */
+  DMDGPU_ERR_TABLES = 9,       /* r-tables do not share one (n, dx) grid        */
+  DMDGPU_ERR_BUDGET = 10       /* instance does not fit the device-memory budget */
 } dmdgpu_status;
 
 typedef struct dmdgpu_ctx dmdgpu_ctx;
@@ -140,6 +141,88 @@ dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* ctx, int k_phase1, int k_phase2, int
  * reference. */
 enum { DMDGPU_FP64 = 0, DMDGPU_SPLIT32 = 1 };
 dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* ctx, int precision);
+
+/* Resident instances -------------------------------------------------------
+ * No counterpart in the reference (it holds one swarm per process).  A context
+ * keeps any number of independent instances (systems) resident in one device
+ * arena; one launch of the residual kernels evaluates all of them and every
+ * instance's result is the same as when it is evaluated alone.
+ * dmdgpu_set_state is the one-instance form: it replaces all instances. */
+
+/* Copies an instance in (same fields as dmdgpu_set_state), builds its neighbour
+ * lists and returns its id.  DMDGPU_ERR_BUDGET when the arena would exceed the
+ * budget set with dmdgpu_set_budget. */
+dmdgpu_status dmdgpu_add_instance(dmdgpu_ctx* ctx, int n, const double* h_mean_q,
+                                  const double* h_stdv_q, const double* h_xi,
+                                  const int32_t* h_specie, const double* h_beta,
+                                  const double* h_gamma, const double* h_box,
+                                  const int32_t* h_periodic, int* id);
+dmdgpu_status dmdgpu_remove_instance(dmdgpu_ctx* ctx, int id);
+/* Sites, active ordered pairs of an instance, and the arena extent (sites). */
+dmdgpu_status dmdgpu_instance_info(dmdgpu_ctx* ctx, int id, int* n, int64_t* pairs,
+                                   int64_t* arena_sites);
+/* Device-memory budget of the arena in bytes (0 = unlimited), and the bytes one
+ * arena site needs (state, lists, records, outputs, solver vectors). */
+dmdgpu_status dmdgpu_set_budget(dmdgpu_ctx* ctx, int64_t bytes);
+int64_t dmdgpu_bytes_per_site(void);
+/* One residual evaluation of every instance (default precision). */
+dmdgpu_status dmdgpu_evaluate(dmdgpu_ctx* ctx);
+/* Results of the last evaluation of an instance, in its caller's site order. */
+dmdgpu_status dmdgpu_instance_residual(dmdgpu_ctx* ctx, int id, double* h_Rq, double* h_Rsigma,
+                                       double* h_Rx, double* h_totals);
+/* Current iterate (mean positions, widths, occupancies) of an instance. */
+dmdgpu_status dmdgpu_instance_state(dmdgpu_ctx* ctx, int id, double* h_mean_q, double* h_stdv_q,
+                                    double* h_xi);
+/* Replaces the unknowns of an instance (NULL leaves a field unchanged). */
+dmdgpu_status dmdgpu_instance_set_unknowns(dmdgpu_ctx* ctx, int id, const double* h_mean_q,
+                                           const double* h_stdv_q, const double* h_xi);
+
+/* Relaxation -------------------------------------------------------------------
+ * Stands in for the reference's SNES solve (SNESSolve,
+ * Mechanical-Relaxation-bulk.cpp:346, NGMRES + line search per
+ * main-BT-AM.cpp:64-67): an inexact Newton-Krylov iteration on the mean
+ * positions and widths (selected by dof_mask) of the active sites of each
+ * instance, occupancies fixed, until the 2-norm of the selected residual
+ * components is <= atol.  Hessian actions are
+ * finite differences of the residual; with `mixed`, those probes use the split
+ * FP32 evaluation while every accepted residual and the stopping test stay
+ * FP64.  Instances advance independently: one tick gives every running
+ * instance exactly one residual evaluation (all instances in one launch). */
+typedef struct {
+  double atol;     /* stop when |(R_q, R_sigma)|_2 <= atol                */
+  int max_newton;  /* Newton steps                                        */
+  int max_cg;      /* CG iterations per Newton step                       */
+  double eta;      /* CG stops at |r| <= eta |R|                          */
+  int mixed;       /* 1: split-precision Hessian probes                   */
+  int max_evals;   /* residual evaluations (0 = no limit)                 */
+  int dof_mask;    /* relaxed unknowns, DMDGPU_DOF_* bits; 0 = Mg q + sigma */
+} dmdgpu_relax_opts;
+/* Relaxed unknowns.  Interstitial H at fixed occupancy has a soft (and at low
+ * occupancy width-unbounded: sigma_eq^2 ~ 1/(beta k x)) landscape, so the
+ * default relaxes the Mg lattice and keeps H sites fixed. */
+enum {
+  DMDGPU_DOF_MG_Q = 1,
+  DMDGPU_DOF_MG_SIGMA = 2,
+  DMDGPU_DOF_H_Q = 4,
+  DMDGPU_DOF_H_SIGMA = 8
+};
+enum {
+  DMDGPU_RELAX_IDLE = 0,
+  DMDGPU_RELAX_RUNNING = 1,
+  DMDGPU_RELAX_CONVERGED = 2,
+  DMDGPU_RELAX_MAXITER = 3,
+  DMDGPU_RELAX_STALLED = 4  /* line search found no decrease */
+};
+dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* ctx, int id, const dmdgpu_relax_opts* opts);
+/* One tick of every running instance; *running = instances still running. */
+dmdgpu_status dmdgpu_relax_tick(dmdgpu_ctx* ctx, int* running);
+dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* ctx, int id, int* state, double* rnorm,
+                                  int* newton_steps, int* residual_evals);
+/* Starts every instance and ticks until all are done (or max_ticks). */
+dmdgpu_status dmdgpu_relax(dmdgpu_ctx* ctx, const dmdgpu_relax_opts* opts, int max_ticks);
+/* 1 (default): a tick's device work is captured once as a hipGraph and
+ * replayed; recaptured when the arena or the tuning changes. */
+dmdgpu_status dmdgpu_set_graph(dmdgpu_ctx* ctx, int on);
 
 /* Instrumentation: when on, every kernel launch is bracketed by HIP events on
  * the context's stream.  dmdgpu_kernel_times returns, per kernel id (see

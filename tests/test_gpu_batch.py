@@ -1,0 +1,147 @@
+"""GPU tests of resident instances (batching) and the Newton-Krylov relaxation.
+
+* Several instances evaluated in one launch give, per instance, bitwise the
+  same residual as that instance evaluated alone, and match the oracle.
+* Relaxation drives |(R_q, R_sigma)|_2 below atol; the oracle confirms the
+  final state's residual.  The mixed mode (split-precision Hessian probes,
+  FP64 residuals) reaches the same tolerance.
+* A relaxation inside a batch follows bitwise the same trajectory as alone.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import RC
+
+import dmdgpu
+import dmdgpu.lattice as lat
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def systems():
+    return [lat.cluster(3, 2, 2, x_h=(0.0, 0.3), seed=41),
+            lat.hcp_mgh(5, 3, 3, x_h=(0.0, 0.2), seed=42),
+            lat.hcp_mgh(4, 3, 3, with_h=False, seed=43),
+            lat.cluster(2, 2, 1, x_h=(0.0, 0.5), seed=44)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dmdgpu.Context(0)
+    yield c
+    c.close()
+
+
+def oracle_residual(oracle, sysd):
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    assert st == 0
+    return oracle.fact_residual(oracle.system(sysd, nnb, nbr))
+
+
+def test_batch_equals_alone_and_oracle(ctx, oracle):
+    say("[gpu] batch vs alone")
+    syss = systems()
+    alone = []
+    for sd in syss:
+        ctx.set_state(sd)
+        ctx.neighbors()
+        alone.append(ctx.residual())
+    ctx.set_state(syss[0])  # clears the arena
+    ctx.remove_instance(0)
+    ids = [ctx.add_instance(sd) for sd in syss]
+    ctx.evaluate()
+    for k, (iid, sd) in enumerate(zip(ids, syss)):
+        out = ctx.instance_residual(iid)
+        for key in ("Rq", "Rs", "Rx"):
+            assert np.array_equal(out[key], alone[k][key]), (k, key)
+        assert out["S0_total"] == alone[k]["S0_total"]
+        ref = oracle_residual(oracle, sd)
+        assert rel(out["Rq"], ref["Rq"]) < 1e-10
+        assert rel(out["Rs"], ref["Rs"]) < 1e-10
+
+
+def test_slot_reuse_and_budget(ctx):
+    say("[gpu] slot reuse + budget")
+    syss = systems()
+    ctx.set_state(syss[0])
+    ctx.remove_instance(0)
+    a = ctx.add_instance(syss[1])
+    b = ctx.add_instance(syss[2])
+    ctx.evaluate()
+    ref_b = ctx.instance_residual(b)
+    ctx.remove_instance(a)
+    c = ctx.add_instance(syss[3])  # fits into a's freed slots
+    ctx.evaluate()
+    assert np.array_equal(ctx.instance_residual(b)["Rq"], ref_b["Rq"])
+    info = ctx.instance_info(c)
+    assert info["n"] == syss[3]["n"]
+    ctx.set_budget(ctx.bytes_per_site() * info["arena_sites"])
+    with pytest.raises(dmdgpu.DmdError) as e:
+        ctx.add_instance(syss[1])  # would grow the arena past the budget
+    assert e.value.code == 10
+    ctx.set_budget(0)
+    ctx.remove_instance(b)
+    ctx.remove_instance(c)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_relaxation_converges(ctx, oracle, mixed):
+    say(f"[gpu] relax mixed={mixed}")
+    sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=7)
+    ctx.set_state(sd)
+    ctx.neighbors()
+    r0 = ctx.residual()
+    n0 = np.sqrt((r0["Rq"] ** 2).sum() + (r0["Rs"] ** 2).sum())
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=1e-3, mixed=mixed)
+    ctx.relax_start(0, opts)
+    ticks = 0
+    while ctx.relax_tick() and ticks < 20000:
+        ticks += 1
+    st = ctx.relax_status(0)
+    say(f"    {st} ticks={ticks} |R0|={n0:.3e}")
+    assert st["state"] == "converged", st
+    q, sg, x = ctx.instance_state(0)
+    fin = dict(sd)
+    fin["q"], fin["sigma"] = q, sg
+    ref = oracle_residual(oracle, fin)
+    # default unknowns: Mg positions and widths (H sites stay at their sites)
+    mg = (sd["species"] == 12) & (sd["x"] >= 5e-4)
+    rn = np.sqrt((ref["Rq"][mg] ** 2).sum() + (ref["Rs"][mg] ** 2).sum())
+    assert rn < 1e-7, rn
+    assert np.abs(q - sd["q"]).max() > 1e-4  # it moved
+
+
+def test_relaxation_batch_equals_alone(ctx):
+    say("[gpu] relax batch vs alone")
+    syss = [lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=s) for s in (11, 12, 13)]
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=1e-3, mixed=True)
+    alone = []
+    for sd in syss:
+        ctx.set_state(sd)
+        ctx.neighbors()
+        ctx.relax(opts)
+        alone.append((ctx.instance_state(0), ctx.relax_status(0)))
+    ctx.remove_instance(0)
+    ids = [ctx.add_instance(sd) for sd in syss]
+    ctx.relax(opts)
+    for k, iid in enumerate(ids):
+        q, sg, _ = ctx.instance_state(iid)
+        st = ctx.relax_status(iid)
+        assert st == alone[k][1], (st, alone[k][1])
+        assert np.array_equal(q, alone[k][0][0])
+        assert np.array_equal(sg, alone[k][0][1])
+        assert st["state"] == "converged"
+    for iid in ids:
+        ctx.remove_instance(iid)
