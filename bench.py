@@ -103,17 +103,15 @@ def split_leg(ctx, args, pqe):
             "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
 
 
-def solve_instances():
-    """The batched relaxation workload: Mg-H nanowires along c (periodic along
-    the axis, R = 15..30 A, 10 cells long) and free spheres (R = 12..25 A),
-    H occupancy x ~ U[0, 0.3], seeded."""
+def solve_instances(count):
+    """The batched relaxation workload: count/2 Mg-H nanowires along c
+    (periodic along the axis, R ~ U[15, 30] A, 10 cells long) and count/2 free
+    spheres (R ~ U[12, 25] A), H occupancy x ~ U[0, 0.3], seeded."""
     import dmdgpu.lattice as lat
-    wires = [lat.nanowire(R, seed=100 + R) for R in (15, 20, 25, 30)]
-    balls = [lat.sphere(R, seed=200 + R) for R in (12, 16, 20, 25)]
-    return wires + balls
+    return lat.mixed_batch(count, seed=2024)
 
 
-def solves_leg(tuning):
+def solves_leg(tuning, count, graph=True):
     """Solves per hour: the plain path (one instance at a time, FP64 Hessian
     probes) against the batched path (all instances resident, one launch per
     tick, split-precision probes, FP64 residuals and stopping test).  Both
@@ -121,14 +119,18 @@ def solves_leg(tuning):
     tolerance that minimises its own residual evaluations (CG eta 0.05 for
     FP64 probes, 0.1 for split-precision probes)."""
     import dmdgpu
-    insts = solve_instances()
+    insts = solve_instances(count)
     k1, k2, un = tuning
-    out = {"instances": len(insts), "sites": [int(s["n"]) for s in insts], "atol": 1e-8}
+    sizes = [int(s["n"]) for s in insts]
+    out = {"instances": len(insts), "sites_total": sum(sizes), "sites_min": min(sizes),
+           "sites_max": max(sizes), "atol": 1e-8}
     ctx = dmdgpu.Context(0)
     ctx.set_tuning(k1, k2, un)
+    ctx.set_graph(graph)
     plain_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, mixed=False)
     t_plain = 0.0
     plain = []
+    per_inst = []
     for sd in insts:
         ctx.set_state(sd)
         ctx.neighbors()
@@ -136,16 +138,23 @@ def solves_leg(tuning):
         t0 = time.perf_counter()
         ctx.relax(plain_opts)
         ctx.sync()
-        t_plain += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        t_plain += dt
         st = ctx.relax_status(0)
         q, sg, _ = ctx.instance_state(0)
         plain.append((st, q, sg))
+        per_inst.append({"sites": int(sd["n"]), "evals": st["evals"],
+                         "ms_per_eval": dt * 1e3 / max(st["evals"], 1)})
     ctx.remove_instance(0)
     ids = [ctx.add_instance(sd) for sd in insts]
     mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.1, mixed=True)
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.relax(mixed_opts)
+    for i in ids:
+        ctx.relax_start(i, mixed_opts)
+    ticks = 0
+    while ctx.relax_tick():
+        ticks += 1
     ctx.sync()
     t_batch = time.perf_counter() - t0
     batch = [ctx.relax_status(i) for i in ids]
@@ -159,10 +168,12 @@ def solves_leg(tuning):
         "solves_per_hour": n * 3600.0 / t_plain, "seconds": t_plain,
         "converged": sum(p[0]["state"] == "converged" for p in plain),
         "residual_evals": sum(p[0]["evals"] for p in plain)}
+    log(f"plain per instance: {per_inst}")
     out["batched_split32_probes"] = {
         "solves_per_hour": n * 3600.0 / t_batch, "seconds": t_batch,
         "converged": sum(s["state"] == "converged" for s in batch),
-        "residual_evals": sum(s["evals"] for s in batch)}
+        "residual_evals": sum(s["evals"] for s in batch), "ticks": ticks,
+        "ms_per_tick": t_batch * 1e3 / max(ticks, 1)}
     out["speedup"] = t_plain / t_batch
     out["max_position_diff_A"] = dq
     return out
@@ -178,6 +189,8 @@ def main():
                     help="auto, or k1,k2,unroll2 (lanes per site in phase 1/2, unroll)")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     ap.add_argument("--no-solves", action="store_true", help="skip the solves/hr leg")
+    ap.add_argument("--instances", type=int, default=64,
+                    help="relaxations in the solves/hr leg (half nanowires, half spheres)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -229,7 +242,7 @@ def main():
 
     solves = None
     if not args.no_solves:
-        solves = solves_leg((k1, k2, un))
+        solves = solves_leg((k1, k2, un), args.instances)
         log(f"solves: {solves}")
 
     split = None

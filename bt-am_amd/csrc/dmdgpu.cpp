@@ -114,9 +114,35 @@ struct Instance {
 
 }  // namespace
 
+// Page-locked host staging for the per-tick descriptor upload and dot
+// download (asynchronous copies without a pageable bounce).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(b, 256), hipHostMallocDefault);
+    if (e == hipSuccess) bytes = std::max<size_t>(b, 256);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 struct dmdgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Second stream for the split-precision chain of a mixed evaluation; forks
+  // from and joins back into `stream` through the two events.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  PinnedBuf pin_up, pin_down;
   double rc = kDefaultRc;
   dmd::PhaseConfig cfg{dmd::kFP64, 8, 8, 0};
   // tables
@@ -667,6 +693,22 @@ Instance* get_inst(dmdgpu_ctx* c, int id) {
   return &c->inst[id];
 }
 
+// FP64 chain on the main stream, split-precision chain on the side stream,
+// joined back into the main stream (also valid under stream capture, where
+// the fork and join become graph edges).
+dmdgpu_status eval_chains(dmdgpu_ctx* c, const dmd::State& s, const dmd::Outputs& o,
+                          bool phase2) {
+  HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  HIPCHK(dmd::launch_phase1(c->side, c->cfg, dmd::kSplit32, s, c->T, o));
+  if (phase2) HIPCHK(dmd::launch_phase2(c->side, c->cfg, dmd::kSplit32, s, c->T, o));
+  HIPCHK(dmd::launch_phase1(c->stream, c->cfg, dmd::kFP64, s, c->T, o));
+  if (phase2) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, dmd::kFP64, s, c->T, o));
+  HIPCHK(hipEventRecord(c->ev_join, c->side));
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  return DMDGPU_OK;
+}
+
 // One residual evaluation of every instance whose h_prec >= 0.
 dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
   CHK(upload_meta(c));
@@ -676,16 +718,24 @@ dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
     if (c->inst[k].used && c->h_prec[k] >= 0) any[c->h_prec[k]] = true;
   const dmd::State s = make_state(c);
   const dmd::Outputs o = make_outputs(c);
+  if (any[0] && any[1] && !c->timing) {
+    // Both precisions present: the two chains touch disjoint instances, so
+    // they run concurrently and fill the device together.
+    CHK(eval_chains(c, s, o, phase2));
+    if (!phase2) return DMDGPU_OK;
+  } else {
+    hipEvent_t a{};
+    CHK(begin_timed(c, &a));
+    for (int p = 0; p < 2; p++)
+      if (any[p]) HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
+    CHK(end_timed(c, KID_PHASE1, a));
+    if (!phase2) return DMDGPU_OK;
+    CHK(begin_timed(c, &a));
+    for (int p = 0; p < 2; p++)
+      if (any[p]) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
+    CHK(end_timed(c, KID_PHASE2, a));
+  }
   hipEvent_t a{};
-  CHK(begin_timed(c, &a));
-  for (int p = 0; p < 2; p++)
-    if (any[p]) HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
-  CHK(end_timed(c, KID_PHASE1, a));
-  if (!phase2) return DMDGPU_OK;
-  CHK(begin_timed(c, &a));
-  for (int p = 0; p < 2; p++)
-    if (any[p]) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
-  CHK(end_timed(c, KID_PHASE2, a));
   CHK(begin_timed(c, &a));
   HIPCHK(dmd::launch_totals(c->stream, s, o, (int)c->inst.size(), c->d_chunk_lo.p,
                             c->d_chunk_hi.p));
@@ -925,8 +975,7 @@ dmdgpu_status tick_device(dmdgpu_ctx* c) {
   const int ni = (int)c->inst.size();
   HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
                                  c->d_ops.p, make_vecs(c), c->d_qs.p, o));
-  for (int p = 0; p < 2; p++) HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
-  for (int p = 0; p < 2; p++) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
+  CHK(eval_chains(c, s, o, true));
   HIPCHK(dmd::launch_totals(c->stream, s, o, ni, c->d_chunk_lo.p, c->d_chunk_hi.p));
   HIPCHK(dmd::launch_solver_dots(c->stream, c->extent, ni, c->d_inst.p, c->d_spc.p, c->d_ne.p,
                                  c->d_ops.p, make_vecs(c), o, c->d_spart.p, c->d_chunk_lo.p,
@@ -938,9 +987,14 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   CHK(upload_meta(c));
   plan_tick(c);
   const int ni = (int)c->inst.size();
-  HIPCHK(hipMemcpyAsync(c->d_ops.p, c->h_ops.data(), sizeof(dmd::SolveOp) * ni,
-                        hipMemcpyHostToDevice, c->stream));
-  CHK(upload_prec(c));
+  const size_t ob = sizeof(dmd::SolveOp) * ni, pb = sizeof(int) * ni;
+  HIPCHK(c->pin_up.ensure(ob + pb));
+  HIPCHK(c->pin_down.ensure(sizeof(double) * 5 * ni));
+  std::memcpy(c->pin_up.p, c->h_ops.data(), ob);
+  std::memcpy((char*)c->pin_up.p + ob, c->h_prec.data(), pb);
+  HIPCHK(hipMemcpyAsync(c->d_ops.p, c->pin_up.p, ob, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_prec.p, (char*)c->pin_up.p + ob, pb, hipMemcpyHostToDevice,
+                        c->stream));
   const bool same_cfg = c->graph_cfg.k1 == c->cfg.k1 && c->graph_cfg.k2 == c->cfg.k2 &&
                         c->graph_cfg.unroll2 == c->cfg.unroll2;
   if (c->use_graph) {
@@ -964,9 +1018,10 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
     CHK(tick_device(c));
   }
   c->h_sdots.resize((size_t)5 * ni);
-  HIPCHK(hipMemcpyAsync(c->h_sdots.data(), c->d_sdots.p, sizeof(double) * 5 * ni,
+  HIPCHK(hipMemcpyAsync(c->pin_down.p, c->d_sdots.p, sizeof(double) * 5 * ni,
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  std::memcpy(c->h_sdots.data(), c->pin_down.p, sizeof(double) * 5 * ni);
   std::vector<int> before(ni);
   for (int k = 0; k < ni; k++) before[k] = c->inst[k].sv.stage;
   digest_tick(c);
@@ -988,8 +1043,8 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
     c->h_ops[k] = op;
   }
   if (flush) {
-    HIPCHK(hipMemcpyAsync(c->d_ops.p, c->h_ops.data(), sizeof(dmd::SolveOp) * ni,
-                          hipMemcpyHostToDevice, c->stream));
+    std::memcpy(c->pin_up.p, c->h_ops.data(), ob);
+    HIPCHK(hipMemcpyAsync(c->d_ops.p, c->pin_up.p, ob, hipMemcpyHostToDevice, c->stream));
     HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
                                    c->d_ops.p, make_vecs(c), c->d_qs.p, make_outputs(c)));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1020,9 +1075,15 @@ dmdgpu_status dmdgpu_create(int device, dmdgpu_ctx** out) {
   dmdgpu_ctx* c = new dmdgpu_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
-    return fail(DMDGPU_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return fail(DMDGPU_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
   }
   *out = c;
   return DMDGPU_OK;
@@ -1055,6 +1116,12 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   c->d_boxes.release();
   c->d_ops.release();
   c->d_pairs.release();
+  (void)hipStreamSynchronize(c->side);
+  c->pin_up.release();
+  c->pin_down.release();
+  (void)hipEventDestroy(c->ev_fork);
+  (void)hipEventDestroy(c->ev_join);
+  (void)hipStreamDestroy(c->side);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return DMDGPU_OK;

@@ -148,3 +148,19 @@ def sphere(radius, a=3.209, c_over_a=1.624, x_h=(0.0, 0.3), seed=0, pos_noise=0.
     keep = lambda q: ((q - ctr) ** 2).sum(1) <= radius ** 2
     return _hcp_region(keep, nx, ny, nz, a, c_over_a, x_h, seed, pos_noise, sigma_jitter, T,
                        (0, 0, 0))
+
+
+def mixed_batch(count=64, seed=2024):
+    """The batched-relaxation workload: count/2 Mg-H nanowires along c
+    (R ~ U[15, 30] A, 10 cells long, periodic along the axis) and count/2 free
+    spheres (R ~ U[12, 25] A), H occupancy x ~ U[0, 0.3], one seed per
+    instance.  Sizes run from about 900 to 19,000 sites."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        s = int(rng.integers(1 << 30))
+        if k % 2 == 0:
+            out.append(nanowire(float(rng.uniform(15.0, 30.0)), seed=s))
+        else:
+            out.append(sphere(float(rng.uniform(12.0, 25.0)), seed=s))
+    return out
