@@ -179,6 +179,51 @@ def solves_leg(tuning, count, graph=True):
     return out
 
 
+def serving_leg(tuning, count, load=0.8, budget_gb=1.0):
+    """Serving loop: relaxation and diffusion-step requests (periodic Mg-H boxes
+    and nanowires, 500..200,000 sites log-uniform, seeded).  Saturation
+    throughput (every request waiting at t = 0) of the plain per-instance FP64
+    server and of the continuous-batching server (split-precision probes and
+    sweeps, HBM budget admission); then Poisson arrivals at `load` x the plain
+    server's capacity, served by each, with p50/p99 latency."""
+    import dmdgpu
+    import dmdgpu.lattice as lat
+    from dmdgpu.serve import Request, latency_stats, poisson_arrivals, serve, serve_plain
+    mix = lat.serving_mix(count, seed=7)
+    budget = int(budget_gb * 2 ** 30)
+    k1, k2, un = tuning
+    ctx = dmdgpu.Context(0)
+    ctx.set_tuning(k1, k2, un)
+    sat = [Request(k, kind, sd, 0.0) for k, (kind, sd) in enumerate(mix)]
+    p_sat, t_p = serve_plain(ctx, sat)
+    b_sat, t_b = serve(ctx, sat, budget_bytes=budget, mixed=True)
+    rate = load * count / t_p
+    arr = poisson_arrivals(count, rate, seed=11)
+    pois = [Request(k, kind, sd, float(a)) for k, ((kind, sd), a) in enumerate(zip(mix, arr))]
+    p_poi, span_p = serve_plain(ctx, pois)
+    b_poi, span_b = serve(ctx, pois, budget_bytes=budget, mixed=True)
+    ctx.close()
+    dq = max(float(np.abs(p.q - b.q).max()) for p, b in zip(p_sat, b_sat))
+    dx = max(float(np.abs(p.x - b.x).max()) for p, b in zip(p_sat, b_sat))
+    sizes = [int(sd["n"]) for _, sd in mix]
+    return {
+        "requests": count, "relax": sum(k == "relax" for k, _ in mix),
+        "diffuse": sum(k == "diffuse" for k, _ in mix), "sites_min": min(sizes),
+        "sites_max": max(sizes), "sites_total": sum(sizes), "budget_bytes": budget,
+        "saturation": {
+            "plain_fp64": {"solves_per_hour": count * 3600.0 / t_p, "seconds": t_p,
+                           "converged": sum(r.state == "converged" for r in p_sat),
+                           "evals": sum(r.evals for r in p_sat)},
+            "batched": {"solves_per_hour": count * 3600.0 / t_b, "seconds": t_b,
+                        "converged": sum(r.state == "converged" for r in b_sat),
+                        "evals": sum(r.evals for r in b_sat)},
+            "speedup": t_p / t_b},
+        "poisson": {"rate_per_s": rate, "load_vs_plain_capacity": load,
+                    "plain_fp64": {"latency_s": latency_stats(p_poi), "makespan_s": span_p},
+                    "batched": {"latency_s": latency_stats(b_poi), "makespan_s": span_b}},
+        "max_position_diff_A": dq, "max_occupancy_diff": dx}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +234,9 @@ def main():
                     help="auto, or k1,k2,unroll2 (lanes per site in phase 1/2, unroll)")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     ap.add_argument("--no-solves", action="store_true", help="skip the solves/hr leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests in the serving leg")
     ap.add_argument("--instances", type=int, default=64,
                     help="relaxations in the solves/hr leg (half nanowires, half spheres)")
     args = ap.parse_args()
@@ -245,6 +293,11 @@ def main():
         solves = solves_leg((k1, k2, un), args.instances)
         log(f"solves: {solves}")
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg((k1, k2, un), args.serve_requests)
+        log(f"serving: {serving}")
+
     split = None
     if not args.no_split:
         split = split_leg(ctx, args, pqe)
@@ -270,6 +323,7 @@ def main():
         "roofline": roof,
         "split32": split,
         "solves": solves,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

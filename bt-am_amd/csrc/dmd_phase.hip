@@ -37,6 +37,12 @@ namespace dmd {
 //   U = sum_j n_j T_j with T_j = sum_ls psi(r_ij + 3 s sigma_j e_l),
 //   rho_bar, pair sum, coincident and self terms on the sqrt(6) sigma points.
 // Writes V0_i, rho_bar_i and the adjoint record of site i.
+//
+// The neighbour row is walked seven times: once per i-shift point (l, s),
+// accumulating only that G_ls (9 values), and once for the other 19 points of
+// each pair.  Re-walking the row (L1/L2-resident neighbour data plus the pair
+// geometry) costs far less than keeping the 54 G_ls accumulators live, which
+// would leave room for a single wavefront per SIMD.
 // ---------------------------------------------------------------------------
 template <typename P> struct PrecCode;
 template <> struct PrecCode<double> { static constexpr int v = kFP64; };
@@ -66,16 +72,54 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   const double4 qi = s.qs[i];
   const P si = (P)qi.w;
   const int spi = s.spc[i];
-  double rho = 0, Ep = 0, Cc = 0, selfA = 0, TPsi = 0;
-  double G0[9], U[9], GA[6][9];
-#pragma unroll
-  for (int c = 0; c < 9; c++) {
-    G0[c] = 0; U[c] = 0;
-#pragma unroll
-    for (int ls = 0; ls < 6; ls++) GA[ls][c] = 0;
-  }
   const int nn = s.nnb[i];
   const int* row = s.nbr + (size_t)i * kCap;
+  double tmp[9];
+
+  // i-shift points, one (l, s) per walk of the row
+  double GG = 0, selfA = 0;
+#pragma unroll 1
+  for (int ls = 0; ls < 6; ls++) {
+    const int l = ls >> 1;
+    const P sh = (ls & 1 ? P(-3) : P(3)) * si;
+    double G[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) G[c] = 0;
+    for (int kk = g; kk < nn; kk += K) {
+      const int j = row[kk];
+      const double nj = s.ne[j];
+      if (nj == 0.0) continue;
+      const double4 qj = s.qs[j];
+      const P* uwr = uwt + ((size_t)pair_slot(spi, s.spc[j]) * nr << 3);
+      double rd[3];
+      min_image(box, qi, qj, rd);
+      PairGeo<P> geo;
+      geo.init(rd);
+      P p[3], off, ir, ps[9];
+      int m;
+      geo.point(T, l, sh, p, m, off, ir);
+      const P* c = uwr + ((size_t)m << 3);
+      psi_from(cval(c, off), cval(c + 4, off), p, ps);
+#pragma unroll
+      for (int e = 0; e < 9; e++) G[e] += nj * (double)ps[e];
+      selfA += nj * nj * (double)dotM(ps, ps);
+    }
+    if (K > 1) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) G[c] = group_sum<K>(G[c]);
+    }
+    GG += dotM(G, G);
+    applyM(1.0 / 9.0, G, tmp);
+#pragma unroll
+    for (int c = 0; c < 9; c++)
+      if (((ls * 9 + c) % K) == g) rec[kRecLA + ls * 9 + c] = tmp[c];
+  }
+
+  // unshifted, j-shift and sqrt(6) sigma points
+  double rho = 0, Ep = 0, Cc = 0, TPsi = 0;
+  double G0[9], U[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) { G0[c] = 0; U[c] = 0; }
   for (int kk = g; kk < nn; kk += K) {
     const int j = row[kk];
     const double nj = s.ne[j];
@@ -96,23 +140,6 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll
     for (int c = 0; c < 9; c++) { G0[c] += nj * (double)psi0[c]; Tj[c] = P(0); }
     const double nj2 = nj * nj;
-    // i-shift points: G_ls accumulators need compile-time indices (unrolled)
-    P selfp = 0;
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-#pragma unroll
-      for (int sg = 0; sg < 2; sg++) {
-        const P sh = sg ? P(-3) : P(3);
-        geo.point(T, l, sh * si, p, m, off, ir);
-        const P* c = uwr + ((size_t)m << 3);
-        psi_from(cval(c, off), cval(c + 4, off), p, ps);
-#pragma unroll
-        for (int e = 0; e < 9; e++) GA[2 * l + sg][e] += nj * (double)ps[e];
-        selfp += dotM(ps, ps);
-      }
-    }
-    selfA += nj2 * (double)selfp;
-    // j-shift points
 #pragma unroll 1
     for (int pt = 0; pt < 6; pt++) {
       const int l = pt >> 1;
@@ -128,7 +155,6 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     P Phi = 0, R = 0, C6 = 0;
     const P* phr = pht + ((size_t)t * nr << 2);
     const P* rhr = rht + ((size_t)spj * nr << 2);
-    // sqrt(6) sigma points
 #pragma unroll 1
     for (int pt = 0; pt < 12; pt++) {
       const int wsel = pt / 6, l = (pt >> 1) % 3;
@@ -151,26 +177,13 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     for (int c = 0; c < 9; c++) {
       G0[c] = group_sum<K>(G0[c]);
       U[c] = group_sum<K>(U[c]);
-#pragma unroll
-      for (int ls = 0; ls < 6; ls++) GA[ls][c] = group_sum<K>(GA[ls][c]);
     }
   }
-  double GG = 0;
-#pragma unroll
-  for (int ls = 0; ls < 6; ls++) GG += dotM(GA[ls], GA[ls]);
   const double At = (GG - selfA + 2.0 * dotM(U, G0) - 2.0 * TPsi) * (1.0 / 18.0) + Cc * (1.0 / 12.0);
   double pe;
   const int me = seg_index(rho, T.dxe[spi], T.inv_dxe[spi], T.ne[spi], pe);
   const double* ce = T.em + ((size_t)spi * T.ne_stride + me) * 4;
   const double F = cval(ce, pe), Fp = cder(ce, pe);
-  double tmp[9];
-#pragma unroll
-  for (int ls = 0; ls < 6; ls++) {
-    applyM(1.0 / 9.0, GA[ls], tmp);
-#pragma unroll
-    for (int c = 0; c < 9; c++)
-      if (((ls * 9 + c) % K) == g) rec[kRecLA + ls * 9 + c] = tmp[c];
-  }
   applyM(1.0 / 9.0, U, tmp);
 #pragma unroll
   for (int c = 0; c < 9; c++)

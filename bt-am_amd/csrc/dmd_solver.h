@@ -16,7 +16,16 @@ enum : int {
   kUpdSdDir = 32      // D = c_sd P
 };
 enum : int { kMakeNone = 0, kMakeX = 1, kMakeXP = 2, kMakeXD = 3 };
-enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3 };
+enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4 };
+
+// Diffusion-step modes (SolveOp.diff): one Picard sweep of the implicit
+// occupancy update, or the bare rates dx/dt written into Xt.
+enum : int { kDiffNone = 0, kDiffStep = 1, kDiffRates = 2 };
+
+// Diffusive neighbours: H sites within 2.6 A of an H site (src/Macros.hpp:860),
+// at most 50 of them (src/Macros.hpp:863).
+constexpr int kDCap = 50;
+constexpr double kRcEb = 2.6;
 
 // Relaxed degrees of freedom (dmdgpu_relax_opts.dof_mask): Mg positions,
 // Mg widths, H positions, H widths.
@@ -29,6 +38,10 @@ struct SolveOp {
   double alpha, beta, c_sd;
   double eps;       // probe step of this tick (kMakeXP, kDotCg)
   double alpha_ls;  // line-search step (kMakeXD)
+  int diff;         // kDiff* mode of a diffusion-step solve
+  double dt;        // time step (s)
+  double d0;        // diffusivity prefactor (A^2/s)
+  double eb;        // migration barrier (eV)
 };
 
 struct SolveVecs {
@@ -38,14 +51,27 @@ struct SolveVecs {
   double4* D;    // Newton step
   double4* Rr;   // CG residual
   double4* Pp;   // CG direction
+  double* Xn;    // occupancy at the start of a diffusion step
+  double* Xt;    // occupancy of the next Picard iterate (or rates)
 };
 
 hipError_t launch_solver_step(hipStream_t st, int n, const int* inst, const int* spc,
                               const double* ne, const SolveOp* ops, const SolveVecs& v,
                               double4* qs, const Outputs& o);
+// Filters each H site's neighbour row down to its diffusive neighbours.
+// flags[0] is set on overflow (> kDCap), flags[1] when an H site has none.
+hipError_t launch_diff_neighbors(hipStream_t st, int base, int n, const Box& box, const int* spc,
+                                 const double4* qs, const int* nnb, const int* nbr, int* dnn,
+                                 int* dnbr, int* flags);
+// Picard sweep of every instance in a diffusion step: Xt = Xn + dt * xdot(x).
+hipError_t launch_diff_flux(hipStream_t st, const State& s, const SolveOp* ops, const SolveVecs& v,
+                            const Outputs& o, const int* dnn, const int* dnbr);
+// Accepts the sweep: x = Xt (raw and effective occupancy) for kDiffStep.
+hipError_t launch_diff_apply(hipStream_t st, int n, const int* inst, const int* spc,
+                             const SolveOp* ops, const SolveVecs& v, double* xr, double* ne);
 hipError_t launch_solver_dots(hipStream_t st, int n, int ninst, const int* inst, const int* spc,
                               const double* ne, const SolveOp* ops, const SolveVecs& v,
-                              const Outputs& o, double* partials, const int* chunk_lo,
-                              const int* chunk_hi, double* out);
+                              const Outputs& o, const double* xr, double* partials,
+                              const int* chunk_lo, const int* chunk_hi, double* out);
 
 }  // namespace dmd

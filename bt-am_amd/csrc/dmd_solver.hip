@@ -82,14 +82,26 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
 //   kDotNorm : a = |Rv|^2
 //   kDotCg   : Ap = (Rv - Rbase)/eps; a = P.Ap, b = Rr.Ap, c = Ap.Ap
 //   kDotLs   : d = max |D_q| (component), e = max(-D_sigma / X_sigma)
+//   kDotDiff : over H sites, a = sum Xt (hydrogen content of the new iterate),
+//              b = sum x (of the current one), d = max |Xt - x|,
+//              e = max(-Xt, Xt - 1) (> 0 when an occupancy leaves [0, 1])
 __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, const int* spc,
                                                     const double* ne, const SolveOp* ops,
-                                                    SolveVecs v, Outputs o, double* partials) {
+                                                    SolveVecs v, Outputs o, const double* xr,
+                                                    double* partials) {
   __shared__ double s[5][256];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k = inst[blockIdx.x * 256];
   double a = 0, b = 0, c = 0, d = 0, e = -1e300;
-  if (i < n && k >= 0 && ne[i] != 0.0) {
+  if (i < n && k >= 0 && ops[k].dot == kDotDiff) {
+    if (spc[i] == 1) {
+      const double xt = v.Xt[i], x = xr[i];
+      a = xt;
+      b = x;
+      d = fabs(xt - x);
+      e = fmax(-xt, xt - 1.0);
+    }
+  } else if (i < n && k >= 0 && ne[i] != 0.0) {
     const SolveOp op = ops[k];
     if (op.dot == kDotNorm) {
       const double4 Rv = ld_rv(o, i, op.dof, spc[i], true);
@@ -152,9 +164,9 @@ hipError_t launch_solver_step(hipStream_t st, int n, const int* inst, const int*
 
 hipError_t launch_solver_dots(hipStream_t st, int n, int ninst, const int* inst, const int* spc,
                               const double* ne, const SolveOp* ops, const SolveVecs& v,
-                              const Outputs& o, double* partials, const int* chunk_lo,
-                              const int* chunk_hi, double* out) {
-  k_solver_dots<<<(n + 255) / 256, 256, 0, st>>>(n, inst, spc, ne, ops, v, o, partials);
+                              const Outputs& o, const double* xr, double* partials,
+                              const int* chunk_lo, const int* chunk_hi, double* out) {
+  k_solver_dots<<<(n + 255) / 256, 256, 0, st>>>(n, inst, spc, ne, ops, v, o, xr, partials);
   k_solver_reduce<<<ninst, 64, 0, st>>>(chunk_lo, chunk_hi, ops, partials, out);
   return hipGetLastError();
 }

@@ -47,8 +47,8 @@ constexpr double kMinOcc = 5e-4;            // src/Macros.hpp:871
 constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
 constexpr int kChunk = dmd::kChunk;
 // Device bytes per arena site (state, lists, records, outputs, solver vectors).
-constexpr size_t kBytesPerSite =
-    32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 + 6 * 32;
+constexpr size_t kBytesPerSite = 32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 +
+                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap;
 
 struct Spline {
   int n = 0;
@@ -86,12 +86,15 @@ struct DevBuf {
   }
 };
 
-enum SolveStage { ST_IDLE = 0, ST_START, ST_BASE32, ST_CG, ST_LSPREP, ST_LS, ST_DONE };
+enum SolveStage { ST_IDLE = 0, ST_START, ST_BASE32, ST_CG, ST_LSPREP, ST_LS, ST_DIFF, ST_DONE };
 
 struct Solve {
   int stage = ST_IDLE;
   int result = 0;  // DMDGPU_RELAX_*
   dmdgpu_relax_opts o{};
+  dmdgpu_diffuse_opts dop{};  // ST_DIFF
+  bool diff64 = false;         // mixed diffusion: sweeps switched to FP64
+  double mass0 = 0, mass = 0;  // hydrogen content at x_n and at the iterate
   int newton = 0, cg = 0, ls = 0, evals = 0;
   double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
   int ndof = 0;
@@ -171,6 +174,8 @@ struct dmdgpu_ctx {
   bool meta_dirty = true;
   // solver
   DevBuf<double4> d_X, d_R0, d_R0f, d_D, d_Rr, d_Pp;
+  DevBuf<double> d_Xn, d_Xt;    // diffusion step: x_n and the next iterate
+  DevBuf<int> d_dnn, d_dnbr;    // diffusive neighbour lists (n x kDCap)
   DevBuf<dmd::SolveOp> d_ops;
   DevBuf<double> d_sdots, d_spart;
   std::vector<dmd::SolveOp> h_ops;
@@ -348,7 +353,8 @@ dmd::Outputs make_outputs(dmdgpu_ctx* c) {
 }
 
 dmd::SolveVecs make_vecs(dmdgpu_ctx* c) {
-  return dmd::SolveVecs{c->d_X.p, c->d_R0.p, c->d_R0f.p, c->d_D.p, c->d_Rr.p, c->d_Pp.p};
+  return dmd::SolveVecs{c->d_X.p, c->d_R0.p, c->d_R0f.p, c->d_D.p,
+                        c->d_Rr.p, c->d_Pp.p, c->d_Xn.p, c->d_Xt.p};
 }
 
 void drop_graph(dmdgpu_ctx* c) {
@@ -393,6 +399,10 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_D.ensure(cap, keep));
   HIPCHK(c->d_Rr.ensure(cap, keep));
   HIPCHK(c->d_Pp.ensure(cap, keep));
+  HIPCHK(c->d_Xn.ensure(cap, keep));
+  HIPCHK(c->d_Xt.ensure(cap, keep));
+  HIPCHK(c->d_dnn.ensure(cap, keep));
+  HIPCHK(c->d_dnbr.ensure(cap * dmd::kDCap, keep * dmd::kDCap));
   HIPCHK(c->d_flags.ensure(2));
   return DMDGPU_OK;
 }
@@ -659,6 +669,9 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   HIPCHK(hipMemcpy(c->d_ne.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_xr.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_nnb.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_dnn.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_Xn.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_Xt.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   {
     std::vector<double4> q0(cp, make_double4(0.0, 0.0, 0.0, 1.0));
     std::vector<double4> z4(cp, make_double4(0.0, 0.0, 0.0, 0.0));
@@ -801,9 +814,26 @@ dmdgpu_status check_primary(dmdgpu_ctx* c) {
 //   LSPREP : step cap so that max |dq| <= 0.2 A and sigma stays >= sigma/2.
 //   LS     : backtracking on |R(X + a d)| (sufficient decrease), FP64.
 // Every stage that needs a residual gets exactly one per tick.
-constexpr double kProbe64 = 1e-7;  // RMS displacement of an FP64 probe (A)
-constexpr double kProbe32 = 3e-4;  // RMS displacement of a split-precision probe (A)
+// RMS displacement of a Hessian probe (A).  The one-sided difference has a
+// truncation error ~ probe |d3 S| / |d2 S| and a rounding error ~ noise /
+// probe: 1e-7 balances them for FP64 residuals, 5e-5 for split-precision
+// ones (RMS noise ~1e-6 per component).
+constexpr double kProbe64 = 1e-7;
+constexpr double kProbe32 = 5e-5;
 constexpr double kMaxStep = 0.2;   // max |dq| of a Newton step (A)
+
+constexpr double kA2PerM2 = 1e20;  // diffusivity m^2/s -> A^2/s
+// Mixed diffusion steps sweep with split-precision residuals until the
+// iterate moves by less than this, then finish with FP64 sweeps (the split
+// residual's occupancy error bounds what FP32 sweeps can resolve).
+constexpr double kDiffSwitch = 1e-7;
+
+void set_diff_op(dmd::SolveOp& op, const dmdgpu_diffuse_opts& d, int mode) {
+  op.diff = mode;
+  op.dt = d.dt;
+  op.d0 = d.D0 * kA2PerM2;
+  op.eb = d.Eb;
+}
 
 void plan_tick(dmdgpu_ctx* c) {
   const int ni = (int)c->inst.size();
@@ -828,7 +858,7 @@ void plan_tick(dmdgpu_ctx* c) {
           c->h_prec[k] = dmd::kSplit32;
           break;
         case ST_CG:
-          S.eps = (S.o.mixed ? kProbe32 : kProbe64) *
+          S.eps = (S.o.probe > 0 ? S.o.probe : (S.o.mixed ? kProbe32 : kProbe64)) *
                   std::sqrt((double)S.ndof / std::max(S.pp, 1e-300));
           op.eps = S.eps;
           op.make = dmd::kMakeXP;
@@ -843,6 +873,11 @@ void plan_tick(dmdgpu_ctx* c) {
           op.alpha_ls = S.alpha_ls;
           op.dot = dmd::kDotNorm;
           c->h_prec[k] = dmd::kFP64;
+          break;
+        case ST_DIFF:
+          set_diff_op(op, S.dop, dmd::kDiffStep);
+          op.dot = dmd::kDotDiff;
+          c->h_prec[k] = S.dop.mixed && !S.diff64 ? dmd::kSplit32 : dmd::kFP64;
           break;
         default:
           break;
@@ -959,6 +994,19 @@ void digest_tick(dmdgpu_ctx* c) {
         }
         break;
       }
+      case ST_DIFF: {
+        // d[0] = H content of the new iterate, d[1] = of the current one,
+        // d[3] = max |dx|, d[4] > 0 when an occupancy left [0, 1]
+        if (S.newton == 0) S.mass0 = d[1];
+        S.mass = d[0];
+        S.rnorm = d[3];
+        S.newton++;
+        if (!(d[4] <= 0.0) || !std::isfinite(d[3])) finish(S, DMDGPU_RELAX_STALLED);
+        else if (d[3] <= S.dop.tol) finish(S, DMDGPU_RELAX_CONVERGED);
+        else if (S.newton >= S.dop.max_iter) finish(S, DMDGPU_RELAX_MAXITER);
+        else if (S.dop.mixed && !S.diff64 && d[3] <= kDiffSwitch) S.diff64 = true;
+        break;
+      }
       default:
         break;
     }
@@ -977,10 +1025,51 @@ dmdgpu_status tick_device(dmdgpu_ctx* c) {
                                  c->d_ops.p, make_vecs(c), c->d_qs.p, o));
   CHK(eval_chains(c, s, o, true));
   HIPCHK(dmd::launch_totals(c->stream, s, o, ni, c->d_chunk_lo.p, c->d_chunk_hi.p));
+  HIPCHK(dmd::launch_diff_flux(c->stream, s, c->d_ops.p, make_vecs(c), o, c->d_dnn.p,
+                               c->d_dnbr.p));
   HIPCHK(dmd::launch_solver_dots(c->stream, c->extent, ni, c->d_inst.p, c->d_spc.p, c->d_ne.p,
-                                 c->d_ops.p, make_vecs(c), o, c->d_spart.p, c->d_chunk_lo.p,
-                                 c->d_chunk_hi.p, c->d_sdots.p));
+                                 c->d_ops.p, make_vecs(c), o, c->d_xr.p, c->d_spart.p,
+                                 c->d_chunk_lo.p, c->d_chunk_hi.p, c->d_sdots.p));
+  HIPCHK(dmd::launch_diff_apply(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ops.p,
+                                make_vecs(c), c->d_xr.p, c->d_ne.p));
   return DMDGPU_OK;
+}
+
+// Re-reads the effective occupancies of an instance after its occupancies
+// changed on the device: active-site flags and active pair count.
+dmdgpu_status refresh_activity(dmdgpu_ctx* c, Instance& I) {
+  std::vector<double> ne(I.n);
+  HIPCHK(hipMemcpyAsync(ne.data(), c->d_ne.p + I.base, sizeof(double) * I.n,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int na = 0;
+  for (int i = 0; i < I.n; i++) {
+    I.h_act[i] = ne[i] != 0.0;
+    na += I.h_act[i];
+  }
+  I.nactive = na;
+  CHK(upload_meta(c));
+  HIPCHK(dmd::launch_active_pairs(c->stream, make_state(c), c->d_pairs.p));
+  const int c0 = I.base / kChunk, nch = I.cap / kChunk;
+  std::vector<long long> pp(nch);
+  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p + c0, sizeof(long long) * nch,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  I.pairs = 0;
+  for (long long v : pp) I.pairs += v;
+  return DMDGPU_OK;
+}
+
+// After a diffusion step: the activity (x >= 5e-4), frozen during the step,
+// follows the new occupancies again.
+dmdgpu_status end_diffusion(dmdgpu_ctx* c, Instance& I) {
+  std::vector<double> x(I.n), ne(I.n);
+  HIPCHK(hipMemcpyAsync(x.data(), c->d_xr.p + I.base, sizeof(double) * I.n,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < I.n; i++) ne[i] = (I.h_slot[i] == 0 || x[i] >= kMinOcc) ? x[i] : 0.0;
+  HIPCHK(hipMemcpy(c->d_ne.p + I.base, ne.data(), sizeof(double) * I.n, hipMemcpyHostToDevice));
+  return refresh_activity(c, I);
 }
 
 dmdgpu_status run_tick(dmdgpu_ctx* c) {
@@ -1032,7 +1121,9 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   for (int k = 0; k < ni; k++) {
     Instance& I = c->inst[k];
     dmd::SolveOp op{};
-    if (I.used && before[k] != ST_DONE && I.sv.stage == ST_DONE) {
+    if (I.used && before[k] == ST_DIFF && I.sv.stage == ST_DONE) {
+      CHK(end_diffusion(c, I));
+    } else if (I.used && before[k] != ST_DONE && I.sv.stage == ST_DONE) {
       op = I.sv.pending;
       op.make = dmd::kMakeX;
       op.mixed = I.sv.o.mixed;
@@ -1104,12 +1195,13 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph,    &c->d_rh,  &c->d_em,       &c->d_ne,
                            &c->d_xr,  &c->d_beta,  &c->d_gamma, &c->d_rec,    &c->d_V0,
                            &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
-                           &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart};
+                           &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart,
+                           &c->d_Xn,  &c->d_Xt};
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
                         &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
                         &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_chunk_lo,
-                        &c->d_chunk_hi};
+                        &c->d_chunk_hi,   &c->d_dnn,        &c->d_dnbr};
   for (auto* b : ibs) b->release();
   DevBuf<double4>* vbs[] = {&c->d_qs, &c->d_X, &c->d_R0, &c->d_R0f, &c->d_D, &c->d_Rr, &c->d_Pp};
   for (auto* b : vbs) b->release();
@@ -1350,9 +1442,9 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
   if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
   Solve S;
-  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0};
+  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0};
   if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
-      S.o.dof_mask < 0 || S.o.dof_mask > 15)
+      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0))
     return fail(DMDGPU_ERR_ARG, "bad relaxation options");
   if (S.o.max_evals <= 0) S.o.max_evals = 100000;
   if (S.o.dof_mask == 0) S.o.dof_mask = DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA;
@@ -1404,6 +1496,93 @@ dmdgpu_status dmdgpu_relax(dmdgpu_ctx* c, const dmdgpu_relax_opts* o, int max_ti
     int run = 0;
     CHK(dmdgpu_relax_tick(c, &run));
     if (run == 0) break;
+  }
+  return DMDGPU_OK;
+}
+
+// ---- diffusion step -------------------------------------------------------------
+
+namespace {
+
+dmdgpu_status check_diffuse_opts(const dmdgpu_diffuse_opts* o) {
+  if (!o) return fail(DMDGPU_ERR_ARG, "null diffusion options");
+  if (!(o->dt > 0) || !(o->D0 >= 0) || !std::isfinite(o->Eb) || !(o->tol > 0) ||
+      o->max_iter <= 0)
+    return fail(DMDGPU_ERR_ARG, "bad diffusion options");
+  return DMDGPU_OK;
+}
+
+dmdgpu_status build_diff_neighbors(dmdgpu_ctx* c, Instance& I) {
+  HIPCHK(hipMemsetAsync(c->d_flags.p, 0, 2 * sizeof(int), c->stream));
+  HIPCHK(dmd::launch_diff_neighbors(c->stream, I.base, I.n, I.box, c->d_spc.p, c->d_qs.p,
+                                    c->d_nnb.p, c->d_nbr.p, c->d_dnn.p, c->d_dnbr.p,
+                                    c->d_flags.p));
+  int flags[2];
+  HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (flags[0]) return fail(DMDGPU_ERR_MAXNEIGH, "an H site has more than 50 diffusive neighbours");
+  if (flags[1]) return fail(DMDGPU_ERR_NONEIGH, "an H site has no diffusive neighbour");
+  return DMDGPU_OK;
+}
+
+}  // namespace
+
+dmdgpu_status dmdgpu_diffuse_start(dmdgpu_ctx* c, int id, const dmdgpu_diffuse_opts* o) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
+  CHK(check_diffuse_opts(o));
+  HIPCHK(hipSetDevice(c->device));
+  CHK(build_diff_neighbors(c, *I));
+  Solve S;
+  S.dop = *o;
+  S.o.max_evals = o->max_iter;  // one residual evaluation per sweep
+  S.stage = ST_DIFF;
+  I->sv = S;
+  HIPCHK(hipMemcpyAsync(c->d_Xn.p + I->base, c->d_xr.p + I->base, sizeof(double) * I->cap,
+                        hipMemcpyDeviceToDevice, c->stream));
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_diffusion_rates(dmdgpu_ctx* c, int id, const dmdgpu_diffuse_opts* o,
+                                     double* xdot) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
+  if (running_count(c) > 0) return fail(DMDGPU_ERR_STATE, "solves are running");
+  CHK(check_diffuse_opts(o));
+  HIPCHK(hipSetDevice(c->device));
+  CHK(build_diff_neighbors(c, *I));
+  CHK(eval_one(c, id));
+  HIPCHK(hipMemcpyAsync(c->d_Xn.p + I->base, c->d_xr.p + I->base, sizeof(double) * I->cap,
+                        hipMemcpyDeviceToDevice, c->stream));
+  const int ni = (int)c->inst.size();
+  std::vector<dmd::SolveOp> ops(ni);
+  set_diff_op(ops[id], *o, dmd::kDiffRates);
+  HIPCHK(hipMemcpy(c->d_ops.p, ops.data(), sizeof(dmd::SolveOp) * ni, hipMemcpyHostToDevice));
+  HIPCHK(dmd::launch_diff_flux(c->stream, make_state(c), c->d_ops.p, make_vecs(c),
+                               make_outputs(c), c->d_dnn.p, c->d_dnbr.p));
+  return d2h_sites(c, *I, xdot, c->d_Xt.p, 1);
+}
+
+dmdgpu_status dmdgpu_diffusion_neighbors(dmdgpu_ctx* c, int id, int* counts, int* lists) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
+  if (!counts || !lists) return fail(DMDGPU_ERR_ARG, "null output");
+  HIPCHK(hipSetDevice(c->device));
+  CHK(build_diff_neighbors(c, *I));
+  const int n = I->n;
+  std::vector<int> cnt(n), lst((size_t)n * dmd::kDCap);
+  HIPCHK(hipMemcpy(cnt.data(), c->d_dnn.p + I->base, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(lst.data(), c->d_dnbr.p + (size_t)I->base * dmd::kDCap,
+                   sizeof(int) * n * dmd::kDCap, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) {
+    const int o = I->perm[i];
+    counts[o] = cnt[i];
+    for (int k = 0; k < dmd::kDCap; k++)
+      lists[(size_t)o * dmd::kDCap + k] =
+          k < cnt[i] ? I->perm[lst[(size_t)i * dmd::kDCap + k] - I->base] : -1;
   }
   return DMDGPU_OK;
 }

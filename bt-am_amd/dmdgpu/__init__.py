@@ -35,7 +35,9 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_set_budget", "dmdgpu_bytes_per_site", "dmdgpu_evaluate",
            "dmdgpu_instance_residual", "dmdgpu_instance_state", "dmdgpu_instance_set_unknowns",
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
-           "dmdgpu_set_graph"]
+           "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
+           "dmdgpu_diffusion_neighbors"]
+DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -46,15 +48,27 @@ class RelaxOpts(C.Structure):
     """dmdgpu_relax_opts."""
     _fields_ = [("atol", C.c_double), ("max_newton", C.c_int), ("max_cg", C.c_int),
                 ("eta", C.c_double), ("mixed", C.c_int), ("max_evals", C.c_int),
-                ("dof_mask", C.c_int)]
+                ("dof_mask", C.c_int), ("probe", C.c_double)]
 
 
 DOF_MG_Q, DOF_MG_SIGMA, DOF_H_Q, DOF_H_SIGMA = 1, 2, 4, 8
 
 
 def relax_opts(atol=1e-8, max_newton=50, max_cg=60, eta=0.05, mixed=False, max_evals=0,
-               dof_mask=DOF_MG_Q | DOF_MG_SIGMA):
-    return RelaxOpts(atol, max_newton, max_cg, eta, 1 if mixed else 0, max_evals, dof_mask)
+               dof_mask=DOF_MG_Q | DOF_MG_SIGMA, probe=0.0):
+    return RelaxOpts(atol, max_newton, max_cg, eta, 1 if mixed else 0, max_evals, dof_mask,
+                     probe)
+
+
+class DiffuseOpts(C.Structure):
+    """dmdgpu_diffuse_opts."""
+    _fields_ = [("dt", C.c_double), ("D0", C.c_double), ("Eb", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("mixed", C.c_int)]
+
+
+def diffuse_opts(dt=1e-9, D0=1e-7, Eb=0.4, tol=1e-10, max_iter=60, mixed=False):
+    """One implicit occupancy step: dt (s), D0 (m^2/s), Eb (eV)."""
+    return DiffuseOpts(dt, D0, Eb, tol, max_iter, 1 if mixed else 0)
 
 
 class DmdError(RuntimeError):
@@ -121,6 +135,9 @@ def load_library():
                                       C.POINTER(C.c_int)]
     L.dmdgpu_relax.argtypes = [vp, C.POINTER(RelaxOpts), C.c_int]
     L.dmdgpu_set_graph.argtypes = [vp, C.c_int]
+    L.dmdgpu_diffuse_start.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts)]
+    L.dmdgpu_diffusion_rates.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts), _dp]
+    L.dmdgpu_diffusion_neighbors.argtypes = [vp, C.c_int, _ip, _ip]
     _lib = L
     return L
 
@@ -340,3 +357,22 @@ class Context:
 
     def set_graph(self, on):
         self._chk(self.L.dmdgpu_set_graph(self.h, 1 if on else 0))
+
+    # -- diffusion step -------------------------------------------------------
+    def diffuse_start(self, iid, opts=None):
+        o = opts if opts is not None else diffuse_opts()
+        self._chk(self.L.dmdgpu_diffuse_start(self.h, iid, C.byref(o)))
+
+    def diffusion_rates(self, iid, opts=None):
+        o = opts if opts is not None else diffuse_opts()
+        n = self.instance_info(iid)["n"]
+        xd = np.zeros(n)
+        self._chk(self.L.dmdgpu_diffusion_rates(self.h, iid, C.byref(o), _d(xd)))
+        return xd
+
+    def diffusion_neighbors(self, iid):
+        n = self.instance_info(iid)["n"]
+        cnt = np.zeros(n, dtype=np.int32)
+        lst = np.zeros((n, DCAP), dtype=np.int32)
+        self._chk(self.L.dmdgpu_diffusion_neighbors(self.h, iid, _i(cnt), _i(lst)))
+        return cnt, lst

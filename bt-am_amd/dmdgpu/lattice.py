@@ -164,3 +164,35 @@ def mixed_batch(count=64, seed=2024):
         else:
             out.append(sphere(float(rng.uniform(12.0, 25.0)), seed=s))
     return out
+
+
+def box_of_size(sites, seed=0, x_h=(0.0, 0.1)):
+    """Periodic hcp Mg-H box of about `sites` sites (12 per orthorhombic cell),
+    near-cubic, at least 4 x 3 x 3 cells (every edge >= 2 r_c)."""
+    a, c = 3.209, 3.209 * 1.624
+    side = (sites / 12.0 * a * np.sqrt(3.0) * a * c) ** (1.0 / 3.0)
+    nx = max(4, int(round(side / a)))
+    ny = max(3, int(round(side / (np.sqrt(3.0) * a))))
+    nz = max(3, int(round(side / c)))
+    return hcp_mgh(nx, ny, nz, x_h=x_h, seed=seed)
+
+
+def wire_of_size(sites, seed=0, x_h=(0.0, 0.3)):
+    """Nanowire (10 cells along c, periodic along c) of about `sites` sites:
+    about 21.1 R^2 sites for radius R in A."""
+    return nanowire(float(np.sqrt(sites / 21.1)), seed=seed, x_h=x_h)
+
+
+def serving_mix(count, seed=7, min_sites=500, max_sites=200_000, diffuse_frac=0.5):
+    """Request mix of the serving loop: (kind, system) pairs, kind "relax" or
+    "diffuse", sizes log-uniform in [min_sites, max_sites], half periodic
+    boxes and half nanowires, one seed per request."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = float(np.exp(rng.uniform(np.log(min_sites), np.log(max_sites))))
+        s = int(rng.integers(1 << 30))
+        sysd = box_of_size(n, seed=s) if rng.random() < 0.5 else wire_of_size(n, seed=s)
+        kind = "diffuse" if rng.random() < diffuse_frac else "relax"
+        out.append((kind, sysd))
+    return out

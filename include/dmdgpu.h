@@ -196,6 +196,8 @@ typedef struct {
   int mixed;       /* 1: split-precision Hessian probes                   */
   int max_evals;   /* residual evaluations (0 = no limit)                 */
   int dof_mask;    /* relaxed unknowns, DMDGPU_DOF_* bits; 0 = Mg q + sigma */
+  double probe;    /* RMS displacement of a Hessian probe (A); 0 = default
+                      (1e-7 with FP64 probes, 5e-5 with split-precision ones) */
 } dmdgpu_relax_opts;
 /* Relaxed unknowns.  Interstitial H at fixed occupancy has a soft (and at low
  * occupancy width-unbounded: sigma_eq^2 ~ 1/(beta k x)) landscape, so the
@@ -223,6 +225,41 @@ dmdgpu_status dmdgpu_relax(dmdgpu_ctx* ctx, const dmdgpu_relax_opts* opts, int m
 /* 1 (default): a tick's device work is captured once as a hipGraph and
  * replayed; recaptured when the arena or the tuning changes. */
 dmdgpu_status dmdgpu_set_graph(dmdgpu_ctx* ctx, int on);
+
+/* Diffusion step -------------------------------------------------------------
+ * One implicit (backward Euler) update of the H occupancies of an instance,
+ * x = x_n + dt xdot(x), positions and widths fixed, solved by Picard sweeps
+ * (one residual evaluation each) until max |x_k+1 - x_k| <= tol.  Diffusive
+ * sites are the H sites; their neighbours are the H sites within 2.6 A
+ * (list_of_diffusive_neighs, src/Atoms/Neighbors.cpp:139-277; r_cutoff_Eb and
+ * max_chemical_neighs = 50, src/Macros.hpp:860,863; parameters declared in
+ * src/Variables.hpp:66-90).  The reference has no rate law; this one is the
+ * mean-field master equation
+ *   xdot_i = sum_j G_ij [x_j (1-x_i) e^{(mu_j-mu_i)/2} - x_i (1-x_j) e^{(mu_i-mu_j)/2}],
+ *   G_ij = D0 / r_ij^2 exp(-Eb (beta_i + beta_j)/2),
+ * mu_i = the occupancy residual R_x(i) without the site's own entropy terms
+ * (0 on masked sites): Fick's law with D = D0 exp(-Eb/k_B T) when mu = 0.
+ * Advanced by dmdgpu_relax_tick together with relaxations; dmdgpu_relax_status
+ * reports the sweeps (newton_steps) and the last max |dx| (rnorm).
+ * Errors: DMDGPU_ERR_MAXNEIGH (> 50 diffusive neighbours),
+ * DMDGPU_ERR_NONEIGH (an H site without one), as the reference. */
+typedef struct {
+  double dt;     /* time step (s)                                  */
+  double D0;     /* diffusivity prefactor (m^2/s)                  */
+  double Eb;     /* migration barrier (eV)                         */
+  double tol;    /* stop when max |x_k+1 - x_k| <= tol             */
+  int max_iter;  /* Picard sweeps                                  */
+  int mixed;     /* 1: split-precision sweeps until max |dx| <= 1e-7,
+                    then FP64 sweeps to tol                        */
+} dmdgpu_diffuse_opts;
+dmdgpu_status dmdgpu_diffuse_start(dmdgpu_ctx* ctx, int id, const dmdgpu_diffuse_opts* opts);
+/* xdot of an instance at its current state (caller's order; 0 on Mg sites),
+ * with the D0 and Eb of opts.  Not while solves are running. */
+dmdgpu_status dmdgpu_diffusion_rates(dmdgpu_ctx* ctx, int id, const dmdgpu_diffuse_opts* opts,
+                                     double* h_xdot);
+/* Diffusive neighbour lists of an instance in the caller's numbering:
+ * h_counts (n), h_lists (n x 50, row order of the neighbour list). */
+dmdgpu_status dmdgpu_diffusion_neighbors(dmdgpu_ctx* ctx, int id, int* h_counts, int* h_lists);
 
 /* Instrumentation: when on, every kernel launch is bracketed by HIP events on
  * the context's stream.  dmdgpu_kernel_times returns, per kernel id (see

@@ -990,3 +990,79 @@ void or_fact_residual(const or_system* s, const or_tables* T, int nthreads, doub
   totals[1] = tV;
   free(p1);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Diffusion step                                                             */
+/* ------------------------------------------------------------------------- */
+
+int or_diff_neighbors(int n, const double* q, const int* spc, const double* lo,
+                      const double* hi, const int* periodic, double rc, int cap, int* cnt,
+                      int* lst) {
+  int over = 0, empty = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : over, empty)
+  for (int i = 0; i < n; i++) {
+    int c = 0;
+    if (spc[i] == 1) {
+      for (int j = 0; j < n; j++) {
+        if (j == i || spc[j] != 1) continue;
+        double d[3];
+        sep(lo, hi, periodic, q + 3 * i, q + 3 * j, d);
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (r <= rc) {
+          if (c == cap) { over = 1; break; }
+          lst[(long)i * cap + c] = j;
+          c++;
+        }
+      }
+      if (c == 0) empty = 1;
+    }
+    cnt[i] = c;
+  }
+  return over ? 1 : (empty ? 2 : 0);
+}
+
+void or_diff_mu(int n, const double* x, const int* spc, const double* gamma, const double* Rx,
+                double* mu) {
+  for (int i = 0; i < n; i++) {
+    if (!(x[i] >= OR_MIN_OCCUPANCY)) {
+      mu[i] = 0.0;
+      continue;
+    }
+    double own = -1.5 / x[i];
+    if (x[i] < 1.0) own -= 1.0 / (1.0 - x[i]);
+    if (spc[i] == 1 && x[i] < 0.9999) own -= gamma[i];
+    mu[i] = Rx[i] - own;
+  }
+}
+
+/* Logarithmic mean of x over l(x) = ln(x / (1 - x)): (b - a) / (l(b) - l(a)),
+ * a (1 - a) in the limit b -> a; 0 unless both sites are active (act) and
+ * 0 < x < 1, so that a masked site exchanges by the Fick term only. */
+static inline double log_mean_mobility(double a, double b, int act) {
+  if (!act || !(a > 0.0 && a < 1.0 && b > 0.0 && b < 1.0)) return 0.0;
+  const double d = log(b / (1.0 - b)) - log(a / (1.0 - a));
+  if (fabs(d) < 1e-8) {
+    const double m = 0.5 * (a + b);
+    return m * (1.0 - m);
+  }
+  return (b - a) / d;
+}
+
+void or_diff_rates(int n, const double* q, const double* x, const double* xa, const double* mu,
+                   const double* beta, const double* lo, const double* hi, const int* periodic,
+                   const int* cnt, const int* lst, int cap, double d0, double eb, double* xdot) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    double f = 0.0;
+    for (int k = 0; k < cnt[i]; k++) {
+      const int j = lst[(long)i * cap + k];
+      double d[3];
+      sep(lo, hi, periodic, q + 3 * i, q + 3 * j, d);
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      const double g = d0 / r2 * exp(-(0.5 * (beta[i] + beta[j])) * eb);
+      const int act = xa[i] >= OR_MIN_OCCUPANCY && xa[j] >= OR_MIN_OCCUPANCY;
+      f += g * ((x[j] - x[i]) + log_mean_mobility(x[i], x[j], act) * (mu[j] - mu[i]));
+    }
+    xdot[i] = f;
+  }
+}

@@ -99,6 +99,31 @@ void or_fact_residual(const or_system* s, const or_tables* t, int nthreads,
                       double* Rq, double* Rs, double* Rx, double* S0, double* V0,
                       double* mf_rho, double* totals);
 
+/* Diffusion step (build-defined rate law; the reference declares only the
+ * topology, src/Atoms/Neighbors.cpp:139-277, and the parameters,
+ * src/Variables.hpp:66-90) ---------------------------------------------------
+ * Diffusive neighbours: H sites (species code 1) within rc of an H site,
+ * brute force over all pairs with the minimum image; at most cap per site.
+ * Returns 0, 1 when a site has more than cap, 2 when an H site has none. */
+int or_diff_neighbors(int n, const double* q, const int* spc, const double* lo,
+                      const double* hi, const int* periodic, double rc, int cap, int* cnt,
+                      int* lst);
+/* Interaction chemical potential (units of k_B T) from the occupancy residual:
+ * mu = Rx - (-3/(2x) - [x<1]/(1-x) - [H, x<0.9999] gamma) on active sites,
+ * 0 on masked ones (x < 5e-4). */
+void or_diff_mu(int n, const double* x, const int* spc, const double* gamma, const double* Rx,
+                double* mu);
+/* Lattice-gas rates in Onsager form with the ideal (Fermi-Dirac) site entropy
+ * l(x) = ln(x/(1-x)) and a logarithmic-mean mobility:
+ *   xdot_i = sum_j G_ij [ (x_j - x_i) + L(x_i, x_j) (mu_j - mu_i) ],
+ *   L(a, b) = (b - a) / (l(b) - l(a)), 0 unless both sites are active
+ *             (xa >= 5e-4: the activity is frozen at the start of a step),
+ *   G_ij = d0 / r_ij^2 exp(-eb (beta_i + beta_j)/2)      (d0 in A^2/s, eb in eV),
+ * i.e. G L (mu_tot_j - mu_tot_i) with mu_tot = mu + l(x). */
+void or_diff_rates(int n, const double* q, const double* x, const double* xa, const double* mu,
+                   const double* beta, const double* lo, const double* hi, const int* periodic,
+                   const int* cnt, const int* lst, int cap, double d0, double eb, double* xdot);
+
 #ifdef __cplusplus
 }
 #endif

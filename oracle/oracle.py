@@ -56,6 +56,11 @@ class Oracle:
         L.or_spline_eval.restype = C.c_double
         L.or_spline_size.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.or_neighbors.argtypes = [C.c_int, _dp, _dp, _dp, _ip, C.c_double, C.c_int, _ip, _ip]
+        L.or_diff_neighbors.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _ip, C.c_double, C.c_int,
+                                        _ip, _ip]
+        L.or_diff_mu.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _dp]
+        L.or_diff_rates.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip,
+                                    C.c_int, C.c_double, C.c_double, _dp]
         sp = C.POINTER(_System)
         for name in ("or_ref_rho0_i", "or_ref_mf_rho_i"):
             getattr(L, name).argtypes = [sp, C.c_void_p, C.c_int]
@@ -178,3 +183,77 @@ class Oracle:
                                   _d(tot))
         out["S0_total"], out["V0_total"] = tot[0], tot[1]
         return out
+
+    # -- diffusion step (build-defined rate law) ---------------------------------
+    DCAP = 50       # max_chemical_neighs, src/Macros.hpp:863
+    RC_EB = 2.6     # r_cutoff_Eb, src/Macros.hpp:860
+    A2_PER_M2 = 1e20
+
+    def diff_neighbors(self, sysd, rc=None):
+        n = sysd["n"]
+        q = np.ascontiguousarray(sysd["q"], dtype=np.float64)
+        spc = np.ascontiguousarray(sysd["species"], dtype=np.int32)
+        lo = np.ascontiguousarray(sysd["lo"], dtype=np.float64)
+        hi = np.ascontiguousarray(sysd["hi"], dtype=np.float64)
+        per = np.ascontiguousarray(sysd["periodic"], dtype=np.int32)
+        cnt = np.zeros(n, dtype=np.int32)
+        lst = np.full((n, self.DCAP), -1, dtype=np.int32)
+        st = self.lib.or_diff_neighbors(n, _d(q), _i(spc), _d(lo), _d(hi), _i(per),
+                                        self.RC_EB if rc is None else rc, self.DCAP, _i(cnt),
+                                        _i(lst))
+        return st, cnt, lst
+
+    def diff_mu(self, sysd, Rx):
+        n = sysd["n"]
+        x = np.ascontiguousarray(sysd["x"], dtype=np.float64)
+        spc = np.ascontiguousarray(sysd["species"], dtype=np.int32)
+        gam = np.ascontiguousarray(sysd["gamma"], dtype=np.float64)
+        Rx = np.ascontiguousarray(Rx, dtype=np.float64)
+        mu = np.zeros(n)
+        self.lib.or_diff_mu(n, _d(x), _i(spc), _d(gam), _d(Rx), _d(mu))
+        return mu
+
+    def diff_rates(self, sysd, mu, cnt, lst, D0=1e-7, Eb=0.4, xa=None):
+        """xdot per site (1/s) for D0 in m^2/s and Eb in eV; xa (default x)
+        decides which sites count as active for the mobility."""
+        n = sysd["n"]
+        arr = {k: np.ascontiguousarray(sysd[k], dtype=np.float64)
+               for k in ("q", "x", "beta", "lo", "hi")}
+        xa = arr["x"] if xa is None else np.ascontiguousarray(xa, dtype=np.float64)
+        per = np.ascontiguousarray(sysd["periodic"], dtype=np.int32)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+        lst = np.ascontiguousarray(lst, dtype=np.int32)
+        xd = np.zeros(n)
+        self.lib.or_diff_rates(n, _d(arr["q"]), _d(arr["x"]), _d(xa), _d(mu), _d(arr["beta"]),
+                               _d(arr["lo"]), _d(arr["hi"]), _i(per), _i(cnt), _i(lst),
+                               lst.shape[1], D0 * self.A2_PER_M2, Eb, _d(xd))
+        return xd
+
+    def diffusion_step(self, sysd, nnb, nbr, dt=1e-9, D0=1e-7, Eb=0.4, tol=1e-10, max_iter=60,
+                       nthreads=0):
+        """Backward-Euler occupancy step x = x_n + dt xdot(x) by Picard sweeps
+        (positions and widths fixed; which sites are active is frozen at x_n),
+        as the GPU solve does.  Returns (x, sweeps, last max |dx|)."""
+        st, cnt, lst = self.diff_neighbors(sysd)
+        if st:
+            raise ValueError(f"diffusive neighbour lists: status {st}")
+        h = sysd["species"] == 1
+        xn = np.asarray(sysd["x"], dtype=np.float64).copy()
+        act = xn >= 5e-4
+        x = xn.copy()
+        for it in range(1, max_iter + 1):
+            s = dict(sysd)
+            s["x"] = np.where(act, x, 0.0)
+            if (s["x"][act] < 5e-4).any():
+                raise ValueError("an active site fell below 5e-4 within the step")
+            res = self.fact_residual(self.system(s, nnb, nbr), nthreads)
+            mu = self.diff_mu(s, res["Rx"])
+            s["x"] = x
+            xd = self.diff_rates(s, mu, cnt, lst, D0, Eb, xa=xn)
+            xt = np.where(h, xn + dt * xd, x)
+            dmax = float(np.abs(xt - x).max())
+            x = xt
+            if dmax <= tol:
+                break
+        return x, it, dmax
