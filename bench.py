@@ -30,6 +30,7 @@ FLOP_PER_PAIR = {"phase1_site_sums": 1814, "phase2_gradient": 4755}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12          # B/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_mgh_20x15x15_fp64.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_mgh_20x15x15.md")
 
 
 def log(*a):
@@ -49,6 +50,19 @@ def traffic_per_launch():
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
+        pass
+    return None
+
+
+def valu_util(kernel):
+    """VALU issue utilisation of a kernel from the committed rocprofv3 SQ
+    counter summary (SQ_INSTS_VALU over the issue capacity of its cycles)."""
+    try:
+        for line in open(VALU_FILE):
+            cells = [c.strip().strip("`") for c in line.split("|")]
+            if len(cells) > 8 and cells[1] == kernel:
+                return float(cells[7])
+    except (OSError, ValueError):
         pass
     return None
 
@@ -184,7 +198,7 @@ def serving_leg(tuning, count, load=0.8, budget_gb=1.0):
     and nanowires, 500..200,000 sites log-uniform, seeded).  Saturation
     throughput (every request waiting at t = 0) of the plain per-instance FP64
     server and of the continuous-batching server (split-precision probes and
-    sweeps, HBM budget admission); then Poisson arrivals at `load` x the plain
+    sweeps, HBM budget admission; also with FP64 probes); then Poisson arrivals at `load` x the plain
     server's capacity, served by each, with p50/p99 latency."""
     import dmdgpu
     import dmdgpu.lattice as lat
@@ -196,6 +210,7 @@ def serving_leg(tuning, count, load=0.8, budget_gb=1.0):
     ctx.set_tuning(k1, k2, un)
     sat = [Request(k, kind, sd, 0.0) for k, (kind, sd) in enumerate(mix)]
     p_sat, t_p = serve_plain(ctx, sat)
+    f_sat, t_f = serve(ctx, sat, budget_bytes=budget, mixed=False)
     b_sat, t_b = serve(ctx, sat, budget_bytes=budget, mixed=True)
     rate = load * count / t_p
     arr = poisson_arrivals(count, rate, seed=11)
@@ -214,10 +229,13 @@ def serving_leg(tuning, count, load=0.8, budget_gb=1.0):
             "plain_fp64": {"solves_per_hour": count * 3600.0 / t_p, "seconds": t_p,
                            "converged": sum(r.state == "converged" for r in p_sat),
                            "evals": sum(r.evals for r in p_sat)},
+            "batched_fp64": {"solves_per_hour": count * 3600.0 / t_f, "seconds": t_f,
+                             "converged": sum(r.state == "converged" for r in f_sat),
+                             "evals": sum(r.evals for r in f_sat)},
             "batched": {"solves_per_hour": count * 3600.0 / t_b, "seconds": t_b,
                         "converged": sum(r.state == "converged" for r in b_sat),
                         "evals": sum(r.evals for r in b_sat)},
-            "speedup": t_p / t_b},
+            "speedup": t_p / t_b, "speedup_fp64_batched": t_p / t_f},
         "poisson": {"rate_per_s": rate, "load_vs_plain_capacity": load,
                     "plain_fp64": {"latency_s": latency_stats(p_poi), "makespan_s": span_p},
                     "batched": {"latency_s": latency_stats(b_poi), "makespan_s": span_b}},
@@ -283,10 +301,15 @@ def main():
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = FLOP_PER_PAIR[dom] * pairs / avg_s
     traffic = traffic_per_launch()
+    kname = {"phase1_site_sums": f"dmd::k_phase1<double, {k1}>",
+             "phase2_gradient": f"dmd::k_phase2<double, {k2}, {'true' if un else 'false'}>"}[dom]
     roof = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
             "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
             "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
-            "flop_per_launch": FLOP_PER_PAIR[dom] * pairs}
+            "flop_per_launch": FLOP_PER_PAIR[dom] * pairs,
+            "hbm_achieved_GBs": traffic / avg_s / 1e9 if traffic else None,
+            "hbm_frac": traffic / avg_s / HBM_PEAK if traffic else None,
+            "valu_issue_util_profiled": valu_util(kname)}
 
     solves = None
     if not args.no_solves:

@@ -63,6 +63,13 @@ struct State {
   const double* gamma;
   const int* nnb;
   const int* nbr;      // n x kCap
+  // Chunks to evaluate, per precision: act[p * act_stride] = count, then the
+  // chunk indices in arena order.  The phase kernels map their blocks onto
+  // these chunks only, so free slots and instances that skip this evaluation
+  // cost no work and the XCD-contiguous block order spreads the real work
+  // over all 8 XCDs.
+  const int* act;
+  int act_stride;
 };
 
 struct Outputs {

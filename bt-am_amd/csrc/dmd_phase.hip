@@ -48,13 +48,23 @@ template <typename P> struct PrecCode;
 template <> struct PrecCode<double> { static constexpr int v = kFP64; };
 template <> struct PrecCode<float> { static constexpr int v = kSplit32; };
 
+// Site and lane of this thread: the block maps onto the precision's list of
+// chunks to evaluate (K blocks per 256-site chunk), in XCD-contiguous order.
+template <typename P, int K>
+DEV bool site_of(const State& s, int& i, int& g) {
+  const int* act = s.act + PrecCode<P>::v * s.act_stride;
+  const int nb = act[0] * K;
+  if ((int)blockIdx.x >= nb) return false;  // uniform per block
+  const int lb = xcd_block(blockIdx.x, nb);
+  i = act[1 + lb / K] * kChunk + (lb % K) * (256 / K) + (int)threadIdx.x / K;
+  g = (int)threadIdx.x % K;
+  return i < s.n;
+}
+
 template <typename P, int K>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int gt = blk * blockDim.x + threadIdx.x;
-  const int i = gt / K;
-  const int g = gt % K;
-  if (i >= s.n) return;
+  int i, g;
+  if (!site_of<P, K>(s, i, g)) return;
   const int inst = s.inst[i];
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
   const Box box = s.boxes[inst];
@@ -219,11 +229,8 @@ DEV void load9(const double* r, bool flip, P* o) {
 // ---------------------------------------------------------------------------
 template <typename P, int K, bool UNR>
 __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int gt = blk * blockDim.x + threadIdx.x;
-  const int u = gt / K;
-  const int g = gt % K;
-  if (u >= s.n) return;
+  int u, g;
+  if (!site_of<P, K>(s, u, g)) return;
   const int inst = s.inst[u];
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
   const Box box = s.boxes[inst];

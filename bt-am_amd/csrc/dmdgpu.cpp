@@ -176,6 +176,8 @@ struct dmdgpu_ctx {
   DevBuf<double4> d_X, d_R0, d_R0f, d_D, d_Rr, d_Pp;
   DevBuf<double> d_Xn, d_Xt;    // diffusion step: x_n and the next iterate
   DevBuf<int> d_dnn, d_dnbr;    // diffusive neighbour lists (n x kDCap)
+  DevBuf<int> d_act;            // chunks to evaluate per precision (State::act)
+  std::vector<int> h_chunks;
   DevBuf<dmd::SolveOp> d_ops;
   DevBuf<double> d_sdots, d_spart;
   std::vector<dmd::SolveOp> h_ops;
@@ -335,6 +337,8 @@ dmd::State make_state(dmdgpu_ctx* c) {
   s.gamma = c->d_gamma.p;
   s.nnb = c->d_nnb.p;
   s.nbr = c->d_nbr.p;
+  s.act = c->d_act.p;
+  s.act_stride = (int)(c->d_act.n / 2);
   return s;
 }
 
@@ -403,6 +407,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_Xt.ensure(cap, keep));
   HIPCHK(c->d_dnn.ensure(cap, keep));
   HIPCHK(c->d_dnbr.ensure(cap * dmd::kDCap, keep * dmd::kDCap));
+  HIPCHK(c->d_act.ensure(2 * (cap / kChunk + 1)));
   HIPCHK(c->d_flags.ensure(2));
   return DMDGPU_OK;
 }
@@ -438,8 +443,26 @@ dmdgpu_status upload_meta(dmdgpu_ctx* c) {
   return DMDGPU_OK;
 }
 
+// Active-chunk lists of both precisions from h_prec (State::act layout).
+void build_act(dmdgpu_ctx* c) {
+  const int stride = (int)(c->d_act.n / 2);
+  c->h_chunks.assign(2 * (size_t)stride, 0);
+  std::vector<std::pair<int, int>> order;  // (base, instance) in arena order
+  for (int k = 0; k < (int)c->inst.size(); k++)
+    if (c->inst[k].used && c->h_prec[k] >= 0) order.push_back({c->inst[k].base, k});
+  std::sort(order.begin(), order.end());
+  for (auto& bk : order) {
+    const Instance& I = c->inst[bk.second];
+    int* a = &c->h_chunks[(size_t)c->h_prec[bk.second] * stride];
+    for (int ch = I.base / kChunk; ch < (I.base + I.cap) / kChunk; ch++) a[1 + a[0]++] = ch;
+  }
+}
+
 dmdgpu_status upload_prec(dmdgpu_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_prec.p, c->h_prec.data(), sizeof(int) * c->inst.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  build_act(c);
+  HIPCHK(hipMemcpyAsync(c->d_act.p, c->h_chunks.data(), sizeof(int) * c->h_chunks.size(),
                         hipMemcpyHostToDevice, c->stream));
   return DMDGPU_OK;
 }
@@ -1076,13 +1099,18 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   CHK(upload_meta(c));
   plan_tick(c);
   const int ni = (int)c->inst.size();
+  build_act(c);
   const size_t ob = sizeof(dmd::SolveOp) * ni, pb = sizeof(int) * ni;
-  HIPCHK(c->pin_up.ensure(ob + pb));
+  const size_t ab = sizeof(int) * c->h_chunks.size();
+  HIPCHK(c->pin_up.ensure(ob + pb + ab));
   HIPCHK(c->pin_down.ensure(sizeof(double) * 5 * ni));
   std::memcpy(c->pin_up.p, c->h_ops.data(), ob);
   std::memcpy((char*)c->pin_up.p + ob, c->h_prec.data(), pb);
+  std::memcpy((char*)c->pin_up.p + ob + pb, c->h_chunks.data(), ab);
   HIPCHK(hipMemcpyAsync(c->d_ops.p, c->pin_up.p, ob, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_prec.p, (char*)c->pin_up.p + ob, pb, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_act.p, (char*)c->pin_up.p + ob + pb, ab, hipMemcpyHostToDevice,
                         c->stream));
   const bool same_cfg = c->graph_cfg.k1 == c->cfg.k1 && c->graph_cfg.k2 == c->cfg.k2 &&
                         c->graph_cfg.unroll2 == c->cfg.unroll2;
