@@ -42,14 +42,15 @@ def workload():
     return lat.hcp_mgh(20, 15, 15, x_h=(0.0, 0.1), seed=2024)
 
 
-def traffic_per_launch():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor)."""
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
-    except OSError:
+            cells = [c.strip().strip("`") for c in line.split("|")]
+            if len(cells) > 5 and cells[1] == kernel:
+                return (float(cells[4]) + float(cells[5])) * 1e6
+    except (OSError, ValueError):
         pass
     return None
 
@@ -84,9 +85,38 @@ def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
         if time.perf_counter() - t0 > budget_s or reps >= 50:
             break
     dt = time.perf_counter() - t0
-    return {"value": pqe_per_res * reps / dt, "unit": "PQE/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} full FP64 residuals of the same 54,000-site workload "
-                      f"({dt:.1f} s, factorised oracle, OpenMP)"}
+    out = {"value": pqe_per_res * reps / dt, "unit": "PQE/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} full FP64 residuals of the same 54,000-site workload "
+                     f"({dt:.1f} s, factorised oracle, OpenMP)"}
+    # The reference's own summation order (O(nb^2) triplets per site), timed on
+    # a sample of sites and extrapolated to a whole residual.
+    rng = np.random.default_rng(1)
+    sites = np.sort(rng.choice(sysd["n"], 128, replace=False)).astype(np.int32)
+    mf = O.fact_residual(s, threads)["mf_rho"]
+    t0 = time.perf_counter()
+    O.ref_residual(s, mf, sites)
+    dts = time.perf_counter() - t0
+    per_res = dts * sysd["n"] / len(sites)
+    out["reference_order"] = {
+        "value": pqe_per_res / per_res, "unit": "PQE/s (same PQE count)", "cores": threads,
+        "seconds_per_residual": per_res,
+        "sample": f"{len(sites)} random sites of the same workload in the reference's O(nb^2) "
+                  f"order ({dts:.1f} s), extrapolated to {sysd['n']} sites"}
+    # one core, factorised, on the 500-site pure-Mg 5x5x5 crystal
+    import dmdgpu.lattice as lat
+    c1 = lat.hcp_mgh(5, 5, 5, with_h=False, seed=1)
+    st, nn1, nb1 = O.neighbors(c1, dmdgpu.R_CUTOFF)
+    s1 = O.system(c1, nn1, nb1)
+    pairs1 = int(nn1.sum())
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < 2.0:
+        O.fact_residual(s1, 1)
+        k += 1
+    d1 = time.perf_counter() - t0
+    out["one_core_500_sites"] = {"value": 50 * pairs1 * k / d1, "unit": "PQE/s", "cores": 1,
+                                 "sample": f"{k} residuals of 500-site hcp Mg 5x5x5 ({d1:.1f} s)"}
+    return out
 
 
 def split_leg(ctx, args, pqe):
@@ -300,9 +330,9 @@ def main():
     launches, ms = kt[dom]
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = FLOP_PER_PAIR[dom] * pairs / avg_s
-    traffic = traffic_per_launch()
     kname = {"phase1_site_sums": f"dmd::k_phase1<double, {k1}>",
              "phase2_gradient": f"dmd::k_phase2<double, {k2}, {'true' if un else 'false'}>"}[dom]
+    traffic = traffic_per_launch(kname)
     roof = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
             "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
             "traffic": traffic, "avg_launch_ms": avg_s * 1e3,

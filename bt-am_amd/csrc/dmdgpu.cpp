@@ -1377,6 +1377,34 @@ dmdgpu_status dmdgpu_residual(dmdgpu_ctx* c, double* h_Rq, double* h_Rs, double*
   return copy_residual(c, c->primary, h_Rq, h_Rs, h_Rx, h_tot);
 }
 
+dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* c, const double* X, const double* V, double* JV) {
+  CHK(check_primary(c));
+  if (!X || !V || !JV) return fail(DMDGPU_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  Instance& I = c->inst[c->primary];
+  const size_t n = (size_t)I.n, m = 4 * n;
+  double vv = 0.0;
+  for (size_t k = 0; k < m; k++) vv += V[k] * V[k];
+  if (!(vv > 0.0)) {
+    std::fill(JV, JV + m, 0.0);
+    return DMDGPU_OK;
+  }
+  // central difference with an RMS displacement of 1e-5 A: truncation
+  // ~1e-10 |d3 S| and rounding ~1e-8 relative
+  const double eps = 1e-5 * std::sqrt((double)m / vv);
+  std::vector<double> Y(m), Rp(m), Rm(m);
+  for (int side = 0; side < 2; side++) {
+    const double s = side == 0 ? eps : -eps;
+    for (size_t k = 0; k < m; k++) Y[k] = X[k] + s * V[k];
+    CHK(upload_unknowns(c, I, Y.data(), Y.data() + 3 * n, nullptr, false));
+    CHK(eval_one(c, c->primary));
+    double* R = side == 0 ? Rp.data() : Rm.data();
+    CHK(copy_residual(c, c->primary, R, R + 3 * n, nullptr, nullptr));
+  }
+  for (size_t k = 0; k < m; k++) JV[k] = (Rp[k] - Rm[k]) / (2.0 * eps);
+  return upload_unknowns(c, I, X, X + 3 * n, nullptr, false);
+}
+
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* c) {
   if (!c) return fail(DMDGPU_ERR_ARG, "null context");
   return eval_all(c);

@@ -36,7 +36,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_instance_residual", "dmdgpu_instance_state", "dmdgpu_instance_set_unknowns",
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
-           "dmdgpu_diffusion_neighbors"]
+           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -138,6 +138,7 @@ def load_library():
     L.dmdgpu_diffuse_start.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts)]
     L.dmdgpu_diffusion_rates.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts), _dp]
     L.dmdgpu_diffusion_neighbors.argtypes = [vp, C.c_int, _ip, _ip]
+    L.dmdgpu_jvp.argtypes = [vp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -357,6 +358,15 @@ class Context:
 
     def set_graph(self, on):
         self._chk(self.L.dmdgpu_set_graph(self.h, 1 if on else 0))
+
+    def jvp(self, q, sigma, vq, vsigma):
+        """J v of the (q, sigma) residual at (q, sigma): returns (Jv_q, Jv_sigma)."""
+        n = self.n
+        X = np.ascontiguousarray(np.concatenate([np.ravel(q), np.ravel(sigma)]), dtype=np.float64)
+        V = np.ascontiguousarray(np.concatenate([np.ravel(vq), np.ravel(vsigma)]), dtype=np.float64)
+        JV = np.zeros(4 * n)
+        self._chk(self.L.dmdgpu_jvp(self.h, _d(X), _d(V), _d(JV)))
+        return JV[:3 * n].reshape(n, 3), JV[3 * n:]
 
     # -- diffusion step -------------------------------------------------------
     def diffuse_start(self, iid, opts=None):

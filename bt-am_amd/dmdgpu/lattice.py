@@ -196,3 +196,30 @@ def serving_mix(count, seed=7, min_sites=500, max_sites=200_000, diffuse_frac=0.
         kind = "diffuse" if rng.random() < diffuse_frac else "relax"
         out.append((kind, sysd))
     return out
+
+
+def write_dump(path, sysd, step=0):
+    """Writes a system in the reference's dump layout (the layout
+    src/IO/dump-input.cpp:51-215 reads; writer src/IO/dump-output.cpp:52-159):
+    TIMESTEP, NUMBER OF ATOMS, BOX BOUNDS with the pp/ff flag of each axis,
+    then ``type x y z stdvq molarfraction chemicalmultp thermalmultp
+    chemicalmultpbcc thermalmultpbcc`` per site.  Two differences from the
+    reference writer, so that the file reads back: the species code is written
+    as is (the writer adds 1, which its reader rejects) and each axis keeps its
+    own boundary flag (the writer always writes pp).  Values are written in their shortest
+    exact decimal form (the file reads back bit for bit)."""
+    n = int(sysd["n"])
+    r = lambda v: repr(float(v))  # shortest exact decimal form
+    flags = " ".join("pp" if p else "ff" for p in sysd["periodic"])
+    lines = ["ITEM: TIMESTEP", str(step), "ITEM: NUMBER OF ATOMS", str(n),
+             f"ITEM: BOX BOUNDS {flags}"]
+    for a in range(3):
+        lines.append(f"{r(sysd['lo'][a])} {r(sysd['hi'][a])}")
+    lines.append("ITEM: ATOMS type x y z stdvq molarfraction chemicalmultp thermalmultp "
+                 "chemicalmultpbcc thermalmultpbcc")
+    q, sg, x = sysd["q"], sysd["sigma"], sysd["x"]
+    for i in range(n):
+        lines.append(f"{int(sysd['species'][i])} {r(q[i, 0])} {r(q[i, 1])} {r(q[i, 2])} "
+                     f"{r(sg[i])} {r(x[i])} {r(sysd['gamma'][i])} {r(sysd['beta'][i])} 0.0 0.0")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

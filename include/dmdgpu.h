@@ -129,6 +129,15 @@ dmdgpu_status dmdgpu_residual(dmdgpu_ctx* ctx, double* h_Rq, double* h_Rsigma, d
  * buffers (no host copies); used by resident solvers and for timing. */
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
 
+/* Jacobian-vector product of the (q, sigma) residual for a MatShell MatMult:
+ * the reference builds its Jacobian by finite-difference colouring
+ * (SNESComputeJacobianDefaultColor + MatFDColoring,
+ * Mechanical-Relaxation-bulk.cpp:282-319); this is the matrix-free form.
+ * h_X = [q (3n) | sigma (n)], h_V likewise, h_JV = [dR_q (3n) | dR_sigma (n)]:
+ * central difference of two residual evaluations at X +- eps V (RMS
+ * displacement 1e-5 A).  Occupancies fixed; the state is left at X. */
+dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* ctx, const double* h_X, const double* h_V, double* h_JV);
+
 /* Kernel tuning: lanes of a wavefront that share one site in the phase-1 and
  * phase-2 kernels (1, 2, 4 or 8) and whether phase 2 unrolls its point loops.
  * The tuner picks them per workload.  No counterpart in the reference. */
