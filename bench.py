@@ -147,6 +147,39 @@ def split_leg(ctx, args, pqe):
             "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
 
 
+def rule5_leg(ctx, args, pqe3):
+    """The same workload under the 27-point tensor Gauss-Hermite pair rule
+    (pair and coincident terms; distinct triplets keep the 3rd-order rule):
+    FP64 and split-precision time per residual and the split error against
+    FP64.  PQE counts use 40 points per ordered pair per phase (13 + 27)."""
+    ctx.set_rule(5)
+    out = {}
+    res = {}
+    try:
+        for prec in ("fp64", "split32"):
+            ctx.set_precision(prec)
+            for _ in range(2):
+                ctx.residual_resident()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                ctx.residual_resident()
+            ctx.sync()
+            dt = (time.perf_counter() - t0) / args.steps
+            res[prec] = ctx.residual()
+            out[prec] = {"ms_per_step": dt * 1e3, "value": pqe3 / 50 * 80 / dt, "unit": "PQE/s"}
+    finally:
+        ctx.set_precision("fp64")
+        ctx.set_rule(3)
+    ref = res["fp64"]
+    err = {k: float(np.abs(res["split32"][k] - ref[k]).max() / np.abs(ref[k]).max())
+           for k in ("Rq", "Rs", "Rx")}
+    err["S0_total"] = float(abs(res["split32"]["S0_total"] - ref["S0_total"]) /
+                            abs(ref["S0_total"]))
+    out["split32"]["max_rel_err_vs_fp64"] = err
+    return out
+
+
 def solve_instances(count):
     """The batched relaxation workload: count/2 Mg-H nanowires along c
     (periodic along the axis, R ~ U[15, 30] A, 10 cells long) and count/2 free
@@ -346,6 +379,9 @@ def main():
         solves = solves_leg((k1, k2, un), args.instances)
         log(f"solves: {solves}")
 
+    rule5 = rule5_leg(ctx, args, pqe)
+    log(f"rule 5: {rule5}")
+
     serving = None
     if not args.no_serve:
         serving = serving_leg((k1, k2, un), args.serve_requests)
@@ -375,6 +411,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "split32": split,
+        "fifth_order_rule": rule5,
         "solves": solves,
         "serving": serving,
         "cpu_baseline": cpu,

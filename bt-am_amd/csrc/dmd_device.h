@@ -67,6 +67,16 @@ template <> struct PairGeo<double> {
     ir = 1.0 / x;
     m = seg_index(x, T.dxr, T.inv_dxr, T.nr, off);
   }
+  // p = r + d for a general displacement d.
+  DEV void point3(const Tables& T, const double* d, double* p, int& m, double& off,
+                  double& ir) const {
+    p[0] = r[0] + d[0];
+    p[1] = r[1] + d[1];
+    p[2] = r[2] + d[2];
+    const double x = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    ir = 1.0 / x;
+    m = seg_index(x, T.dxr, T.inv_dxr, T.nr, off);
+  }
 };
 
 template <> struct PairGeo<float> {
@@ -92,6 +102,71 @@ template <> struct PairGeo<float> {
     off = o > T.dxr32 ? T.dxr32 : o;
     m = mm;
     ir = __builtin_amdgcn_rcpf(pn);
+  }
+  // p = r + d for a general displacement d; |p| = |r| + (2 r.d + d.d)/(|p| + |r|).
+  DEV void point3(const Tables& T, const float* d, float* p, int& m, float& off, float& ir) const {
+    p[0] = r[0] + d[0];
+    p[1] = r[1] + d[1];
+    p[2] = r[2] + d[2];
+    const float pn = __builtin_amdgcn_sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const float rd = r[0] * d[0] + r[1] * d[1] + r[2] * d[2];
+    const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const float D = __fdividef(2.0f * rd + dd, pn + R32);
+    int mm = (int)((R32 + D) * T.inv_dxr32);
+    mm = mm > T.nr - 1 ? T.nr - 1 : mm;
+    mm = mm < 0 ? 0 : mm;
+    float o = (float)(R - (double)mm * T.dxr) + D;
+    off = o > T.dxr32 ? T.dxr32 : o;
+    m = mm;
+    ir = __builtin_amdgcn_rcpf(pn);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Pair (and coincident-triplet) rules.  Point t of pair (i, j) lies at
+// p = r_ij + delta_t with weight w_t (returned in FP64 for both precisions, so
+// the weights sum to 1 to FP64 accuracy); dsi_t = d delta_t / d sigma_i.
+//   PairRule<3>: the reference's 6D Stroud rule -- sqrt(6) sigma shifts of
+//                site i (t < 6) or site j along one axis, weight 1/12.
+//   PairRule<5>: 3-point Gauss-Hermite per axis on the relative coordinate
+//                r ~ N(r_ij, (sigma_i^2 + sigma_j^2) I): nodes 0 and
+//                +-sqrt(3) sigma_rel, weights 2/3 and 1/6 (27 points, exact to
+//                degree 5).
+// ---------------------------------------------------------------------------
+template <int RULE> struct PairRule;
+template <> struct PairRule<3> {
+  static constexpr int NP = 12;
+  template <typename P>
+  DEV static double point(const PairGeo<P>& geo, const Tables& T, int t, P si, P sj, P* p, int& m,
+                          P& off, P& ir, P* dsi) {
+    const int wsel = t / 6, l = (t >> 1) % 3;
+    const P d = (t & 1) ? P(-kSqrt6) : P(kSqrt6);
+    geo.point(T, l, d * (wsel ? sj : si), p, m, off, ir);
+    dsi[0] = (!wsel && l == 0) ? d : P(0);
+    dsi[1] = (!wsel && l == 1) ? d : P(0);
+    dsi[2] = (!wsel && l == 2) ? d : P(0);
+    return 1.0 / 12.0;
+  }
+};
+template <> struct PairRule<5> {
+  static constexpr int NP = 27;
+  template <typename P>
+  DEV static double point(const PairGeo<P>& geo, const Tables& T, int t, P si, P sj, P* p, int& m,
+                          P& off, P& ir, P* dsi) {
+    const P s3 = P(1.7320508075688772);
+    const int a0 = t % 3, a1 = (t / 3) % 3, a2 = t / 9;
+    const P x0 = a0 == 0 ? P(0) : (a0 == 1 ? s3 : -s3);
+    const P x1 = a1 == 0 ? P(0) : (a1 == 1 ? s3 : -s3);
+    const P x2 = a2 == 0 ? P(0) : (a2 == 1 ? s3 : -s3);
+    const P srel = sqrt(si * si + sj * sj);
+    const P d[3] = {srel * x0, srel * x1, srel * x2};
+    geo.point3(T, d, p, m, off, ir);
+    const P f = si / srel;
+    dsi[0] = f * x0; dsi[1] = f * x1; dsi[2] = f * x2;
+    const double w0 = a0 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
+    const double w1 = a1 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
+    const double w2 = a2 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
+    return w0 * w1 * w2;
   }
 };
 

@@ -103,8 +103,10 @@ enum Precision { kFP64 = 0, kSplit32 = 1 };
 
 struct PhaseConfig {
   int precision;   // Precision
-  int k1, k2;      // lanes per site in phase 1 / phase 2 (1, 2, 4, 8)
+  int k1, k2;      // lanes per site in phase 1 / phase 2 (1, 2, 4, 8, 16)
   int unroll2;     // 1: fully unrolled point loops in phase 2, 0: rolled
+  int rule = 3;    // pair / coincident rule: 3 (reference, 12 points) or 5
+                   // (27-point tensor Gauss-Hermite on the relative coordinate)
 };
 
 // Evaluates every instance whose prec[] equals the given precision (both

@@ -61,7 +61,7 @@ DEV bool site_of(const State& s, int& i, int& g) {
   return i < s.n;
 }
 
-template <typename P, int K>
+template <typename P, int K, int RULE>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   int i, g;
   if (!site_of<P, K>(s, i, g)) return;
@@ -162,23 +162,23 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll
     for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
     TPsi += nj2 * (double)dotM(Tj, psi0);
-    P Phi = 0, R = 0, C6 = 0;
+    double Phi = 0, R = 0, C6 = 0;  // per-pair sums over the rule's points in FP64
     const P* phr = pht + ((size_t)t * nr << 2);
     const P* rhr = rht + ((size_t)spj * nr << 2);
+    // pair and coincident points of the pair rule
 #pragma unroll 1
-    for (int pt = 0; pt < 12; pt++) {
-      const int wsel = pt / 6, l = (pt >> 1) % 3;
-      const P sd = P(kSqrt6) * (wsel ? sj : si);
-      geo.point(T, l, pt & 1 ? -sd : sd, p, m, off, ir);
+    for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
+      P dsi[3];
+      const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi);
       const P* c = uwr + ((size_t)m << 3);
       psi_from(cval(c, off), cval(c + 4, off), p, ps);
-      C6 += dotM(ps, ps);
-      Phi += cval(phr + ((size_t)m << 2), off);
-      R += cval(rhr + ((size_t)m << 2), off);
+      C6 += wt * (double)dotM(ps, ps);
+      Phi += wt * (double)cval(phr + ((size_t)m << 2), off);
+      R += wt * (double)cval(rhr + ((size_t)m << 2), off);
     }
-    Cc += nj2 * (double)C6;
-    rho += nj * (double)R * (1.0 / 12.0);
-    Ep += nj * (double)Phi * (1.0 / 12.0);
+    Cc += nj2 * C6;
+    rho += nj * R;
+    Ep += nj * Phi;
   }
   if (K > 1) {
     rho = group_sum<K>(rho); Ep = group_sum<K>(Ep); Cc = group_sum<K>(Cc);
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
       U[c] = group_sum<K>(U[c]);
     }
   }
-  const double At = (GG - selfA + 2.0 * dotM(U, G0) - 2.0 * TPsi) * (1.0 / 18.0) + Cc * (1.0 / 12.0);
+  const double At = (GG - selfA + 2.0 * dotM(U, G0) - 2.0 * TPsi) * (1.0 / 18.0) + Cc;
   double pe;
   const int me = seg_index(rho, T.dxe[spi], T.inv_dxe[spi], T.ne[spi], pe);
   const double* ce = T.em + ((size_t)spi * T.ne_stride + me) * 4;
@@ -227,7 +227,7 @@ DEV void load9(const double* r, bool flip, P* o) {
 // p = r_uk + shift; their adjoints are combined before one gradient per point.
 // psi(-p) = flip(psi(p)) (the dipole part is odd, the quadrupole part even).
 // ---------------------------------------------------------------------------
-template <typename P, int K, bool UNR>
+template <typename P, int K, bool UNR, int RULE>
 __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
   int u, g;
   if (!site_of<P, K>(s, u, g)) return;
@@ -369,48 +369,45 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
       xacc += xb * (double)dot9(lb, psi0);
     }
-    // sqrt(6) sigma points: coincident angular term, pair and density
+    // pair-rule points: coincident angular term, pair and density
     {
       const double FPk = Rk[kRecFP];
-      const P cphi = (P)(0.5 * nu * nk * (bu + bk) * (1.0 / 12.0));
-      const P cra = (P)(bu * nu * FPu * nk * (1.0 / 12.0));
-      const P crb = (P)(bk * nk * FPk * nu * (1.0 / 12.0));
-      const P cang = (P)((wa * nk + wb * nu) * (1.0 / 6.0));
-      const double xang = xb * nu * (1.0 / 6.0);
-      const P xrho = (P)(bk * nk * FPk * (1.0 / 12.0));
-      const P xphi = (P)(bk * 0.5 * nk * (1.0 / 12.0));
+      const P cphi = (P)(0.5 * nu * nk * (bu + bk));
+      const P cra = (P)(bu * nu * FPu * nk);
+      const P crb = (P)(bk * nk * FPk * nu);
+      const P cang = (P)(2.0 * (wa * nk + wb * nu));
+      const double xang = 2.0 * xb * nu;
+      const P xrho = (P)(bk * nk * FPk);
+      const P xphi = (P)(bk * 0.5 * nk);
       const P* phr = pht + ((size_t)t * nr << 2);
       const P* rkr = rht + ((size_t)spk * nr << 2);
       const P* rur = rht + ((size_t)spu * nr << 2);
       auto cpt = [&](int pt) {
-        const int wsel = pt / 6, l = (pt >> 1) % 3, sg = pt & 1;
-        const P sd = P(kSqrt6) * (wsel ? skP : suP);
-        P p[3], ir, o2, ps[9], L[9], g3[3];
+        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3];
         int mm;
-        geo.point(T, l, sg ? -sd : sd, p, mm, o2, ir);
+        const double wt = PairRule<RULE>::point(geo, T, pt, suP, skP, p, mm, o2, ir, dsu);
+        const P wtP = (P)wt;
         const P* c = uwr + ((size_t)mm << 3);
         const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
         const P* cp = phr + ((size_t)mm << 2);
         const P* ck = rkr + ((size_t)mm << 2);
         const P* cu = rur + ((size_t)mm << 2);
         psi_from(uu, ww, p, ps);
-        applyM(cang, ps, L);
+        applyM(wtP * cang, ps, L);
         grad_psi(L, p, ir, uu, du, ww, dw, g3);
-        const P cs = (cphi * cder(cp, o2) + cra * cder(ck, o2) + crb * cder(cu, o2)) * ir;
+        const P cs = wtP * (cphi * cder(cp, o2) + cra * cder(ck, o2) + crb * cder(cu, o2)) * ir;
         g3[0] += cs * p[0]; g3[1] += cs * p[1]; g3[2] += cs * p[2];
         q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-        if (!wsel) {
-          const P gl = l == 0 ? g3[0] : (l == 1 ? g3[1] : g3[2]);
-          sacc += (double)((sg ? P(-kSqrt6) : P(kSqrt6)) * gl);
-        }
-        xacc += xang * (double)dotM(ps, ps) + (double)(xrho * cval(cu, o2) + xphi * cval(cp, o2));
+        sacc += (double)(g3[0] * dsu[0] + g3[1] * dsu[1] + g3[2] * dsu[2]);
+        xacc += wt * (xang * (double)dotM(ps, ps) +
+                      (double)(xrho * cval(cu, o2) + xphi * cval(cp, o2)));
       };
       if (UNR) {
 #pragma unroll
-        for (int pt = 0; pt < 12; pt++) cpt(pt);
+        for (int pt = 0; pt < PairRule<RULE>::NP; pt++) cpt(pt);
       } else {
 #pragma unroll 1
-        for (int pt = 0; pt < 12; pt++) cpt(pt);
+        for (int pt = 0; pt < PairRule<RULE>::NP; pt++) cpt(pt);
       }
     }
   }
@@ -443,60 +440,74 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <typename P, int K>
+template <typename P, int K, int RULE>
 static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
-  k_phase1<P, K><<<nb, 256, 0, st>>>(s, T, o);
+  k_phase1<P, K, RULE><<<nb, 256, 0, st>>>(s, T, o);
 }
-template <typename P, int K, bool UNR>
+template <typename P, int K, bool UNR, int RULE>
 static void p2(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
-  k_phase2<P, K, UNR><<<nb, 256, 0, st>>>(s, T, o);
+  k_phase2<P, K, UNR, RULE><<<nb, 256, 0, st>>>(s, T, o);
 }
 
-template <typename P>
+template <typename P, int RULE>
 static hipError_t phase1_p(hipStream_t st, int K, const State& s, const Tables& T,
                            const Outputs& o) {
   switch (K) {
-    case 1: p1<P, 1>(st, s, T, o); break;
-    case 2: p1<P, 2>(st, s, T, o); break;
-    case 4: p1<P, 4>(st, s, T, o); break;
-    case 8: p1<P, 8>(st, s, T, o); break;
-    case 16: p1<P, 16>(st, s, T, o); break;
+    case 1: p1<P, 1, RULE>(st, s, T, o); break;
+    case 2: p1<P, 2, RULE>(st, s, T, o); break;
+    case 4: p1<P, 4, RULE>(st, s, T, o); break;
+    case 8: p1<P, 8, RULE>(st, s, T, o); break;
+    case 16: p1<P, 16, RULE>(st, s, T, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <typename P, bool UNR>
+template <typename P, bool UNR, int RULE>
 static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Tables& T,
                             const Outputs& o) {
   switch (K) {
-    case 1: p2<P, 1, UNR>(st, s, T, o); break;
-    case 2: p2<P, 2, UNR>(st, s, T, o); break;
-    case 4: p2<P, 4, UNR>(st, s, T, o); break;
-    case 8: p2<P, 8, UNR>(st, s, T, o); break;
-    case 16: p2<P, 16, UNR>(st, s, T, o); break;
+    case 1: p2<P, 1, UNR, RULE>(st, s, T, o); break;
+    case 2: p2<P, 2, UNR, RULE>(st, s, T, o); break;
+    case 4: p2<P, 4, UNR, RULE>(st, s, T, o); break;
+    case 8: p2<P, 8, UNR, RULE>(st, s, T, o); break;
+    case 16: p2<P, 16, UNR, RULE>(st, s, T, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <typename P>
+static hipError_t phase1_r(hipStream_t st, const PhaseConfig& cfg, const State& s,
+                           const Tables& T, const Outputs& o) {
+  if (cfg.rule == 5) return phase1_p<P, 5>(st, cfg.k1, s, T, o);
+  return phase1_p<P, 3>(st, cfg.k1, s, T, o);
+}
+
+template <typename P>
+static hipError_t phase2_r(hipStream_t st, const PhaseConfig& cfg, const State& s,
+                           const Tables& T, const Outputs& o) {
+  if (cfg.rule == 5)
+    return cfg.unroll2 ? phase2_pu<P, true, 5>(st, cfg.k2, s, T, o)
+                       : phase2_pu<P, false, 5>(st, cfg.k2, s, T, o);
+  return cfg.unroll2 ? phase2_pu<P, true, 3>(st, cfg.k2, s, T, o)
+                     : phase2_pu<P, false, 3>(st, cfg.k2, s, T, o);
 }
 
 hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,
                          const Tables& T, const Outputs& o) {
-  if (precision == kSplit32) return phase1_p<float>(st, cfg.k1, s, T, o);
-  return phase1_p<double>(st, cfg.k1, s, T, o);
+  if (precision == kSplit32) return phase1_r<float>(st, cfg, s, T, o);
+  return phase1_r<double>(st, cfg, s, T, o);
 }
 
 hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,
                          const Tables& T, const Outputs& o) {
-  if (precision == kSplit32)
-    return cfg.unroll2 ? phase2_pu<float, true>(st, cfg.k2, s, T, o)
-                       : phase2_pu<float, false>(st, cfg.k2, s, T, o);
-  return cfg.unroll2 ? phase2_pu<double, true>(st, cfg.k2, s, T, o)
-                     : phase2_pu<double, false>(st, cfg.k2, s, T, o);
+  if (precision == kSplit32) return phase2_r<float>(st, cfg, s, T, o);
+  return phase2_r<double>(st, cfg, s, T, o);
 }
 
 }  // namespace dmd

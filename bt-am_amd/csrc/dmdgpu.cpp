@@ -147,7 +147,7 @@ struct dmdgpu_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   PinnedBuf pin_up, pin_down;
   double rc = kDefaultRc;
-  dmd::PhaseConfig cfg{dmd::kFP64, 8, 8, 0};
+  dmd::PhaseConfig cfg{dmd::kFP64, 8, 8, 0, 3};
   // tables
   bool have[3] = {false, false, false};
   Spline rho[2], embed[2], pair[3], u[3], w[3];
@@ -1113,7 +1113,8 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_act.p, (char*)c->pin_up.p + ob + pb, ab, hipMemcpyHostToDevice,
                         c->stream));
   const bool same_cfg = c->graph_cfg.k1 == c->cfg.k1 && c->graph_cfg.k2 == c->cfg.k2 &&
-                        c->graph_cfg.unroll2 == c->cfg.unroll2;
+                        c->graph_cfg.unroll2 == c->cfg.unroll2 &&
+                        c->graph_cfg.rule == c->cfg.rule;
   if (c->use_graph) {
     if (!c->gexec || c->graph_extent != c->extent || c->graph_ninst != ni || !same_cfg) {
       drop_graph(c);
@@ -1302,6 +1303,12 @@ dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* c, int precision) {
   if (!c || (precision != DMDGPU_FP64 && precision != DMDGPU_SPLIT32))
     return fail(DMDGPU_ERR_ARG, "precision must be DMDGPU_FP64 or DMDGPU_SPLIT32");
   c->cfg.precision = precision == DMDGPU_SPLIT32 ? dmd::kSplit32 : dmd::kFP64;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* c, int order) {
+  if (!c || (order != 3 && order != 5)) return fail(DMDGPU_ERR_ARG, "rule order must be 3 or 5");
+  c->cfg.rule = order;
   return DMDGPU_OK;
 }
 

@@ -151,6 +151,16 @@ dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* ctx, int k_phase1, int k_phase2, int
 enum { DMDGPU_FP64 = 0, DMDGPU_SPLIT32 = 1 };
 dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* ctx, int precision);
 
+/* Quadrature of the pair terms (density, pair potential) and of the
+ * coincident angular terms.  3 (default): the reference's 12-point 6D Stroud
+ * rule (src/Numerical/Quadrature-Hermitian-3th.cpp:36-110).  5: 27-point tensor
+ * Gauss-Hermite rule on the relative coordinate r ~ N(r_ij, (s_i^2+s_j^2) I)
+ * (nodes 0, +-sqrt(3) s_rel; weights 2/3, 1/6 per axis; exact to degree 5).
+ * Distinct angular triplets keep the 3rd-order 9D rule under both: the
+ * reference has no higher-order 9D rule (its 5th-order rule,
+ * Quadrature-Hermitian-5th.cpp:31-157, is a pair rule). */
+dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* ctx, int order);
+
 /* Resident instances -------------------------------------------------------
  * No counterpart in the reference (it holds one swarm per process).  A context
  * keeps any number of independent instances (systems) resident in one device

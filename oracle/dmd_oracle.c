@@ -770,6 +770,41 @@ static inline void grad_at(const or_tables* T, int t, const double* p, const dou
 
 static inline int act(const or_system* s, int j) { return s->x[j] >= OR_MIN_OCCUPANCY; }
 
+/* Pair (and coincident-triplet) rule of the factorised form.  Point t of the
+ * pair (i, j): p = r_ij + delta, weight w, and dsi = d delta / d sigma_i.
+ *   rule 3: the reference's 6D Stroud rule (Quadrature-Hermitian-3th.cpp:
+ *           36-110): sqrt(6) sigma shifts of site i (t < 6) or of site j,
+ *           one axis at a time, weight 1/12 each (12 points);
+ *   rule 5: 3-point Gauss-Hermite per axis on the relative coordinate
+ *           r ~ N(r_ij, (sigma_i^2 + sigma_j^2) I): nodes 0, +-sqrt(3)
+ *           sigma_rel with weights 2/3, 1/6 (27 points, exact to degree 5). */
+#define OR_SQRT3 1.7320508075688772
+static inline int pair_rule_points(int rule) { return rule == 5 ? 27 : 12; }
+static inline double pair_rule_point(int rule, int t, const double* r, double si, double sj,
+                                     double* p, double* dsi) {
+  p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
+  if (rule == 5) {
+    static const double node[3] = {0.0, OR_SQRT3, -OR_SQRT3};
+    static const double wn[3] = {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0};
+    const int a[3] = {t % 3, (t / 3) % 3, t / 9};
+    const double srel = sqrt(si * si + sj * sj);
+    double w = 1.0;
+    for (int k = 0; k < 3; k++) {
+      const double xi = node[a[k]];
+      p[k] += srel * xi;
+      dsi[k] = si / srel * xi;
+      w *= wn[a[k]];
+    }
+    return w;
+  }
+  const int wsel = t / 6, l = (t >> 1) % 3;
+  const double d = (t & 1) ? -OR_SQRT6 : OR_SQRT6;
+  p[l] += d * (wsel ? sj : si);
+  dsi[0] = dsi[1] = dsi[2] = 0.0;
+  if (!wsel) dsi[l] = d;
+  return 1.0 / 12.0;
+}
+
 /* Phase-1 record layout (OR_P1_STRIDE doubles per site). */
 enum { P1_LA = 0, P1_LU = 54, P1_LG0 = 63, P1_FP = 72, P1_DVDN = 73, P1_RHO = 74 };
 
@@ -808,24 +843,22 @@ static void phase1_site(const or_system* s, const or_tables* T, int i, double* r
       }
     for (int c = 0; c < 9; c++) U[c] += nj * T9[c];
     TPsi += nj * nj * dotM(T9, psi0);
-    double Phi = 0, R = 0;
-    for (int w = 0; w < 2; w++)
-      for (int l = 0; l < 3; l++)
-        for (int sg = 0; sg < 2; sg++) {
-          p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
-          p[l] += (sg ? -OR_SQRT6 : OR_SQRT6) * (w ? sj : si);
-          psi_at(T, t, p, ps);
-          C += nj * nj * dotM(ps, ps);
-          double pr = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-          Phi += sp_val(sphi, pr);
-          R += sp_val(srho, pr);
-        }
-    rho += nj * R / 12.0;
-    Ep += nj * Phi / 12.0;
+    double Phi = 0, R = 0, Cp = 0, dsi[3];
+    for (int pt = 0; pt < pair_rule_points(s->rule); pt++) {
+      const double wt = pair_rule_point(s->rule, pt, r, si, sj, p, dsi);
+      psi_at(T, t, p, ps);
+      Cp += wt * dotM(ps, ps);
+      double pr = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+      Phi += wt * sp_val(sphi, pr);
+      R += wt * sp_val(srho, pr);
+    }
+    C += nj * nj * Cp;
+    rho += nj * R;
+    Ep += nj * Phi;
   }
   double GG = 0;
   for (int ls = 0; ls < 6; ls++) GG += dotM(GA[ls], GA[ls]);
-  double At = (GG - selfA + 2.0 * dotM(U, G0) - 2.0 * TPsi) / 18.0 + C / 12.0;
+  double At = (GG - selfA + 2.0 * dotM(U, G0) - 2.0 * TPsi) / 18.0 + C;
   const or_spline* emb = &T->embed[sslot(spi)];
   double F = sp_val(emb, rho);
   *rho_out = rho;
@@ -930,27 +963,24 @@ static void phase2_site(const or_system* s, const or_tables* T, const double* p1
     const or_spline* sphi = &T->pair[t];
     const or_spline* srk = &T->rho[sslot(s->spc[k])];
     const or_spline* sru = &T->rho[sslot(spu)];
-    double cphi = 0.5 * nu * nk * (bu + bk) / 12.0;
-    double cra = bu * nu * Ru[P1_FP] * nk / 12.0;
-    double crb = bk * nk * Rk[P1_FP] * nu / 12.0;
-    double cang = (wa * nk + wb * nu) / 6.0;
-    for (int w = 0; w < 2; w++)
-      for (int l = 0; l < 3; l++)
-        for (int sg = 0; sg < 2; sg++) {
-          double sh = sg ? -OR_SQRT6 : OR_SQRT6;
-          p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
-          p[l] += sh * (w ? sk : su);
-          psi_at(T, t, p, ps);
-          applyM(cang, ps, L);
-          grad_at(T, t, p, L, g);
-          double pr = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-          double cs = (cphi * sp_d1(sphi, pr) + cra * sp_d1(srk, pr) + crb * sp_d1(sru, pr)) / pr;
-          for (int a = 0; a < 3; a++) g[a] += cs * p[a];
-          q0 += g[0]; q1 += g[1]; q2 += g[2];
-          if (!w) sacc += sh * g[l];
-          xacc += bk * nk * nk * (nu / 6.0) * dotM(ps, ps) +
-                  bk * (nk * Rk[P1_FP] * sp_val(sru, pr) + 0.5 * nk * sp_val(sphi, pr)) / 12.0;
-        }
+    double cphi = 0.5 * nu * nk * (bu + bk);
+    double cra = bu * nu * Ru[P1_FP] * nk;
+    double crb = bk * nk * Rk[P1_FP] * nu;
+    double cang = 2.0 * (wa * nk + wb * nu);
+    double dsu[3];
+    for (int pt = 0; pt < pair_rule_points(s->rule); pt++) {
+      const double wt = pair_rule_point(s->rule, pt, r, su, sk, p, dsu);
+      psi_at(T, t, p, ps);
+      applyM(wt * cang, ps, L);
+      grad_at(T, t, p, L, g);
+      double pr = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+      double cs = wt * (cphi * sp_d1(sphi, pr) + cra * sp_d1(srk, pr) + crb * sp_d1(sru, pr)) / pr;
+      for (int a = 0; a < 3; a++) g[a] += cs * p[a];
+      q0 += g[0]; q1 += g[1]; q2 += g[2];
+      sacc += g[0] * dsu[0] + g[1] * dsu[1] + g[2] * dsu[2];
+      xacc += wt * (bk * nk * nk * (2.0 * nu) * dotM(ps, ps) +
+                    bk * (nk * Rk[P1_FP] * sp_val(sru, pr) + 0.5 * nk * sp_val(sphi, pr)));
+    }
   }
   gq[0] = q0; gq[1] = q1; gq[2] = q2;
   *gs = sacc;

@@ -62,6 +62,10 @@ typedef struct {
   int cap;              /* row stride of nbr                 */
   const int* nnb;       /* n                                 */
   const int* nbr;       /* n x cap                           */
+  int rule;             /* pair/coincident rule of the factorised form:
+                           3 (default, 0 too) = the reference's 12-point 6D
+                           Stroud rule; 5 = 27-point tensor Gauss-Hermite on
+                           the relative coordinate (degree 5)             */
 } or_system;
 
 /* Tables ------------------------------------------------------------------ */

@@ -29,7 +29,8 @@ def build():
 class _System(C.Structure):
     _fields_ = [("n", C.c_int), ("q", _dp), ("sig", _dp), ("x", _dp), ("spc", _ip),
                 ("beta", _dp), ("gamma", _dp), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
-                ("periodic", C.c_int * 3), ("cap", C.c_int), ("nnb", _ip), ("nbr", _ip)]
+                ("periodic", C.c_int * 3), ("cap", C.c_int), ("nnb", _ip), ("nbr", _ip),
+                ("rule", C.c_int)]
 
 
 def _d(a):
@@ -108,9 +109,11 @@ class Oracle:
                                    _i(nnb), _i(nbr))
         return st, nnb, nbr
 
-    def system(self, sysd, nnb, nbr):
+    def system(self, sysd, nnb, nbr, rule=3):
         """Pack a system dict (see dmdgpu.lattice) into the C struct.  The
-        returned object keeps the arrays alive."""
+        returned object keeps the arrays alive.  rule: 3 (the reference's
+        pair rule) or 5 (27-point tensor Gauss-Hermite pair rule) for the
+        factorised functions."""
         keep = {}
         for k in ("q", "sigma", "x", "beta", "gamma"):
             keep[k] = np.ascontiguousarray(sysd[k], dtype=np.float64)
@@ -128,6 +131,7 @@ class Oracle:
             s.periodic[a] = int(sysd["periodic"][a])
         s.cap = keep["nbr"].shape[1]
         s.nnb, s.nbr = _i(keep["nnb"]), _i(keep["nbr"])
+        s.rule = int(rule)
         s._keep = keep
         return s
 

@@ -191,3 +191,38 @@ def test_error_codes(gpu):
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.neighbors()
     assert e.value.code == 5
+
+
+# Fifth-order pair rule (27-point tensor Gauss-Hermite on the relative
+# coordinate) for the pair and coincident terms: FP64 GPU against the oracle's
+# restatement of the same rule at the FP64 bar, and the split evaluation
+# against FP64 at the split bars.
+@pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "cluster_mgh"])
+def test_fifth_order_rule(gpu, oracle, case):
+    say(f"[gpu] rule 5 {case}")
+    sysd = CASES[case]()
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr, 5))
+    gpu.set_state(sysd)
+    gpu.set_tuning(8, 8, 0)
+    gpu.neighbors()
+    gpu.set_rule(5)
+    try:
+        gpu.set_precision("fp64")
+        out = gpu.residual()
+        for k in ("Rq", "Rs", "Rx"):
+            assert rel(out[k], ref[k]) < TOL, k
+        assert out["S0_total"] == pytest.approx(ref["S0_total"], rel=TOL)
+        assert rel(gpu.mf_rho(), ref["mf_rho"]) < TOL
+        gpu.set_precision("split32")
+        out = gpu.residual()
+        err = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
+        err["mf_rho"] = rel(gpu.mf_rho(), ref["mf_rho"])
+        err["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"])
+        say(f"    rule-5 split32 errors {err}")
+        for k, tol in SPLIT_TOL.items():
+            assert err[k] < tol, (k, err[k])
+    finally:
+        gpu.set_precision("fp64")
+        gpu.set_rule(3)
+        gpu.set_tuning(4, 4, 1)

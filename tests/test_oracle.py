@@ -190,6 +190,34 @@ def test_quadrature_closed_form_density_and_dipole(tmp_path):
     assert rel(f["V0"], V0_cf) < 1e-12
 
 
+def test_fifth_order_pair_rule_closed_form(tmp_path):
+    """Coincident quadrupole term of a dimer with w = c (all other functions
+    zero): V0_i = n_i^2 n_j^2 c^2/3 E|r|^4 over r ~ N(r_ij, s^2 I),
+    s^2 = sigma_i^2 + sigma_j^2, whose Gaussian moment is
+    |r_ij|^4 + 10 |r_ij|^2 s^2 + 15 s^4.  The 27-point tensor rule (degree 5)
+    is exact; the reference's 12-point rule (degree 3) gives 18 (sigma_i^4 +
+    sigma_j^4) in place of 15 s^4, which the test also pins."""
+    c = 0.7
+    O = _poly_tables(tmp_path, rho=(0,), F=(0,), phi=(0,), u=(0,), w=(c,))
+    sysd = lat.cluster(1, 1, 1, with_h=False, seed=3, sigma_jitter=0.3)
+    keep = [0, 1]
+    for k in ("q", "sigma", "x", "species", "beta", "gamma"):
+        sysd[k] = sysd[k][keep]
+    sysd["n"] = 2
+    sysd["x"] = np.array([0.9, 0.6])
+    st, nnb, nbr = O.neighbors(sysd, RC)
+    assert st == 0 and list(nnb) == [1, 1]
+    A = ((sysd["q"][0] - sysd["q"][1]) ** 2).sum()
+    si, sj = sysd["sigma"]
+    s2 = si ** 2 + sj ** 2
+    pre = (sysd["x"][0] * sysd["x"][1]) ** 2 * c * c / 3.0
+    exact = pre * (A * A + 10 * A * s2 + 15 * s2 * s2)
+    third = pre * (A * A + 10 * A * s2 + 18 * (si ** 4 + sj ** 4))
+    for rule, want in ((5, exact), (3, third)):
+        V0 = O.fact_residual(O.system(sysd, nnb, nbr, rule))["V0"]
+        assert V0 == pytest.approx([want, want], rel=1e-12)
+
+
 def numpy_stroud_V0(sysd, nnb, nbr, rho, F, phi, u, w):
     """Independent enumeration of the reference's rule for V0_i: pair integrals
     with sqrt(6) sigma shifts (weight 1/12), distinct triplets with 3 sigma
@@ -321,12 +349,13 @@ def synth(tmp_path_factory):
     return Oracle(str(d))
 
 
-def test_finite_differences_of_free_entropy(synth):
+@pytest.mark.parametrize("rule", [3, 5])
+def test_finite_differences_of_free_entropy(synth, rule):
     """R = d(sum_i S0_i)/d(q, sigma, x) on smooth tables (consistent clamp)."""
     sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21)
     st, nnb, nbr = synth.neighbors(sysd, 6.5)
     assert st == 0
-    base = synth.fact_residual(synth.system(sysd, nnb, nbr))
+    base = synth.fact_residual(synth.system(sysd, nnb, nbr, rule))
 
     def tot(key, u, a, h):
         sd = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sysd.items()}
@@ -334,7 +363,7 @@ def test_finite_differences_of_free_entropy(synth):
             sd["q"][u, a] += h
         else:
             sd[key][u] += h
-        return synth.fact_residual(synth.system(sd, nnb, nbr))["S0_total"]
+        return synth.fact_residual(synth.system(sd, nnb, nbr, rule))["S0_total"]
 
     rng = np.random.default_rng(1)
     for u in rng.choice(sysd["n"], 4, replace=False):
