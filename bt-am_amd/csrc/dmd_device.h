@@ -55,6 +55,22 @@ template <> DEV const float* tab_rh<float>(const Tables& T) { return T.rh32; }
 // ---------------------------------------------------------------------------
 template <typename P> struct PairGeo;
 
+// |p| and 1/|p| from |p|^2 in FP64: the hardware reciprocal square-root
+// estimate refined by Newton's iteration y <- y (3 - a y^2) / 2 (each step
+// doubles the correct bits of the ~2^-23-accurate estimate, so three steps
+// reach the FP64 rounding level).  One transcendental and ~10 FMA-class
+// operations instead of a square root and a division.
+DEV void norm_and_inv(double a, double& x, double& ir) {
+  double y = __builtin_amdgcn_rsq(a);
+#pragma unroll
+  for (int it = 0; it < 3; it++) {
+    const double h = 0.5 * a * y;
+    y = fma(-h, y, 1.5) * y;
+  }
+  ir = y;
+  x = a * y;
+}
+
 template <> struct PairGeo<double> {
   double r[3];
   DEV void init(const double* rd) { r[0] = rd[0]; r[1] = rd[1]; r[2] = rd[2]; }
@@ -63,8 +79,8 @@ template <> struct PairGeo<double> {
     p[0] = r[0] + (l == 0 ? d : 0.0);
     p[1] = r[1] + (l == 1 ? d : 0.0);
     p[2] = r[2] + (l == 2 ? d : 0.0);
-    const double x = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-    ir = 1.0 / x;
+    double x;
+    norm_and_inv(p[0] * p[0] + p[1] * p[1] + p[2] * p[2], x, ir);
     m = seg_index(x, T.dxr, T.inv_dxr, T.nr, off);
   }
   // p = r + d for a general displacement d.
@@ -73,8 +89,8 @@ template <> struct PairGeo<double> {
     p[0] = r[0] + d[0];
     p[1] = r[1] + d[1];
     p[2] = r[2] + d[2];
-    const double x = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-    ir = 1.0 / x;
+    double x;
+    norm_and_inv(p[0] * p[0] + p[1] * p[1] + p[2] * p[2], x, ir);
     m = seg_index(x, T.dxr, T.inv_dxr, T.nr, off);
   }
 };
@@ -94,7 +110,7 @@ template <> struct PairGeo<float> {
     p[2] = r[2] + (l == 2 ? d : 0.0f);
     const float rl = l == 0 ? r[0] : (l == 1 ? r[1] : r[2]);
     const float pn = __builtin_amdgcn_sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-    const float D = __fdividef(2.0f * rl * d + d * d, pn + R32);
+    const float D = (2.0f * rl * d + d * d) * __builtin_amdgcn_rcpf(pn + R32);
     int mm = (int)((R32 + D) * T.inv_dxr32);
     mm = mm > T.nr - 1 ? T.nr - 1 : mm;
     mm = mm < 0 ? 0 : mm;
@@ -111,7 +127,7 @@ template <> struct PairGeo<float> {
     const float pn = __builtin_amdgcn_sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
     const float rd = r[0] * d[0] + r[1] * d[1] + r[2] * d[2];
     const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const float D = __fdividef(2.0f * rd + dd, pn + R32);
+    const float D = (2.0f * rd + dd) * __builtin_amdgcn_rcpf(pn + R32);
     int mm = (int)((R32 + D) * T.inv_dxr32);
     mm = mm > T.nr - 1 ? T.nr - 1 : mm;
     mm = mm < 0 ? 0 : mm;

@@ -15,6 +15,7 @@ constexpr int kRecLG0 = 63;        // 9     : (1/9) M G0       (unshifted sums)
 constexpr int kRecFP = 72;         // F'(rho_bar)
 constexpr int kRecDVDN = 73;       // dV0_i / dx_i (own occupancy)
 constexpr int kRecRHO = 74;        // rho_bar
+constexpr int kRec32 = 72;         // floats per FP32 copy of LA, LU, LG0 (split path)
 
 // Spline tables resident in HBM (L2-resident in practice: ~0.8 MB for the
 // reference tables).  All r-tables share one (nr, dx) grid.  Coefficient rows
@@ -74,6 +75,8 @@ struct State {
 
 struct Outputs {
   double* rec;         // n x kRec
+  float* rec32;        // n x kRec32: LA, LU, LG0 in FP32 for split-precision instances
+                       // (phase 2 of the split path reads them as FP32 anyway)
   double* V0;
   double* rho;
   double* Rq;          // n x 3

@@ -69,9 +69,13 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
   const Box box = s.boxes[inst];
   double* rec = o.rec + (size_t)i * kRec;
+  float* rec32 = o.rec32 + (size_t)i * kRec32;
+  constexpr bool kF32 = PrecCode<P>::v == kSplit32;
   const double ni = s.ne[i];
   if (ni == 0.0) {
     for (int c = g; c < kRec; c += K) rec[c] = 0.0;
+    if (kF32)
+      for (int c = g; c < kRec32; c += K) rec32[c] = 0.0f;
     if (g == 0) { o.V0[i] = 0.0; o.rho[i] = 0.0; }
     return;
   }
@@ -122,7 +126,10 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     applyM(1.0 / 9.0, G, tmp);
 #pragma unroll
     for (int c = 0; c < 9; c++)
-      if (((ls * 9 + c) % K) == g) rec[kRecLA + ls * 9 + c] = tmp[c];
+      if (((ls * 9 + c) % K) == g) {
+        if (kF32) rec32[kRecLA + ls * 9 + c] = (float)tmp[c];
+        else rec[kRecLA + ls * 9 + c] = tmp[c];
+      }
   }
 
   // unshifted, j-shift and sqrt(6) sigma points
@@ -197,11 +204,17 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   applyM(1.0 / 9.0, U, tmp);
 #pragma unroll
   for (int c = 0; c < 9; c++)
-    if (((kRecLU + c) % K) == g) rec[kRecLU + c] = tmp[c];
+    if (((kRecLU + c) % K) == g) {
+      if (kF32) rec32[kRecLU + c] = (float)tmp[c];
+      else rec[kRecLU + c] = tmp[c];
+    }
   applyM(1.0 / 9.0, G0, tmp);
 #pragma unroll
   for (int c = 0; c < 9; c++)
-    if (((kRecLG0 + c) % K) == g) rec[kRecLG0 + c] = tmp[c];
+    if (((kRecLG0 + c) % K) == g) {
+      if (kF32) rec32[kRecLG0 + c] = (float)tmp[c];
+      else rec[kRecLG0 + c] = tmp[c];
+    }
   if (g == 0) {
     rec[kRecFP] = Fp;
     rec[kRecDVDN] = F + 0.5 * Ep + 2.0 * ni * At;
@@ -212,13 +225,18 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 }
 
 // Loads 9 record values as P, negating the dipole part when flip is set.
-template <typename P>
-DEV void load9(const double* r, bool flip, P* o) {
+template <typename S, typename P>
+DEV void load9(const S* r, bool flip, P* o) {
   const P sgn = flip ? P(-1) : P(1);
   o[0] = sgn * (P)r[0]; o[1] = sgn * (P)r[1]; o[2] = sgn * (P)r[2];
 #pragma unroll
   for (int c = 3; c < 9; c++) o[c] = (P)r[c];
 }
+
+// The adjoint blocks (LA, LU, LG0) of site i in the precision phase 2 reads:
+// the FP64 record for FP64, the FP32 copy for the split path.
+DEV const double* adj_rec(const Outputs& o, int i, double) { return o.rec + (size_t)i * kRec; }
+DEV const float* adj_rec(const Outputs& o, int i, float) { return o.rec32 + (size_t)i * kRec32; }
 
 // ---------------------------------------------------------------------------
 // Phase 2: gradient of sum_c beta_c V0_c with respect to (q_u, sigma_u, x_u).
@@ -251,6 +269,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
   const int spu = s.spc[u];
   const double bu = s.beta[u];
   const double* Ru = o.rec + (size_t)u * kRec;
+  const P* Au = adj_rec(o, u, P(0));
   const double FPu = Ru[kRecFP];
   double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
   const int nn = s.nnb[u];
@@ -264,6 +283,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
     const int spk = s.spc[k];
     const double bk = s.beta[k];
     const double* Rk = o.rec + (size_t)k * kRec;
+    const P* Ak = adj_rec(o, k, P(0));
     const double wa = bu * nu * nu * nk, wb = bk * nk * nk * nu;
     const double xb = bk * nk * nk;
     const P waP = (P)wa, wbP = (P)wb;
@@ -283,11 +303,11 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
     // Role-(a) adjoint at the j-shift points (constant over them) and role-(b)
     // adjoint at the i-shift points (constant over them).
     P laB[9], lbA[9];
-    load9(Ru + kRecLG0, false, laB);
+    load9(Au + kRecLG0, false, laB);
     applyM(cnk, psi0, tmp);
 #pragma unroll
     for (int c = 0; c < 9; c++) laB[c] = waP * (laB[c] + tmp[c]);
-    load9(Rk + kRecLG0, true, lbA);
+    load9(Ak + kRecLG0, true, lbA);
     applyM(cnu, psi0, tmp);
 #pragma unroll
     for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; Tuk[c] = P(0); SA[c] = P(0); }
@@ -305,7 +325,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
       psi_from(uu, ww, p, ps);
 #pragma unroll
       for (int e = 0; e < 9; e++) Tuk[e] += ps[e];
-      load9(Rk + kRecLA + 9 * (2 * l + (1 - sg)), true, lb);
+      load9(Ak + kRecLA + 9 * (2 * l + (1 - sg)), true, lb);
       P mps[9];
       applyM(cnu, ps, mps);
 #pragma unroll
@@ -329,7 +349,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
       psi_from(uu, ww, p, ps);
 #pragma unroll
       for (int e = 0; e < 9; e++) SA[e] += ps[e];
-      load9(Ru + kRecLA + 9 * (2 * l + sg), false, la);
+      load9(Au + kRecLA + 9 * (2 * l + sg), false, la);
       P mps[9];
       applyM(cnk, ps, mps);
 #pragma unroll
@@ -354,11 +374,11 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
     // unshifted point
     {
       P la[9], lb[9], L[9], g3[3];
-      load9(Ru + kRecLU, false, la);
+      load9(Au + kRecLU, false, la);
       applyM(cnk, Tuk, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) la[c] += tmp[c];
-      load9(Rk + kRecLU, true, lb);
+      load9(Ak + kRecLU, true, lb);
       applyM(cnu, SA, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) {

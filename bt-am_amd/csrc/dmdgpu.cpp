@@ -48,7 +48,7 @@ constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
 constexpr int kChunk = dmd::kChunk;
 // Device bytes per arena site (state, lists, records, outputs, solver vectors).
 constexpr size_t kBytesPerSite = 32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 +
-                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap;
+                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32;
 
 struct Spline {
   int n = 0;
@@ -154,7 +154,7 @@ struct dmdgpu_ctx {
   double mass[2] = {0.0, 0.0};
   bool tables_dirty = true;
   DevBuf<double> d_uw, d_ph, d_rh, d_em;
-  DevBuf<float> d_uw32, d_ph32, d_rh32;
+  DevBuf<float> d_uw32, d_ph32, d_rh32, d_rec32;
   dmd::Tables T{};
   // arena
   int extent = 0;     // sites in use, multiple of kChunk
@@ -345,6 +345,7 @@ dmd::State make_state(dmdgpu_ctx* c) {
 dmd::Outputs make_outputs(dmdgpu_ctx* c) {
   dmd::Outputs o;
   o.rec = c->d_rec.p;
+  o.rec32 = c->d_rec32.p;
   o.V0 = c->d_V0.p;
   o.rho = c->d_rho.p;
   o.Rq = c->d_Rq.p;
@@ -388,6 +389,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_cell_of.ensure(cap));
   HIPCHK(c->d_cell_atoms.ensure(cap));
   HIPCHK(c->d_rec.ensure(cap * dmd::kRec, keep * dmd::kRec));
+  HIPCHK(c->d_rec32.ensure(cap * dmd::kRec32, keep * dmd::kRec32));
   HIPCHK(c->d_V0.ensure(cap, keep));
   HIPCHK(c->d_rho.ensure(cap, keep));
   HIPCHK(c->d_Rq.ensure(cap * 3, keep * 3));
@@ -1221,6 +1223,7 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   c->d_uw32.release();
   c->d_ph32.release();
   c->d_rh32.release();
+  c->d_rec32.release();
   DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph,    &c->d_rh,  &c->d_em,       &c->d_ne,
                            &c->d_xr,  &c->d_beta,  &c->d_gamma, &c->d_rec,    &c->d_V0,
                            &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,

@@ -5,7 +5,7 @@ whether phase 2 unrolls its 25 point evaluations.  Each candidate is timed with
 the context's per-kernel HIP events on the current state and lists; phase 1 and
 phase 2 are independent kernels, so each picks its own best setting.
 """
-LANES = (1, 2, 4, 8)
+LANES = (1, 2, 4, 8, 16)
 
 
 def autotune(ctx, reps=3, lanes=LANES, unrolls=(0, 1)):
