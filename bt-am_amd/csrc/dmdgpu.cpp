@@ -94,6 +94,8 @@ struct Solve {
   dmdgpu_relax_opts o{};
   dmdgpu_diffuse_opts dop{};  // ST_DIFF
   bool diff64 = false;         // mixed diffusion: sweeps switched to FP64
+  bool early = false;          // mixed relaxation, far from convergence: split residuals throughout
+  bool have_r0f = false;       // R0f already holds the split residual at X
   double mass0 = 0, mass = 0;  // hydrogen content at x_n and at the iterate
   int newton = 0, cg = 0, ls = 0, evals = 0;
   double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
@@ -846,6 +848,13 @@ dmdgpu_status check_primary(dmdgpu_ctx* c) {
 constexpr double kProbe64 = 1e-7;
 constexpr double kProbe32 = 5e-5;
 constexpr double kMaxStep = 0.2;   // max |dq| of a Newton step (A)
+// Mixed relaxation far from convergence: while |R|_2 > kSplitSwitch sqrt(ndof)
+// -- a thousand times the split residual's RMS noise of ~1e-6 per component --
+// the residuals of the Newton iteration and of its line search are split
+// evaluations too (they also serve as the base of the split probes, so no
+// separate base evaluation is needed).  Below it every accepted residual and
+// the stopping test are FP64.
+constexpr double kSplitSwitch = 1e-3;
 
 constexpr double kA2PerM2 = 1e20;  // diffusivity m^2/s -> A^2/s
 // Mixed diffusion steps sweep with split-precision residuals until the
@@ -876,7 +885,7 @@ void plan_tick(dmdgpu_ctx* c) {
         case ST_START:
           op.make = dmd::kMakeX;
           op.dot = dmd::kDotNorm;
-          c->h_prec[k] = dmd::kFP64;
+          c->h_prec[k] = S.early ? dmd::kSplit32 : dmd::kFP64;
           break;
         case ST_BASE32:
           op.make = dmd::kMakeX;
@@ -897,7 +906,7 @@ void plan_tick(dmdgpu_ctx* c) {
           op.make = dmd::kMakeXD;
           op.alpha_ls = S.alpha_ls;
           op.dot = dmd::kDotNorm;
-          c->h_prec[k] = dmd::kFP64;
+          c->h_prec[k] = S.early ? dmd::kSplit32 : dmd::kFP64;
           break;
         case ST_DIFF:
           set_diff_op(op, S.dop, dmd::kDiffStep);
@@ -932,10 +941,32 @@ void digest_tick(dmdgpu_ctx* c) {
     switch (S.stage) {
       case ST_START: {
         S.rnorm = std::sqrt(d[0]);
-        S.rnorm0 = S.rnorm;
+        if (S.newton == 0) S.rnorm0 = S.rnorm;
         nx.upd |= dmd::kUpdR0FromRv;
+        if (S.early) {
+          // split residual at X: base of the probes as well
+          nx.upd |= dmd::kUpdR0fFromRv;
+          if (S.rnorm <= kSplitSwitch * std::sqrt((double)S.ndof)) {
+            S.early = false;
+            S.have_r0f = true;  // FP64 residual at X next, then CG
+            break;
+          }
+          nx.upd |= dmd::kUpdCgInit;
+          S.rr = S.pp = S.rnorm * S.rnorm;
+          S.cg = 0;
+          S.stage = ST_CG;
+          break;
+        }
         if (S.rnorm <= S.o.atol) {
           finish(S, DMDGPU_RELAX_CONVERGED);
+          break;
+        }
+        if (S.o.mixed && S.have_r0f) {
+          S.have_r0f = false;
+          nx.upd |= dmd::kUpdCgInit;
+          S.rr = S.pp = S.rnorm * S.rnorm;
+          S.cg = 0;
+          S.stage = ST_CG;
           break;
         }
         if (S.o.mixed) {
@@ -997,6 +1028,24 @@ void digest_tick(dmdgpu_ctx* c) {
           nx.upd |= dmd::kUpdAcceptY | dmd::kUpdR0FromRv;
           S.rnorm = rn;
           S.newton++;
+          if (S.early) {
+            nx.upd |= dmd::kUpdR0fFromRv;
+            if (S.newton >= S.o.max_newton) {
+              finish(S, DMDGPU_RELAX_MAXITER);
+              break;
+            }
+            if (rn <= kSplitSwitch * std::sqrt((double)S.ndof)) {
+              S.early = false;
+              S.have_r0f = true;
+              S.stage = ST_START;  // FP64 residual at the accepted iterate
+              break;
+            }
+            nx.upd |= dmd::kUpdCgInit;
+            S.rr = S.pp = rn * rn;
+            S.cg = 0;
+            S.stage = ST_CG;
+            break;
+          }
           if (rn <= S.o.atol) {
             finish(S, DMDGPU_RELAX_CONVERGED);
             break;
@@ -1524,6 +1573,7 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   }
   if (nd == 0) return fail(DMDGPU_ERR_ARG, "no degree of freedom selected");
   S.ndof = nd;
+  S.early = S.o.mixed != 0;
   I->sv = S;
   // the iterate starts from the current evaluation state of the instance
   HIPCHK(hipMemcpyAsync(c->d_X.p + I->base, c->d_qs.p + I->base, sizeof(double4) * I->cap,

@@ -204,8 +204,10 @@ dmdgpu_status dmdgpu_instance_set_unknowns(dmdgpu_ctx* ctx, int id, const double
  * instance, occupancies fixed, until the 2-norm of the selected residual
  * components is <= atol.  Hessian actions are
  * finite differences of the residual; with `mixed`, those probes use the split
- * FP32 evaluation while every accepted residual and the stopping test stay
- * FP64.  Instances advance independently: one tick gives every running
+ * FP32 evaluation; while |R|_2 > 1e-3 sqrt(ndof) (a thousand times the split
+ * residual's noise) the Newton and line-search residuals are split
+ * evaluations too, and below it every accepted residual and the stopping test
+ * are FP64.  Instances advance independently: one tick gives every running
  * instance exactly one residual evaluation (all instances in one launch). */
 typedef struct {
   double atol;     /* stop when |(R_q, R_sigma)|_2 <= atol                */
