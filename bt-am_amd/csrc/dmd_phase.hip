@@ -61,6 +61,11 @@ DEV bool site_of(const State& s, int& i, int& g) {
   return i < s.n;
 }
 
+// i-shift points per walk of the neighbour row in phase 1 (1, 2, 3 or 6;
+// measured on MI355X at the 54k-site workload: 3 is fastest in both precisions):
+// fewer walks recompute the pair geometry less often, more accumulators live.
+constexpr int kShiftsPerWalk = 3;
+
 template <typename P, int K, int RULE>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   int i, g;
@@ -90,15 +95,15 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   const int* row = s.nbr + (size_t)i * kCap;
   double tmp[9];
 
-  // i-shift points, one (l, s) per walk of the row
+  // i-shift points, kShiftsPerWalk of the six (l, s) per walk of the row
   double GG = 0, selfA = 0;
 #pragma unroll 1
-  for (int ls = 0; ls < 6; ls++) {
-    const int l = ls >> 1;
-    const P sh = (ls & 1 ? P(-3) : P(3)) * si;
-    double G[9];
+  for (int ls0 = 0; ls0 < 6; ls0 += kShiftsPerWalk) {
+    double G[kShiftsPerWalk][9];
 #pragma unroll
-    for (int c = 0; c < 9; c++) G[c] = 0;
+    for (int b = 0; b < kShiftsPerWalk; b++)
+#pragma unroll
+      for (int c = 0; c < 9; c++) G[b][c] = 0;
     for (int kk = g; kk < nn; kk += K) {
       const int j = row[kk];
       const double nj = s.ne[j];
@@ -109,27 +114,36 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
       min_image(box, qi, qj, rd);
       PairGeo<P> geo;
       geo.init(rd);
-      P p[3], off, ir, ps[9];
-      int m;
-      geo.point(T, l, sh, p, m, off, ir);
-      const P* c = uwr + ((size_t)m << 3);
-      psi_from(cval(c, off), cval(c + 4, off), p, ps);
 #pragma unroll
-      for (int e = 0; e < 9; e++) G[e] += nj * (double)ps[e];
-      selfA += nj * nj * (double)dotM(ps, ps);
-    }
-    if (K > 1) {
+      for (int b = 0; b < kShiftsPerWalk; b++) {
+        const int ls = ls0 + b;
+        const P sh = (ls & 1 ? P(-3) : P(3)) * si;
+        P p[3], off, ir, ps[9];
+        int m;
+        geo.point(T, ls >> 1, sh, p, m, off, ir);
+        const P* c = uwr + ((size_t)m << 3);
+        psi_from(cval(c, off), cval(c + 4, off), p, ps);
 #pragma unroll
-      for (int c = 0; c < 9; c++) G[c] = group_sum<K>(G[c]);
-    }
-    GG += dotM(G, G);
-    applyM(1.0 / 9.0, G, tmp);
-#pragma unroll
-    for (int c = 0; c < 9; c++)
-      if (((ls * 9 + c) % K) == g) {
-        if (kF32) rec32[kRecLA + ls * 9 + c] = (float)tmp[c];
-        else rec[kRecLA + ls * 9 + c] = tmp[c];
+        for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
+        selfA += nj * nj * (double)dotM(ps, ps);
       }
+    }
+#pragma unroll
+    for (int b = 0; b < kShiftsPerWalk; b++) {
+      const int ls = ls0 + b;
+      if (K > 1) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) G[b][c] = group_sum<K>(G[b][c]);
+      }
+      GG += dotM(G[b], G[b]);
+      applyM(1.0 / 9.0, G[b], tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++)
+        if (((ls * 9 + c) % K) == g) {
+          if (kF32) rec32[kRecLA + ls * 9 + c] = (float)tmp[c];
+          else rec[kRecLA + ls * 9 + c] = tmp[c];
+        }
+    }
   }
 
   // unshifted, j-shift and sqrt(6) sigma points
