@@ -122,14 +122,14 @@ def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
 def split_leg(ctx, args, pqe):
     """The split-precision evaluation (FP32 points, FP64 sums/totals) of the
     same workload: its throughput and its error against the FP64 result."""
-    from dmdgpu.tune import autotune
     ref = ctx.residual()
     ctx.set_precision("split32")
-    cfg, _ = autotune(ctx)
+    cfg = ctx.tuning32
     out = ctx.residual()
     for _ in range(args.warmup):
         ctx.residual_resident()
     ctx.sync()
+    ctx.reset_kernel_times()
     ctx.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -189,6 +189,7 @@ def solve_instances(count):
 
 
 def solves_leg(tuning, count, graph=True):
+    # tuning = ((k1, k2, unroll2) FP64, (k1, k2, unroll2) split)
     """Solves per hour: the plain path (one instance at a time, FP64 Hessian
     probes) against the batched path (all instances resident, one launch per
     tick, split-precision probes, FP64 residuals and stopping test).  Both
@@ -197,12 +198,12 @@ def solves_leg(tuning, count, graph=True):
     FP64 probes, 0.1 for split-precision probes)."""
     import dmdgpu
     insts = solve_instances(count)
-    k1, k2, un = tuning
     sizes = [int(s["n"]) for s in insts]
     out = {"instances": len(insts), "sites_total": sum(sizes), "sites_min": min(sizes),
            "sites_max": max(sizes), "atol": 1e-8}
     ctx = dmdgpu.Context(0)
-    ctx.set_tuning(k1, k2, un)
+    ctx.set_tuning_precision("fp64", *tuning[0])
+    ctx.set_tuning_precision("split32", *tuning[1])
     ctx.set_graph(graph)
     plain_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, mixed=False)
     t_plain = 0.0
@@ -268,9 +269,9 @@ def serving_leg(tuning, count, load=0.8, budget_gb=1.0):
     from dmdgpu.serve import Request, latency_stats, poisson_arrivals, serve, serve_plain
     mix = lat.serving_mix(count, seed=7)
     budget = int(budget_gb * 2 ** 30)
-    k1, k2, un = tuning
     ctx = dmdgpu.Context(0)
-    ctx.set_tuning(k1, k2, un)
+    ctx.set_tuning_precision("fp64", *tuning[0])
+    ctx.set_tuning_precision("split32", *tuning[1])
     sat = [Request(k, kind, sd, 0.0) for k, (kind, sd) in enumerate(mix)]
     p_sat, t_p = serve_plain(ctx, sat)
     f_sat, t_f = serve(ctx, sat, budget_bytes=budget, mixed=False)
@@ -312,7 +313,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tuning", type=str, default="auto",
-                    help="auto, or k1,k2,unroll2 (lanes per site in phase 1/2, unroll)")
+                    help="auto, or k1,k2,unroll2[/k1,k2,unroll2] (lanes per site in phase 1/2, "
+                         "unroll; FP64 then split precision)")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     ap.add_argument("--no-solves", action="store_true", help="skip the solves/hr leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
@@ -334,11 +336,18 @@ def main():
     ctx.neighbors()
     from dmdgpu.tune import autotune
     if args.tuning == "auto":
-        (k1, k2, un), table = autotune(ctx)
+        cfg32, table32 = autotune(ctx, "split32")
+        log(f"autotune split32: {cfg32} {table32}")
+        (k1, k2, un), table = autotune(ctx, "fp64")
         log(f"autotune fp64: {(k1, k2, un)} {table}")
     else:
-        k1, k2, un = (int(v) for v in args.tuning.split(","))
-        ctx.set_tuning(k1, k2, un)
+        parts = args.tuning.split("/")
+        k1, k2, un = (int(v) for v in parts[0].split(","))
+        cfg32 = tuple(int(v) for v in parts[-1].split(","))
+        ctx.set_tuning_precision("fp64", k1, k2, un)
+        ctx.set_tuning_precision("split32", *cfg32)
+        ctx.set_precision("fp64")
+    ctx.tuning32 = cfg32
     pairs, pqe = ctx.work_counts()
     log(f"active ordered pairs {pairs}, PQE per residual {pqe}")
 
@@ -376,7 +385,7 @@ def main():
 
     solves = None
     if not args.no_solves:
-        solves = solves_leg((k1, k2, un), args.instances)
+        solves = solves_leg(((k1, k2, un), cfg32), args.instances)
         log(f"solves: {solves}")
 
     rule5 = rule5_leg(ctx, args, pqe)
@@ -384,7 +393,7 @@ def main():
 
     serving = None
     if not args.no_serve:
-        serving = serving_leg((k1, k2, un), args.serve_requests)
+        serving = serving_leg(((k1, k2, un), cfg32), args.serve_requests)
         log(f"serving: {serving}")
 
     split = None

@@ -105,11 +105,16 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
 enum Precision { kFP64 = 0, kSplit32 = 1 };
 
 struct PhaseConfig {
-  int precision;   // Precision
-  int k1, k2;      // lanes per site in phase 1 / phase 2 (1, 2, 4, 8, 16)
-  int unroll2;     // 1: fully unrolled point loops in phase 2, 0: rolled
-  int rule = 3;    // pair / coincident rule: 3 (reference, 12 points) or 5
-                   // (27-point tensor Gauss-Hermite on the relative coordinate)
+  int precision;     // Precision of one-instance evaluations
+  // per Precision (index kFP64 / kSplit32):
+  int k1[2], k2[2];  // lanes per site in phase 1 / phase 2 (1, 2, 4, 8, 16)
+  int unroll2[2];    // 1: fully unrolled point loops in phase 2, 0: rolled
+  int rule = 3;      // pair / coincident rule: 3 (reference, 12 points) or 5
+                     // (27-point tensor Gauss-Hermite on the relative coordinate)
+  bool operator==(const PhaseConfig& o) const {
+    return k1[0] == o.k1[0] && k1[1] == o.k1[1] && k2[0] == o.k2[0] && k2[1] == o.k2[1] &&
+           unroll2[0] == o.unroll2[0] && unroll2[1] == o.unroll2[1] && rule == o.rule;
+  }
 };
 
 // Evaluates every instance whose prec[] equals the given precision (both

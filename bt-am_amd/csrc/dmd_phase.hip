@@ -518,18 +518,19 @@ static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Tables&
 template <typename P>
 static hipError_t phase1_r(hipStream_t st, const PhaseConfig& cfg, const State& s,
                            const Tables& T, const Outputs& o) {
-  if (cfg.rule == 5) return phase1_p<P, 5>(st, cfg.k1, s, T, o);
-  return phase1_p<P, 3>(st, cfg.k1, s, T, o);
+  const int k1 = cfg.k1[PrecCode<P>::v];
+  if (cfg.rule == 5) return phase1_p<P, 5>(st, k1, s, T, o);
+  return phase1_p<P, 3>(st, k1, s, T, o);
 }
 
 template <typename P>
 static hipError_t phase2_r(hipStream_t st, const PhaseConfig& cfg, const State& s,
                            const Tables& T, const Outputs& o) {
+  const int k2 = cfg.k2[PrecCode<P>::v];
+  const bool un = cfg.unroll2[PrecCode<P>::v] != 0;
   if (cfg.rule == 5)
-    return cfg.unroll2 ? phase2_pu<P, true, 5>(st, cfg.k2, s, T, o)
-                       : phase2_pu<P, false, 5>(st, cfg.k2, s, T, o);
-  return cfg.unroll2 ? phase2_pu<P, true, 3>(st, cfg.k2, s, T, o)
-                     : phase2_pu<P, false, 3>(st, cfg.k2, s, T, o);
+    return un ? phase2_pu<P, true, 5>(st, k2, s, T, o) : phase2_pu<P, false, 5>(st, k2, s, T, o);
+  return un ? phase2_pu<P, true, 3>(st, k2, s, T, o) : phase2_pu<P, false, 3>(st, k2, s, T, o);
 }
 
 hipError_t launch_phase1(hipStream_t st, const PhaseConfig& cfg, int precision, const State& s,

@@ -149,7 +149,7 @@ struct dmdgpu_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   PinnedBuf pin_up, pin_down;
   double rc = kDefaultRc;
-  dmd::PhaseConfig cfg{dmd::kFP64, 8, 8, 0, 3};
+  dmd::PhaseConfig cfg{dmd::kFP64, {16, 16}, {8, 8}, {0, 1}, 3};
   // tables
   bool have[3] = {false, false, false};
   Spline rho[2], embed[2], pair[3], u[3], w[3];
@@ -1163,9 +1163,7 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
                         c->stream));
   HIPCHK(hipMemcpyAsync(c->d_act.p, (char*)c->pin_up.p + ob + pb, ab, hipMemcpyHostToDevice,
                         c->stream));
-  const bool same_cfg = c->graph_cfg.k1 == c->cfg.k1 && c->graph_cfg.k2 == c->cfg.k2 &&
-                        c->graph_cfg.unroll2 == c->cfg.unroll2 &&
-                        c->graph_cfg.rule == c->cfg.rule;
+  const bool same_cfg = c->graph_cfg == c->cfg;
   if (c->use_graph) {
     if (!c->gexec || c->graph_extent != c->extent || c->graph_ninst != ni || !same_cfg) {
       drop_graph(c);
@@ -1345,9 +1343,23 @@ dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
   auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; };
   if (!c || !ok(k1) || !ok(k2))
     return fail(DMDGPU_ERR_ARG, "lanes per site must be 1, 2, 4, 8 or 16");
-  c->cfg.k1 = k1;
-  c->cfg.k2 = k2;
-  c->cfg.unroll2 = unroll2 ? 1 : 0;
+  for (int p = 0; p < 2; p++) {
+    c->cfg.k1[p] = k1;
+    c->cfg.k2[p] = k2;
+    c->cfg.unroll2[p] = unroll2 ? 1 : 0;
+  }
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_set_tuning_precision(dmdgpu_ctx* c, int precision, int k1, int k2,
+                                          int unroll2) {
+  auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; };
+  if (!c || !ok(k1) || !ok(k2) || (precision != DMDGPU_FP64 && precision != DMDGPU_SPLIT32))
+    return fail(DMDGPU_ERR_ARG, "bad tuning");
+  const int p = precision == DMDGPU_SPLIT32 ? dmd::kSplit32 : dmd::kFP64;
+  c->cfg.k1[p] = k1;
+  c->cfg.k2[p] = k2;
+  c->cfg.unroll2[p] = unroll2 ? 1 : 0;
   return DMDGPU_OK;
 }
 

@@ -36,7 +36,8 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_instance_residual", "dmdgpu_instance_state", "dmdgpu_instance_set_unknowns",
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
-           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule"]
+           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule",
+           "dmdgpu_set_tuning_precision"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -140,6 +141,7 @@ def load_library():
     L.dmdgpu_diffusion_neighbors.argtypes = [vp, C.c_int, _ip, _ip]
     L.dmdgpu_jvp.argtypes = [vp, _dp, _dp, _dp]
     L.dmdgpu_set_rule.argtypes = [vp, C.c_int]
+    L.dmdgpu_set_tuning_precision.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -215,6 +217,11 @@ class Context:
         """Pair/coincident quadrature: 3 (the reference's rule) or 5 (27-point
         tensor Gauss-Hermite on the relative coordinate)."""
         self._chk(self.L.dmdgpu_set_rule(self.h, int(order)))
+
+    def set_tuning_precision(self, precision, k1, k2, unroll2):
+        """Tuning of one precision ('fp64' or 'split32') only."""
+        code = {"fp64": FP64, "split32": SPLIT32}[precision]
+        self._chk(self.L.dmdgpu_set_tuning_precision(self.h, code, k1, k2, unroll2))
 
     def set_precision(self, precision):
         """'fp64' or 'split32' (FP32 points, FP64 sums and totals)."""
