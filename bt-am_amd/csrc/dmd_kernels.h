@@ -56,6 +56,7 @@ struct State {
   const int* inst;     // instance of each site, -1 for a free slot
   const Box* boxes;    // per instance
   const int* prec;     // per instance: Precision to evaluate with, -1 = skip
+  const int* noxr;     // per instance: 1 = skip the occupancy residual (nullptr: none)
   const double4* qs;   // (x, y, z, sigma)
   const double* ne;    // effective occupancy: x if active (x >= 5e-4) else 0
   const double* xr;    // raw occupancy

@@ -259,12 +259,10 @@ DEV const float* adj_rec(const Outputs& o, int i, float) { return o.rec32 + (siz
 // p = r_uk + shift; their adjoints are combined before one gradient per point.
 // psi(-p) = flip(psi(p)) (the dipole part is odd, the quadrupole part even).
 // ---------------------------------------------------------------------------
-template <typename P, int K, bool UNR, int RULE>
-__global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
-  int u, g;
-  if (!site_of<P, K>(s, u, g)) return;
-  const int inst = s.inst[u];
-  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+// XR: also form the occupancy residual R_x (instances that relax positions and
+// widths only skip it; their R_x output is left at zero).
+template <typename P, int K, bool UNR, int RULE, bool XR>
+DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst) {
   const Box box = s.boxes[inst];
   const double nu = s.ne[u];
   if (nu == 0.0) {
@@ -349,7 +347,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
       }
       grad_psi(L, p, ir, uu, du, ww, dw, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-      xacc += xb * (double)dot9(lb, ps);
+      if (XR) xacc += xb * (double)dot9(lb, ps);
     };
     // i-shift points of centre u (shift sigma_u) == j-shift points of k
     auto apt = [&](int pt) {
@@ -372,7 +370,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
       const P gl = l == 0 ? g3[0] : (l == 1 ? g3[1] : g3[2]);
       sacc += (double)(sh * gl);
-      xacc += xb * (double)dot9(lbA, ps);
+      if (XR) xacc += xb * (double)dot9(lbA, ps);
     };
     if (UNR) {
 #pragma unroll
@@ -401,7 +399,7 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
       }
       grad_psi(L, p0, ir0, u0, du0, w0, dw0, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-      xacc += xb * (double)dot9(lb, psi0);
+      if (XR) xacc += xb * (double)dot9(lb, psi0);
     }
     // pair-rule points: coincident angular term, pair and density
     {
@@ -433,8 +431,9 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
         g3[0] += cs * p[0]; g3[1] += cs * p[1]; g3[2] += cs * p[2];
         q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
         sacc += (double)(g3[0] * dsu[0] + g3[1] * dsu[1] + g3[2] * dsu[2]);
-        xacc += wt * (xang * (double)dotM(ps, ps) +
-                      (double)(xrho * cval(cu, o2) + xphi * cval(cp, o2)));
+        if (XR)
+          xacc += wt * (xang * (double)dotM(ps, ps) +
+                        (double)(xrho * cval(cu, o2) + xphi * cval(cp, o2)));
       };
       if (UNR) {
 #pragma unroll
@@ -447,7 +446,8 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
   }
   if (K > 1) {
     q0 = group_sum<K>(q0); q1 = group_sum<K>(q1); q2 = group_sum<K>(q2);
-    sacc = group_sum<K>(sacc); xacc = group_sum<K>(xacc);
+    sacc = group_sum<K>(sacc);
+    if (XR) xacc = group_sum<K>(xacc);
   }
   if (g == 0) {
     const double xu = nu;
@@ -467,8 +467,20 @@ __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
     o.S0[u] = -logZ + bu * (1.0 / (2.0 * bu) + V0) - gx;
     o.Rq[3 * u] = q0; o.Rq[3 * u + 1] = q1; o.Rq[3 * u + 2] = q2;
     o.Rs[u] = sacc - 3.0 / su;
-    o.Rx[u] = bu * Ru[kRecDVDN] + xacc + dx;
+    o.Rx[u] = XR ? bu * Ru[kRecDVDN] + xacc + dx : 0.0;
   }
+}
+
+template <typename P, int K, bool UNR, int RULE>
+__global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
+  int u, g;
+  if (!site_of<P, K>(s, u, g)) return;
+  const int inst = s.inst[u];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  if (s.noxr && s.noxr[inst])
+    phase2_site<P, K, UNR, RULE, false>(s, T, o, u, g, inst);
+  else
+    phase2_site<P, K, UNR, RULE, true>(s, T, o, u, g, inst);
 }
 
 // ---------------------------------------------------------------------------

@@ -115,6 +115,10 @@ struct Instance {
   long long pairs = 0;
   int nactive = 0;
   Solve sv;
+  // The residual buffers hold a relaxation tick's evaluation, which leaves
+  // R_x out; rx_prec is the precision to re-evaluate in for a full residual.
+  bool rx_stale = false;
+  int rx_prec = 0;
 };
 
 }  // namespace
@@ -164,7 +168,7 @@ struct dmdgpu_ctx {
   std::vector<Instance> inst;
   int primary = -1;         // instance of the one-instance API
   std::vector<int> h_inst;  // arena site -> instance
-  DevBuf<int> d_inst, d_prec, d_chunk_lo, d_chunk_hi;
+  DevBuf<int> d_inst, d_prec, d_chunk_lo, d_chunk_hi, d_noxr;
   DevBuf<dmd::Box> d_boxes;
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
@@ -173,6 +177,7 @@ struct dmdgpu_ctx {
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals;
   DevBuf<long long> d_pairs;
   std::vector<int> h_prec;
+  std::vector<int> h_noxr;  // per instance, solver ticks: 1 = no occupancy residual needed
   bool meta_dirty = true;
   // solver
   DevBuf<double4> d_X, d_R0, d_R0f, d_D, d_Rr, d_Pp;
@@ -331,6 +336,7 @@ dmd::State make_state(dmdgpu_ctx* c) {
   s.inst = c->d_inst.p;
   s.boxes = c->d_boxes.p;
   s.prec = c->d_prec.p;
+  s.noxr = nullptr;  // every evaluation outside solver ticks forms R_x
   s.qs = c->d_qs.p;
   s.ne = c->d_ne.p;
   s.xr = c->d_xr.p;
@@ -424,6 +430,7 @@ dmdgpu_status upload_meta(dmdgpu_ctx* c) {
   if (c->d_boxes.n < cap) drop_graph(c);
   HIPCHK(c->d_boxes.ensure(cap));
   HIPCHK(c->d_prec.ensure(cap));
+  HIPCHK(c->d_noxr.ensure(cap));
   HIPCHK(c->d_chunk_lo.ensure(cap));
   HIPCHK(c->d_chunk_hi.ensure(cap));
   HIPCHK(c->d_totals.ensure(cap * 2));
@@ -755,7 +762,10 @@ dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
   CHK(upload_prec(c));
   bool any[2] = {false, false};
   for (int k = 0; k < (int)c->inst.size(); k++)
-    if (c->inst[k].used && c->h_prec[k] >= 0) any[c->h_prec[k]] = true;
+    if (c->inst[k].used && c->h_prec[k] >= 0) {
+      any[c->h_prec[k]] = true;
+      if (phase2) c->inst[k].rx_stale = false;
+    }
   const dmd::State s = make_state(c);
   const dmd::Outputs o = make_outputs(c);
   if (any[0] && any[1] && !c->timing) {
@@ -811,7 +821,14 @@ dmdgpu_status d2h_sites(dmdgpu_ctx* c, const Instance& I, double* h, const doubl
 
 dmdgpu_status copy_residual(dmdgpu_ctx* c, int id, double* Rq, double* Rs, double* Rx,
                             double* tot) {
-  const Instance& I = c->inst[id];
+  Instance& I = c->inst[id];
+  if (Rx && I.rx_stale) {
+    // Last evaluated by a relaxation tick: evaluate again, in the same
+    // precision, at the same point, with R_x.
+    std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+    c->h_prec[id] = I.rx_prec;
+    CHK(run_eval(c, true));
+  }
   CHK(d2h_sites(c, I, Rq, c->d_Rq.p, 3));
   CHK(d2h_sites(c, I, Rs, c->d_Rs.p, 1));
   CHK(d2h_sites(c, I, Rx, c->d_Rx.p, 1));
@@ -1092,7 +1109,8 @@ void digest_tick(dmdgpu_ctx* c) {
 // The device work of one tick: vector updates + evaluation points, residual
 // of every instance that needs one (both precisions), totals, dots.
 dmdgpu_status tick_device(dmdgpu_ctx* c) {
-  const dmd::State s = make_state(c);
+  dmd::State s = make_state(c);
+  s.noxr = c->d_noxr.p;  // relaxations need no occupancy residual
   const dmd::Outputs o = make_outputs(c);
   const int ni = (int)c->inst.size();
   HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
@@ -1151,18 +1169,29 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   plan_tick(c);
   const int ni = (int)c->inst.size();
   build_act(c);
+  c->h_noxr.assign(ni, 0);
+  for (int k = 0; k < ni; k++) {
+    Instance& I = c->inst[k];
+    if (!I.used || c->h_prec[k] < 0) continue;
+    c->h_noxr[k] = I.sv.stage != ST_DIFF;
+    I.rx_stale = c->h_noxr[k] != 0;
+    I.rx_prec = c->h_prec[k];
+  }
   const size_t ob = sizeof(dmd::SolveOp) * ni, pb = sizeof(int) * ni;
   const size_t ab = sizeof(int) * c->h_chunks.size();
-  HIPCHK(c->pin_up.ensure(ob + pb + ab));
+  HIPCHK(c->pin_up.ensure(ob + 2 * pb + ab));
   HIPCHK(c->pin_down.ensure(sizeof(double) * 5 * ni));
   std::memcpy(c->pin_up.p, c->h_ops.data(), ob);
   std::memcpy((char*)c->pin_up.p + ob, c->h_prec.data(), pb);
-  std::memcpy((char*)c->pin_up.p + ob + pb, c->h_chunks.data(), ab);
+  std::memcpy((char*)c->pin_up.p + ob + pb, c->h_noxr.data(), pb);
+  std::memcpy((char*)c->pin_up.p + ob + 2 * pb, c->h_chunks.data(), ab);
   HIPCHK(hipMemcpyAsync(c->d_ops.p, c->pin_up.p, ob, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_prec.p, (char*)c->pin_up.p + ob, pb, hipMemcpyHostToDevice,
                         c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_act.p, (char*)c->pin_up.p + ob + pb, ab, hipMemcpyHostToDevice,
+  HIPCHK(hipMemcpyAsync(c->d_noxr.p, (char*)c->pin_up.p + ob + pb, pb, hipMemcpyHostToDevice,
                         c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_act.p, (char*)c->pin_up.p + ob + 2 * pb, ab,
+                        hipMemcpyHostToDevice, c->stream));
   const bool same_cfg = c->graph_cfg == c->cfg;
   if (c->use_graph) {
     if (!c->gexec || c->graph_extent != c->extent || c->graph_ninst != ni || !same_cfg) {
@@ -1279,7 +1308,8 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
                         &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
-                        &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_chunk_lo,
+                        &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_noxr,
+                        &c->d_chunk_lo,
                         &c->d_chunk_hi,   &c->d_dnn,        &c->d_dnbr};
   for (auto* b : ibs) b->release();
   DevBuf<double4>* vbs[] = {&c->d_qs, &c->d_X, &c->d_R0, &c->d_R0f, &c->d_D, &c->d_Rr, &c->d_Pp};
