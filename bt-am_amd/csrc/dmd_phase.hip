@@ -261,6 +261,11 @@ DEV const float* adj_rec(const Outputs& o, int i, float) { return o.rec32 + (siz
 // ---------------------------------------------------------------------------
 // XR: also form the occupancy residual R_x (instances that relax positions and
 // widths only skip it; their R_x output is left at zero).
+// Returns x unchanged, through an empty asm statement the compiler cannot see
+// into, so a value recomputed from it is not merged with an earlier copy.
+DEV double kept_out(double x) { asm volatile("" : "+v"(x)); return x; }
+DEV float kept_out(float x) { asm volatile("" : "+v"(x)); return x; }
+
 template <typename P, int K, bool UNR, int RULE, bool XR>
 DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst) {
   const Box box = s.boxes[inst];
@@ -306,23 +311,28 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     min_image(box, qu, qk, rd);
     PairGeo<P> geo;
     geo.init(rd);
-    P p0[3], ir0, off, psi0[9], Tuk[9], SA[9], tmp[9];
+    P p0[3], ir0, off, tmp[9];
     int m;
     geo.point(T, 0, P(0), p0, m, off, ir0);
     const P* c0 = uwr + ((size_t)m << 3);
     const P u0 = cval(c0, off), du0 = cder(c0, off), w0 = cval(c0 + 4, off), dw0 = cder(c0 + 4, off);
-    psi_from(u0, w0, p0, psi0);
-    // Role-(a) adjoint at the j-shift points (constant over them) and role-(b)
-    // adjoint at the i-shift points (constant over them).
-    P laB[9], lbA[9];
-    load9(Au + kRecLG0, false, laB);
-    applyM(cnk, psi0, tmp);
+    // The two shifted-point walks each hold one constant adjoint and one
+    // accumulator: the j-shift walk the role-(a) adjoint laB and the sum Tuk of
+    // psi over it, the i-shift walk the role-(b) adjoint lbA and the sum SA.
+    // The unshifted point's gradient is linear in its adjoint, so its role-(a)
+    // half is taken after the first walk and its role-(b) half after the
+    // second; psi at the unshifted point is formed again where it is needed
+    // (kept_out hides the earlier copy from CSE so it is not held in registers
+    // across a walk).
+    P laB[9], Tuk[9];
+    {
+      P psi0[9];
+      psi_from(u0, w0, p0, psi0);
+      load9(Au + kRecLG0, false, laB);
+      applyM(cnk, psi0, tmp);
 #pragma unroll
-    for (int c = 0; c < 9; c++) laB[c] = waP * (laB[c] + tmp[c]);
-    load9(Ak + kRecLG0, true, lbA);
-    applyM(cnu, psi0, tmp);
-#pragma unroll
-    for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; Tuk[c] = P(0); SA[c] = P(0); }
+      for (int c = 0; c < 9; c++) { laB[c] = waP * (laB[c] + tmp[c]); Tuk[c] = P(0); }
+    }
     const P skP = (P)sk, suP = (P)su;
 
     // j-shift points of centre u (shift sigma_k) == i-shift points (l,-s) of k
@@ -349,6 +359,32 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
       if (XR) xacc += xb * (double)dot9(lb, ps);
     };
+    if (UNR) {
+#pragma unroll
+      for (int pt = 0; pt < 6; pt++) bpt(pt);
+    } else {
+#pragma unroll 1
+      for (int pt = 0; pt < 6; pt++) bpt(pt);
+    }
+    // unshifted point, role (a): waP * (LU_u - nk/9 M Tuk)
+    {
+      P la[9], g3[3];
+      load9(Au + kRecLU, false, la);
+      applyM(cnk, Tuk, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) la[c] = waP * (la[c] + tmp[c]);
+      grad_psi(la, p0, ir0, u0, du0, w0, dw0, g3);
+      q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
+    }
+    P lbA[9], SA[9];
+    {
+      P psi0[9];
+      psi_from(kept_out(u0), w0, p0, psi0);
+      load9(Ak + kRecLG0, true, lbA);
+      applyM(cnu, psi0, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; SA[c] = P(0); }
+    }
     // i-shift points of centre u (shift sigma_u) == j-shift points of k
     auto apt = [&](int pt) {
       const int l = pt >> 1, sg = pt & 1;
@@ -374,32 +410,28 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     };
     if (UNR) {
 #pragma unroll
-      for (int pt = 0; pt < 6; pt++) bpt(pt);
-#pragma unroll
       for (int pt = 0; pt < 6; pt++) apt(pt);
     } else {
 #pragma unroll 1
-      for (int pt = 0; pt < 6; pt++) bpt(pt);
-#pragma unroll 1
       for (int pt = 0; pt < 6; pt++) apt(pt);
     }
-    // unshifted point
+    // unshifted point, role (b): wbP * (LU_k - nu/9 M SA)
     {
-      P la[9], lb[9], L[9], g3[3];
-      load9(Au + kRecLU, false, la);
-      applyM(cnk, Tuk, tmp);
-#pragma unroll
-      for (int c = 0; c < 9; c++) la[c] += tmp[c];
+      P lb[9], L[9], g3[3];
       load9(Ak + kRecLU, true, lb);
       applyM(cnu, SA, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) {
         lb[c] += tmp[c];
-        L[c] = waP * la[c] + wbP * lb[c];
+        L[c] = wbP * lb[c];
       }
       grad_psi(L, p0, ir0, u0, du0, w0, dw0, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-      if (XR) xacc += xb * (double)dot9(lb, psi0);
+      if (XR) {
+        P psi0[9];
+        psi_from(kept_out(u0), w0, p0, psi0);
+        xacc += xb * (double)dot9(lb, psi0);
+      }
     }
     // pair-rule points: coincident angular term, pair and density
     {
