@@ -256,6 +256,151 @@ DEV double group_sum(double v) {
   return v;
 }
 
+// Neighbour walks.  With cooperative gathers (below) a lane past the end of
+// its site's neighbour row keeps walking while any lane of the wavefront
+// does, since the gathers need every lane of a group: it takes the site
+// itself as the neighbour, with zero occupancy, and this fixed displacement
+// (|r| = 2.4 A, inside every table), so all it forms is finite and enters the
+// sums with weight zero.  Without them each lane stops at its own row's end
+// (the loop condition is divergent, which costs fewer registers).
+template <typename P> DEV bool walk_on(bool more);
+DEV void idle_disp(bool inb, double* r) {
+  if (!inb) { r[0] = 1.1; r[1] = 1.3; r[2] = 1.7; }
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative row gathers.  Each lane needs whole spline rows (8 or 4
+// coefficients) at lane-dependent indices.  A per-lane load touches one cache
+// line per lane and per 16-byte load, and the vector memory pipeline's time
+// grows with the number of distinct lines an instruction touches.  Here the
+// NQ = row bytes / 16 lanes of an aligned group load the NQ chunks of one
+// member's row with one instruction, member by member (NQ rounds: 64/NQ lines
+// per instruction instead of 64), and the chunks pass through a per-wavefront
+// LDS area to the lane that owns the row.  FP64 rows (64 and 32 B) go this
+// way; FP32 rows (32 and 16 B) gain at most a factor two in lines and are
+// loaded directly (measured faster on MI355X).
+//
+// Every lane of a group must execute the gather (the phase kernels walk the
+// neighbour rows in wavefront-uniform loops, and whole groups leave early).
+// Wavefront-scope fences keep the LDS writes before the reads and the reads
+// before the next gather's writes; they emit no instructions.  All global
+// loads of a gather are issued before its first fence.
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kStageSlots = 65;  // 16-byte slots per round: 64 lanes + 1 (LDS bank skew)
+template <typename P> struct Coop { static constexpr bool v = sizeof(P) == 8; };
+// 16-byte chunks per row of W values; stage rounds a gather of N8 8-value and
+// N4 4-value rows uses.
+template <typename P, int W> struct RowQ { static constexpr int v = (int)(W * sizeof(P) / 16); };
+template <typename P, int N8, int N4> struct StageRounds {
+  static constexpr int v = Coop<P>::v ? N8 * RowQ<P, 8>::v + N4 * RowQ<P, 4>::v : 1;
+};
+
+// Value of v at lane R of this lane's group of NQ (quad permutation).
+template <int NQ, int R>
+DEV int group_bcast(int v) {
+  constexpr int ctrl = NQ == 4 ? R * 0x55 : (R | (R << 2) | ((2 + R) << 4) | ((2 + R) << 6));
+  return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xf, 0xf, false);
+}
+
+template <int NQ>
+DEV void coop_rounds(const u32x4* base, int row, int a, u32x4* v) {
+  if constexpr (NQ == 1) {
+    v[0] = base[row];
+  } else if constexpr (NQ == 4) {
+    v[0] = base[(size_t)group_bcast<4, 0>(row) * 4 + a];
+    v[1] = base[(size_t)group_bcast<4, 1>(row) * 4 + a];
+    v[2] = base[(size_t)group_bcast<4, 2>(row) * 4 + a];
+    v[3] = base[(size_t)group_bcast<4, 3>(row) * 4 + a];
+  } else {
+    v[0] = base[(size_t)group_bcast<2, 0>(row) * 2 + a];
+    v[1] = base[(size_t)group_bcast<2, 1>(row) * 2 + a];
+  }
+}
+
+// o8[q] = row r8[q] of t8 (rows of 8 values), q < N8, and o4[q] = row r4[q] of
+// t4[q] (rows of 4 values), q < N4, with one LDS round trip.  stage: this
+// wavefront's LDS area, StageRounds<P, N8, N4> * kStageSlots slots.
+template <typename P, int N8, int N4>
+DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const int* r4,
+                    u32x4* stage, P (*o8)[8], P (*o4)[4]) {
+  if constexpr (!Coop<P>::v) {
+#pragma unroll
+    for (int q = 0; q < N8; q++) {
+      const P* c = t8[q] + ((size_t)r8[q] << 3);
+#pragma unroll
+      for (int e = 0; e < 8; e++) o8[q][e] = c[e];
+    }
+#pragma unroll
+    for (int q = 0; q < N4; q++) {
+      const P* c = t4[q] + ((size_t)r4[q] << 2);
+#pragma unroll
+      for (int e = 0; e < 4; e++) o4[q][e] = c[e];
+    }
+  } else {
+    constexpr int Q8 = RowQ<P, 8>::v, Q4 = RowQ<P, 4>::v;
+    const int lane = (int)__lane_id();
+    const int a8 = lane & (Q8 - 1), a4 = lane & (Q4 - 1);
+    u32x4 v8[N8 > 0 ? N8 : 1][Q8], v4[N4 > 0 ? N4 : 1][Q4];
+#pragma unroll
+    for (int q = 0; q < N8; q++)
+      coop_rounds<Q8>(reinterpret_cast<const u32x4*>(t8[q]), r8[q], a8, v8[q]);
+#pragma unroll
+    for (int q = 0; q < N4; q++)
+      coop_rounds<Q4>(reinterpret_cast<const u32x4*>(t4[q]), r4[q], a4, v4[q]);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int q = 0; q < N8; q++)
+#pragma unroll
+      for (int r = 0; r < Q8; r++) stage[(q * Q8 + r) * kStageSlots + lane] = v8[q][r];
+#pragma unroll
+    for (int q = 0; q < N4; q++)
+#pragma unroll
+      for (int r = 0; r < Q4; r++)
+        stage[(N8 * Q8 + q * Q4 + r) * kStageSlots + lane] = v4[q][r];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int q = 0; q < N8; q++) {
+      u32x4 w[Q8];
+#pragma unroll
+      for (int c = 0; c < Q8; c++) w[c] = stage[(q * Q8 + a8) * kStageSlots + (lane - a8) + c];
+      __builtin_memcpy(o8[q], w, sizeof(w));
+    }
+#pragma unroll
+    for (int q = 0; q < N4; q++) {
+      u32x4 w[Q4];
+#pragma unroll
+      for (int c = 0; c < Q4; c++)
+        w[c] = stage[(N8 * Q8 + q * Q4 + a4) * kStageSlots + (lane - a4) + c];
+      __builtin_memcpy(o4[q], w, sizeof(w));
+    }
+  }
+}
+
+template <typename P, int NR>
+DEV void gather_rows8(const P* tab, const int* rows, u32x4* stage, P (*out)[8]) {
+  const P* t8[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) t8[q] = tab;
+  gather_mix<P, NR, 0>(t8, rows, nullptr, nullptr, stage, out, nullptr);
+}
+
+template <typename P>
+DEV void gather_row8(const P* tab, int row, u32x4* stage, P* out) {
+  gather_rows8<P, 1>(tab, &row, stage, reinterpret_cast<P(*)[8]>(out));
+}
+
+template <typename P> DEV bool walk_on(bool more) {
+  if constexpr (Coop<P>::v) return __any(more);
+  else return more;
+}
+
+// A 4-value row into registers (issued ahead of a gather so both overlap).
+template <typename P>
+DEV void load4(const P* c, P* o) {
+  o[0] = c[0]; o[1] = c[1]; o[2] = c[2]; o[3] = c[3];
+}
+
 // XCD-aware block order: the hardware deals consecutive workgroups to the 8
 // XCDs round-robin; remap so each XCD walks one contiguous range of sites and
 // the neighbour records it gathers stay in its own L2.

@@ -8,14 +8,24 @@ namespace dmd {
 // Species slots: 0 = Mg, 1 = H.  Pair-table slots: 0 = MgMg, 1 = HH, 2 = MgH.
 constexpr int kCap = 256;          // neighbour row stride (reference maxneigh = 250)
 constexpr int kMaxNeigh = 250;     // src/Macros.hpp:862
-constexpr int kRec = 80;           // doubles per phase-1 record
-constexpr int kRecLA = 0;          // 6 x 9 : (1/9) M G_ls     (i-shift sums)
-constexpr int kRecLU = 54;         // 9     : (1/9) M U        (j-shift sums)
-constexpr int kRecLG0 = 63;        // 9     : (1/9) M G0       (unshifted sums)
+// Phase-1 record of a site: eight 9-component adjoint blocks
+//   blocks 0..5 : (1/9) M G_ls   (i-shift sums, block = 2 l + s)
+//   block 6     : (1/9) M U      (j-shift sums)
+//   block 7     : (1/9) M G0     (unshifted sums)
+// stored as aligned 8-value rows (components 0..7 of block b at 8 b) and one
+// tail row (component 8 of block b at kRecTail + b), then three scalars.  A
+// block is one aligned row plus one value, so gathers of it touch few lines.
+constexpr int kRec = 80;           // doubles per phase-1 record (10 rows of 8)
+constexpr int kRecRows = kRec / 8;
+constexpr int kBlkLU = 6;
+constexpr int kBlkLG0 = 7;
+constexpr int kRecTail = 64;
 constexpr int kRecFP = 72;         // F'(rho_bar)
 constexpr int kRecDVDN = 73;       // dV0_i / dx_i (own occupancy)
 constexpr int kRecRHO = 74;        // rho_bar
-constexpr int kRec32 = 72;         // floats per FP32 copy of LA, LU, LG0 (split path)
+constexpr int kRec32 = 72;         // floats per FP32 copy of the eight blocks (split path)
+// Slot of component c of block b in a record (FP64 or FP32 copy).
+__host__ __device__ constexpr int rec_slot(int b, int c) { return c < 8 ? 8 * b + c : kRecTail + b; }
 
 // Spline tables resident in HBM (L2-resident in practice: ~0.8 MB for the
 // reference tables).  All r-tables share one (nr, dx) grid.  Coefficient rows

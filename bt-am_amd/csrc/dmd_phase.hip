@@ -38,11 +38,12 @@ namespace dmd {
 //   rho_bar, pair sum, coincident and self terms on the sqrt(6) sigma points.
 // Writes V0_i, rho_bar_i and the adjoint record of site i.
 //
-// The neighbour row is walked seven times: once per i-shift point (l, s),
-// accumulating only that G_ls (9 values), and once for the other 19 points of
-// each pair.  Re-walking the row (L1/L2-resident neighbour data plus the pair
-// geometry) costs far less than keeping the 54 G_ls accumulators live, which
-// would leave room for a single wavefront per SIMD.
+// The neighbour row is walked 6 / kShiftsPerWalk times for the i-shift points
+// (l, s), each walk accumulating only its kShiftsPerWalk G_ls (9 values each),
+// and once more for the other 19 points of each pair.  Re-walking the row
+// (L1/L2-resident neighbour data plus the pair geometry) costs far less than
+// keeping all 54 G_ls accumulators live, which would leave room for a single
+// wavefront per SIMD.
 // ---------------------------------------------------------------------------
 template <typename P> struct PrecCode;
 template <> struct PrecCode<double> { static constexpr int v = kFP64; };
@@ -66,8 +67,12 @@ DEV bool site_of(const State& s, int& i, int& g) {
 // fewer walks recompute the pair geometry less often, more accumulators live.
 constexpr int kShiftsPerWalk = 3;
 
+
 template <typename P, int K, int RULE>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
+  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
+  __shared__ u32x4 stage_all[4][StageRounds<P, kShiftsPerWalk, 0>::v * kStageSlots];
+  u32x4* stage = stage_all[threadIdx.x >> 6];
   int i, g;
   if (!site_of<P, K>(s, i, g)) return;
   const int inst = s.inst[i];
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   float* rec32 = o.rec32 + (size_t)i * kRec32;
   constexpr bool kF32 = PrecCode<P>::v == kSplit32;
   const double ni = s.ne[i];
-  if (ni == 0.0) {
+  if (ni == 0.0) {  // all K lanes of the site leave together
     for (int c = g; c < kRec; c += K) rec[c] = 0.0;
     if (kF32)
       for (int c = g; c < kRec32; c += K) rec32[c] = 0.0f;
@@ -104,28 +109,52 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     for (int b = 0; b < kShiftsPerWalk; b++)
 #pragma unroll
       for (int c = 0; c < 9; c++) G[b][c] = 0;
-    for (int kk = g; kk < nn; kk += K) {
-      const int j = row[kk];
-      const double nj = s.ne[j];
-      if (nj == 0.0) continue;
+    for (int kk = g; walk_on<P>(kk < nn); kk += K) {
+      const bool inb = kk < nn;
+      const int j = inb ? row[kk] : i;
+      const double nj = inb ? s.ne[j] : 0.0;
       const double4 qj = s.qs[j];
-      const P* uwr = uwt + ((size_t)pair_slot(spi, s.spc[j]) * nr << 3);
+      const int rb = pair_slot(spi, s.spc[j]) * nr;
       double rd[3];
       min_image(box, qi, qj, rd);
+      idle_disp(inb, rd);
       PairGeo<P> geo;
       geo.init(rd);
+      if constexpr (Coop<P>::v) {  // the walk's rows in one gather
+        P p[kShiftsPerWalk][3], off[kShiftsPerWalk], cw[kShiftsPerWalk][8];
+        int rows[kShiftsPerWalk];
 #pragma unroll
-      for (int b = 0; b < kShiftsPerWalk; b++) {
-        const int ls = ls0 + b;
-        const P sh = (ls & 1 ? P(-3) : P(3)) * si;
-        P p[3], off, ir, ps[9];
-        int m;
-        geo.point(T, ls >> 1, sh, p, m, off, ir);
-        const P* c = uwr + ((size_t)m << 3);
-        psi_from(cval(c, off), cval(c + 4, off), p, ps);
+        for (int b = 0; b < kShiftsPerWalk; b++) {
+          const int ls = ls0 + b;
+          const P sh = (ls & 1 ? P(-3) : P(3)) * si;
+          P ir;
+          int m;
+          geo.point(T, ls >> 1, sh, p[b], m, off[b], ir);
+          rows[b] = rb + m;
+        }
+        gather_rows8<P, kShiftsPerWalk>(uwt, rows, stage, cw);
 #pragma unroll
-        for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
-        selfA += nj * nj * (double)dotM(ps, ps);
+        for (int b = 0; b < kShiftsPerWalk; b++) {
+          P ps[9];
+          psi_from(cval(cw[b], off[b]), cval(cw[b] + 4, off[b]), p[b], ps);
+#pragma unroll
+          for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
+          selfA += nj * nj * (double)dotM(ps, ps);
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < kShiftsPerWalk; b++) {
+          const int ls = ls0 + b;
+          const P sh = (ls & 1 ? P(-3) : P(3)) * si;
+          P p[3], off, ir, ps[9];
+          int m;
+          geo.point(T, ls >> 1, sh, p, m, off, ir);
+          const P* c = uwt + ((size_t)(rb + m) << 3);
+          psi_from(cval(c, off), cval(c + 4, off), p, ps);
+#pragma unroll
+          for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
+          selfA += nj * nj * (double)dotM(ps, ps);
+        }
       }
     }
 #pragma unroll
@@ -140,43 +169,44 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll
       for (int c = 0; c < 9; c++)
         if (((ls * 9 + c) % K) == g) {
-          if (kF32) rec32[kRecLA + ls * 9 + c] = (float)tmp[c];
-          else rec[kRecLA + ls * 9 + c] = tmp[c];
+          if (kF32) rec32[rec_slot(ls, c)] = (float)tmp[c];
+          else rec[rec_slot(ls, c)] = tmp[c];
         }
     }
   }
 
-  // unshifted, j-shift and sqrt(6) sigma points
+  // unshifted, j-shift and pair-rule points
   double rho = 0, Ep = 0, Cc = 0, TPsi = 0;
   double G0[9], U[9];
 #pragma unroll
   for (int c = 0; c < 9; c++) { G0[c] = 0; U[c] = 0; }
-  for (int kk = g; kk < nn; kk += K) {
-    const int j = row[kk];
-    const double nj = s.ne[j];
-    if (nj == 0.0) continue;
+  for (int kk = g; walk_on<P>(kk < nn); kk += K) {
+    const bool inb = kk < nn;
+    const int j = inb ? row[kk] : i;
+    const double nj = inb ? s.ne[j] : 0.0;
     const double4 qj = s.qs[j];
     const P sj = (P)qj.w;
     const int spj = s.spc[j];
     const int t = pair_slot(spi, spj);
-    const P* uwr = uwt + ((size_t)t * nr << 3);
+    const int rb = t * nr;
     double rd[3];
     min_image(box, qi, qj, rd);
+    idle_disp(inb, rd);
     PairGeo<P> geo;
     geo.init(rd);
-    P p[3], off, ir, psi0[9], ps[9], Tj[9];
+    P p[3], off, ir, psi0[9], ps[9], Tj[9], cw[8];
     int m;
     geo.point(T, 0, P(0), p, m, off, ir);
-    psi_from(cval(uwr + ((size_t)m << 3), off), cval(uwr + ((size_t)m << 3) + 4, off), p, psi0);
+    gather_row8<P>(uwt, rb + m, stage, cw);
+    psi_from(cval(cw, off), cval(cw + 4, off), p, psi0);
 #pragma unroll
     for (int c = 0; c < 9; c++) { G0[c] += nj * (double)psi0[c]; Tj[c] = P(0); }
     const double nj2 = nj * nj;
 #pragma unroll 1
     for (int pt = 0; pt < 6; pt++) {
-      const int l = pt >> 1;
-      geo.point(T, l, (pt & 1 ? P(-3) : P(3)) * sj, p, m, off, ir);
-      const P* cb = uwr + ((size_t)m << 3);
-      psi_from(cval(cb, off), cval(cb + 4, off), p, ps);
+      geo.point(T, pt >> 1, (pt & 1 ? P(-3) : P(3)) * sj, p, m, off, ir);
+      gather_row8<P>(uwt, rb + m, stage, cw);
+      psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
 #pragma unroll
       for (int e = 0; e < 9; e++) Tj[e] += ps[e];
     }
@@ -184,18 +214,18 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
     TPsi += nj2 * (double)dotM(Tj, psi0);
     double Phi = 0, R = 0, C6 = 0;  // per-pair sums over the rule's points in FP64
-    const P* phr = pht + ((size_t)t * nr << 2);
-    const P* rhr = rht + ((size_t)spj * nr << 2);
     // pair and coincident points of the pair rule
 #pragma unroll 1
     for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      P dsi[3];
+      P dsi[3], c4[2][4];
       const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi);
-      const P* c = uwr + ((size_t)m << 3);
-      psi_from(cval(c, off), cval(c + 4, off), p, ps);
+      const int r8 = rb + m, r4[2] = {t * nr + m, spj * nr + m};
+      const P* t4[2] = {pht, rht};
+      gather_mix<P, 1, 2>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
+      psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
       C6 += wt * (double)dotM(ps, ps);
-      Phi += wt * (double)cval(phr + ((size_t)m << 2), off);
-      R += wt * (double)cval(rhr + ((size_t)m << 2), off);
+      Phi += wt * (double)cval(c4[0], off);
+      R += wt * (double)cval(c4[1], off);
     }
     Cc += nj2 * C6;
     rho += nj * R;
@@ -218,16 +248,16 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   applyM(1.0 / 9.0, U, tmp);
 #pragma unroll
   for (int c = 0; c < 9; c++)
-    if (((kRecLU + c) % K) == g) {
-      if (kF32) rec32[kRecLU + c] = (float)tmp[c];
-      else rec[kRecLU + c] = tmp[c];
+    if (((kBlkLU * 9 + c) % K) == g) {
+      if (kF32) rec32[rec_slot(kBlkLU, c)] = (float)tmp[c];
+      else rec[rec_slot(kBlkLU, c)] = tmp[c];
     }
   applyM(1.0 / 9.0, G0, tmp);
 #pragma unroll
   for (int c = 0; c < 9; c++)
-    if (((kRecLG0 + c) % K) == g) {
-      if (kF32) rec32[kRecLG0 + c] = (float)tmp[c];
-      else rec[kRecLG0 + c] = tmp[c];
+    if (((kBlkLG0 * 9 + c) % K) == g) {
+      if (kF32) rec32[rec_slot(kBlkLG0, c)] = (float)tmp[c];
+      else rec[rec_slot(kBlkLG0, c)] = tmp[c];
     }
   if (g == 0) {
     rec[kRecFP] = Fp;
@@ -238,19 +268,47 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   }
 }
 
-// Loads 9 record values as P, negating the dipole part when flip is set.
+// Block b of a record as P, negating the dipole part when flip is set.
 template <typename S, typename P>
-DEV void load9(const S* r, bool flip, P* o) {
+DEV void load_blk(const S* r, int b, bool flip, P* o) {
+  const S* m = r + 8 * b;
   const P sgn = flip ? P(-1) : P(1);
-  o[0] = sgn * (P)r[0]; o[1] = sgn * (P)r[1]; o[2] = sgn * (P)r[2];
+  o[0] = sgn * (P)m[0]; o[1] = sgn * (P)m[1]; o[2] = sgn * (P)m[2];
 #pragma unroll
-  for (int c = 3; c < 9; c++) o[c] = (P)r[c];
+  for (int c = 3; c < 8; c++) o[c] = (P)m[c];
+  o[8] = (P)r[kRecTail + b];
 }
 
-// The adjoint blocks (LA, LU, LG0) of site i in the precision phase 2 reads:
-// the FP64 record for FP64, the FP32 copy for the split path.
+// The adjoint blocks of site i in the precision phase 2 reads: the FP64 record
+// for FP64, the FP32 copy for the split path (adj_base + adj_rows: the same in
+// 8-value rows).
 DEV const double* adj_rec(const Outputs& o, int i, double) { return o.rec + (size_t)i * kRec; }
 DEV const float* adj_rec(const Outputs& o, int i, float) { return o.rec32 + (size_t)i * kRec32; }
+DEV const double* adj_base(const Outputs& o, double) { return o.rec; }
+DEV const float* adj_base(const Outputs& o, float) { return o.rec32; }
+template <typename P> constexpr int adj_rows() { return sizeof(P) == 8 ? kRec / 8 : kRec32 / 8; }
+
+// The 9 components of a block from its 8-value row and its tail value.
+template <typename P>
+DEV void blk_from(const P* row, P tail, bool flip, P* o) {
+  const P sgn = flip ? P(-1) : P(1);
+  o[0] = sgn * row[0]; o[1] = sgn * row[1]; o[2] = sgn * row[2];
+#pragma unroll
+  for (int c = 3; c < 8; c++) o[c] = row[c];
+  o[8] = tail;
+}
+
+// Block b of neighbour k's record (k differs between lanes): a cooperative
+// gather of its row plus the tail value.
+template <typename P>
+DEV void nbr_blk(const Outputs& o, int k, int b, bool flip, u32x4* stage, P* out) {
+  const P* base = adj_base(o, P(0));
+  const P tail = adj_rec(o, k, P(0))[kRecTail + b];
+  const int r = k * adj_rows<P>() + b;
+  P row[1][8];
+  gather_mix<P, 1, 0>(&base, &r, nullptr, nullptr, stage, row, nullptr);
+  blk_from(row[0], tail, flip, out);
+}
 
 // ---------------------------------------------------------------------------
 // Phase 2: gradient of sum_c beta_c V0_c with respect to (q_u, sigma_u, x_u).
@@ -267,10 +325,11 @@ DEV double kept_out(double x) { asm volatile("" : "+v"(x)); return x; }
 DEV float kept_out(float x) { asm volatile("" : "+v"(x)); return x; }
 
 template <typename P, int K, bool UNR, int RULE, bool XR>
-DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst) {
+DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst,
+                     u32x4* stage) {
   const Box box = s.boxes[inst];
   const double nu = s.ne[u];
-  if (nu == 0.0) {
+  if (nu == 0.0) {  // all K lanes of the site leave together
     if (g == 0) {
       o.Rq[3 * u] = 0.0; o.Rq[3 * u + 1] = 0.0; o.Rq[3 * u + 2] = 0.0;
       o.Rs[u] = 0.0; o.Rx[u] = 0.0; o.S0[u] = 0.0;
@@ -291,10 +350,10 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
   double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
   const int nn = s.nnb[u];
   const int* row = s.nbr + (size_t)u * kCap;
-  for (int kk = g; kk < nn; kk += K) {
-    const int k = row[kk];
-    const double nk = s.ne[k];
-    if (nk == 0.0) continue;
+  for (int kk = g; walk_on<P>(kk < nn); kk += K) {
+    const bool inb = kk < nn;
+    const int k = inb ? row[kk] : u;
+    const double nk = inb ? s.ne[k] : 0.0;  // zero: every weight below vanishes
     const double4 qk = s.qs[k];
     const double sk = qk.w;
     const int spk = s.spc[k];
@@ -306,15 +365,17 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     const P waP = (P)wa, wbP = (P)wb;
     const P cnk = P(-nk * (1.0 / 9.0)), cnu = P(-nu * (1.0 / 9.0));
     const int t = pair_slot(spu, spk);
-    const P* uwr = uwt + ((size_t)t * nr << 3);
+    const int rb = t * nr;
     double rd[3];
     min_image(box, qu, qk, rd);
+    idle_disp(inb, rd);
     PairGeo<P> geo;
     geo.init(rd);
     P p0[3], ir0, off, tmp[9];
     int m;
     geo.point(T, 0, P(0), p0, m, off, ir0);
-    const P* c0 = uwr + ((size_t)m << 3);
+    P c0[8];
+    gather_row8<P>(uwt, rb + m, stage, c0);
     const P u0 = cval(c0, off), du0 = cder(c0, off), w0 = cval(c0 + 4, off), dw0 = cder(c0 + 4, off);
     // The two shifted-point walks each hold one constant adjoint and one
     // accumulator: the j-shift walk the role-(a) adjoint laB and the sum Tuk of
@@ -328,7 +389,7 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     {
       P psi0[9];
       psi_from(u0, w0, p0, psi0);
-      load9(Au + kRecLG0, false, laB);
+      load_blk(Au, kBlkLG0, false, laB);
       applyM(cnk, psi0, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) { laB[c] = waP * (laB[c] + tmp[c]); Tuk[c] = P(0); }
@@ -339,15 +400,20 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     auto bpt = [&](int pt) {
       const int l = pt >> 1, sg = pt & 1;
       const P sh = sg ? P(-3) : P(3);
-      P p[3], ir, o2, ps[9], lb[9], L[9], g3[3];
+      P p[3], ir, o2, ps[9], lb[9], L[9], g3[3], c[2][8];
       int mm;
       geo.point(T, l, sh * skP, p, mm, o2, ir);
-      const P* c = uwr + ((size_t)mm << 3);
-      const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
+      const int blk = 2 * l + (1 - sg);
+      const P tail = Ak[kRecTail + blk];
+      const int r8[2] = {rb + mm, k * adj_rows<P>() + blk};
+      const P* t8[2] = {uwt, adj_base(o, P(0))};
+      gather_mix<P, 2, 0>(t8, r8, nullptr, nullptr, stage, c, nullptr);
+      blk_from(c[1], tail, true, lb);
+      const P uu = cval(c[0], o2), du = cder(c[0], o2), ww = cval(c[0] + 4, o2),
+              dw = cder(c[0] + 4, o2);
       psi_from(uu, ww, p, ps);
 #pragma unroll
       for (int e = 0; e < 9; e++) Tuk[e] += ps[e];
-      load9(Ak + kRecLA + 9 * (2 * l + (1 - sg)), true, lb);
       P mps[9];
       applyM(cnu, ps, mps);
 #pragma unroll
@@ -369,7 +435,7 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     // unshifted point, role (a): waP * (LU_u - nk/9 M Tuk)
     {
       P la[9], g3[3];
-      load9(Au + kRecLU, false, la);
+      load_blk(Au, kBlkLU, false, la);
       applyM(cnk, Tuk, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) la[c] = waP * (la[c] + tmp[c]);
@@ -380,7 +446,7 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     {
       P psi0[9];
       psi_from(kept_out(u0), w0, p0, psi0);
-      load9(Ak + kRecLG0, true, lbA);
+      nbr_blk<P>(o, k, kBlkLG0, true, stage, lbA);
       applyM(cnu, psi0, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; SA[c] = P(0); }
@@ -389,15 +455,15 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     auto apt = [&](int pt) {
       const int l = pt >> 1, sg = pt & 1;
       const P sh = sg ? P(-3) : P(3);
-      P p[3], ir, o2, ps[9], la[9], L[9], g3[3];
+      P p[3], ir, o2, ps[9], la[9], L[9], g3[3], c[8];
       int mm;
       geo.point(T, l, sh * suP, p, mm, o2, ir);
-      const P* c = uwr + ((size_t)mm << 3);
+      load_blk(Au, 2 * l + sg, false, la);
+      gather_row8<P>(uwt, rb + mm, stage, c);
       const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
       psi_from(uu, ww, p, ps);
 #pragma unroll
       for (int e = 0; e < 9; e++) SA[e] += ps[e];
-      load9(Au + kRecLA + 9 * (2 * l + sg), false, la);
       P mps[9];
       applyM(cnk, ps, mps);
 #pragma unroll
@@ -418,7 +484,7 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     // unshifted point, role (b): wbP * (LU_k - nu/9 M SA)
     {
       P lb[9], L[9], g3[3];
-      load9(Ak + kRecLU, true, lb);
+      nbr_blk<P>(o, k, kBlkLU, true, stage, lb);
       applyM(cnu, SA, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) {
@@ -443,19 +509,18 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       const double xang = 2.0 * xb * nu;
       const P xrho = (P)(bk * nk * FPk);
       const P xphi = (P)(bk * 0.5 * nk);
-      const P* phr = pht + ((size_t)t * nr << 2);
-      const P* rkr = rht + ((size_t)spk * nr << 2);
-      const P* rur = rht + ((size_t)spu * nr << 2);
       auto cpt = [&](int pt) {
-        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3];
+        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3], c[8], c4[3][4];
         int mm;
         const double wt = PairRule<RULE>::point(geo, T, pt, suP, skP, p, mm, o2, ir, dsu);
         const P wtP = (P)wt;
-        const P* c = uwr + ((size_t)mm << 3);
+        const int r8 = rb + mm, r4[3] = {t * nr + mm, spk * nr + mm, spu * nr + mm};
+        const P* t4[3] = {pht, rht, rht};
+        gather_mix<P, 1, 3>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(c), c4);
+        const P* cp = c4[0];
+        const P* ck = c4[1];
+        const P* cu = c4[2];
         const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
-        const P* cp = phr + ((size_t)mm << 2);
-        const P* ck = rkr + ((size_t)mm << 2);
-        const P* cu = rur + ((size_t)mm << 2);
         psi_from(uu, ww, p, ps);
         applyM(wtP * cang, ps, L);
         grad_psi(L, p, ir, uu, du, ww, dw, g3);
@@ -505,14 +570,21 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
 
 template <typename P, int K, bool UNR, int RULE>
 __global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
+  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
+  // largest gathers: two 8-value rows (j-shift points), one 8- and three
+  // 4-value rows (pair-rule points)
+  constexpr int kRounds = StageRounds<P, 2, 0>::v > StageRounds<P, 1, 3>::v
+                              ? StageRounds<P, 2, 0>::v : StageRounds<P, 1, 3>::v;
+  __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
+  u32x4* stage = stage_all[threadIdx.x >> 6];
   int u, g;
   if (!site_of<P, K>(s, u, g)) return;
   const int inst = s.inst[u];
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
   if (s.noxr && s.noxr[inst])
-    phase2_site<P, K, UNR, RULE, false>(s, T, o, u, g, inst);
+    phase2_site<P, K, UNR, RULE, false>(s, T, o, u, g, inst, stage);
   else
-    phase2_site<P, K, UNR, RULE, true>(s, T, o, u, g, inst);
+    phase2_site<P, K, UNR, RULE, true>(s, T, o, u, g, inst, stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -535,8 +607,6 @@ template <typename P, int RULE>
 static hipError_t phase1_p(hipStream_t st, int K, const State& s, const Tables& T,
                            const Outputs& o) {
   switch (K) {
-    case 1: p1<P, 1, RULE>(st, s, T, o); break;
-    case 2: p1<P, 2, RULE>(st, s, T, o); break;
     case 4: p1<P, 4, RULE>(st, s, T, o); break;
     case 8: p1<P, 8, RULE>(st, s, T, o); break;
     case 16: p1<P, 16, RULE>(st, s, T, o); break;
@@ -549,8 +619,6 @@ template <typename P, bool UNR, int RULE>
 static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Tables& T,
                             const Outputs& o) {
   switch (K) {
-    case 1: p2<P, 1, UNR, RULE>(st, s, T, o); break;
-    case 2: p2<P, 2, UNR, RULE>(st, s, T, o); break;
     case 4: p2<P, 4, UNR, RULE>(st, s, T, o); break;
     case 8: p2<P, 8, UNR, RULE>(st, s, T, o); break;
     case 16: p2<P, 16, UNR, RULE>(st, s, T, o); break;

@@ -1370,9 +1370,9 @@ dmdgpu_status dmdgpu_set_cutoff(dmdgpu_ctx* c, double rc) {
 }
 
 dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
-  auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; };
+  auto ok = [](int k) { return k == 4 || k == 8 || k == 16; };
   if (!c || !ok(k1) || !ok(k2))
-    return fail(DMDGPU_ERR_ARG, "lanes per site must be 1, 2, 4, 8 or 16");
+    return fail(DMDGPU_ERR_ARG, "lanes per site must be 4, 8 or 16");
   for (int p = 0; p < 2; p++) {
     c->cfg.k1[p] = k1;
     c->cfg.k2[p] = k2;
@@ -1383,7 +1383,7 @@ dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
 
 dmdgpu_status dmdgpu_set_tuning_precision(dmdgpu_ctx* c, int precision, int k1, int k2,
                                           int unroll2) {
-  auto ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; };
+  auto ok = [](int k) { return k == 4 || k == 8 || k == 16; };
   if (!c || !ok(k1) || !ok(k2) || (precision != DMDGPU_FP64 && precision != DMDGPU_SPLIT32))
     return fail(DMDGPU_ERR_ARG, "bad tuning");
   const int p = precision == DMDGPU_SPLIT32 ? dmd::kSplit32 : dmd::kFP64;

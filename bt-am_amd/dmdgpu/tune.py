@@ -6,7 +6,7 @@ candidate is timed with the context's per-kernel HIP events on the current
 state and lists; phase 1 and phase 2 are independent kernels, so each picks its
 own best setting.
 """
-LANES = (1, 2, 4, 8, 16)
+LANES = (4, 8, 16)
 
 
 def autotune(ctx, precision="fp64", reps=3, lanes=LANES, unrolls=(0, 1)):

@@ -139,7 +139,7 @@ dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
 dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* ctx, const double* h_X, const double* h_V, double* h_JV);
 
 /* Kernel tuning: lanes of a wavefront that share one site in the phase-1 and
- * phase-2 kernels (1, 2, 4, 8 or 16) and whether phase 2 unrolls its point loops,
+ * phase-2 kernels (4, 8 or 16) and whether phase 2 unrolls its point loops,
  * for both precisions.
  * The tuner picks them per workload.  No counterpart in the reference. */
 dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* ctx, int k_phase1, int k_phase2, int unroll2);

@@ -76,7 +76,7 @@ def test_residual_parity_fp64(gpu, oracle, case):
     assert rel(S0, ref["S0"]) < TOL
 
 
-@pytest.mark.parametrize("k1,k2,un", [(1, 1, 1), (2, 8, 0), (8, 2, 1), (4, 4, 0)])
+@pytest.mark.parametrize("k1,k2,un", [(4, 16, 1), (16, 4, 0), (8, 8, 1), (4, 4, 0)])
 def test_tuning_variants_agree(gpu, oracle, k1, k2, un):
     say(f"[gpu] tuning {k1} {k2} {un}")
     sysd = CASES["hcp_mgh_5x3x3"]()
