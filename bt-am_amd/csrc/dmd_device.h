@@ -141,7 +141,8 @@ template <> struct PairGeo<float> {
 // ---------------------------------------------------------------------------
 // Pair (and coincident-triplet) rules.  Point t of pair (i, j) lies at
 // p = r_ij + delta_t with weight w_t (returned in FP64 for both precisions, so
-// the weights sum to 1 to FP64 accuracy); dsi_t = d delta_t / d sigma_i.
+// the weights sum to 1 to FP64 accuracy); dsi_t = d delta_t / d sigma_i and
+// dsj_t = d delta_t / d sigma_j.
 //   PairRule<3>: the reference's 6D Stroud rule -- sqrt(6) sigma shifts of
 //                site i (t < 6) or site j along one axis, weight 1/12.
 //   PairRule<5>: 3-point Gauss-Hermite per axis on the relative coordinate
@@ -154,13 +155,16 @@ template <> struct PairRule<3> {
   static constexpr int NP = 12;
   template <typename P>
   DEV static double point(const PairGeo<P>& geo, const Tables& T, int t, P si, P sj, P* p, int& m,
-                          P& off, P& ir, P* dsi) {
+                          P& off, P& ir, P* dsi, P* dsj) {
     const int wsel = t / 6, l = (t >> 1) % 3;
     const P d = (t & 1) ? P(-kSqrt6) : P(kSqrt6);
     geo.point(T, l, d * (wsel ? sj : si), p, m, off, ir);
     dsi[0] = (!wsel && l == 0) ? d : P(0);
     dsi[1] = (!wsel && l == 1) ? d : P(0);
     dsi[2] = (!wsel && l == 2) ? d : P(0);
+    dsj[0] = (wsel && l == 0) ? d : P(0);
+    dsj[1] = (wsel && l == 1) ? d : P(0);
+    dsj[2] = (wsel && l == 2) ? d : P(0);
     return 1.0 / 12.0;
   }
 };
@@ -168,7 +172,7 @@ template <> struct PairRule<5> {
   static constexpr int NP = 27;
   template <typename P>
   DEV static double point(const PairGeo<P>& geo, const Tables& T, int t, P si, P sj, P* p, int& m,
-                          P& off, P& ir, P* dsi) {
+                          P& off, P& ir, P* dsi, P* dsj) {
     const P s3 = P(1.7320508075688772);
     const int a0 = t % 3, a1 = (t / 3) % 3, a2 = t / 9;
     const P x0 = a0 == 0 ? P(0) : (a0 == 1 ? s3 : -s3);
@@ -177,8 +181,9 @@ template <> struct PairRule<5> {
     const P srel = sqrt(si * si + sj * sj);
     const P d[3] = {srel * x0, srel * x1, srel * x2};
     geo.point3(T, d, p, m, off, ir);
-    const P f = si / srel;
+    const P f = si / srel, fj = sj / srel;
     dsi[0] = f * x0; dsi[1] = f * x1; dsi[2] = f * x2;
+    dsj[0] = fj * x0; dsj[1] = fj * x1; dsj[2] = fj * x2;
     const double w0 = a0 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
     const double w1 = a1 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
     const double w2 = a2 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
@@ -249,6 +254,10 @@ DEV void min_image(const Box& b, const double4& qi, const double4& qj, double* r
 
 DEV int pair_slot(int si, int sj) { return si == sj ? si : 2; }
 
+// Owner of the unordered pair {i, j}: the lower index when i + j is odd, the
+// higher when even, so each site owns about half of its pairs.
+DEV bool pair_owner(int i, int j) { return ((i ^ j) & 1) ? i < j : i > j; }
+
 template <int K>
 DEV double group_sum(double v) {
 #pragma unroll
@@ -256,14 +265,12 @@ DEV double group_sum(double v) {
   return v;
 }
 
-// Neighbour walks.  With cooperative gathers (below) a lane past the end of
-// its site's neighbour row keeps walking while any lane of the wavefront
-// does, since the gathers need every lane of a group: it takes the site
-// itself as the neighbour, with zero occupancy, and this fixed displacement
-// (|r| = 2.4 A, inside every table), so all it forms is finite and enters the
-// sums with weight zero.  Without them each lane stops at its own row's end
-// (the loop condition is divergent, which costs fewer registers).
-template <typename P> DEV bool walk_on(bool more);
+// Neighbour walks.  The K >= 4 lanes of a site walk its row together for
+// ceil(nn / K) steps (the cooperative gathers below need every lane of a
+// group of four): a lane past the row's end takes the site itself as the
+// neighbour, with zero occupancy, and this fixed displacement (|r| = 2.4 A,
+// inside every table), so all it forms is finite and enters the sums with
+// weight zero.
 DEV void idle_disp(bool inb, double* r) {
   if (!inb) { r[0] = 1.1; r[1] = 1.3; r[2] = 1.7; }
 }
@@ -293,7 +300,8 @@ template <typename P> struct Coop { static constexpr bool v = sizeof(P) == 8; };
 // N4 4-value rows uses.
 template <typename P, int W> struct RowQ { static constexpr int v = (int)(W * sizeof(P) / 16); };
 template <typename P, int N8, int N4> struct StageRounds {
-  static constexpr int v = Coop<P>::v ? N8 * RowQ<P, 8>::v + N4 * RowQ<P, 4>::v : 1;
+  static constexpr int q4 = RowQ<P, 4>::v > 1 ? RowQ<P, 4>::v : 0;  // 16-B rows: direct
+  static constexpr int v = Coop<P>::v ? N8 * RowQ<P, 8>::v + N4 * q4 : 1;
 };
 
 // Value of v at lane R of this lane's group of NQ (quad permutation).
@@ -339,6 +347,7 @@ DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const
     }
   } else {
     constexpr int Q8 = RowQ<P, 8>::v, Q4 = RowQ<P, 4>::v;
+    constexpr int S4 = StageRounds<P, 0, 1>::q4;  // stage rounds per 4-value row
     const int lane = (int)__lane_id();
     const int a8 = lane & (Q8 - 1), a4 = lane & (Q4 - 1);
     u32x4 v8[N8 > 0 ? N8 : 1][Q8], v4[N4 > 0 ? N4 : 1][Q4];
@@ -348,16 +357,22 @@ DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const
 #pragma unroll
     for (int q = 0; q < N4; q++)
       coop_rounds<Q4>(reinterpret_cast<const u32x4*>(t4[q]), r4[q], a4, v4[q]);
+    if constexpr (S4 == 0) {  // 16-B rows were loaded whole by their own lanes
+#pragma unroll
+      for (int q = 0; q < N4; q++) __builtin_memcpy(o4[q], &v4[q][0], sizeof(v4[q][0]));
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
     for (int q = 0; q < N8; q++)
 #pragma unroll
       for (int r = 0; r < Q8; r++) stage[(q * Q8 + r) * kStageSlots + lane] = v8[q][r];
+    if constexpr (S4 > 0) {
 #pragma unroll
-    for (int q = 0; q < N4; q++)
+      for (int q = 0; q < N4; q++)
 #pragma unroll
-      for (int r = 0; r < Q4; r++)
-        stage[(N8 * Q8 + q * Q4 + r) * kStageSlots + lane] = v4[q][r];
+        for (int r = 0; r < Q4; r++)
+          stage[(N8 * Q8 + q * Q4 + r) * kStageSlots + lane] = v4[q][r];
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
     for (int q = 0; q < N8; q++) {
@@ -366,13 +381,15 @@ DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const
       for (int c = 0; c < Q8; c++) w[c] = stage[(q * Q8 + a8) * kStageSlots + (lane - a8) + c];
       __builtin_memcpy(o8[q], w, sizeof(w));
     }
+    if constexpr (S4 > 0) {
 #pragma unroll
-    for (int q = 0; q < N4; q++) {
-      u32x4 w[Q4];
+      for (int q = 0; q < N4; q++) {
+        u32x4 w[Q4];
 #pragma unroll
-      for (int c = 0; c < Q4; c++)
-        w[c] = stage[(N8 * Q8 + q * Q4 + a4) * kStageSlots + (lane - a4) + c];
-      __builtin_memcpy(o4[q], w, sizeof(w));
+        for (int c = 0; c < Q4; c++)
+          w[c] = stage[(N8 * Q8 + q * Q4 + a4) * kStageSlots + (lane - a4) + c];
+        __builtin_memcpy(o4[q], w, sizeof(w));
+      }
     }
   }
 }
@@ -388,11 +405,6 @@ DEV void gather_rows8(const P* tab, const int* rows, u32x4* stage, P (*out)[8]) 
 template <typename P>
 DEV void gather_row8(const P* tab, int row, u32x4* stage, P* out) {
   gather_rows8<P, 1>(tab, &row, stage, reinterpret_cast<P(*)[8]>(out));
-}
-
-template <typename P> DEV bool walk_on(bool more) {
-  if constexpr (Coop<P>::v) return __any(more);
-  else return more;
 }
 
 // A 4-value row into registers (issued ahead of a gather so both overlap).

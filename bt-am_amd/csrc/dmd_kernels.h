@@ -60,6 +60,7 @@ struct Box {
 // at a multiple of kChunk sites, so a kChunk-aligned block of sites belongs to
 // a single instance.
 constexpr int kChunk = 256;
+constexpr int kSlotW = 5;
 
 struct State {
   int n;               // arena extent (sites)
@@ -75,6 +76,11 @@ struct State {
   const double* gamma;
   const int* nnb;
   const int* nbr;      // n x kCap
+  // Pairs a site owns (phase 2 evaluates each unordered pair once, at its
+  // owner): own_n[i] entries own[i * kCap + j] = kk | pos << 8, where kk is
+  // the slot of the neighbour k in i's row and pos the slot of i in k's row.
+  const int* own_n;
+  const int* own;
   // Chunks to evaluate, per precision: act[p * act_stride] = count, then the
   // chunk indices in arena order.  The phase kernels map their blocks onto
   // these chunks only, so free slots and instances that skip this evaluation
@@ -94,6 +100,8 @@ struct Outputs {
   double* Rs;
   double* Rx;
   double* S0;
+  double* slot;        // n x kCap x kSlotW: a pair's phase-2 contribution to the site
+                       // of that row (dR_q x3, dR_sigma, dR_x), written by the owner
   double* partials;    // per-chunk [S0, V0] partial sums
   double* totals;      // per instance [sum S0, sum V0]
 };
@@ -110,6 +118,11 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
                             int n, const double4* qs, int* cell_of, int* cell_count,
                             int* cell_start, int* cell_cursor, int* cell_atoms, int* nnb, int* nbr,
                             int* flags);
+
+// Owned-pair lists of one instance (after launch_neighbors on the same
+// stream): flags[2] is set if a neighbour relation is not symmetric.
+hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, const int* nbr,
+                            int* own_n, int* own, int* flags);
 
 // Precision of the quadrature-point evaluation: FP64 throughout, or the split
 // form (FP32 points and splines, FP64 distances, sums, records and totals).

@@ -223,6 +223,38 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n, const int* nnb, int* n
   if (t < cnt) row[t] = key[t];
 }
 
+// Pairs of site i that i owns, in row order: (slot kk of the neighbour k in
+// i's row) | (slot of i in k's row) << 8.  Rows are sorted, so the reverse slot
+// is a binary search.
+__global__ __launch_bounds__(256) void k_own_lists(int base, int n, const int* nnb, const int* nbr,
+                                                   int* own_n, int* own, int* flags) {
+  const int li = blockIdx.x * 256 + threadIdx.x;
+  if (li >= n) return;
+  const int i = base + li;
+  const int cnt = nnb[i];
+  const int* row = nbr + (size_t)i * kCap;
+  int* ow = own + (size_t)i * kCap;
+  int no = 0;
+  for (int kk = 0; kk < cnt; kk++) {
+    const int k = row[kk];
+    if (!pair_owner(i, k)) continue;
+    const int* rk = nbr + (size_t)k * kCap;
+    const int nk = nnb[k];
+    int lo = 0, hi = nk;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (rk[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= nk || rk[lo] != i) {
+      flags[2] = 1;
+      continue;
+    }
+    ow[no++] = kk | (lo << 8);
+  }
+  own_n[i] = no;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -236,7 +268,7 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
   int nc = g.nc[0] * g.nc[1] * g.nc[2];
   hipError_t e = hipMemsetAsync(cell_count, 0, sizeof(int) * nc, st);
   if (e != hipSuccess) return e;
-  e = hipMemsetAsync(flags, 0, sizeof(int) * 2, st);
+  e = hipMemsetAsync(flags, 0, sizeof(int) * 3, st);
   if (e != hipSuccess) return e;
   int nb = (n + 255) / 256;
   k_cell_assign<<<nb, 256, 0, st>>>(n, box, g, qs, cell_of, cell_count);
@@ -245,6 +277,12 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
   k_build<<<nb, 256, 0, st>>>(base, n, box, g, rc, qs, cell_of, cell_start, cell_atoms, nnb, nbr,
                               flags);
   k_sort_rows<<<n, kCap, 0, st>>>(n, nnb, nbr);
+  return hipGetLastError();
+}
+
+hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, const int* nbr,
+                            int* own_n, int* own, int* flags) {
+  k_own_lists<<<(n + 255) / 256, 256, 0, st>>>(base, n, nnb, nbr, own_n, own, flags);
   return hipGetLastError();
 }
 

@@ -109,7 +109,8 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     for (int b = 0; b < kShiftsPerWalk; b++)
 #pragma unroll
       for (int c = 0; c < 9; c++) G[b][c] = 0;
-    for (int kk = g; walk_on<P>(kk < nn); kk += K) {
+    for (int kk0 = 0; kk0 < nn; kk0 += K) {  // uniform over the site's lanes
+      const int kk = kk0 + g;
       const bool inb = kk < nn;
       const int j = inb ? row[kk] : i;
       const double nj = inb ? s.ne[j] : 0.0;
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   double G0[9], U[9];
 #pragma unroll
   for (int c = 0; c < 9; c++) { G0[c] = 0; U[c] = 0; }
-  for (int kk = g; walk_on<P>(kk < nn); kk += K) {
+  for (int kk0 = 0; kk0 < nn; kk0 += K) {  // uniform over the site's lanes
+      const int kk = kk0 + g;
     const bool inb = kk < nn;
     const int j = inb ? row[kk] : i;
     const double nj = inb ? s.ne[j] : 0.0;
@@ -217,8 +219,8 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     // pair and coincident points of the pair rule
 #pragma unroll 1
     for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      P dsi[3], c4[2][4];
-      const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi);
+      P dsi[3], dsj[3], c4[2][4];
+      const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi, dsj);
       const int r8 = rb + m, r4[2] = {t * nr + m, spj * nr + m};
       const P* t4[2] = {pht, rht};
       gather_mix<P, 1, 2>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
@@ -311,31 +313,34 @@ DEV void nbr_blk(const Outputs& o, int k, int b, bool flip, u32x4* stage, P* out
 }
 
 // ---------------------------------------------------------------------------
-// Phase 2: gradient of sum_c beta_c V0_c with respect to (q_u, sigma_u, x_u).
+// Phase 2: gradient of sum_c beta_c V0_c with respect to (q, sigma, x).
 // Role (a): V0_u (u is the centre); role (b): V0_k of each neighbour k (u is a
 // neighbour).  Both roles are evaluated on the same 25 points
 // p = r_uk + shift; their adjoints are combined before one gradient per point.
 // psi(-p) = flip(psi(p)) (the dipole part is odd, the quadrupole part even).
+//
+// The points of pair (u, k) seen from k are those seen from u mirrored
+// (p -> -p), with the same combined adjoint, so each unordered pair is
+// evaluated once, by its owner (pair_owner): k_phase2 writes the pair's
+// contributions to both ends into their row slots -- position gradient +g at
+// u and -g at k, each end's width derivative from the points that shift with
+// its own sigma, and each end's occupancy derivative from the other end's
+// adjoints -- and k_phase2_sum adds up each site's slots in row order.  No
+// atomics: every slot has one writer and the sums run in a fixed order.
 // ---------------------------------------------------------------------------
-// XR: also form the occupancy residual R_x (instances that relax positions and
-// widths only skip it; their R_x output is left at zero).
 // Returns x unchanged, through an empty asm statement the compiler cannot see
 // into, so a value recomputed from it is not merged with an earlier copy.
 DEV double kept_out(double x) { asm volatile("" : "+v"(x)); return x; }
 DEV float kept_out(float x) { asm volatile("" : "+v"(x)); return x; }
 
+// XR: also form the occupancy terms (instances that relax positions and widths
+// only skip them; their R_x output is zero).
 template <typename P, int K, bool UNR, int RULE, bool XR>
-DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst,
-                     u32x4* stage) {
+DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst,
+                      u32x4* stage) {
   const Box box = s.boxes[inst];
   const double nu = s.ne[u];
-  if (nu == 0.0) {  // all K lanes of the site leave together
-    if (g == 0) {
-      o.Rq[3 * u] = 0.0; o.Rq[3 * u + 1] = 0.0; o.Rq[3 * u + 2] = 0.0;
-      o.Rs[u] = 0.0; o.Rx[u] = 0.0; o.S0[u] = 0.0;
-    }
-    return;
-  }
+  if (nu == 0.0) return;  // all K lanes of the site leave together; its pairs add nothing
   const P* uwt = tab_uw<P>(T);
   const P* pht = tab_ph<P>(T);
   const P* rht = tab_rh<P>(T);
@@ -347,11 +352,14 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
   const double* Ru = o.rec + (size_t)u * kRec;
   const P* Au = adj_rec(o, u, P(0));
   const double FPu = Ru[kRecFP];
-  double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
-  const int nn = s.nnb[u];
+  const int no = s.own_n[u];
+  const int* own = s.own + (size_t)u * kCap;
   const int* row = s.nbr + (size_t)u * kCap;
-  for (int kk = g; walk_on<P>(kk < nn); kk += K) {
-    const bool inb = kk < nn;
+  for (int j0 = 0; j0 < no; j0 += K) {  // uniform over the site's lanes
+    const int j = j0 + g;
+    const bool inb = j < no;
+    const int ent = inb ? own[j] : 0;
+    const int kk = ent & 255, pos = ent >> 8;
     const int k = inb ? row[kk] : u;
     const double nk = inb ? s.ne[k] : 0.0;  // zero: every weight below vanishes
     const double4 qk = s.qs[k];
@@ -361,11 +369,12 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     const double* Rk = o.rec + (size_t)k * kRec;
     const P* Ak = adj_rec(o, k, P(0));
     const double wa = bu * nu * nu * nk, wb = bk * nk * nk * nu;
-    const double xb = bk * nk * nk;
+    const double xb = bk * nk * nk, xbk = bu * nu * nu;  // occupancy weights, ends u and k
     const P waP = (P)wa, wbP = (P)wb;
     const P cnk = P(-nk * (1.0 / 9.0)), cnu = P(-nu * (1.0 / 9.0));
     const int t = pair_slot(spu, spk);
     const int rb = t * nr;
+    double q0 = 0, q1 = 0, q2 = 0, sgu = 0, sgk = 0, xu = 0, xk = 0;  // this pair
     double rd[3];
     min_image(box, qu, qk, rd);
     idle_disp(inb, rd);
@@ -392,7 +401,7 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       load_blk(Au, kBlkLG0, false, laB);
       applyM(cnk, psi0, tmp);
 #pragma unroll
-      for (int c = 0; c < 9; c++) { laB[c] = waP * (laB[c] + tmp[c]); Tuk[c] = P(0); }
+      for (int c = 0; c < 9; c++) { laB[c] += tmp[c]; Tuk[c] = P(0); }  // unscaled by waP
     }
     const P skP = (P)sk, suP = (P)su;
 
@@ -419,14 +428,19 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
 #pragma unroll
       for (int e = 0; e < 9; e++) {
         lb[e] += mps[e];
-        L[e] = laB[e] + wbP * lb[e];
+        L[e] = waP * laB[e] + wbP * lb[e];
       }
       grad_psi(L, p, ir, uu, du, ww, dw, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-      if (XR) xacc += xb * (double)dot9(lb, ps);
+      const P gl = l == 0 ? g3[0] : (l == 1 ? g3[1] : g3[2]);
+      sgk += (double)(sh * gl);
+      if (XR) {
+        xu += xb * (double)dot9(lb, ps);
+        xk += xbk * (double)dot9(laB, ps);
+      }
     };
     if (UNR) {
-#pragma unroll
+#pragma unroll 2
       for (int pt = 0; pt < 6; pt++) bpt(pt);
     } else {
 #pragma unroll 1
@@ -438,7 +452,14 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       load_blk(Au, kBlkLU, false, la);
       applyM(cnk, Tuk, tmp);
 #pragma unroll
-      for (int c = 0; c < 9; c++) la[c] = waP * (la[c] + tmp[c]);
+      for (int c = 0; c < 9; c++) la[c] += tmp[c];
+      if (XR) {
+        P psi0[9];
+        psi_from(kept_out(u0), w0, p0, psi0);
+        xk += xbk * (double)dot9(la, psi0);
+      }
+#pragma unroll
+      for (int c = 0; c < 9; c++) la[c] *= waP;
       grad_psi(la, p0, ir0, u0, du0, w0, dw0, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
     }
@@ -467,15 +488,21 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       P mps[9];
       applyM(cnk, ps, mps);
 #pragma unroll
-      for (int e = 0; e < 9; e++) L[e] = waP * (la[e] + mps[e]) + wbP * lbA[e];
+      for (int e = 0; e < 9; e++) {
+        la[e] += mps[e];
+        L[e] = waP * la[e] + wbP * lbA[e];
+      }
       grad_psi(L, p, ir, uu, du, ww, dw, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
       const P gl = l == 0 ? g3[0] : (l == 1 ? g3[1] : g3[2]);
-      sacc += (double)(sh * gl);
-      if (XR) xacc += xb * (double)dot9(lbA, ps);
+      sgu += (double)(sh * gl);
+      if (XR) {
+        xu += xb * (double)dot9(lbA, ps);
+        xk += xbk * (double)dot9(la, ps);
+      }
     };
     if (UNR) {
-#pragma unroll
+#pragma unroll 2
       for (int pt = 0; pt < 6; pt++) apt(pt);
     } else {
 #pragma unroll 1
@@ -496,7 +523,7 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       if (XR) {
         P psi0[9];
         psi_from(kept_out(u0), w0, p0, psi0);
-        xacc += xb * (double)dot9(lb, psi0);
+        xu += xb * (double)dot9(lb, psi0);
       }
     }
     // pair-rule points: coincident angular term, pair and density
@@ -506,13 +533,11 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
       const P cra = (P)(bu * nu * FPu * nk);
       const P crb = (P)(bk * nk * FPk * nu);
       const P cang = (P)(2.0 * (wa * nk + wb * nu));
-      const double xang = 2.0 * xb * nu;
-      const P xrho = (P)(bk * nk * FPk);
-      const P xphi = (P)(bk * 0.5 * nk);
       auto cpt = [&](int pt) {
-        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3], c[8], c4[3][4];
+        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3], dsk[3], c[8], c4[3][4];
         int mm;
-        const double wt = PairRule<RULE>::point(geo, T, pt, suP, skP, p, mm, o2, ir, dsu);
+        const double wt =
+            PairRule<RULE>::point(geo, T, pt, suP, skP, p, mm, o2, ir, dsu, dsk);
         const P wtP = (P)wt;
         const int r8 = rb + mm, r4[3] = {t * nr + mm, spk * nr + mm, spu * nr + mm};
         const P* t4[3] = {pht, rht, rht};
@@ -527,26 +552,91 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
         const P cs = wtP * (cphi * cder(cp, o2) + cra * cder(ck, o2) + crb * cder(cu, o2)) * ir;
         g3[0] += cs * p[0]; g3[1] += cs * p[1]; g3[2] += cs * p[2];
         q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-        sacc += (double)(g3[0] * dsu[0] + g3[1] * dsu[1] + g3[2] * dsu[2]);
-        if (XR)
-          xacc += wt * (xang * (double)dotM(ps, ps) +
-                        (double)(xrho * cval(cu, o2) + xphi * cval(cp, o2)));
+        sgu += (double)(g3[0] * dsu[0] + g3[1] * dsu[1] + g3[2] * dsu[2]);
+        sgk += (double)(g3[0] * dsk[0] + g3[1] * dsk[1] + g3[2] * dsk[2]);
+        if (XR) {  // angular, density and pair terms of the other end's V0
+          const double pp = (double)dotM(ps, ps), hp = 0.5 * (double)cval(cp, o2);
+          xu += wt * (2.0 * xb * nu * pp + bk * nk * (FPk * (double)cval(cu, o2) + hp));
+          xk += wt * (2.0 * xbk * nk * pp + bu * nu * (FPu * (double)cval(ck, o2) + hp));
+        }
       };
       if (UNR) {
-#pragma unroll
+#pragma unroll 3
         for (int pt = 0; pt < PairRule<RULE>::NP; pt++) cpt(pt);
       } else {
 #pragma unroll 1
         for (int pt = 0; pt < PairRule<RULE>::NP; pt++) cpt(pt);
       }
     }
+    if (inb && nk != 0.0) {
+      double* eu = o.slot + ((size_t)u * kCap + kk) * kSlotW;
+      double* ek = o.slot + ((size_t)k * kCap + pos) * kSlotW;
+      eu[0] = q0; eu[1] = q1; eu[2] = q2; eu[3] = sgu;
+      ek[0] = -q0; ek[1] = -q1; ek[2] = -q2; ek[3] = sgk;
+      if (XR) { eu[4] = xu; ek[4] = xk; }
+    }
   }
-  if (K > 1) {
-    q0 = group_sum<K>(q0); q1 = group_sum<K>(q1); q2 = group_sum<K>(q2);
-    sacc = group_sum<K>(sacc);
-    if (XR) xacc = group_sum<K>(xacc);
+}
+
+// Two wavefronts per SIMD: the FP64 pair body fits 256 VGPRs without spills.
+// UNR: the point loops advance two (shifted) or three (pair-rule) points per
+// iteration, so one point's arithmetic overlaps the next point's gathers.
+template <typename P, int K, bool UNR, int RULE>
+__global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o) {
+  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
+  // largest gathers: two 8-value rows (j-shift points), one 8- and three
+  // 4-value rows (pair-rule points)
+  constexpr int kRounds = StageRounds<P, 2, 0>::v > StageRounds<P, 1, 3>::v
+                              ? StageRounds<P, 2, 0>::v : StageRounds<P, 1, 3>::v;
+  __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
+  u32x4* stage = stage_all[threadIdx.x >> 6];
+  int u, g;
+  if (!site_of<P, K>(s, u, g)) return;
+  const int inst = s.inst[u];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  if (s.noxr && s.noxr[inst])
+    phase2_pairs<P, K, UNR, RULE, false>(s, T, o, u, g, inst, stage);
+  else
+    phase2_pairs<P, K, UNR, RULE, true>(s, T, o, u, g, inst, stage);
+}
+
+// Sums of a site's slots (row order, fixed lane order) and the site's own
+// terms: entropy, width entropy, occupancy terms of its own V0 and S0.
+constexpr int kSumLanes = 8;
+template <typename P>
+__global__ __launch_bounds__(256) void k_phase2_sum(State s, Tables T, Outputs o) {
+  constexpr int K = kSumLanes;
+  int u, g;
+  if (!site_of<P, K>(s, u, g)) return;
+  const int inst = s.inst[u];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  const bool xr = !(s.noxr && s.noxr[inst]);
+  const double nu = s.ne[u];
+  if (nu == 0.0) {
+    if (g == 0) {
+      o.Rq[3 * u] = 0.0; o.Rq[3 * u + 1] = 0.0; o.Rq[3 * u + 2] = 0.0;
+      o.Rs[u] = 0.0; o.Rx[u] = 0.0; o.S0[u] = 0.0;
+    }
+    return;
   }
+  const int nn = s.nnb[u];
+  const int* row = s.nbr + (size_t)u * kCap;
+  const double* sl = o.slot + (size_t)u * kCap * kSlotW;
+  double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
+  for (int kk = g; kk < nn; kk += K) {
+    if (s.ne[row[kk]] == 0.0) continue;  // no contribution, slot not written
+    const double* e = sl + (size_t)kk * kSlotW;
+    q0 += e[0]; q1 += e[1]; q2 += e[2]; sacc += e[3];
+    if (xr) xacc += e[4];
+  }
+  q0 = group_sum<K>(q0); q1 = group_sum<K>(q1); q2 = group_sum<K>(q2);
+  sacc = group_sum<K>(sacc);
+  xacc = group_sum<K>(xacc);
   if (g == 0) {
+    const double4 qu = s.qs[u];
+    const double su = qu.w;
+    const int spu = s.spc[u];
+    const double bu = s.beta[u];
     const double xu = nu;
     const double mi = kUma * xu * T.mass[spu];
     double logZ = 3.0 * log((su * sqrt(mi / bu)) / kHbar);
@@ -564,27 +654,8 @@ DEV void phase2_site(const State& s, const Tables& T, const Outputs& o, int u, i
     o.S0[u] = -logZ + bu * (1.0 / (2.0 * bu) + V0) - gx;
     o.Rq[3 * u] = q0; o.Rq[3 * u + 1] = q1; o.Rq[3 * u + 2] = q2;
     o.Rs[u] = sacc - 3.0 / su;
-    o.Rx[u] = XR ? bu * Ru[kRecDVDN] + xacc + dx : 0.0;
+    o.Rx[u] = xr ? bu * o.rec[(size_t)u * kRec + kRecDVDN] + xacc + dx : 0.0;
   }
-}
-
-template <typename P, int K, bool UNR, int RULE>
-__global__ __launch_bounds__(256) void k_phase2(State s, Tables T, Outputs o) {
-  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
-  // largest gathers: two 8-value rows (j-shift points), one 8- and three
-  // 4-value rows (pair-rule points)
-  constexpr int kRounds = StageRounds<P, 2, 0>::v > StageRounds<P, 1, 3>::v
-                              ? StageRounds<P, 2, 0>::v : StageRounds<P, 1, 3>::v;
-  __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
-  u32x4* stage = stage_all[threadIdx.x >> 6];
-  int u, g;
-  if (!site_of<P, K>(s, u, g)) return;
-  const int inst = s.inst[u];
-  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
-  if (s.noxr && s.noxr[inst])
-    phase2_site<P, K, UNR, RULE, false>(s, T, o, u, g, inst, stage);
-  else
-    phase2_site<P, K, UNR, RULE, true>(s, T, o, u, g, inst, stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,6 +672,8 @@ static void p2(hipStream_t st, const State& s, const Tables& T, const Outputs& o
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
   k_phase2<P, K, UNR, RULE><<<nb, 256, 0, st>>>(s, T, o);
+  const int nbs = (int)(((long long)s.n * kSumLanes + 255) / 256);
+  k_phase2_sum<P><<<nbs, 256, 0, st>>>(s, T, o);
 }
 
 template <typename P, int RULE>

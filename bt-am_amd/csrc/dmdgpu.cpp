@@ -48,7 +48,8 @@ constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
 constexpr int kChunk = dmd::kChunk;
 // Device bytes per arena site (state, lists, records, outputs, solver vectors).
 constexpr size_t kBytesPerSite = 32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 +
-                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32;
+                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32 + 4 +
+                                 4 * dmd::kCap + 8 * dmd::kSlotW * dmd::kCap;
 
 struct Spline {
   int n = 0;
@@ -172,9 +173,9 @@ struct dmdgpu_ctx {
   DevBuf<dmd::Box> d_boxes;
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
-  DevBuf<int> d_spc, d_nnb, d_nbr;
+  DevBuf<int> d_spc, d_nnb, d_nbr, d_own_n, d_own;
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
-  DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals;
+  DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<long long> d_pairs;
   std::vector<int> h_prec;
   std::vector<int> h_noxr;  // per instance, solver ticks: 1 = no occupancy residual needed
@@ -345,6 +346,8 @@ dmd::State make_state(dmdgpu_ctx* c) {
   s.gamma = c->d_gamma.p;
   s.nnb = c->d_nnb.p;
   s.nbr = c->d_nbr.p;
+  s.own_n = c->d_own_n.p;
+  s.own = c->d_own.p;
   s.act = c->d_act.p;
   s.act_stride = (int)(c->d_act.n / 2);
   return s;
@@ -360,6 +363,7 @@ dmd::Outputs make_outputs(dmdgpu_ctx* c) {
   o.Rs = c->d_Rs.p;
   o.Rx = c->d_Rx.p;
   o.S0 = c->d_S0.p;
+  o.slot = c->d_slot.p;
   o.partials = c->d_partials.p;
   o.totals = c->d_totals.p;
   return o;
@@ -394,6 +398,9 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_inst.ensure(cap, keep));
   HIPCHK(c->d_nnb.ensure(cap, keep));
   HIPCHK(c->d_nbr.ensure(cap * dmd::kCap, keep * dmd::kCap));
+  HIPCHK(c->d_own_n.ensure(cap, keep));
+  HIPCHK(c->d_own.ensure(cap * dmd::kCap, keep * dmd::kCap));
+  HIPCHK(c->d_slot.ensure(cap * dmd::kCap * dmd::kSlotW));
   HIPCHK(c->d_cell_of.ensure(cap));
   HIPCHK(c->d_cell_atoms.ensure(cap));
   HIPCHK(c->d_rec.ensure(cap * dmd::kRec, keep * dmd::kRec));
@@ -418,7 +425,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_dnn.ensure(cap, keep));
   HIPCHK(c->d_dnbr.ensure(cap * dmd::kDCap, keep * dmd::kDCap));
   HIPCHK(c->d_act.ensure(2 * (cap / kChunk + 1)));
-  HIPCHK(c->d_flags.ensure(2));
+  HIPCHK(c->d_flags.ensure(3));
   return DMDGPU_OK;
 }
 
@@ -610,12 +617,15 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   HIPCHK(dmd::launch_neighbors(c->stream, I.box, g, c->rc, I.base, I.n, c->d_qs.p, c->d_cell_of.p,
                                c->d_cell_count.p, c->d_cell_start.p, c->d_cell_cursor.p,
                                c->d_cell_atoms.p, c->d_nnb.p, c->d_nbr.p, c->d_flags.p));
+  HIPCHK(dmd::launch_own_lists(c->stream, I.base, I.n, c->d_nnb.p, c->d_nbr.p, c->d_own_n.p,
+                               c->d_own.p, c->d_flags.p));
   CHK(end_timed(c, KID_NEIGH, ev));
-  int flags[2];
+  int flags[3];
   HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (flags[0]) return fail(DMDGPU_ERR_MAXNEIGH, "a site has more than 250 neighbours");
   if (flags[1]) return fail(DMDGPU_ERR_NONEIGH, "a site has no neighbours");
+  if (flags[2]) return fail(DMDGPU_ERR_HIP, "neighbour lists are not symmetric");
   // active ordered pairs of this instance
   CHK(upload_meta(c));
   HIPCHK(dmd::launch_active_pairs(c->stream, make_state(c), c->d_pairs.p));
@@ -1303,10 +1313,12 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph,    &c->d_rh,  &c->d_em,       &c->d_ne,
                            &c->d_xr,  &c->d_beta,  &c->d_gamma, &c->d_rec,    &c->d_V0,
                            &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
+                           &c->d_slot,
                            &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart,
                            &c->d_Xn,  &c->d_Xt};
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
+                        &c->d_own_n,      &c->d_own,
                         &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
                         &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_noxr,
                         &c->d_chunk_lo,
