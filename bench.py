@@ -257,7 +257,7 @@ def solves_leg(tuning, count, graph=True):
     return out
 
 
-def serving_leg(tuning, count, load=0.8, budget_gb=1.0):
+def serving_leg(tuning, count, load=0.8, budget_gb=6.0):
     """Serving loop: relaxation and diffusion-step requests (periodic Mg-H boxes
     and nanowires, 500..200,000 sites log-uniform, seeded).  Saturation
     throughput (every request waiting at t = 0) of the plain per-instance FP64

@@ -283,9 +283,9 @@ DEV void idle_disp(bool inb, double* r) {
 // NQ = row bytes / 16 lanes of an aligned group load the NQ chunks of one
 // member's row with one instruction, member by member (NQ rounds: 64/NQ lines
 // per instruction instead of 64), and the chunks pass through a per-wavefront
-// LDS area to the lane that owns the row.  FP64 rows (64 and 32 B) go this
-// way; FP32 rows (32 and 16 B) gain at most a factor two in lines and are
-// loaded directly (measured faster on MI355X).
+// LDS area to the lane that owns the row.  FP32 rows (32 and 16 B) gain at
+// most a factor two in lines; the kernels choose per phase and precision
+// (COOP) what measured faster on MI355X.
 //
 // Every lane of a group must execute the gather (the phase kernels walk the
 // neighbour rows in wavefront-uniform loops, and whole groups leave early).
@@ -295,13 +295,12 @@ DEV void idle_disp(bool inb, double* r) {
 // ---------------------------------------------------------------------------
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kStageSlots = 65;  // 16-byte slots per round: 64 lanes + 1 (LDS bank skew)
-template <typename P> struct Coop { static constexpr bool v = sizeof(P) == 8; };
 // 16-byte chunks per row of W values; stage rounds a gather of N8 8-value and
 // N4 4-value rows uses.
 template <typename P, int W> struct RowQ { static constexpr int v = (int)(W * sizeof(P) / 16); };
-template <typename P, int N8, int N4> struct StageRounds {
+template <typename P, int N8, int N4, bool COOP> struct StageRounds {
   static constexpr int q4 = RowQ<P, 4>::v > 1 ? RowQ<P, 4>::v : 0;  // 16-B rows: direct
-  static constexpr int v = Coop<P>::v ? N8 * RowQ<P, 8>::v + N4 * q4 : 1;
+  static constexpr int v = COOP ? N8 * RowQ<P, 8>::v + N4 * q4 : 1;
 };
 
 // Value of v at lane R of this lane's group of NQ (quad permutation).
@@ -329,10 +328,10 @@ DEV void coop_rounds(const u32x4* base, int row, int a, u32x4* v) {
 // o8[q] = row r8[q] of t8 (rows of 8 values), q < N8, and o4[q] = row r4[q] of
 // t4[q] (rows of 4 values), q < N4, with one LDS round trip.  stage: this
 // wavefront's LDS area, StageRounds<P, N8, N4> * kStageSlots slots.
-template <typename P, int N8, int N4>
+template <typename P, int N8, int N4, bool COOP>
 DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const int* r4,
                     u32x4* stage, P (*o8)[8], P (*o4)[4]) {
-  if constexpr (!Coop<P>::v) {
+  if constexpr (!COOP) {
 #pragma unroll
     for (int q = 0; q < N8; q++) {
       const P* c = t8[q] + ((size_t)r8[q] << 3);
@@ -347,7 +346,7 @@ DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const
     }
   } else {
     constexpr int Q8 = RowQ<P, 8>::v, Q4 = RowQ<P, 4>::v;
-    constexpr int S4 = StageRounds<P, 0, 1>::q4;  // stage rounds per 4-value row
+    constexpr int S4 = StageRounds<P, 0, 1, true>::q4;  // stage rounds per 4-value row
     const int lane = (int)__lane_id();
     const int a8 = lane & (Q8 - 1), a4 = lane & (Q4 - 1);
     u32x4 v8[N8 > 0 ? N8 : 1][Q8], v4[N4 > 0 ? N4 : 1][Q4];
@@ -394,17 +393,17 @@ DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const
   }
 }
 
-template <typename P, int NR>
+template <typename P, int NR, bool COOP>
 DEV void gather_rows8(const P* tab, const int* rows, u32x4* stage, P (*out)[8]) {
   const P* t8[NR];
 #pragma unroll
   for (int q = 0; q < NR; q++) t8[q] = tab;
-  gather_mix<P, NR, 0>(t8, rows, nullptr, nullptr, stage, out, nullptr);
+  gather_mix<P, NR, 0, COOP>(t8, rows, nullptr, nullptr, stage, out, nullptr);
 }
 
-template <typename P>
+template <typename P, bool COOP>
 DEV void gather_row8(const P* tab, int row, u32x4* stage, P* out) {
-  gather_rows8<P, 1>(tab, &row, stage, reinterpret_cast<P(*)[8]>(out));
+  gather_rows8<P, 1, COOP>(tab, &row, stage, reinterpret_cast<P(*)[8]>(out));
 }
 
 // A 4-value row into registers (issued ahead of a gather so both overlap).

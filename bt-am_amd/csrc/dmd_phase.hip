@@ -67,11 +67,18 @@ DEV bool site_of(const State& s, int& i, int& g) {
 // fewer walks recompute the pair geometry less often, more accumulators live.
 constexpr int kShiftsPerWalk = 3;
 
+// Cooperative gathers (dmd_device.h) per phase and precision, as measured on
+// MI355X at the 54k-site workload: phase 1 gains in both precisions; phase 2
+// gains in FP64 only (its FP32 body is already VALU-heavy).
+template <typename P> constexpr bool kCoop1 = true;
+template <typename P> constexpr bool kCoop2 = sizeof(P) == 8;
+
 
 template <typename P, int K, int RULE>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   static_assert(K >= 4, "a site's lanes must cover whole gather groups");
-  __shared__ u32x4 stage_all[4][StageRounds<P, kShiftsPerWalk, 0>::v * kStageSlots];
+  constexpr bool CO = kCoop1<P>;
+  __shared__ u32x4 stage_all[4][StageRounds<P, kShiftsPerWalk, 0, CO>::v * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   int i, g;
   if (!site_of<P, K>(s, i, g)) return;
@@ -121,7 +128,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
       idle_disp(inb, rd);
       PairGeo<P> geo;
       geo.init(rd);
-      if constexpr (Coop<P>::v) {  // the walk's rows in one gather
+      if constexpr (CO) {  // the walk's rows in one gather
         P p[kShiftsPerWalk][3], off[kShiftsPerWalk], cw[kShiftsPerWalk][8];
         int rows[kShiftsPerWalk];
 #pragma unroll
@@ -133,7 +140,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
           geo.point(T, ls >> 1, sh, p[b], m, off[b], ir);
           rows[b] = rb + m;
         }
-        gather_rows8<P, kShiftsPerWalk>(uwt, rows, stage, cw);
+        gather_rows8<P, kShiftsPerWalk, CO>(uwt, rows, stage, cw);
 #pragma unroll
         for (int b = 0; b < kShiftsPerWalk; b++) {
           P ps[9];
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     P p[3], off, ir, psi0[9], ps[9], Tj[9], cw[8];
     int m;
     geo.point(T, 0, P(0), p, m, off, ir);
-    gather_row8<P>(uwt, rb + m, stage, cw);
+    gather_row8<P, CO>(uwt, rb + m, stage, cw);
     psi_from(cval(cw, off), cval(cw + 4, off), p, psi0);
 #pragma unroll
     for (int c = 0; c < 9; c++) { G0[c] += nj * (double)psi0[c]; Tj[c] = P(0); }
@@ -207,7 +214,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll 1
     for (int pt = 0; pt < 6; pt++) {
       geo.point(T, pt >> 1, (pt & 1 ? P(-3) : P(3)) * sj, p, m, off, ir);
-      gather_row8<P>(uwt, rb + m, stage, cw);
+      gather_row8<P, CO>(uwt, rb + m, stage, cw);
       psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
 #pragma unroll
       for (int e = 0; e < 9; e++) Tj[e] += ps[e];
@@ -223,7 +230,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
       const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi, dsj);
       const int r8 = rb + m, r4[2] = {t * nr + m, spj * nr + m};
       const P* t4[2] = {pht, rht};
-      gather_mix<P, 1, 2>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
+      gather_mix<P, 1, 2, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
       psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
       C6 += wt * (double)dotM(ps, ps);
       Phi += wt * (double)cval(c4[0], off);
@@ -302,13 +309,13 @@ DEV void blk_from(const P* row, P tail, bool flip, P* o) {
 
 // Block b of neighbour k's record (k differs between lanes): a cooperative
 // gather of its row plus the tail value.
-template <typename P>
+template <typename P, bool CO>
 DEV void nbr_blk(const Outputs& o, int k, int b, bool flip, u32x4* stage, P* out) {
   const P* base = adj_base(o, P(0));
   const P tail = adj_rec(o, k, P(0))[kRecTail + b];
   const int r = k * adj_rows<P>() + b;
   P row[1][8];
-  gather_mix<P, 1, 0>(&base, &r, nullptr, nullptr, stage, row, nullptr);
+  gather_mix<P, 1, 0, CO>(&base, &r, nullptr, nullptr, stage, row, nullptr);
   blk_from(row[0], tail, flip, out);
 }
 
@@ -338,9 +345,20 @@ DEV float kept_out(float x) { asm volatile("" : "+v"(x)); return x; }
 template <typename P, int K, bool UNR, int RULE, bool XR>
 DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst,
                       u32x4* stage) {
+  constexpr bool CO = kCoop2<P>;
   const Box box = s.boxes[inst];
   const double nu = s.ne[u];
-  if (nu == 0.0) return;  // all K lanes of the site leave together; its pairs add nothing
+  const int no = s.own_n[u];
+  const int* own = s.own + (size_t)u * kCap;
+  const int* row = s.nbr + (size_t)u * kCap;
+  if (nu == 0.0) {  // its pairs add nothing: zero the other ends' slots
+    for (int j = g; j < no; j += K) {
+      const int ent = own[j];
+      double* ek = o.slot + ((size_t)row[ent & 255] * kCap + (ent >> 8)) * kSlotW;
+      ek[0] = 0.0; ek[1] = 0.0; ek[2] = 0.0; ek[3] = 0.0; ek[4] = 0.0;
+    }
+    return;  // all K lanes of the site leave together
+  }
   const P* uwt = tab_uw<P>(T);
   const P* pht = tab_ph<P>(T);
   const P* rht = tab_rh<P>(T);
@@ -352,9 +370,6 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
   const double* Ru = o.rec + (size_t)u * kRec;
   const P* Au = adj_rec(o, u, P(0));
   const double FPu = Ru[kRecFP];
-  const int no = s.own_n[u];
-  const int* own = s.own + (size_t)u * kCap;
-  const int* row = s.nbr + (size_t)u * kCap;
   for (int j0 = 0; j0 < no; j0 += K) {  // uniform over the site's lanes
     const int j = j0 + g;
     const bool inb = j < no;
@@ -384,7 +399,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
     int m;
     geo.point(T, 0, P(0), p0, m, off, ir0);
     P c0[8];
-    gather_row8<P>(uwt, rb + m, stage, c0);
+    gather_row8<P, CO>(uwt, rb + m, stage, c0);
     const P u0 = cval(c0, off), du0 = cder(c0, off), w0 = cval(c0 + 4, off), dw0 = cder(c0 + 4, off);
     // The two shifted-point walks each hold one constant adjoint and one
     // accumulator: the j-shift walk the role-(a) adjoint laB and the sum Tuk of
@@ -416,7 +431,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
       const P tail = Ak[kRecTail + blk];
       const int r8[2] = {rb + mm, k * adj_rows<P>() + blk};
       const P* t8[2] = {uwt, adj_base(o, P(0))};
-      gather_mix<P, 2, 0>(t8, r8, nullptr, nullptr, stage, c, nullptr);
+      gather_mix<P, 2, 0, CO>(t8, r8, nullptr, nullptr, stage, c, nullptr);
       blk_from(c[1], tail, true, lb);
       const P uu = cval(c[0], o2), du = cder(c[0], o2), ww = cval(c[0] + 4, o2),
               dw = cder(c[0] + 4, o2);
@@ -467,7 +482,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
     {
       P psi0[9];
       psi_from(kept_out(u0), w0, p0, psi0);
-      nbr_blk<P>(o, k, kBlkLG0, true, stage, lbA);
+      nbr_blk<P, CO>(o, k, kBlkLG0, true, stage, lbA);
       applyM(cnu, psi0, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; SA[c] = P(0); }
@@ -480,7 +495,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
       int mm;
       geo.point(T, l, sh * suP, p, mm, o2, ir);
       load_blk(Au, 2 * l + sg, false, la);
-      gather_row8<P>(uwt, rb + mm, stage, c);
+      gather_row8<P, CO>(uwt, rb + mm, stage, c);
       const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
       psi_from(uu, ww, p, ps);
 #pragma unroll
@@ -511,7 +526,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
     // unshifted point, role (b): wbP * (LU_k - nu/9 M SA)
     {
       P lb[9], L[9], g3[3];
-      nbr_blk<P>(o, k, kBlkLU, true, stage, lb);
+      nbr_blk<P, CO>(o, k, kBlkLU, true, stage, lb);
       applyM(cnu, SA, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) {
@@ -541,7 +556,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
         const P wtP = (P)wt;
         const int r8 = rb + mm, r4[3] = {t * nr + mm, spk * nr + mm, spu * nr + mm};
         const P* t4[3] = {pht, rht, rht};
-        gather_mix<P, 1, 3>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(c), c4);
+        gather_mix<P, 1, 3, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(c), c4);
         const P* cp = c4[0];
         const P* ck = c4[1];
         const P* cu = c4[2];
@@ -568,7 +583,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
         for (int pt = 0; pt < PairRule<RULE>::NP; pt++) cpt(pt);
       }
     }
-    if (inb && nk != 0.0) {
+    if (inb) {  // with nk = 0 the values at u are exact zeros; k reads none
       double* eu = o.slot + ((size_t)u * kCap + kk) * kSlotW;
       double* ek = o.slot + ((size_t)k * kCap + pos) * kSlotW;
       eu[0] = q0; eu[1] = q1; eu[2] = q2; eu[3] = sgu;
@@ -586,8 +601,9 @@ __global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o)
   static_assert(K >= 4, "a site's lanes must cover whole gather groups");
   // largest gathers: two 8-value rows (j-shift points), one 8- and three
   // 4-value rows (pair-rule points)
-  constexpr int kRounds = StageRounds<P, 2, 0>::v > StageRounds<P, 1, 3>::v
-                              ? StageRounds<P, 2, 0>::v : StageRounds<P, 1, 3>::v;
+  constexpr bool CO = kCoop2<P>;
+  constexpr int kRounds = StageRounds<P, 2, 0, CO>::v > StageRounds<P, 1, 3, CO>::v
+                              ? StageRounds<P, 2, 0, CO>::v : StageRounds<P, 1, 3, CO>::v;
   __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   int u, g;
@@ -620,11 +636,9 @@ __global__ __launch_bounds__(256) void k_phase2_sum(State s, Tables T, Outputs o
     return;
   }
   const int nn = s.nnb[u];
-  const int* row = s.nbr + (size_t)u * kCap;
   const double* sl = o.slot + (size_t)u * kCap * kSlotW;
   double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
-  for (int kk = g; kk < nn; kk += K) {
-    if (s.ne[row[kk]] == 0.0) continue;  // no contribution, slot not written
+  for (int kk = g; kk < nn; kk += K) {  // every slot is written each evaluation
     const double* e = sl + (size_t)kk * kSlotW;
     q0 += e[0]; q1 += e[1]; q2 += e[2]; sacc += e[3];
     if (xr) xacc += e[4];

@@ -713,6 +713,7 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   HIPCHK(hipMemcpy(c->d_ne.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_xr.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_nnb.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_own_n.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_dnn.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_Xn.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_Xt.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
