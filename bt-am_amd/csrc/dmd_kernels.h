@@ -76,10 +76,12 @@ struct State {
   const double* gamma;
   const int* nnb;
   const int* nbr;      // n x kCap
-  // Pairs a site owns (phase 2 evaluates each unordered pair once, at its
-  // owner): own_n[i] entries own[i * kCap + j] = kk | pos << 8, where kk is
-  // the slot of the neighbour k in i's row and pos the slot of i in k's row.
-  const int* own_n;
+  // Pairs owned by the sites of each chunk (phase 2 evaluates each unordered
+  // pair once, at its owner), in site order: chunk c holds chunk_np[c] entries
+  // own[c * kChunk * kCap + j] = li | kk << 8 | pos << 16 -- owner site
+  // c * kChunk + li, kk the slot of the neighbour k in its row, pos the slot of
+  // the owner in k's row.
+  const int* chunk_np;
   const int* own;
   // Chunks to evaluate, per precision: act[p * act_stride] = count, then the
   // chunk indices in arena order.  The phase kernels map their blocks onto
@@ -122,7 +124,7 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
 // Owned-pair lists of one instance (after launch_neighbors on the same
 // stream): flags[2] is set if a neighbour relation is not symmetric.
 hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, const int* nbr,
-                            int* own_n, int* own, int* flags);
+                            int* chunk_np, int* own, int* flags);
 
 // Precision of the quadrature-point evaluation: FP64 throughout, or the split
 // form (FP32 points and splines, FP64 distances, sums, records and totals).

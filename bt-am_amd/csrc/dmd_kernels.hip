@@ -223,36 +223,55 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n, const int* nnb, int* n
   if (t < cnt) row[t] = key[t];
 }
 
-// Pairs of site i that i owns, in row order: (slot kk of the neighbour k in
-// i's row) | (slot of i in k's row) << 8.  Rows are sorted, so the reverse slot
-// is a binary search.
+// Owned pairs of one chunk (block = chunk, thread = site), in site order and
+// row order within a site: li | kk << 8 | pos << 16 (owner li of the chunk, kk
+// the slot of the neighbour k in its row, pos the slot of the owner in k's
+// row).  Rows are sorted, so the reverse slot is a binary search.
+DEV int rev_slot(const int* nnb, const int* nbr, int k, int i) {
+  const int* rk = nbr + (size_t)k * kCap;
+  const int nk = nnb[k];
+  int lo = 0, hi = nk;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (rk[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nk && rk[lo] == i ? lo : -1;
+}
+
 __global__ __launch_bounds__(256) void k_own_lists(int base, int n, const int* nnb, const int* nbr,
-                                                   int* own_n, int* own, int* flags) {
+                                                   int* chunk_np, int* own, int* flags) {
+  __shared__ int scan[256];
   const int li = blockIdx.x * 256 + threadIdx.x;
-  if (li >= n) return;
   const int i = base + li;
-  const int cnt = nnb[i];
+  const int cnt = li < n ? nnb[i] : 0;
   const int* row = nbr + (size_t)i * kCap;
-  int* ow = own + (size_t)i * kCap;
   int no = 0;
+  for (int kk = 0; kk < cnt; kk++) no += pair_owner(i, row[kk]);
+  // exclusive scan over the chunk's sites (Hillis-Steele in LDS)
+  scan[threadIdx.x] = no;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const int v = threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const int ch = (base >> 8) + (int)blockIdx.x;
+  int* out = own + (size_t)ch * kChunk * kCap;
+  int j = scan[threadIdx.x] - no;
   for (int kk = 0; kk < cnt; kk++) {
     const int k = row[kk];
     if (!pair_owner(i, k)) continue;
-    const int* rk = nbr + (size_t)k * kCap;
-    const int nk = nnb[k];
-    int lo = 0, hi = nk;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (rk[mid] < i) lo = mid + 1;
-      else hi = mid;
-    }
-    if (lo >= nk || rk[lo] != i) {
+    const int pos = rev_slot(nnb, nbr, k, i);
+    if (pos < 0) {
       flags[2] = 1;
+      out[j++] = (int)threadIdx.x | (kk << 8);  // counted: keep the list dense
       continue;
     }
-    ow[no++] = kk | (lo << 8);
+    out[j++] = (int)threadIdx.x | (kk << 8) | (pos << 16);
   }
-  own_n[i] = no;
+  if (threadIdx.x == 255) chunk_np[ch] = scan[255];
 }
 
 // ---------------------------------------------------------------------------
@@ -281,8 +300,8 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
 }
 
 hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, const int* nbr,
-                            int* own_n, int* own, int* flags) {
-  k_own_lists<<<(n + 255) / 256, 256, 0, st>>>(base, n, nnb, nbr, own_n, own, flags);
+                            int* chunk_np, int* own, int* flags) {
+  k_own_lists<<<(n + 255) / 256, 256, 0, st>>>(base, n, nnb, nbr, chunk_np, own, flags);
   return hipGetLastError();
 }
 

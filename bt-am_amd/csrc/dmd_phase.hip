@@ -342,41 +342,39 @@ DEV float kept_out(float x) { asm volatile("" : "+v"(x)); return x; }
 
 // XR: also form the occupancy terms (instances that relax positions and widths
 // only skip them; their R_x output is zero).
-template <typename P, int K, bool UNR, int RULE, bool XR>
-DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, int g, int inst,
-                      u32x4* stage) {
+//
+// Work: the owned pairs of a chunk form one list; the chunk's B blocks stride
+// through it, each group of four lanes taking four consecutive pairs (the
+// cooperative gathers need the four lanes of a group together).  A pair with
+// an inactive end has exact-zero contributions at the active end (every
+// weight carries both occupancies); the inactive end reads no slots.
+template <typename P, int B, bool UNR, int RULE, bool XR>
+DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int inst, int ch,
+                      int part, u32x4* stage) {
   constexpr bool CO = kCoop2<P>;
   const Box box = s.boxes[inst];
-  const double nu = s.ne[u];
-  const int no = s.own_n[u];
-  const int* own = s.own + (size_t)u * kCap;
-  const int* row = s.nbr + (size_t)u * kCap;
-  if (nu == 0.0) {  // its pairs add nothing: zero the other ends' slots
-    for (int j = g; j < no; j += K) {
-      const int ent = own[j];
-      double* ek = o.slot + ((size_t)row[ent & 255] * kCap + (ent >> 8)) * kSlotW;
-      ek[0] = 0.0; ek[1] = 0.0; ek[2] = 0.0; ek[3] = 0.0; ek[4] = 0.0;
-    }
-    return;  // all K lanes of the site leave together
-  }
   const P* uwt = tab_uw<P>(T);
   const P* pht = tab_ph<P>(T);
   const P* rht = tab_rh<P>(T);
   const int nr = T.nr;
-  const double4 qu = s.qs[u];
-  const double su = qu.w;
-  const int spu = s.spc[u];
-  const double bu = s.beta[u];
-  const double* Ru = o.rec + (size_t)u * kRec;
-  const P* Au = adj_rec(o, u, P(0));
-  const double FPu = Ru[kRecFP];
-  for (int j0 = 0; j0 < no; j0 += K) {  // uniform over the site's lanes
-    const int j = j0 + g;
-    const bool inb = j < no;
-    const int ent = inb ? own[j] : 0;
-    const int kk = ent & 255, pos = ent >> 8;
-    const int k = inb ? row[kk] : u;
-    const double nk = inb ? s.ne[k] : 0.0;  // zero: every weight below vanishes
+  const int c0 = ch * kChunk;
+  const int np = s.chunk_np[ch];
+  const int* own = s.own + (size_t)ch * kChunk * kCap;
+  const int tq = (int)threadIdx.x & 3;
+  for (int p0 = part * 256 + ((int)threadIdx.x & ~3); p0 < np; p0 += B * 256) {
+    const bool inb = p0 + tq < np;  // lanes past the end repeat the group's first pair
+    const int ent = own[inb ? p0 + tq : p0];
+    const int u = c0 + (ent & 255), kk = (ent >> 8) & 255, pos = ent >> 16;
+    const double nu = s.ne[u];
+    const double4 qu = s.qs[u];
+    const double su = qu.w;
+    const int spu = s.spc[u];
+    const double bu = s.beta[u];
+    const double* Ru = o.rec + (size_t)u * kRec;
+    const P* Au = adj_rec(o, u, P(0));
+    const double FPu = Ru[kRecFP];
+    const int k = s.nbr[(size_t)u * kCap + kk];
+    const double nk = s.ne[k];
     const double4 qk = s.qs[k];
     const double sk = qk.w;
     const int spk = s.spc[k];
@@ -392,15 +390,14 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
     double q0 = 0, q1 = 0, q2 = 0, sgu = 0, sgk = 0, xu = 0, xk = 0;  // this pair
     double rd[3];
     min_image(box, qu, qk, rd);
-    idle_disp(inb, rd);
     PairGeo<P> geo;
     geo.init(rd);
-    P p0[3], ir0, off, tmp[9];
+    P pz[3], ir0, off, tmp[9];
     int m;
-    geo.point(T, 0, P(0), p0, m, off, ir0);
-    P c0[8];
-    gather_row8<P, CO>(uwt, rb + m, stage, c0);
-    const P u0 = cval(c0, off), du0 = cder(c0, off), w0 = cval(c0 + 4, off), dw0 = cder(c0 + 4, off);
+    geo.point(T, 0, P(0), pz, m, off, ir0);
+    P cz[8];
+    gather_row8<P, CO>(uwt, rb + m, stage, cz);
+    const P u0 = cval(cz, off), du0 = cder(cz, off), w0 = cval(cz + 4, off), dw0 = cder(cz + 4, off);
     // The two shifted-point walks each hold one constant adjoint and one
     // accumulator: the j-shift walk the role-(a) adjoint laB and the sum Tuk of
     // psi over it, the i-shift walk the role-(b) adjoint lbA and the sum SA.
@@ -412,7 +409,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
     P laB[9], Tuk[9];
     {
       P psi0[9];
-      psi_from(u0, w0, p0, psi0);
+      psi_from(u0, w0, pz, psi0);
       load_blk(Au, kBlkLG0, false, laB);
       applyM(cnk, psi0, tmp);
 #pragma unroll
@@ -470,18 +467,18 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
       for (int c = 0; c < 9; c++) la[c] += tmp[c];
       if (XR) {
         P psi0[9];
-        psi_from(kept_out(u0), w0, p0, psi0);
+        psi_from(kept_out(u0), w0, pz, psi0);
         xk += xbk * (double)dot9(la, psi0);
       }
 #pragma unroll
       for (int c = 0; c < 9; c++) la[c] *= waP;
-      grad_psi(la, p0, ir0, u0, du0, w0, dw0, g3);
+      grad_psi(la, pz, ir0, u0, du0, w0, dw0, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
     }
     P lbA[9], SA[9];
     {
       P psi0[9];
-      psi_from(kept_out(u0), w0, p0, psi0);
+      psi_from(kept_out(u0), w0, pz, psi0);
       nbr_blk<P, CO>(o, k, kBlkLG0, true, stage, lbA);
       applyM(cnu, psi0, tmp);
 #pragma unroll
@@ -533,11 +530,11 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
         lb[c] += tmp[c];
         L[c] = wbP * lb[c];
       }
-      grad_psi(L, p0, ir0, u0, du0, w0, dw0, g3);
+      grad_psi(L, pz, ir0, u0, du0, w0, dw0, g3);
       q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
       if (XR) {
         P psi0[9];
-        psi_from(kept_out(u0), w0, p0, psi0);
+        psi_from(kept_out(u0), w0, pz, psi0);
         xu += xb * (double)dot9(lb, psi0);
       }
     }
@@ -596,9 +593,8 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int u, 
 // Two wavefronts per SIMD: the FP64 pair body fits 256 VGPRs without spills.
 // UNR: the point loops advance two (shifted) or three (pair-rule) points per
 // iteration, so one point's arithmetic overlaps the next point's gathers.
-template <typename P, int K, bool UNR, int RULE>
+template <typename P, int B, bool UNR, int RULE>
 __global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o) {
-  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
   // largest gathers: two 8-value rows (j-shift points), one 8- and three
   // 4-value rows (pair-rule points)
   constexpr bool CO = kCoop2<P>;
@@ -606,14 +602,17 @@ __global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o)
                               ? StageRounds<P, 2, 0, CO>::v : StageRounds<P, 1, 3, CO>::v;
   __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
-  int u, g;
-  if (!site_of<P, K>(s, u, g)) return;
-  const int inst = s.inst[u];
-  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  const int* act = s.act + PrecCode<P>::v * s.act_stride;
+  const int nb = act[0] * B;
+  if ((int)blockIdx.x >= nb) return;  // uniform per block
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int ch = act[1 + lb / B];
+  const int inst = s.inst[ch * kChunk];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;
   if (s.noxr && s.noxr[inst])
-    phase2_pairs<P, K, UNR, RULE, false>(s, T, o, u, g, inst, stage);
+    phase2_pairs<P, B, UNR, RULE, false>(s, T, o, inst, ch, lb % B, stage);
   else
-    phase2_pairs<P, K, UNR, RULE, true>(s, T, o, u, g, inst, stage);
+    phase2_pairs<P, B, UNR, RULE, true>(s, T, o, inst, ch, lb % B, stage);
 }
 
 // Sums of a site's slots (row order, fixed lane order) and the site's own

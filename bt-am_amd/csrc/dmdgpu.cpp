@@ -48,7 +48,7 @@ constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
 constexpr int kChunk = dmd::kChunk;
 // Device bytes per arena site (state, lists, records, outputs, solver vectors).
 constexpr size_t kBytesPerSite = 32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 +
-                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32 + 4 +
+                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32 + 1 +
                                  4 * dmd::kCap + 8 * dmd::kSlotW * dmd::kCap;
 
 struct Spline {
@@ -173,7 +173,7 @@ struct dmdgpu_ctx {
   DevBuf<dmd::Box> d_boxes;
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
-  DevBuf<int> d_spc, d_nnb, d_nbr, d_own_n, d_own;
+  DevBuf<int> d_spc, d_nnb, d_nbr, d_chunk_np, d_own;
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<long long> d_pairs;
@@ -346,7 +346,7 @@ dmd::State make_state(dmdgpu_ctx* c) {
   s.gamma = c->d_gamma.p;
   s.nnb = c->d_nnb.p;
   s.nbr = c->d_nbr.p;
-  s.own_n = c->d_own_n.p;
+  s.chunk_np = c->d_chunk_np.p;
   s.own = c->d_own.p;
   s.act = c->d_act.p;
   s.act_stride = (int)(c->d_act.n / 2);
@@ -398,7 +398,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_inst.ensure(cap, keep));
   HIPCHK(c->d_nnb.ensure(cap, keep));
   HIPCHK(c->d_nbr.ensure(cap * dmd::kCap, keep * dmd::kCap));
-  HIPCHK(c->d_own_n.ensure(cap, keep));
+  HIPCHK(c->d_chunk_np.ensure(cap / kChunk, keep / kChunk));
   HIPCHK(c->d_own.ensure(cap * dmd::kCap, keep * dmd::kCap));
   HIPCHK(c->d_slot.ensure(cap * dmd::kCap * dmd::kSlotW));
   HIPCHK(c->d_cell_of.ensure(cap));
@@ -617,7 +617,7 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   HIPCHK(dmd::launch_neighbors(c->stream, I.box, g, c->rc, I.base, I.n, c->d_qs.p, c->d_cell_of.p,
                                c->d_cell_count.p, c->d_cell_start.p, c->d_cell_cursor.p,
                                c->d_cell_atoms.p, c->d_nnb.p, c->d_nbr.p, c->d_flags.p));
-  HIPCHK(dmd::launch_own_lists(c->stream, I.base, I.n, c->d_nnb.p, c->d_nbr.p, c->d_own_n.p,
+  HIPCHK(dmd::launch_own_lists(c->stream, I.base, I.n, c->d_nnb.p, c->d_nbr.p, c->d_chunk_np.p,
                                c->d_own.p, c->d_flags.p));
   CHK(end_timed(c, KID_NEIGH, ev));
   int flags[3];
@@ -713,7 +713,8 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   HIPCHK(hipMemcpy(c->d_ne.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_xr.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_nnb.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_own_n.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_chunk_np.p + b / kChunk, zi.data(), sizeof(int) * (cp / kChunk),
+                   hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_dnn.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_Xn.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_Xt.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
@@ -1319,7 +1320,7 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
                            &c->d_Xn,  &c->d_Xt};
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
-                        &c->d_own_n,      &c->d_own,
+                        &c->d_chunk_np,   &c->d_own,
                         &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
                         &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_noxr,
                         &c->d_chunk_lo,
