@@ -72,13 +72,85 @@ constexpr int kShiftsPerWalk = 3;
 // gains in FP64 only (its FP32 body is already VALU-heavy).
 template <typename P> constexpr bool kCoop1 = true;
 template <typename P> constexpr bool kCoop2 = sizeof(P) == 8;
+// Phase-1 pair-rule points in their own pass, once per unordered pair
+// (k_phase1_pairs): faster in FP64; in FP32 the extra pass costs more than
+// the halved point count saves (measured on MI355X), so k_phase1 keeps them.
+template <typename P> constexpr bool kPairs1 = sizeof(P) == 8;
 
+
+// Pair-rule points of phase 1, once per unordered pair: their sums are
+// symmetric in the pair (C6 = sum w <psi,psi>, Phi = sum w phi) except the
+// densities, one per end (rho of the other end's species), and both come from
+// the same points.  The owner writes (C6, Phi, density at that end) into the
+// slots of both ends (the slots are free until phase 2); k_phase1 adds them up
+// in row order.  Work layout as in phase 2 (per-chunk owned-pair lists).
+template <typename P, int B, int RULE>
+__global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs o) {
+  constexpr bool CO = kCoop1<P>;
+  __shared__ u32x4 stage_all[4][StageRounds<P, 1, 3, CO>::v * kStageSlots];
+  u32x4* stage = stage_all[threadIdx.x >> 6];
+  const int* act = s.act + PrecCode<P>::v * s.act_stride;
+  const int nb = act[0] * B;
+  if ((int)blockIdx.x >= nb) return;  // uniform per block
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int ch = act[1 + lb / B];
+  const int inst = s.inst[ch * kChunk];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;
+  const Box box = s.boxes[inst];
+  const P* uwt = tab_uw<P>(T);
+  const P* pht = tab_ph<P>(T);
+  const P* rht = tab_rh<P>(T);
+  const int nr = T.nr;
+  const int c0 = ch * kChunk;
+  const int np = s.chunk_np[ch];
+  const int* own = s.own + (size_t)ch * kChunk * kCap;
+  const int tq = (int)threadIdx.x & 3;
+  for (int p0 = (lb % B) * 256 + ((int)threadIdx.x & ~3); p0 < np; p0 += B * 256) {
+    const bool inb = p0 + tq < np;  // lanes past the end repeat the group's first pair
+    const int ent = own[inb ? p0 + tq : p0];
+    const int u = c0 + (ent & 255), kk = (ent >> 8) & 255, pos = ent >> 16;
+    const int k = s.nbr[(size_t)u * kCap + kk];
+    const double4 qu = s.qs[u], qk = s.qs[k];
+    const int spu = s.spc[u], spk = s.spc[k];
+    const int t = pair_slot(spu, spk);
+    double rd[3];
+    min_image(box, qu, qk, rd);
+    PairGeo<P> geo;
+    geo.init(rd);
+    const P su = (P)qu.w, sk = (P)qk.w;
+    double C6 = 0, Phi = 0, Ru = 0, Rk = 0;  // per-pair sums in FP64
+#pragma unroll 1
+    for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
+      P p[3], off, ir, dsi[3], dsj[3], cw[8], c4[3][4], ps[9];
+      int m;
+      const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
+      const int r8 = t * nr + m, r4[3] = {t * nr + m, spk * nr + m, spu * nr + m};
+      const P* t4[3] = {pht, rht, rht};
+      gather_mix<P, 1, 3, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
+      psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
+      C6 += wt * (double)dotM(ps, ps);
+      Phi += wt * (double)cval(c4[0], off);
+      Ru += wt * (double)cval(c4[1], off);
+      Rk += wt * (double)cval(c4[2], off);
+    }
+    if (inb) {
+      double* eu = o.slot + ((size_t)u * kCap + kk) * kSlotW;
+      double* ek = o.slot + ((size_t)k * kCap + pos) * kSlotW;
+      eu[0] = C6; eu[1] = Phi; eu[2] = Ru;
+      ek[0] = C6; ek[1] = Phi; ek[2] = Rk;
+    }
+  }
+}
 
 template <typename P, int K, int RULE>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   static_assert(K >= 4, "a site's lanes must cover whole gather groups");
   constexpr bool CO = kCoop1<P>;
-  __shared__ u32x4 stage_all[4][StageRounds<P, kShiftsPerWalk, 0, CO>::v * kStageSlots];
+  // largest gathers: the rows of one i-shift walk; one 8- and two 4-value rows
+  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, 0, CO>::v > StageRounds<P, 1, 2, CO>::v
+                              ? StageRounds<P, kShiftsPerWalk, 0, CO>::v
+                              : StageRounds<P, 1, 2, CO>::v;
+  __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   int i, g;
   if (!site_of<P, K>(s, i, g)) return;
@@ -97,8 +169,6 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     return;
   }
   const P* uwt = tab_uw<P>(T);
-  const P* pht = tab_ph<P>(T);
-  const P* rht = tab_rh<P>(T);
   const int nr = T.nr;
   const double4 qi = s.qs[i];
   const P si = (P)qi.w;
@@ -222,23 +292,34 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll
     for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
     TPsi += nj2 * (double)dotM(Tj, psi0);
-    double Phi = 0, R = 0, C6 = 0;  // per-pair sums over the rule's points in FP64
     // pair and coincident points of the pair rule
+    if constexpr (kPairs1<P>) {  // per-pair sums from k_phase1_pairs
+      if (inb) {
+        const double* e = o.slot + ((size_t)i * kCap + kk) * kSlotW;
+        Cc += nj2 * e[0];
+        Ep += nj * e[1];
+        rho += nj * e[2];
+      }
+    } else {
+      const P* pht = tab_ph<P>(T);
+      const P* rht = tab_rh<P>(T);
+      double Phi = 0, R = 0, C6 = 0;  // per-pair sums over the rule's points in FP64
 #pragma unroll 1
-    for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      P dsi[3], dsj[3], c4[2][4];
-      const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi, dsj);
-      const int r8 = rb + m, r4[2] = {t * nr + m, spj * nr + m};
-      const P* t4[2] = {pht, rht};
-      gather_mix<P, 1, 2, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
-      psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
-      C6 += wt * (double)dotM(ps, ps);
-      Phi += wt * (double)cval(c4[0], off);
-      R += wt * (double)cval(c4[1], off);
+      for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
+        P dsi[3], dsj[3], c4[2][4];
+        const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi, dsj);
+        const int r8 = rb + m, r4[2] = {t * nr + m, spj * nr + m};
+        const P* t4[2] = {pht, rht};
+        gather_mix<P, 1, 2, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
+        psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
+        C6 += wt * (double)dotM(ps, ps);
+        Phi += wt * (double)cval(c4[0], off);
+        R += wt * (double)cval(c4[1], off);
+      }
+      Cc += nj2 * C6;
+      rho += nj * R;
+      Ep += nj * Phi;
     }
-    Cc += nj2 * C6;
-    rho += nj * R;
-    Ep += nj * Phi;
   }
   if (K > 1) {
     rho = group_sum<K>(rho); Ep = group_sum<K>(Ep); Cc = group_sum<K>(Cc);
@@ -678,6 +759,9 @@ template <typename P, int K, int RULE>
 static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
+  constexpr int kB = 16;  // blocks per chunk of the pair-rule kernel
+  if (kPairs1<P>)
+    k_phase1_pairs<P, kB, RULE><<<(int)((long long)s.n * kB / 256), 256, 0, st>>>(s, T, o);
   k_phase1<P, K, RULE><<<nb, 256, 0, st>>>(s, T, o);
 }
 template <typename P, int K, bool UNR, int RULE>
