@@ -23,10 +23,14 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-# Algorithmic FP64 operations per active ordered pair (FMA = 2, others 1,
-# sqrt / reciprocal = 1), counted from the formulas of each kernel.  25 PQE per
-# pair in each phase.
-FLOP_PER_PAIR = {"phase1_site_sums": 1814, "phase2_gradient": 4755}
+# FP64 floating-point operations per active ordered pair of each phase of the
+# FP64 evaluation (FMA = 2), counted once with the hardware counter
+# SQ_INSTS_VALU_FLOPS_FP64 (x 64 lanes) on this workload and frozen here
+# (profiles/valu_mgh_20x15x15.md).  Phase 1: site sums plus the pair-rule pass
+# (once per unordered pair); phase 2: the pair kernel (each unordered pair
+# once) plus the slot sums.  The PQE count per residual stays the workload's
+# fixed 50 per active ordered pair.
+FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2946}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12          # B/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_mgh_20x15x15_fp64.md")
@@ -42,30 +46,29 @@ def workload():
     return lat.hcp_mgh(20, 15, 15, x_h=(0.0, 0.1), seed=2024)
 
 
-def traffic_per_launch(kernel):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor)."""
+def _phase_row(path, phase, col):
+    """Cell `col` of the per-phase row `phase` of a committed profile summary."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             cells = [c.strip().strip("`") for c in line.split("|")]
-            if len(cells) > 5 and cells[1] == kernel:
-                return (float(cells[4]) + float(cells[5])) * 1e6
+            if len(cells) > col + 1 and cells[1] == phase:
+                return float(cells[col])
     except (OSError, ValueError):
         pass
     return None
 
 
-def valu_util(kernel):
-    """VALU issue utilisation of a kernel from the committed rocprofv3 SQ
-    counter summary (SQ_INSTS_VALU over the issue capacity of its cycles)."""
-    try:
-        for line in open(VALU_FILE):
-            cells = [c.strip().strip("`") for c in line.split("|")]
-            if len(cells) > 8 and cells[1] == kernel:
-                return float(cells[7])
-    except (OSError, ValueError):
-        pass
-    return None
+def traffic_per_launch(phase):
+    """HBM bytes per launch of a phase's kernels from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel
+    factor)."""
+    return _phase_row(TRAFFIC_FILE, phase, 2)
+
+
+def valu_util(phase):
+    """VALU issue utilisation of a phase's kernels from the committed rocprofv3
+    SQ counter summary (SQ_INSTS_VALU over the issue capacity of their cycles)."""
+    return _phase_row(VALU_FILE, phase, 3)
 
 
 def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
@@ -372,16 +375,14 @@ def main():
     launches, ms = kt[dom]
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = FLOP_PER_PAIR[dom] * pairs / avg_s
-    kname = {"phase1_site_sums": f"dmd::k_phase1<double, {k1}>",
-             "phase2_gradient": f"dmd::k_phase2<double, {k2}, {'true' if un else 'false'}>"}[dom]
-    traffic = traffic_per_launch(kname)
+    traffic = traffic_per_launch(dom)
     roof = {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
             "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
             "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
             "flop_per_launch": FLOP_PER_PAIR[dom] * pairs,
             "hbm_achieved_GBs": traffic / avg_s / 1e9 if traffic else None,
             "hbm_frac": traffic / avg_s / HBM_PEAK if traffic else None,
-            "valu_issue_util_profiled": valu_util(kname)}
+            "valu_issue_util_profiled": valu_util(dom)}
 
     solves = None
     if not args.no_solves:
