@@ -1,7 +1,8 @@
 """Autotuner for the phase kernels.
 
-Tunables, per precision: lanes of a wavefront per site in phase 1 (k1) and
-phase 2 (k2), and whether phase 2 unrolls its point evaluations.  Each
+Tunables, per precision: lanes of a wavefront per site in phase 1 (k1),
+blocks per 256-site chunk of the pair kernels (k2), and whether phase 2
+advances two or three points per loop iteration (unroll2).  Each
 candidate is timed with the context's per-kernel HIP events on the current
 state and lists; phase 1 and phase 2 are independent kernels, so each picks its
 own best setting.

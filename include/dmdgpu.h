@@ -138,9 +138,11 @@ dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
  * displacement 1e-5 A).  Occupancies fixed; the state is left at X. */
 dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* ctx, const double* h_X, const double* h_V, double* h_JV);
 
-/* Kernel tuning: lanes of a wavefront that share one site in the phase-1 and
- * phase-2 kernels (4, 8 or 16) and whether phase 2 unrolls its point loops,
- * for both precisions.
+/* Kernel tuning, for both precisions: k_phase1 = lanes of a wavefront that
+ * share one site in phase 1 (4, 8 or 16); k_phase2 = blocks per 256-site chunk
+ * of the pair kernels (4, 8 or 16: on average that many lanes per site);
+ * unroll2 = phase 2 advances two shifted or three pair-rule points per loop
+ * iteration (0: one).
  * The tuner picks them per workload.  No counterpart in the reference. */
 dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* ctx, int k_phase1, int k_phase2, int unroll2);
 /* The same for one precision only (DMDGPU_FP64 or DMDGPU_SPLIT32): batched
