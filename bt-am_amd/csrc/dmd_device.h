@@ -44,6 +44,9 @@ template <> DEV const float* tab_ph<float>(const Tables& T) { return T.ph32; }
 template <typename P> DEV const P* tab_rh(const Tables& T);
 template <> DEV const double* tab_rh<double>(const Tables& T) { return T.rh; }
 template <> DEV const float* tab_rh<float>(const Tables& T) { return T.rh32; }
+template <typename P> DEV const P* tab_pr(const Tables& T);
+template <> DEV const double* tab_pr<double>(const Tables& T) { return T.pr; }
+template <> DEV const float* tab_pr<float>(const Tables& T) { return T.pr32; }
 
 // ---------------------------------------------------------------------------
 // Quadrature-point geometry.  A point is p = r + d e_l (one axis shifted).
@@ -303,17 +306,32 @@ template <typename P, int N8, int N4, bool COOP> struct StageRounds {
   static constexpr int v = COOP ? N8 * RowQ<P, 8>::v + N4 * q4 : 1;
 };
 
-// Value of v at lane R of this lane's group of NQ (quad permutation).
+// Value of v at lane R of this lane's group of NQ: a quad permutation (DPP)
+// for NQ <= 4, a bit-mask swizzle (lane = (lane & 0x18) | R within each half
+// wavefront) for NQ = 8.
 template <int NQ, int R>
 DEV int group_bcast(int v) {
-  constexpr int ctrl = NQ == 4 ? R * 0x55 : (R | (R << 2) | ((2 + R) << 4) | ((2 + R) << 6));
-  return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xf, 0xf, false);
+  if constexpr (NQ == 8) {
+    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (R << 5));
+  } else {
+    constexpr int ctrl = NQ == 4 ? R * 0x55 : (R | (R << 2) | ((2 + R) << 4) | ((2 + R) << 6));
+    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xf, 0xf, false);
+  }
 }
 
 template <int NQ>
 DEV void coop_rounds(const u32x4* base, int row, int a, u32x4* v) {
   if constexpr (NQ == 1) {
     v[0] = base[row];
+  } else if constexpr (NQ == 8) {
+    v[0] = base[(size_t)group_bcast<8, 0>(row) * 8 + a];
+    v[1] = base[(size_t)group_bcast<8, 1>(row) * 8 + a];
+    v[2] = base[(size_t)group_bcast<8, 2>(row) * 8 + a];
+    v[3] = base[(size_t)group_bcast<8, 3>(row) * 8 + a];
+    v[4] = base[(size_t)group_bcast<8, 4>(row) * 8 + a];
+    v[5] = base[(size_t)group_bcast<8, 5>(row) * 8 + a];
+    v[6] = base[(size_t)group_bcast<8, 6>(row) * 8 + a];
+    v[7] = base[(size_t)group_bcast<8, 7>(row) * 8 + a];
   } else if constexpr (NQ == 4) {
     v[0] = base[(size_t)group_bcast<4, 0>(row) * 4 + a];
     v[1] = base[(size_t)group_bcast<4, 1>(row) * 4 + a];
@@ -391,6 +409,66 @@ DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const
       }
     }
   }
+}
+
+// A pair-rule point's rows: o8 = u/w row r8 of t8 (8 values), o16 = fused
+// row r16 of t16 (16 values: phi, rho_Mg, rho_H, 0), with one LDS round trip.
+// C8 / C16: cooperative or direct for each.  stage: CptRounds<P, C8, C16> *
+// kStageSlots slots.
+template <typename P, bool C8, bool C16> struct CptRounds {
+  static constexpr int v = (C8 ? RowQ<P, 8>::v : 0) + (C16 ? RowQ<P, 16>::v : 0) > 0
+                               ? (C8 ? RowQ<P, 8>::v : 0) + (C16 ? RowQ<P, 16>::v : 0) : 1;
+};
+template <typename P, bool C8, bool C16>
+DEV void gather_cpt(const P* t8, int r8, const P* t16, int r16, u32x4* stage, P* o8, P* o16) {
+  constexpr int Q8 = RowQ<P, 8>::v, Q16 = RowQ<P, 16>::v;
+  const int lane = (int)__lane_id();
+  const int a8 = lane & (Q8 - 1), a16 = lane & (Q16 - 1);
+  const u32x4* b8 = reinterpret_cast<const u32x4*>(t8);
+  const u32x4* b16 = reinterpret_cast<const u32x4*>(t16);
+  u32x4 v8[Q8], v16[Q16];
+  if constexpr (C8) coop_rounds<Q8>(b8, r8, a8, v8);
+  else {
+#pragma unroll
+    for (int c = 0; c < Q8; c++) v8[c] = b8[(size_t)r8 * Q8 + c];
+  }
+  if constexpr (C16) coop_rounds<Q16>(b16, r16, a16, v16);
+  else {
+#pragma unroll
+    for (int c = 0; c < Q16 - 1; c++) v16[c] = b16[(size_t)r16 * Q16 + c];  // last chunk: padding
+    v16[Q16 - 1] = u32x4{0, 0, 0, 0};
+  }
+  if constexpr (C8 || C16) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if constexpr (C8) {
+#pragma unroll
+      for (int r = 0; r < Q8; r++) stage[r * kStageSlots + lane] = v8[r];
+    }
+    if constexpr (C16) {
+      constexpr int o = C8 ? Q8 : 0;
+#pragma unroll
+      for (int r = 0; r < Q16; r++) stage[(o + r) * kStageSlots + lane] = v16[r];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if constexpr (C8) {
+#pragma unroll
+      for (int c = 0; c < Q8; c++) v8[c] = stage[a8 * kStageSlots + (lane - a8) + c];
+    }
+    if constexpr (C16) {
+      constexpr int o = C8 ? Q8 : 0;
+#pragma unroll
+      for (int c = 0; c < Q16; c++) v16[c] = stage[(o + a16) * kStageSlots + (lane - a16) + c];
+    }
+  }
+  __builtin_memcpy(o8, v8, sizeof(v8));
+  __builtin_memcpy(o16, v16, sizeof(v16));
+}
+
+// rho coefficients of species slot sp (0 Mg, 1 H) in a fused row.
+template <typename P>
+DEV void fused_rho(const P* f, int sp, P* o) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) o[e] = sp ? f[8 + e] : f[4 + e];
 }
 
 template <typename P, int NR, bool COOP>

@@ -42,6 +42,8 @@ struct Tables {
   const float* uw32;   // FP32 copies of uw, ph, rh (split-precision path)
   const float* ph32;
   const float* rh32;
+  const double* pr;    // fused pair-rule rows [t][m]: phi_t | rho_Mg | rho_H | 0 (16 values)
+  const float* pr32;
   int nr;
   double dxr, inv_dxr;
   float dxr32, inv_dxr32;

@@ -76,6 +76,13 @@ template <typename P> constexpr bool kCoop2 = sizeof(P) == 8;
 // (k_phase1_pairs): faster in FP64; in FP32 the extra pass costs more than
 // the halved point count saves (measured on MI355X), so k_phase1 keeps them.
 template <typename P> constexpr bool kPairs1 = sizeof(P) == 8;
+// Fused pair-rule rows (phi | rho_Mg | rho_H) through the cooperative gather
+// in phase 2 (FP64 always; FP32 as measured).
+template <typename P> constexpr bool kCoopF2 = sizeof(P) == 8;
+// The pair kernels step through their pair lists in groups of this many
+// consecutive lanes (the largest cooperative gather group: 128-B fused rows in
+// FP64), so every gather group runs each step together.
+constexpr int kPairGroup = 8;
 
 
 // Pair-rule points of phase 1, once per unordered pair: their sums are
@@ -87,7 +94,7 @@ template <typename P> constexpr bool kPairs1 = sizeof(P) == 8;
 template <typename P, int B, int RULE>
 __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs o) {
   constexpr bool CO = kCoop1<P>;
-  __shared__ u32x4 stage_all[4][StageRounds<P, 1, 3, CO>::v * kStageSlots];
+  __shared__ u32x4 stage_all[4][CptRounds<P, CO, CO>::v * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   const int* act = s.act + PrecCode<P>::v * s.act_stride;
   const int nb = act[0] * B;
@@ -98,14 +105,13 @@ __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;
   const Box box = s.boxes[inst];
   const P* uwt = tab_uw<P>(T);
-  const P* pht = tab_ph<P>(T);
-  const P* rht = tab_rh<P>(T);
+  const P* prt = tab_pr<P>(T);
   const int nr = T.nr;
   const int c0 = ch * kChunk;
   const int np = s.chunk_np[ch];
   const int* own = s.own + (size_t)ch * kChunk * kCap;
-  const int tq = (int)threadIdx.x & 3;
-  for (int p0 = (lb % B) * 256 + ((int)threadIdx.x & ~3); p0 < np; p0 += B * 256) {
+  const int tq = (int)threadIdx.x & (kPairGroup - 1);
+  for (int p0 = (lb % B) * 256 + ((int)threadIdx.x & ~(kPairGroup - 1)); p0 < np; p0 += B * 256) {
     const bool inb = p0 + tq < np;  // lanes past the end repeat the group's first pair
     const int ent = own[inb ? p0 + tq : p0];
     const int u = c0 + (ent & 255), kk = (ent >> 8) & 255, pos = ent >> 16;
@@ -121,17 +127,17 @@ __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs
     double C6 = 0, Phi = 0, Ru = 0, Rk = 0;  // per-pair sums in FP64
 #pragma unroll 1
     for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      P p[3], off, ir, dsi[3], dsj[3], cw[8], c4[3][4], ps[9];
+      P p[3], off, ir, dsi[3], dsj[3], cw[8], f[16], cr[4], ps[9];
       int m;
       const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
-      const int r8 = t * nr + m, r4[3] = {t * nr + m, spk * nr + m, spu * nr + m};
-      const P* t4[3] = {pht, rht, rht};
-      gather_mix<P, 1, 3, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(cw), c4);
+      gather_cpt<P, CO, CO>(uwt, t * nr + m, prt, t * nr + m, stage, cw, f);
       psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
       C6 += wt * (double)dotM(ps, ps);
-      Phi += wt * (double)cval(c4[0], off);
-      Ru += wt * (double)cval(c4[1], off);
-      Rk += wt * (double)cval(c4[2], off);
+      Phi += wt * (double)cval(f, off);
+      fused_rho(f, spk, cr);
+      Ru += wt * (double)cval(cr, off);
+      fused_rho(f, spu, cr);
+      Rk += wt * (double)cval(cr, off);
     }
     if (inb) {
       double* eu = o.slot + ((size_t)u * kCap + kk) * kSlotW;
@@ -425,8 +431,8 @@ DEV float kept_out(float x) { asm volatile("" : "+v"(x)); return x; }
 // only skip them; their R_x output is zero).
 //
 // Work: the owned pairs of a chunk form one list; the chunk's B blocks stride
-// through it, each group of four lanes taking four consecutive pairs (the
-// cooperative gathers need the four lanes of a group together).  A pair with
+// through it, each group of kPairGroup lanes taking as many consecutive pairs
+// (the cooperative gathers need the lanes of a group together).  A pair with
 // an inactive end has exact-zero contributions at the active end (every
 // weight carries both occupancies); the inactive end reads no slots.
 template <typename P, int B, bool UNR, int RULE, bool XR>
@@ -435,14 +441,13 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
   constexpr bool CO = kCoop2<P>;
   const Box box = s.boxes[inst];
   const P* uwt = tab_uw<P>(T);
-  const P* pht = tab_ph<P>(T);
-  const P* rht = tab_rh<P>(T);
+  const P* prt = tab_pr<P>(T);
   const int nr = T.nr;
   const int c0 = ch * kChunk;
   const int np = s.chunk_np[ch];
   const int* own = s.own + (size_t)ch * kChunk * kCap;
-  const int tq = (int)threadIdx.x & 3;
-  for (int p0 = part * 256 + ((int)threadIdx.x & ~3); p0 < np; p0 += B * 256) {
+  const int tq = (int)threadIdx.x & (kPairGroup - 1);
+  for (int p0 = part * 256 + ((int)threadIdx.x & ~(kPairGroup - 1)); p0 < np; p0 += B * 256) {
     const bool inb = p0 + tq < np;  // lanes past the end repeat the group's first pair
     const int ent = own[inb ? p0 + tq : p0];
     const int u = c0 + (ent & 255), kk = (ent >> 8) & 255, pos = ent >> 16;
@@ -627,17 +632,15 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
       const P crb = (P)(bk * nk * FPk * nu);
       const P cang = (P)(2.0 * (wa * nk + wb * nu));
       auto cpt = [&](int pt) {
-        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3], dsk[3], c[8], c4[3][4];
+        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3], dsk[3], c[8], f[16], ck[4], cu[4];
         int mm;
         const double wt =
             PairRule<RULE>::point(geo, T, pt, suP, skP, p, mm, o2, ir, dsu, dsk);
         const P wtP = (P)wt;
-        const int r8 = rb + mm, r4[3] = {t * nr + mm, spk * nr + mm, spu * nr + mm};
-        const P* t4[3] = {pht, rht, rht};
-        gather_mix<P, 1, 3, CO>(&uwt, &r8, t4, r4, stage, reinterpret_cast<P(*)[8]>(c), c4);
-        const P* cp = c4[0];
-        const P* ck = c4[1];
-        const P* cu = c4[2];
+        gather_cpt<P, CO, kCoopF2<P>>(uwt, rb + mm, prt, rb + mm, stage, c, f);
+        const P* cp = f;
+        fused_rho(f, spk, ck);
+        fused_rho(f, spu, cu);
         const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
         psi_from(uu, ww, p, ps);
         applyM(wtP * cang, ps, L);
@@ -676,11 +679,11 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
 // iteration, so one point's arithmetic overlaps the next point's gathers.
 template <typename P, int B, bool UNR, int RULE>
 __global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o) {
-  // largest gathers: two 8-value rows (j-shift points), one 8- and three
-  // 4-value rows (pair-rule points)
+  // largest gathers: two 8-value rows (j-shift points), an 8- and a 16-value
+  // row (pair-rule points)
   constexpr bool CO = kCoop2<P>;
-  constexpr int kRounds = StageRounds<P, 2, 0, CO>::v > StageRounds<P, 1, 3, CO>::v
-                              ? StageRounds<P, 2, 0, CO>::v : StageRounds<P, 1, 3, CO>::v;
+  constexpr int kRounds = StageRounds<P, 2, 0, CO>::v > CptRounds<P, CO, kCoopF2<P>>::v
+                              ? StageRounds<P, 2, 0, CO>::v : CptRounds<P, CO, kCoopF2<P>>::v;
   __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   const int* act = s.act + PrecCode<P>::v * s.act_stride;

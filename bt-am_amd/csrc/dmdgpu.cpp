@@ -160,8 +160,8 @@ struct dmdgpu_ctx {
   Spline rho[2], embed[2], pair[3], u[3], w[3];
   double mass[2] = {0.0, 0.0};
   bool tables_dirty = true;
-  DevBuf<double> d_uw, d_ph, d_rh, d_em;
-  DevBuf<float> d_uw32, d_ph32, d_rh32, d_rec32;
+  DevBuf<double> d_uw, d_ph, d_rh, d_em, d_pr;
+  DevBuf<float> d_uw32, d_ph32, d_rh32, d_pr32, d_rec32;
   dmd::Tables T{};
   // arena
   int extent = 0;     // sites in use, multiple of kChunk
@@ -286,6 +286,17 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
     pack4(c->pair[t], &ph[(size_t)t * nr * 4]);
   }
   for (int s = 0; s < 2; s++) pack4(c->rho[s], &rh[(size_t)s * nr * 4]);
+  // fused rows for the pair-rule points: phi of the pair slot, rho of Mg, rho of H
+  std::vector<double> pr((size_t)3 * nr * 16, 0.0);
+  for (int t = 0; t < 3; t++)
+    for (int m = 0; m < nr; m++) {
+      double* row = &pr[((size_t)t * nr + m) * 16];
+      for (int k = 0; k < 4; k++) {
+        row[k] = ph[((size_t)t * nr + m) * 4 + k];
+        row[4 + k] = rh[((size_t)0 * nr + m) * 4 + k];
+        row[8 + k] = rh[((size_t)1 * nr + m) * 4 + k];
+      }
+    }
   const int ne_stride = std::max(c->embed[0].n, c->embed[1].n);
   std::vector<double> em((size_t)2 * ne_stride * 4, 0.0);
   for (int s = 0; s < 2; s++) pack4(c->embed[s], &em[(size_t)s * ne_stride * 4]);
@@ -297,9 +308,13 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   HIPCHK(hipMemcpy(c->d_ph.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_rh.p, rh.data(), rh.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_em.p, em.data(), em.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c->d_pr.ensure(pr.size()));
+  HIPCHK(hipMemcpy(c->d_pr.p, pr.data(), pr.size() * 8, hipMemcpyHostToDevice));
   // FP32 copies for the split-precision evaluation
   std::vector<float> uw32(uw.begin(), uw.end()), ph32(ph.begin(), ph.end()),
-      rh32(rh.begin(), rh.end());
+      rh32(rh.begin(), rh.end()), pr32(pr.begin(), pr.end());
+  HIPCHK(c->d_pr32.ensure(pr32.size()));
+  HIPCHK(hipMemcpy(c->d_pr32.p, pr32.data(), pr32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(c->d_uw32.ensure(uw32.size()));
   HIPCHK(c->d_ph32.ensure(ph32.size()));
   HIPCHK(c->d_rh32.ensure(rh32.size()));
@@ -314,6 +329,8 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   T.uw32 = c->d_uw32.p;
   T.ph32 = c->d_ph32.p;
   T.rh32 = c->d_rh32.p;
+  T.pr = c->d_pr.p;
+  T.pr32 = c->d_pr32.p;
   T.nr = nr;
   T.dxr = dx;
   T.inv_dxr = 1.0 / dx;
@@ -1311,8 +1328,10 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   c->d_uw32.release();
   c->d_ph32.release();
   c->d_rh32.release();
+  c->d_pr32.release();
   c->d_rec32.release();
   DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph,    &c->d_rh,  &c->d_em,       &c->d_ne,
+                           &c->d_pr,
                            &c->d_xr,  &c->d_beta,  &c->d_gamma, &c->d_rec,    &c->d_V0,
                            &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
                            &c->d_slot,
