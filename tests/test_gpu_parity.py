@@ -191,6 +191,12 @@ def test_error_codes(gpu):
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.neighbors()
     assert e.value.code == 5
+    empty = {k: (v[:0] if isinstance(v, np.ndarray) and v.ndim and k not in
+                 ("lo", "hi", "periodic") else v) for k, v in small.items()}
+    empty["n"] = 0
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.add_instance(empty)  # an instance needs at least one site
+    assert e.value.code == 1
 
 
 # Fifth-order pair rule (27-point tensor Gauss-Hermite on the relative
