@@ -169,6 +169,22 @@ def test_full_size_mgh_20x15x15(gpu, oracle):
     assert np.abs(out["Rq"].sum(0)).max() < 1e-9 * np.abs(out["Rq"]).max() * sysd["n"]
 
 
+@pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "cluster_mgh"])
+def test_zero_temperature_density(gpu, oracle, case):
+    """The zero-temperature density (reference evaluate_rho_i_adp_MgHx,
+    src/ADP/MgHx-mf-V-bulk.cpp:83-143) as the sigma -> 0 limit of the GPU
+    mean-field density: at sigma = 1e-9 A the rule's points sit within 3e-9 A
+    of r_ij and, being symmetric, cancel the first-order change, so dmdgpu_mf_rho
+    equals the oracle's zero-T density to rounding."""
+    say(f"[gpu] zero-T density {case}")
+    sysd = CASES[case]()
+    sysd["sigma"][:] = 1e-9
+    (nnb, nbr), _, _ = run_both(gpu, oracle, sysd)
+    rho = gpu.mf_rho()
+    ref = oracle.ref_rho0(oracle.system(sysd, nnb, nbr))
+    assert rel(rho, ref) < 1e-12
+
+
 def test_error_codes(gpu):
     say("[gpu] error codes")
     small = lat.hcp_mgh(3, 2, 2, seed=0)  # periodic edges < 2 r_c
