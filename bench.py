@@ -30,7 +30,7 @@ import numpy as np  # noqa: E402
 # (once per unordered pair); phase 2: the pair kernel (each unordered pair
 # once) plus the slot sums.  The PQE count per residual stays the workload's
 # fixed 50 per active ordered pair.
-FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2946}
+FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2943}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12          # B/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_mgh_20x15x15_fp64.md")
