@@ -1,9 +1,13 @@
 // dmd_phase.hip -- phase-1 / phase-2 kernels of the mean-field DMD residual.
 //
-// Work decomposition: K consecutive lanes of a wavefront own one site; each
-// lane walks every K-th neighbour of the site and the K partial sums are
-// combined with a butterfly of cross-lane shuffles.  All reductions run in a
-// fixed order, so a site's result does not depend on what else is in flight.
+// Work decomposition: in k_phase1, K consecutive lanes of a wavefront own one
+// site; each lane walks every K-th neighbour of the site and the K partial sums
+// are combined with a butterfly of cross-lane shuffles.  The pair kernels
+// (k_phase1_pairs, k_phase2) evaluate each unordered pair once, at its owner,
+// from per-chunk lists of owned pairs, and write both ends' contributions into
+// their neighbour-row slots; the site sums read the slots in row order.  All
+// reductions run in a fixed order, so a site's result does not depend on what
+// else is in flight.
 //
 // The per-point quantities are the reference's Gaussian-averaged ADP terms
 //   rho_ij  = n_j rho(r)                 (src/ADP/MgHx-ADP.cpp:526-553)
