@@ -279,7 +279,7 @@ DEV void idle_disp(bool inb, double* r) {
 }
 
 // ---------------------------------------------------------------------------
-// Cooperative row gathers.  Each lane needs whole spline rows (8 or 4
+// Cooperative row gathers.  Each lane needs whole table rows (8 or 16
 // coefficients) at lane-dependent indices.  A per-lane load touches one cache
 // line per lane and per 16-byte load, and the vector memory pipeline's time
 // grows with the number of distinct lines an instruction touches.  Here the
@@ -298,12 +298,11 @@ DEV void idle_disp(bool inb, double* r) {
 // ---------------------------------------------------------------------------
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kStageSlots = 65;  // 16-byte slots per round: 64 lanes + 1 (LDS bank skew)
-// 16-byte chunks per row of W values; stage rounds a gather of N8 8-value and
-// N4 4-value rows uses.
+// 16-byte chunks per row of W values; stage rounds a gather of N 8-value rows
+// uses.
 template <typename P, int W> struct RowQ { static constexpr int v = (int)(W * sizeof(P) / 16); };
-template <typename P, int N8, int N4, bool COOP> struct StageRounds {
-  static constexpr int q4 = RowQ<P, 4>::v > 1 ? RowQ<P, 4>::v : 0;  // 16-B rows: direct
-  static constexpr int v = COOP ? N8 * RowQ<P, 8>::v + N4 * q4 : 1;
+template <typename P, int N, bool COOP> struct StageRounds {
+  static constexpr int v = COOP ? N * RowQ<P, 8>::v : 1;
 };
 
 // Value of v at lane R of this lane's group of NQ: a quad permutation (DPP)
@@ -343,70 +342,38 @@ DEV void coop_rounds(const u32x4* base, int row, int a, u32x4* v) {
   }
 }
 
-// o8[q] = row r8[q] of t8 (rows of 8 values), q < N8, and o4[q] = row r4[q] of
-// t4[q] (rows of 4 values), q < N4, with one LDS round trip.  stage: this
-// wavefront's LDS area, StageRounds<P, N8, N4> * kStageSlots slots.
-template <typename P, int N8, int N4, bool COOP>
-DEV void gather_mix(const P* const* t8, const int* r8, const P* const* t4, const int* r4,
-                    u32x4* stage, P (*o8)[8], P (*o4)[4]) {
+// o8[q] = row r8[q] of t8[q] (rows of 8 values), q < N, with one LDS round
+// trip.  stage: this wavefront's LDS area, StageRounds<P, N, COOP> *
+// kStageSlots slots.
+template <typename P, int N, bool COOP>
+DEV void gather8(const P* const* t8, const int* r8, u32x4* stage, P (*o8)[8]) {
   if constexpr (!COOP) {
 #pragma unroll
-    for (int q = 0; q < N8; q++) {
+    for (int q = 0; q < N; q++) {
       const P* c = t8[q] + ((size_t)r8[q] << 3);
 #pragma unroll
       for (int e = 0; e < 8; e++) o8[q][e] = c[e];
     }
-#pragma unroll
-    for (int q = 0; q < N4; q++) {
-      const P* c = t4[q] + ((size_t)r4[q] << 2);
-#pragma unroll
-      for (int e = 0; e < 4; e++) o4[q][e] = c[e];
-    }
   } else {
-    constexpr int Q8 = RowQ<P, 8>::v, Q4 = RowQ<P, 4>::v;
-    constexpr int S4 = StageRounds<P, 0, 1, true>::q4;  // stage rounds per 4-value row
+    constexpr int Q8 = RowQ<P, 8>::v;
     const int lane = (int)__lane_id();
-    const int a8 = lane & (Q8 - 1), a4 = lane & (Q4 - 1);
-    u32x4 v8[N8 > 0 ? N8 : 1][Q8], v4[N4 > 0 ? N4 : 1][Q4];
+    const int a8 = lane & (Q8 - 1);
+    u32x4 v8[N][Q8];
 #pragma unroll
-    for (int q = 0; q < N8; q++)
+    for (int q = 0; q < N; q++)
       coop_rounds<Q8>(reinterpret_cast<const u32x4*>(t8[q]), r8[q], a8, v8[q]);
-#pragma unroll
-    for (int q = 0; q < N4; q++)
-      coop_rounds<Q4>(reinterpret_cast<const u32x4*>(t4[q]), r4[q], a4, v4[q]);
-    if constexpr (S4 == 0) {  // 16-B rows were loaded whole by their own lanes
-#pragma unroll
-      for (int q = 0; q < N4; q++) __builtin_memcpy(o4[q], &v4[q][0], sizeof(v4[q][0]));
-    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-    for (int q = 0; q < N8; q++)
+    for (int q = 0; q < N; q++)
 #pragma unroll
       for (int r = 0; r < Q8; r++) stage[(q * Q8 + r) * kStageSlots + lane] = v8[q][r];
-    if constexpr (S4 > 0) {
-#pragma unroll
-      for (int q = 0; q < N4; q++)
-#pragma unroll
-        for (int r = 0; r < Q4; r++)
-          stage[(N8 * Q8 + q * Q4 + r) * kStageSlots + lane] = v4[q][r];
-    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-    for (int q = 0; q < N8; q++) {
+    for (int q = 0; q < N; q++) {
       u32x4 w[Q8];
 #pragma unroll
       for (int c = 0; c < Q8; c++) w[c] = stage[(q * Q8 + a8) * kStageSlots + (lane - a8) + c];
       __builtin_memcpy(o8[q], w, sizeof(w));
-    }
-    if constexpr (S4 > 0) {
-#pragma unroll
-      for (int q = 0; q < N4; q++) {
-        u32x4 w[Q4];
-#pragma unroll
-        for (int c = 0; c < Q4; c++)
-          w[c] = stage[(N8 * Q8 + q * Q4 + a4) * kStageSlots + (lane - a4) + c];
-        __builtin_memcpy(o4[q], w, sizeof(w));
-      }
     }
   }
 }
@@ -476,18 +443,12 @@ DEV void gather_rows8(const P* tab, const int* rows, u32x4* stage, P (*out)[8]) 
   const P* t8[NR];
 #pragma unroll
   for (int q = 0; q < NR; q++) t8[q] = tab;
-  gather_mix<P, NR, 0, COOP>(t8, rows, nullptr, nullptr, stage, out, nullptr);
+  gather8<P, NR, COOP>(t8, rows, stage, out);
 }
 
 template <typename P, bool COOP>
 DEV void gather_row8(const P* tab, int row, u32x4* stage, P* out) {
   gather_rows8<P, 1, COOP>(tab, &row, stage, reinterpret_cast<P(*)[8]>(out));
-}
-
-// A 4-value row into registers (issued ahead of a gather so both overlap).
-template <typename P>
-DEV void load4(const P* c, P* o) {
-  o[0] = c[0]; o[1] = c[1]; o[2] = c[2]; o[3] = c[3];
 }
 
 // XCD-aware block order: the hardware deals consecutive workgroups to the 8

@@ -157,8 +157,8 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   static_assert(K >= 4, "a site's lanes must cover whole gather groups");
   constexpr bool CO = kCoop1<P>;
   // largest gathers: the rows of one i-shift walk; a u/w and a fused row
-  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, 0, CO>::v > CptRounds<P, CO, CO>::v
-                              ? StageRounds<P, kShiftsPerWalk, 0, CO>::v
+  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, CO>::v > CptRounds<P, CO, CO>::v
+                              ? StageRounds<P, kShiftsPerWalk, CO>::v
                               : CptRounds<P, CO, CO>::v;
   __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
@@ -404,7 +404,7 @@ DEV void nbr_blk(const Outputs& o, int k, int b, bool flip, u32x4* stage, P* out
   const P tail = adj_rec(o, k, P(0))[kRecTail + b];
   const int r = k * adj_rows<P>() + b;
   P row[1][8];
-  gather_mix<P, 1, 0, CO>(&base, &r, nullptr, nullptr, stage, row, nullptr);
+  gather8<P, 1, CO>(&base, &r, stage, row);
   blk_from(row[0], tail, flip, out);
 }
 
@@ -516,7 +516,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
       const P tail = Ak[kRecTail + blk];
       const int r8[2] = {rb + mm, k * adj_rows<P>() + blk};
       const P* t8[2] = {uwt, adj_base(o, P(0))};
-      gather_mix<P, 2, 0, CO>(t8, r8, nullptr, nullptr, stage, c, nullptr);
+      gather8<P, 2, CO>(t8, r8, stage, c);
       blk_from(c[1], tail, true, lb);
       const P uu = cval(c[0], o2), du = cder(c[0], o2), ww = cval(c[0] + 4, o2),
               dw = cder(c[0] + 4, o2);
@@ -684,8 +684,8 @@ __global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o)
   // largest gathers: two 8-value rows (j-shift points), an 8- and a 16-value
   // row (pair-rule points)
   constexpr bool CO = kCoop2<P>;
-  constexpr int kRounds = StageRounds<P, 2, 0, CO>::v > CptRounds<P, CO, kCoopF2<P>>::v
-                              ? StageRounds<P, 2, 0, CO>::v : CptRounds<P, CO, kCoopF2<P>>::v;
+  constexpr int kRounds = StageRounds<P, 2, CO>::v > CptRounds<P, CO, kCoopF2<P>>::v
+                              ? StageRounds<P, 2, CO>::v : CptRounds<P, CO, kCoopF2<P>>::v;
   __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   const int* act = s.act + PrecCode<P>::v * s.act_stride;
