@@ -227,6 +227,7 @@ typedef struct {
   void (*F)(const or_tables*, double*, const double*, const double*, const int*);
   void (*dF_dq)(const or_tables*, int, double*, const double*, const double*, const int*);
   void (*dF_dn)(const or_tables*, int, double*, const double*, const double*, const int*);
+  void (*d2F_dq2)(const or_tables*, int, double*, const double*, const double*, const int*);
 } or_fn;
 
 static inline double rvec(const double* q, int a, int b, double* r) {
@@ -374,10 +375,110 @@ static void f_quad_dn(const or_tables* T, int dir, double* out, const double* n,
   }
 }
 
-static const or_fn FN_RHO = {f_rho, f_rho_dq, f_rho_dn};
-static const or_fn FN_PAIR = {f_pair, f_pair_dq, f_pair_dn};
-static const or_fn FN_DIP = {f_dip, f_dip_dq, f_dip_dn};
-static const or_fn FN_QUAD = {f_quad, f_quad_dq, f_quad_dn};
+/* Second derivatives.  code = ns * a + b is d2F / dq_a dq_b (3 x 3, row
+ * index from q_a); the reference's Hessians are d2_rho_ij_dq2 (:664-731),
+ * d2V_pair_ij_dq2 (:1013-1087), dV2_dipole_ij1j2_dq2 (:1435-1601) and
+ * dV2_quadrupole_ij1j2_dq2 (:2000-2266), bound at :514, :855, :1220, :1769.  Written here as products of radial
+ * factors: for c(|r|), d2c/dr2 = c'' rr^T/r^2 + c' (I - rr^T/r^2)/r. */
+static inline void rad_hess(double d1, double d2, const double* r, double nr, double k,
+                            double* H) {
+  double inv = 1.0 / nr;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      double tt = r[a] * r[b] * inv * inv;
+      H[3 * a + b] += k * (d2 * tt + d1 * ((a == b) - tt) * inv);
+    }
+}
+/* Pair functions depend on r = q_0 - q_1 only: codes 0 and 3 carry +, 1 and 2 -. */
+static inline double pair_code_sign(int code) { return (code == 0 || code == 3) ? 1.0 : -1.0; }
+static void f_rho_d2q(const or_tables* T, int code, double* out, const double* n, const double* q,
+                      const int* spc) {
+  double r[3]; double nr = sqrt(rvec(q, 0, 1, r));
+  const or_spline* s = &T->rho[sslot(spc[1])];
+  memset(out, 0, 9 * sizeof(double));
+  rad_hess(sp_d1(s, nr), sp_d2(s, nr), r, nr, pair_code_sign(code) * n[1], out);
+}
+static void f_pair_d2q(const or_tables* T, int code, double* out, const double* n, const double* q,
+                       const int* spc) {
+  double r[3]; double nr = sqrt(rvec(q, 0, 1, r));
+  const or_spline* s = &T->pair[pslot(spc[0], spc[1])];
+  memset(out, 0, 9 * sizeof(double));
+  rad_hess(sp_d1(s, nr), sp_d2(s, nr), r, nr, pair_code_sign(code) * 0.5 * n[0] * n[1], out);
+}
+
+/* Triplet functions f = c1(|r1|) c2(|r2|) G(r1, r2), r_k = q_0 - q_k.  With
+ * e_k = dc_k/dr_k, E_k = d2c_k/dr_k2, g_k = dG/dr_k and G_kl = d2G/dr_k dr_l:
+ *   H11 = c2 (G E1 + e1 g1^T + g1 e1^T + c1 G11),  H22 likewise,
+ *   H12 = G e1 e2^T + c2 e1 g2^T + c1 g1 e2^T + c1 c2 G12   (= d2f/dr1 dr2),
+ * H21 = H12^T.  Since d/dq_0 = d/dr1 + d/dr2 and d/dq_k = -d/dr_k, code
+ * 3 a + b is sum_kl w_ak w_bl H_kl with w_0 = (1, 1), w_1 = (-1, 0),
+ * w_2 = (0, -1): code 0 = H11 + H12 + H21 + H22, code 4 = H11, code 8 = H22. */
+typedef struct { double G, g1[3], g2[3], G11[9], G22[9], G12[9]; } bil;
+static void trip_d2(const trip* t, const or_spline* s1, const or_spline* s2, double k1, double k2,
+                    const bil* b, int code, double* out) {
+  double c1 = k1 * sp_val(s1, t->n1), d1 = k1 * sp_d1(s1, t->n1), dd1 = k1 * sp_d2(s1, t->n1);
+  double c2 = k2 * sp_val(s2, t->n2), d2 = k2 * sp_d1(s2, t->n2), dd2 = k2 * sp_d2(s2, t->n2);
+  double e1[3], e2[3], E1[9] = {0}, E2[9] = {0}, H11[9], H22[9], H12[9];
+  for (int a = 0; a < 3; a++) { e1[a] = d1 * t->r1[a] / t->n1; e2[a] = d2 * t->r2[a] / t->n2; }
+  rad_hess(d1, dd1, t->r1, t->n1, 1.0, E1);
+  rad_hess(d2, dd2, t->r2, t->n2, 1.0, E2);
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) {
+      int ac = 3 * a + c;
+      H11[ac] = c2 * (b->G * E1[ac] + e1[a] * b->g1[c] + b->g1[a] * e1[c] + c1 * b->G11[ac]);
+      H22[ac] = c1 * (b->G * E2[ac] + e2[a] * b->g2[c] + b->g2[a] * e2[c] + c2 * b->G22[ac]);
+      H12[ac] = b->G * e1[a] * e2[c] + c2 * e1[a] * b->g2[c] + c1 * b->g1[a] * e2[c] +
+                c1 * c2 * b->G12[ac];
+    }
+  /* d/dq_0 = d/dr1 + d/dr2, d/dq_1 = -d/dr1, d/dq_2 = -d/dr2 */
+  static const double wt[3][2] = {{1, 1}, {-1, 0}, {0, -1}};
+  int sa = code / 3, sb = code % 3;
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      out[3 * a + c] = wt[sa][0] * wt[sb][0] * H11[3 * a + c] +
+                       wt[sa][0] * wt[sb][1] * H12[3 * a + c] +
+                       wt[sa][1] * wt[sb][0] * H12[3 * c + a] +
+                       wt[sa][1] * wt[sb][1] * H22[3 * a + c];
+}
+/* dipole: G = 1/2 r1.r2 */
+static void f_dip_d2q(const or_tables* T, int code, double* out, const double* n, const double* q,
+                      const int* spc) {
+  trip g; trip_geo(q, &g);
+  bil b;
+  memset(&b, 0, sizeof(b));
+  b.G = 0.5 * g.dot;
+  for (int a = 0; a < 3; a++) {
+    b.g1[a] = 0.5 * g.r2[a];
+    b.g2[a] = 0.5 * g.r1[a];
+    b.G12[4 * a] = 0.5;
+  }
+  trip_d2(&g, &T->u[pslot(spc[0], spc[1])], &T->u[pslot(spc[0], spc[2])], n[0] * n[1],
+          n[0] * n[2], &b, code, out);
+}
+/* quadrupole: G = 1/2 (r1.r2)^2 - 1/6 r1^2 r2^2 */
+static void f_quad_d2q(const or_tables* T, int code, double* out, const double* n, const double* q,
+                       const int* spc) {
+  trip g; trip_geo(q, &g);
+  bil b;
+  b.G = 0.5 * g.dot * g.dot - g.r1s * g.r2s / 6.0;
+  for (int a = 0; a < 3; a++) {
+    b.g1[a] = g.dot * g.r2[a] - g.r2s * g.r1[a] / 3.0;
+    b.g2[a] = g.dot * g.r1[a] - g.r1s * g.r2[a] / 3.0;
+    for (int c = 0; c < 3; c++) {
+      double id = (a == c);
+      b.G11[3 * a + c] = g.r2[a] * g.r2[c] - g.r2s * id / 3.0;
+      b.G22[3 * a + c] = g.r1[a] * g.r1[c] - g.r1s * id / 3.0;
+      b.G12[3 * a + c] = g.r2[a] * g.r1[c] + g.dot * id - 2.0 * g.r1[a] * g.r2[c] / 3.0;
+    }
+  }
+  trip_d2(&g, &T->w[pslot(spc[0], spc[1])], &T->w[pslot(spc[0], spc[2])], n[0] * n[1],
+          n[0] * n[2], &b, code, out);
+}
+
+static const or_fn FN_RHO = {f_rho, f_rho_dq, f_rho_dn, f_rho_d2q};
+static const or_fn FN_PAIR = {f_pair, f_pair_dq, f_pair_dn, f_pair_d2q};
+static const or_fn FN_DIP = {f_dip, f_dip_dq, f_dip_dn, f_dip_d2q};
+static const or_fn FN_QUAD = {f_quad, f_quad_dq, f_quad_dn, f_quad_d2q};
 
 /* ------------------------------------------------------------------------- */
 /* Gaussian measure: src/Numerical/Quadrature-Measure.cpp:30-110              */
@@ -473,6 +574,34 @@ static double gh3_dxi(const or_tables* T, const measure* m, const or_fn* f, int 
       acc += v * W;
     }
   return acc;
+}
+
+/* One pair / triplet function at explicit positions (for tests): which 0 rho,
+ * 1 pair, 2 dipole, 3 quadrupole; deriv 0 value (out[0]), 1 dF/dq_code (3),
+ * 2 d2F for code (3 x 3).  Species are slots here (0 Mg, 1 H). */
+int or_fn_eval(const or_tables* T, int which, int deriv, int code, const double* n,
+               const double* q, const int* slots, double* out) {
+  static const or_fn* fns[4] = {&FN_RHO, &FN_PAIR, &FN_DIP, &FN_QUAD};
+  if (which < 0 || which > 3) return 1;
+  int spc[3];
+  for (int k = 0; k < (which < 2 ? 2 : 3); k++) spc[k] = slots[k] ? SPC_H : SPC_MG;
+  const or_fn* f = fns[which];
+  if (deriv == 0) f->F(T, out, n, q, spc);
+  else if (deriv == 1) f->dF_dq(T, code, out, n, q, spc);
+  else f->d2F_dq2(T, code, out, n, q, spc);
+  return 0;
+}
+
+/* meanfield_integral_gh_3th_d2mq (:382-463): the rule applied to d2F/dq2. */
+static void gh3_d2mq(const or_tables* T, const measure* m, const or_fn* f, int code, double* out) {
+  double W = 1.0 / (2.0 * m->D), ql[9], h[9];
+  for (int a = 0; a < 9; a++) out[a] = 0.0;
+  for (int sg = 0; sg < 2; sg++)
+    for (int l = 0; l < m->D; l++) {
+      gh3_point(m, l, sg ? -1.0 : 1.0, ql);
+      f->d2F_dq2(T, code, h, m->n, ql, m->spc);
+      for (int a = 0; a < 9; a++) out[a] += h[a] * W;
+    }
 }
 
 /* Measure set-up for the pair (i, j) and the triplet (i, j1, j2), with
@@ -662,6 +791,76 @@ double or_ref_DV0_i_Dxi_u(const or_system* s, const or_tables* T, int u, int i,
   if (s->spc[i] == SPC_H && i != u) de = s->x[i] * sp_d1(emb, mf_rho[i]) * drho;
   if (s->spc[i] == SPC_MG) de = sp_d1(emb, mf_rho[i]) * drho; /* (:2311-2315) */
   return de + dv;
+}
+
+/* evaluate_D2V0_i_Dmeanq2_u_MgHx (:1366-1599): d2 V0_i / dq_u2 (3 x 3).  Only
+ * the diagonal codes of each measure enter (2 dir + dir on pairs, 4 dir on
+ * triplets), so on the coincident triplet (i, u, u) the two codes of u are
+ * summed and the mixed codes 5, 7 are not; masks are the H-only ones of that
+ * function (:1400, :1425, :1439 "<=", :1513 "<="). */
+void or_ref_D2V0_i_Dq2_u(const or_system* s, const or_tables* T, int u, int i,
+                         const double* mf_rho, double out[9]) {
+  for (int a = 0; a < 9; a++) out[a] = 0.0;
+  if (s->spc[u] == SPC_H && s->x[u] < OR_MIN_OCCUPANCY) return;
+  if (s->spc[i] == SPC_H && s->x[i] < OR_MIN_OCCUPANCY) return;
+  double drho[3] = {0, 0, 0}, d2rho[9] = {0}, hv[9] = {0}, g[3], h[9];
+  for (int k1 = 0; k1 < s->nnb[i]; k1++) {
+    int j1 = NB(s, i, k1);
+    if (s->spc[j1] == SPC_H && s->x[j1] <= OR_MIN_OCCUPANCY) continue;
+    measure m;
+    pair_measure(s, i, j1, &m);
+    int sites2[2] = {i, j1};
+    for (int dir = 0; dir < 2; dir++) {
+      if (sites2[dir] != u) continue;
+      gh3_dmq(T, &m, &FN_RHO, dir, g);
+      for (int a = 0; a < 3; a++) drho[a] += g[a];
+      gh3_d2mq(T, &m, &FN_RHO, 3 * dir, h);
+      for (int a = 0; a < 9; a++) d2rho[a] += h[a];
+      gh3_d2mq(T, &m, &FN_PAIR, 3 * dir, h);
+      for (int a = 0; a < 9; a++) hv[a] += h[a];
+    }
+    for (int k2 = k1; k2 < s->nnb[i]; k2++) {
+      int j2 = NB(s, i, k2);
+      if (s->spc[j2] == SPC_H && s->x[j2] <= OR_MIN_OCCUPANCY) continue;
+      int sites3[3] = {i, j1, j2};
+      if (sites3[0] != u && sites3[1] != u && sites3[2] != u) continue;
+      double f = (k1 == k2) ? 1.0 : 2.0;
+      measure mt;
+      trip_measure(s, i, j1, j2, &mt);
+      for (int dir = 0; dir < 3; dir++) {
+        if (sites3[dir] != u) continue;
+        gh3_d2mq(T, &mt, &FN_DIP, 4 * dir, h);
+        for (int a = 0; a < 9; a++) hv[a] += f * h[a];
+        gh3_d2mq(T, &mt, &FN_QUAD, 4 * dir, h);
+        for (int a = 0; a < 9; a++) hv[a] += f * h[a];
+      }
+    }
+  }
+  const or_spline* emb = &T->embed[sslot(s->spc[i])];
+  double xi = s->x[i], Fp = sp_d1(emb, mf_rho[i]), Fpp = sp_d2(emb, mf_rho[i]);
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      out[3 * a + c] = xi * Fpp * drho[a] * drho[c] + xi * Fp * d2rho[3 * a + c] + hv[3 * a + c];
+}
+
+/* Diagonal blocks of the force-residual Jacobian as the reference's per-site
+ * Hessians form them: D_u = sum_{i in {u} U N(u)} beta_i d2V0_i/dq_u2 (the
+ * beta_i weights are those of the residual R_q(u) = sum_i beta_i dV0_i/dq_u).
+ * No reference function forms this sum; a-21 is its per-(u, i) term. */
+void or_ref_diag_blocks(const or_system* s, const or_tables* T, int nsites, const int* sites,
+                        const double* mf_rho, double* D) {
+  int cnt = sites ? nsites : s->n;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < cnt; k++) {
+    int u = sites ? sites[k] : k;
+    double acc[9] = {0}, h[9];
+    for (int c = -1; c < s->nnb[u]; c++) {
+      int i = c < 0 ? u : NB(s, u, c);
+      or_ref_D2V0_i_Dq2_u(s, T, u, i, mf_rho, h);
+      for (int a = 0; a < 9; a++) acc[a] += s->beta[i] * h[a];
+    }
+    for (int a = 0; a < 9; a++) D[9 * (long)k + a] = acc[a];
+  }
 }
 
 /* d(-log Z0_u)/d sigma_u and d(-log Z0_u - gamma min(x,0.9999))/dx_u for the
@@ -1019,6 +1218,198 @@ void or_fact_residual(const or_system* s, const or_tables* T, int nthreads, doub
   totals[0] = tS;
   totals[1] = tV;
   free(p1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Factorised diagonal Hessian blocks                                         */
+/*                                                                            */
+/* d2/dq_u2 of sum_c beta_c V0_c.  Every term that is linear in one feature   */
+/* psi(p) has the form L . psi(p) with the same adjoint L as the gradient in  */
+/* phase2_site, so its Hessian is sum_a L_a d2psi_a/dp2 (hess_at).  What is  */
+/* quadratic in features of the same centre u adds J^T M J terms, J = dpsi/dp */
+/* summed over the neighbours: 1/9 [sum_ls JG_ls^T M JG_ls + JU^T M JG0 +     */
+/* JG0^T M JU], less the j1 = j2 parts of those sums, plus 2 n_k^2 J^T M J on */
+/* the coincident points.  For a neighbour centre k, only psi_u of k moves,   */
+/* and the coincident triplet (k, u, u) contributes codes 4 + 8 only (no J^T  */
+/* M J), as evaluate_D2V0_i_Dmeanq2_u_MgHx sums it (:1540-1570); exact = 1    */
+/* adds its mixed codes 5 + 7 (2 n_k^2 n_u^2 J^T M J), which makes D_u the    */
+/* exact diagonal block of the residual's Jacobian.  Embedding:               */
+/* x F''(rho) g g^T + x F'(rho) d2rho, with g the density gradient of the     */
+/* centre with respect to q_u.                                                 */
+/* ------------------------------------------------------------------------- */
+
+/* J[3 a + g] = d psi_a / dp_g (no occupancy factor). */
+static inline void jac_at(const or_tables* T, int t, const double* p, double* J) {
+  static const int qi[6] = {0, 1, 2, 0, 0, 1}, qj[6] = {0, 1, 2, 1, 2, 2};
+  double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  double u = sp_val(&T->u[t], r), du = sp_d1(&T->u[t], r) / r;
+  double w = sp_val(&T->w[t], r), dw = sp_d1(&T->w[t], r) / r;
+  for (int a = 0; a < 3; a++)
+    for (int g = 0; g < 3; g++) J[3 * a + g] = du * p[a] * p[g] + (a == g ? u : 0.0);
+  for (int m = 0; m < 6; m++) {
+    int i = qi[m], j = qj[m];
+    for (int g = 0; g < 3; g++)
+      J[3 * (3 + m) + g] = dw * p[g] * p[i] * p[j] + w * ((i == g ? p[j] : 0.0) + (j == g ? p[i] : 0.0));
+  }
+}
+
+/* H += k sum_a L_a d2psi_a/dp2.  L . psi = u(r) (La . p) + w(r) Q(p), Q the
+ * quadratic form of L[3..8]. */
+static inline void hess_at(const or_tables* T, int t, const double* p, const double* L, double k,
+                           double* H) {
+  double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  const or_spline *su = &T->u[t], *sw = &T->w[t];
+  double u1 = sp_d1(su, r), u2 = sp_d2(su, r), w = sp_val(sw, r), w1 = sp_d1(sw, r),
+         w2 = sp_d2(sw, r);
+  double la = L[0] * p[0] + L[1] * p[1] + L[2] * p[2];
+  double HQ[9] = {2 * L[3], L[6], L[7], L[6], 2 * L[4], L[8], L[7], L[8], 2 * L[5]};
+  double gQ[3];
+  for (int a = 0; a < 3; a++) gQ[a] = HQ[3 * a] * p[0] + HQ[3 * a + 1] * p[1] + HQ[3 * a + 2] * p[2];
+  double Q = 0.5 * (p[0] * gQ[0] + p[1] * gQ[1] + p[2] * gQ[2]);
+  double R[9] = {0};
+  rad_hess(u1, u2, p, r, la, R);
+  rad_hess(w1, w2, p, r, Q, R);
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      H[3 * a + c] += k * (R[3 * a + c] + u1 / r * (p[a] * L[c] + L[a] * p[c]) +
+                           w1 / r * (p[a] * gQ[c] + gQ[a] * p[c]) + w * HQ[3 * a + c]);
+}
+
+/* H += k A^T M B for 9 x 3 Jacobians. */
+static inline void jtmj(const double* A, const double* B, double k, double* H) {
+  for (int c = 0; c < 3; c++) {
+    double col[9], mc[9];
+    for (int a = 0; a < 9; a++) col[a] = B[3 * a + c];
+    applyM(1.0, col, mc);
+    for (int b = 0; b < 3; b++) {
+      double acc = 0.0;
+      for (int a = 0; a < 9; a++) acc += A[3 * a + b] * mc[a];
+      H[3 * b + c] += k * acc;
+    }
+  }
+}
+
+static void diag_site(const or_system* s, const or_tables* T, const double* p1, int u, int exact,
+                      double* D) {
+  for (int a = 0; a < 9; a++) D[a] = 0.0;
+  if (!act(s, u)) return;
+  const double* qu = s->q + 3 * u;
+  const double* Ru = p1 + (long)u * OR_P1_STRIDE;
+  double su = s->sig[u], nu = s->x[u], bu = s->beta[u];
+  int spu = s->spc[u];
+  const or_spline* sru = &T->rho[sslot(spu)];
+  double H[9] = {0}, HJ[9] = {0}, gu[3] = {0, 0, 0};
+  double JG[6][27], JU[27] = {0}, JG0[27] = {0};
+  memset(JG, 0, sizeof(JG));
+  for (int kk = 0; kk < s->nnb[u]; kk++) {
+    int k = NB(s, u, kk);
+    if (!act(s, k)) continue;
+    const double* Rk = p1 + (long)k * OR_P1_STRIDE;
+    double nk = s->x[k], sk = s->sig[k], bk = s->beta[k];
+    double wa = bu * nu * nu * nk, wb = bk * nk * nk * nu;
+    int t = pslot(spu, s->spc[k]);
+    double r[3], p[3], ps[9], psi0[9], Tuk[9] = {0}, SA[9] = {0}, la[9], lb[9], L[9], tmp[9];
+    double J0[27], J[27], JT[27] = {0};
+    sep(s->lo, s->hi, s->periodic, qu, s->q + 3 * k, r);
+    psi_at(T, t, r, psi0);
+    jac_at(T, t, r, J0);
+    for (int c = 0; c < 27; c++) JG0[c] += nk * J0[c];
+    /* j-shift points of centre u (shift sigma_k) = i-shift points (l,-s) of k */
+    for (int l = 0; l < 3; l++)
+      for (int sg = 0; sg < 2; sg++) {
+        double sh = sg ? -3.0 : 3.0;
+        p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
+        p[l] += sh * sk;
+        psi_at(T, t, p, ps);
+        jac_at(T, t, p, J);
+        for (int c = 0; c < 9; c++) Tuk[c] += ps[c];
+        for (int c = 0; c < 27; c++) JT[c] += J[c];
+        applyM(-nk / 9.0, psi0, tmp);
+        for (int c = 0; c < 9; c++) la[c] = Ru[P1_LG0 + c] + tmp[c];
+        flip(Rk + P1_LA + 9 * (2 * l + (1 - sg)), lb);
+        applyM(-nu / 9.0, ps, tmp);
+        for (int c = 0; c < 9; c++) lb[c] += tmp[c];
+        for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * lb[c];
+        hess_at(T, t, p, L, 1.0, H);
+      }
+    for (int c = 0; c < 27; c++) JU[c] += nk * JT[c];
+    jtmj(JT, J0, -nk * nk / 9.0, HJ);
+    jtmj(J0, JT, -nk * nk / 9.0, HJ);
+    /* i-shift points of centre u (shift sigma_u) = j-shift points of k */
+    for (int l = 0; l < 3; l++)
+      for (int sg = 0; sg < 2; sg++) {
+        double sh = sg ? -3.0 : 3.0;
+        p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
+        p[l] += sh * su;
+        psi_at(T, t, p, ps);
+        jac_at(T, t, p, J);
+        for (int c = 0; c < 9; c++) SA[c] += ps[c];
+        for (int c = 0; c < 27; c++) JG[2 * l + sg][c] += nk * J[c];
+        jtmj(J, J, -nk * nk / 9.0, HJ);
+        applyM(-nk / 9.0, ps, tmp);
+        for (int c = 0; c < 9; c++) la[c] = Ru[P1_LA + 9 * (2 * l + sg) + c] + tmp[c];
+        flip(Rk + P1_LG0, lb);
+        applyM(-nu / 9.0, psi0, tmp);
+        for (int c = 0; c < 9; c++) lb[c] += tmp[c];
+        for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * lb[c];
+        hess_at(T, t, p, L, 1.0, H);
+      }
+    /* unshifted point */
+    applyM(-nk / 9.0, Tuk, tmp);
+    for (int c = 0; c < 9; c++) la[c] = Ru[P1_LU + c] + tmp[c];
+    flip(Rk + P1_LU, lb);
+    applyM(-nu / 9.0, SA, tmp);
+    for (int c = 0; c < 9; c++) lb[c] += tmp[c];
+    for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * lb[c];
+    hess_at(T, t, r, L, 1.0, H);
+    /* pair-rule points: coincident angular term, pair and density */
+    const or_spline* sphi = &T->pair[t];
+    const or_spline* srk = &T->rho[sslot(s->spc[k])];
+    const or_spline* emk = &T->embed[sslot(s->spc[k])];
+    double cphi = 0.5 * nu * nk * (bu + bk);
+    double cra = bu * nu * Ru[P1_FP] * nk;
+    double crb = bk * nk * Rk[P1_FP] * nu;
+    double cang = 2.0 * (wa * nk + wb * nu);
+    double gk[3] = {0, 0, 0}, dsu[3];
+    for (int pt = 0; pt < pair_rule_points(s->rule); pt++) {
+      const double wt = pair_rule_point(s->rule, pt, r, su, sk, p, dsu);
+      psi_at(T, t, p, ps);
+      jac_at(T, t, p, J);
+      applyM(wt * cang, ps, L);
+      hess_at(T, t, p, L, 1.0, H);
+      jtmj(J, J, 2.0 * wt * nk * nk, HJ);
+      if (exact) jtmj(J, J, 2.0 * wt * bk * nk * nk * nu * nu, H); /* codes 5 + 7 of (k, u, u) */
+      double pr = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+      rad_hess(sp_d1(sphi, pr), sp_d2(sphi, pr), p, pr, wt * cphi, H);
+      rad_hess(sp_d1(srk, pr), sp_d2(srk, pr), p, pr, wt * cra, H);
+      rad_hess(sp_d1(sru, pr), sp_d2(sru, pr), p, pr, wt * crb, H);
+      double a1 = wt * nk * sp_d1(srk, pr) / pr, b1 = wt * nu * sp_d1(sru, pr) / pr;
+      for (int a = 0; a < 3; a++) { gu[a] += a1 * p[a]; gk[a] += b1 * p[a]; }
+    }
+    double ck = bk * nk * sp_d2(emk, Rk[P1_RHO]);
+    for (int a = 0; a < 3; a++)
+      for (int c = 0; c < 3; c++) H[3 * a + c] += ck * gk[a] * gk[c];
+  }
+  for (int ls = 0; ls < 6; ls++) jtmj(JG[ls], JG[ls], 1.0 / 9.0, HJ);
+  jtmj(JU, JG0, 1.0 / 9.0, HJ);
+  jtmj(JG0, JU, 1.0 / 9.0, HJ);
+  double cu = bu * nu * sp_d2(&T->embed[sslot(spu)], Ru[P1_RHO]);
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      D[3 * a + c] = H[3 * a + c] + bu * nu * nu * HJ[3 * a + c] + cu * gu[a] * gu[c];
+}
+
+void or_fact_diag_blocks(const or_system* s, const or_tables* T, int nthreads, int exact,
+                         double* D) {
+  double* p1 = (double*)malloc(sizeof(double) * OR_P1_STRIDE * (size_t)s->n);
+  double* rho = (double*)malloc(sizeof(double) * (size_t)s->n);
+  double* V0 = (double*)malloc(sizeof(double) * (size_t)s->n);
+  or_fact_phase1(s, T, nthreads, rho, V0, p1);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int u = 0; u < s->n; u++) diag_site(s, T, p1, u, exact, D + 9 * (long)u);
+  free(p1);
+  free(rho);
+  free(V0);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -75,6 +75,13 @@ void or_free_tables(or_tables* t);
 double or_spline_eval(const or_tables* t, int which, int slot, int deriv, double x);
 int or_spline_size(const or_tables* t, int which, int slot, double* dx);
 
+/* One pair (which 0 rho, 1 pair) or triplet (2 dipole, 3 quadrupole) function
+ * at explicit positions q (3 per site) and occupancies n; slots 0 Mg / 1 H.
+ * deriv 0: value; 1: gradient with respect to site `code`; 2: Hessian block
+ * `code` = ns * a + b.  For tests. */
+int or_fn_eval(const or_tables* t, int which, int deriv, int code, const double* n,
+               const double* q, const int* slots, double* out);
+
 /* Neighbours (brute force, src/Atoms/Neighbors.cpp:39-134) ----------------- */
 int or_neighbors(int n, const double* q, const double* lo, const double* hi,
                  const int* periodic, double rc, int cap, int* nnb, int* nbr);
@@ -88,6 +95,12 @@ void or_ref_DV0_i_Dq_u(const or_system* s, const or_tables* t, int u, int i,
                        const double* mf_rho, double out[4]);
 double or_ref_DV0_i_Dxi_u(const or_system* s, const or_tables* t, int u, int i,
                           const double* mf_rho);
+/* d2 V0_i / dq_u2 as a 3 x 3 row-major block (evaluate_D2V0_i_Dmeanq2_u_MgHx). */
+void or_ref_D2V0_i_Dq2_u(const or_system* s, const or_tables* t, int u, int i,
+                         const double* mf_rho, double out[9]);
+/* D_u = sum_{i in {u} U N(u)} beta_i d2V0_i/dq_u2 for the listed sites (n x 9). */
+void or_ref_diag_blocks(const or_system* s, const or_tables* t, int nsites, const int* sites,
+                        const double* mf_rho, double* D);
 /* Residual of sum_i S0_i for the listed sites u (all sites when sites==NULL). */
 void or_ref_residual(const or_system* s, const or_tables* t, int nsites, const int* sites,
                      const double* mf_rho, double* Rq, double* Rs, double* Rx);
@@ -102,6 +115,11 @@ void or_fact_phase1(const or_system* s, const or_tables* t, int nthreads,
 void or_fact_residual(const or_system* s, const or_tables* t, int nthreads,
                       double* Rq, double* Rs, double* Rx, double* S0, double* V0,
                       double* mf_rho, double* totals);
+/* Diagonal Hessian blocks D_u (n x 9), same definition as or_ref_diag_blocks
+ * (exact = 0), or the exact diagonal blocks of dR_q/dq (exact = 1: adds the
+ * mixed codes of the coincident triplets (i, u, u) that the reference omits). */
+void or_fact_diag_blocks(const or_system* s, const or_tables* t, int nthreads, int exact,
+                         double* D);
 
 /* Diffusion step (build-defined rate law; the reference declares only the
  * topology, src/Atoms/Neighbors.cpp:139-277, and the parameters,

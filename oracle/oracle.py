@@ -75,6 +75,10 @@ class Oracle:
         L.or_ref_residual.argtypes = [sp, C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, _dp]
         L.or_fact_phase1.argtypes = [sp, C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.or_fact_residual.argtypes = [sp, C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.or_ref_D2V0_i_Dq2_u.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.or_ref_diag_blocks.argtypes = [sp, C.c_void_p, C.c_int, _ip, _dp, _dp]
+        L.or_fact_diag_blocks.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp]
+        L.or_fn_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _dp]
         h = C.c_void_p()
         rc = L.or_load_tables(C.byref(h), table_dir.encode())
         if rc != 0:
@@ -176,7 +180,40 @@ class Oracle:
         self.lib.or_ref_residual(C.byref(s), self.tab, cnt, sp, _d(mf_rho), _d(Rq), _d(Rs), _d(Rx))
         return Rq, Rs, Rx
 
+    def ref_D2V0_Dq2(self, s, u, i, mf_rho):
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        out = np.zeros(9)
+        self.lib.or_ref_D2V0_i_Dq2_u(C.byref(s), self.tab, u, i, _d(mf_rho), _d(out))
+        return out.reshape(3, 3)
+
+    def ref_diag_blocks(self, s, mf_rho, sites=None):
+        """D_u = sum_i beta_i d2V0_i/dq_u2 for the listed sites, (cnt, 3, 3)."""
+        mf_rho = np.ascontiguousarray(mf_rho, dtype=np.float64)
+        if sites is None:
+            cnt, sp = s.n, None
+        else:
+            sarr = np.ascontiguousarray(sites, dtype=np.int32)
+            cnt, sp = len(sarr), _i(sarr)
+        D = np.zeros((cnt, 9))
+        self.lib.or_ref_diag_blocks(C.byref(s), self.tab, cnt, sp, _d(mf_rho), _d(D))
+        return D.reshape(cnt, 3, 3)
+
+    def fn_eval(self, which, deriv, code, n, q, slots):
+        """One ADP pair/triplet function (which: 0 rho, 1 pair, 2 dipole,
+        3 quadrupole) at explicit positions q (ns x 3)."""
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        out = np.zeros(9)
+        self.lib.or_fn_eval(self.tab, which, deriv, code, _d(n), _d(q), _i(sl), _d(out))
+        return out[0] if deriv == 0 else (out[:3] if deriv == 1 else out.reshape(3, 3))
+
     # -- factorised ----------------------------------------------------------
+    def fact_diag_blocks(self, s, exact=False, nthreads=0):
+        D = np.zeros((s.n, 9))
+        self.lib.or_fact_diag_blocks(C.byref(s), self.tab, nthreads, int(bool(exact)), _d(D))
+        return D.reshape(s.n, 3, 3)
+
     def fact_residual(self, s, nthreads=0):
         n = s.n
         out = {k: np.zeros(n) for k in ("Rs", "Rx", "S0", "V0", "mf_rho")}
