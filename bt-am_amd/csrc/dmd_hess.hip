@@ -1,0 +1,353 @@
+// dmd_hess.hip -- diagonal 3x3 blocks of the force residual's Jacobian (gfx950).
+//
+// D_u = sum_{i in {u} U N(u)} beta_i d2V0_i/dq_u2, the per-(u, i) Hessians of
+// evaluate_D2V0_i_Dmeanq2_u_MgHx (src/ADP/MgHx-mf-V-bulk.cpp:1366-1599)
+// summed with the weights of the residual R_q(u) = sum_i beta_i dV0_i/dq_u.
+//
+// Factorised form (FP64).  The phase-1 record of every site holds the adjoint
+// blocks L of its dipole / quadrupole sums (1/9 M G_ls, 1/9 M U, 1/9 M G0).
+// Phase 2 forms the gradient as dpsi/dp^T L at the 25 points p = r_uk + shift
+// of each pair; here the same adjoints contract the second derivatives:
+// sum_a L_a d2psi_a/dp2.  What is quadratic in features of the centre u adds
+// J^T M J terms, J = dpsi/dp: 1/9 (sum_ls JG_ls^T M JG_ls + JU^T M JG0 +
+// JG0^T M JU) with the Jacobian sums JG_ls = sum_k n_k J(r_uk + 3 s sigma_u
+// e_l), JU = sum_k n_k sum_ls J(r_uk + 3 s sigma_k e_l), JG0 = sum_k n_k J(r_uk),
+// less their j1 = j2 parts, plus 2 n_k^2 J^T M J on the coincident points.
+// For a neighbour centre k only psi_u moves; the reference sums codes 4 and 8
+// of the coincident triplet (k, u, u) and not the mixed codes 5 and 7
+// (:1540-1570), so its 2 n_k^2 n_u^2 J^T M J is added only when `exact` is
+// set (then D_u is the exact diagonal block of dR_q/dq).  Pair and density:
+// radial Hessians on the pair-rule points; embedding: x F''(rho) g g^T +
+// x F'(rho) d2rho for u and for every neighbour centre.
+//
+// One wavefront per site u; the lanes take the neighbours k = lane, lane + 64,
+// ... and every sum over k closes with a butterfly reduction (the same value
+// on every lane, independent of the order the lanes ran).  The Jacobian sums
+// are built one block at a time (27 values each) so that the per-lane state
+// stays in registers.
+#include "dmd_device.h"
+
+namespace dmd {
+namespace {
+
+// Spline second derivative 2c + 6d p of a coefficient row a b c d.
+DEV double cder2(const double* c, double p) { return 2.0 * c[2] + 6.0 * c[3] * p; }
+
+// u, u', u'', w, w', w'' of pair table t at segment m, offset off.
+DEV void uw_at(const Tables& T, int t, int m, double off, double* uv) {
+  const double* c = T.uw + ((size_t)t * T.nr + m) * 8;
+  uv[0] = cval(c, off); uv[1] = cder(c, off); uv[2] = cder2(c, off);
+  uv[3] = cval(c + 4, off); uv[4] = cder(c + 4, off); uv[5] = cder2(c + 4, off);
+}
+
+// Symmetric 3x3 blocks are kept as (xx, yy, zz, xy, xz, yz).
+// H += k d2c(|p|)/dp2 from c' = d1 and c'' = d2 (ir = 1/|p|).
+DEV void rad_hess(double d1, double d2, const double* p, double ir, double k, double* H) {
+  const double a = k * (d2 - d1 * ir) * ir * ir, b = k * d1 * ir;
+  H[0] += a * p[0] * p[0] + b; H[1] += a * p[1] * p[1] + b; H[2] += a * p[2] * p[2] + b;
+  H[3] += a * p[0] * p[1]; H[4] += a * p[0] * p[2]; H[5] += a * p[1] * p[2];
+}
+
+// H += k sum_a L_a d2psi_a/dp2.  L . psi = u(r) (La . p) + w(r) Q(p) with Q the
+// quadratic form of L[3..8] (gradient gQ, constant Hessian HQ).
+DEV void hess_psi(const double* L, const double* p, double ir, const double* uv, double k,
+                  double* H) {
+  const double la = L[0] * p[0] + L[1] * p[1] + L[2] * p[2];
+  const double g0 = 2.0 * L[3] * p[0] + L[6] * p[1] + L[7] * p[2];
+  const double g1 = L[6] * p[0] + 2.0 * L[4] * p[1] + L[8] * p[2];
+  const double g2 = L[7] * p[0] + L[8] * p[1] + 2.0 * L[5] * p[2];
+  const double Q = 0.5 * (p[0] * g0 + p[1] * g1 + p[2] * g2);
+  rad_hess(la * uv[1] + Q * uv[4], la * uv[2] + Q * uv[5], p, ir, k, H);
+  const double cu = k * uv[1] * ir, cw = k * uv[4] * ir, w = k * uv[3];
+  H[0] += 2.0 * (cu * p[0] * L[0] + cw * p[0] * g0 + w * L[3]);
+  H[1] += 2.0 * (cu * p[1] * L[1] + cw * p[1] * g1 + w * L[4]);
+  H[2] += 2.0 * (cu * p[2] * L[2] + cw * p[2] * g2 + w * L[5]);
+  H[3] += cu * (p[0] * L[1] + L[0] * p[1]) + cw * (p[0] * g1 + g0 * p[1]) + w * L[6];
+  H[4] += cu * (p[0] * L[2] + L[0] * p[2]) + cw * (p[0] * g2 + g0 * p[2]) + w * L[7];
+  H[5] += cu * (p[1] * L[2] + L[1] * p[2]) + cw * (p[1] * g2 + g1 * p[2]) + w * L[8];
+}
+
+// J[3 a + g] = dpsi_a/dp_g.
+DEV void jac_psi(const double* p, double ir, const double* uv, double* J) {
+  const double du = uv[1] * ir, u = uv[0], dw = uv[4] * ir, w = uv[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int g = 0; g < 3; g++) J[3 * a + g] = du * p[a] * p[g] + (a == g ? u : 0.0);
+  constexpr int qi[6] = {0, 1, 2, 0, 0, 1}, qj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const double pij = p[qi[m]] * p[qj[m]];
+#pragma unroll
+    for (int g = 0; g < 3; g++)
+      J[3 * (3 + m) + g] = dw * p[g] * pij + w * ((qi[m] == g ? p[qj[m]] : 0.0) +
+                                                  (qj[m] == g ? p[qi[m]] : 0.0));
+  }
+}
+
+// H += k (A^T M B + B^T M A) for 9 x 3 Jacobians.
+DEV void jtmj_sym(const double* A, const double* B, double k, double* H) {
+  double X[9];
+#pragma unroll
+  for (int g = 0; g < 3; g++) {
+    double col[9], mb[9];
+#pragma unroll
+    for (int a = 0; a < 9; a++) col[a] = B[3 * a + g];
+    applyM(1.0, col, mb);
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int a = 0; a < 9; a++) acc += A[3 * a + b] * mb[a];
+      X[3 * b + g] = acc;
+    }
+  }
+  H[0] += 2.0 * k * X[0]; H[1] += 2.0 * k * X[4]; H[2] += 2.0 * k * X[8];
+  H[3] += k * (X[1] + X[3]); H[4] += k * (X[2] + X[6]); H[5] += k * (X[5] + X[7]);
+}
+
+// Block b (9 components) of a phase-1 record, dipole part negated if flip.
+DEV void rec_blk(const double* r, int b, bool flip, double* o) {
+  const double sgn = flip ? -1.0 : 1.0;
+  o[0] = sgn * r[8 * b]; o[1] = sgn * r[8 * b + 1]; o[2] = sgn * r[8 * b + 2];
+#pragma unroll
+  for (int c = 3; c < 8; c++) o[c] = r[8 * b + c];
+  o[8] = r[kRecTail + b];
+}
+
+DEV double emb_d2(const Tables& T, int sp, double rho) {
+  double pe;
+  const int me = seg_index(rho, T.dxe[sp], T.inv_dxe[sp], T.ne[sp], pe);
+  return cder2(T.em + ((size_t)sp * T.ne_stride + me) * 4, pe);
+}
+
+template <int RULE>
+__global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __restrict__ rec,
+                                             int base, int exact, double* __restrict__ D) {
+  const int u = base + (int)blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  double* Du = D + (size_t)blockIdx.x * 9;
+  const double nu = s.ne[u];
+  if (!(nu > 0.0)) {  // inactive site (wave-uniform)
+    if (lane < 9) Du[lane] = 0.0;
+    return;
+  }
+  const Box box = s.boxes[s.inst[u]];
+  const double4 qu = s.qs[u];
+  const double su = qu.w, bu = s.beta[u];
+  const int spu = s.spc[u];
+  const int nn = s.nnb[u];
+  const int* row = s.nbr + (size_t)u * kCap;
+  const double* Ru = rec + (size_t)u * kRec;
+
+  // ---- pass A: adjoint-contracted Hessians, pair, density, embedding ----
+  double H[6] = {0, 0, 0, 0, 0, 0}, HJ[6] = {0, 0, 0, 0, 0, 0}, gu[3] = {0, 0, 0};
+#pragma unroll 1
+  for (int kk = lane; kk < nn; kk += 64) {
+    const int k = row[kk];
+    const double nk = s.ne[k];
+    if (!(nk > 0.0)) continue;
+    const double4 qk = s.qs[k];
+    const double sk = qk.w, bk = s.beta[k];
+    const int spk = s.spc[k];
+    const int t = pair_slot(spu, spk);
+    const double* Rk = rec + (size_t)k * kRec;
+    const double wa = bu * nu * nu * nk, wb = bk * nk * nk * nu;
+    double rd[3];
+    min_image(box, qu, qk, rd);
+    PairGeo<double> geo;
+    geo.init(rd);
+    double p[3], ir, off, uv[6], ps[9], psi0[9], la[9], lb[9], L[9], tmp[9];
+    double Tuk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, SA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int m;
+    geo.point(T, 0, 0.0, p, m, off, ir);
+    double uv0[6], ir0 = ir, p0[3] = {p[0], p[1], p[2]};
+    uw_at(T, t, m, off, uv0);
+    psi_from(uv0[0], uv0[3], p0, psi0);
+    // j-shift points of centre u (shift sigma_k) = i-shift points (l, -s) of k
+#pragma unroll 1
+    for (int ls = 0; ls < 6; ls++) {
+      const int l = ls >> 1, sg = ls & 1;
+      geo.point(T, l, (sg ? -3.0 : 3.0) * sk, p, m, off, ir);
+      uw_at(T, t, m, off, uv);
+      psi_from(uv[0], uv[3], p, ps);
+#pragma unroll
+      for (int c = 0; c < 9; c++) Tuk[c] += ps[c];
+      rec_blk(Ru, kBlkLG0, false, la);
+      applyM(-nk / 9.0, psi0, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) la[c] += tmp[c];
+      rec_blk(Rk, 2 * l + (1 - sg), true, lb);
+      applyM(-nu / 9.0, ps, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * (lb[c] + tmp[c]);
+      hess_psi(L, p, ir, uv, 1.0, H);
+    }
+    // i-shift points of centre u (shift sigma_u) = j-shift points of k
+#pragma unroll 1
+    for (int ls = 0; ls < 6; ls++) {
+      const int l = ls >> 1, sg = ls & 1;
+      geo.point(T, l, (sg ? -3.0 : 3.0) * su, p, m, off, ir);
+      uw_at(T, t, m, off, uv);
+      psi_from(uv[0], uv[3], p, ps);
+#pragma unroll
+      for (int c = 0; c < 9; c++) SA[c] += ps[c];
+      rec_blk(Ru, ls, false, la);
+      applyM(-nk / 9.0, ps, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) la[c] += tmp[c];
+      rec_blk(Rk, kBlkLG0, true, lb);
+      applyM(-nu / 9.0, psi0, tmp);
+#pragma unroll
+      for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * (lb[c] + tmp[c]);
+      hess_psi(L, p, ir, uv, 1.0, H);
+    }
+    // unshifted point
+    rec_blk(Ru, kBlkLU, false, la);
+    applyM(-nk / 9.0, Tuk, tmp);
+#pragma unroll
+    for (int c = 0; c < 9; c++) la[c] += tmp[c];
+    rec_blk(Rk, kBlkLU, true, lb);
+    applyM(-nu / 9.0, SA, tmp);
+#pragma unroll
+    for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * (lb[c] + tmp[c]);
+    hess_psi(L, p0, ir0, uv0, 1.0, H);
+    // pair-rule points: coincident angular term, pair and density
+    const double cphi = 0.5 * nu * nk * (bu + bk);
+    const double cra = bu * nu * Ru[kRecFP] * nk;
+    const double crb = bk * nk * Rk[kRecFP] * nu;
+    const double cang = 2.0 * (wa * nk + wb * nu);
+    const double cjj = exact ? 2.0 * bk * nk * nk * nu * nu : 0.0;
+    double gk[3] = {0, 0, 0};
+#pragma unroll 1
+    for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
+      double dsi[3], dsj[3], J[27];
+      const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
+      uw_at(T, t, m, off, uv);
+      psi_from(uv[0], uv[3], p, ps);
+      applyM(wt * cang, ps, L);
+      hess_psi(L, p, ir, uv, 1.0, H);
+      jac_psi(p, ir, uv, J);
+      jtmj_sym(J, J, wt * nk * nk, HJ);                 // 2 wt n_k^2 J^T M J
+      if (exact) jtmj_sym(J, J, 0.5 * wt * cjj, H);     // codes 5 + 7 of (k, u, u)
+      const double* f = T.pr + ((size_t)t * T.nr + m) * 16;
+      const double* fk = f + 4 + 4 * spk;
+      const double* fu = f + 4 + 4 * spu;
+      rad_hess(cder(f, off), cder2(f, off), p, ir, wt * cphi, H);
+      rad_hess(cder(fk, off), cder2(fk, off), p, ir, wt * cra, H);
+      rad_hess(cder(fu, off), cder2(fu, off), p, ir, wt * crb, H);
+      const double a1 = wt * nk * cder(fk, off) * ir, b1 = wt * nu * cder(fu, off) * ir;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        gu[a] += a1 * p[a];
+        gk[a] += b1 * p[a];
+      }
+    }
+    const double ck = bk * nk * emb_d2(T, spk, Rk[kRecRHO]);
+    H[0] += ck * gk[0] * gk[0]; H[1] += ck * gk[1] * gk[1]; H[2] += ck * gk[2] * gk[2];
+    H[3] += ck * gk[0] * gk[1]; H[4] += ck * gk[0] * gk[2]; H[5] += ck * gk[1] * gk[2];
+  }
+
+  // ---- Jacobian sums of centre u, one block at a time ----
+  // blk 0..5: JG_ls; blk 6: JU (and the j1 = j2 part of JU^T M JG0);
+  // blk 7: JG0, closing JU^T M JG0 + JG0^T M JU.
+  __shared__ double sJU[27];
+  double Hw[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform
+#pragma unroll 1
+  for (int blk = 0; blk < 8; blk++) {
+    double S[27];
+#pragma unroll
+    for (int c = 0; c < 27; c++) S[c] = 0.0;
+#pragma unroll 1
+    for (int kk = lane; kk < nn; kk += 64) {
+      const int k = row[kk];
+      const double nk = s.ne[k];
+      if (!(nk > 0.0)) continue;
+      const double4 qk = s.qs[k];
+      const int t = pair_slot(spu, s.spc[k]);
+      double rd[3], p[3], ir, off, uv[6], J[27];
+      int m;
+      min_image(box, qu, qk, rd);
+      PairGeo<double> geo;
+      geo.init(rd);
+      if (blk < 6) {
+        geo.point(T, blk >> 1, ((blk & 1) ? -3.0 : 3.0) * su, p, m, off, ir);
+        uw_at(T, t, m, off, uv);
+        jac_psi(p, ir, uv, J);
+#pragma unroll
+        for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+        jtmj_sym(J, J, -nk * nk / 18.0, HJ);  // -n_k^2/9 J^T M J
+      } else if (blk == 6) {
+        double J0[27];
+        geo.point(T, 0, 0.0, p, m, off, ir);
+        uw_at(T, t, m, off, uv);
+        jac_psi(p, ir, uv, J0);
+#pragma unroll 1
+        for (int ls = 0; ls < 6; ls++) {
+          geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
+          uw_at(T, t, m, off, uv);
+          jac_psi(p, ir, uv, J);
+#pragma unroll
+          for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+          jtmj_sym(J, J0, -nk * nk / 9.0, HJ);
+        }
+      } else {
+        geo.point(T, 0, 0.0, p, m, off, ir);
+        uw_at(T, t, m, off, uv);
+        jac_psi(p, ir, uv, J);
+#pragma unroll
+        for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 27; c++) S[c] = group_sum<64>(S[c]);
+    if (blk < 6) {
+      jtmj_sym(S, S, 1.0 / 18.0, Hw);
+    } else if (blk == 6) {
+      if (lane == 0)  // S is uniform after the reduction
+#pragma unroll
+        for (int c = 0; c < 27; c++) sJU[c] = S[c];
+      __syncthreads();
+    } else {
+      double JU[27];
+#pragma unroll
+      for (int c = 0; c < 27; c++) JU[c] = sJU[c];
+      jtmj_sym(JU, S, 1.0 / 9.0, Hw);
+    }
+  }
+
+  // ---- close ----
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    H[c] = group_sum<64>(H[c]);
+    HJ[c] = group_sum<64>(HJ[c]);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) gu[a] = group_sum<64>(gu[a]);
+  if (lane == 0) {
+    const double cu = bu * nu * emb_d2(T, spu, Ru[kRecRHO]);
+    const double w2 = bu * nu * nu;
+    double O[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) O[c] = H[c] + w2 * (HJ[c] + Hw[c]);
+    O[0] += cu * gu[0] * gu[0]; O[1] += cu * gu[1] * gu[1]; O[2] += cu * gu[2] * gu[2];
+    O[3] += cu * gu[0] * gu[1]; O[4] += cu * gu[0] * gu[2]; O[5] += cu * gu[1] * gu[2];
+    Du[0] = O[0]; Du[1] = O[3]; Du[2] = O[4];
+    Du[3] = O[3]; Du[4] = O[1]; Du[5] = O[5];
+    Du[6] = O[4]; Du[7] = O[5]; Du[8] = O[2];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_diag(hipStream_t st, int rule, int exact, const State& s, const Tables& T,
+                       const double* rec, int base, int n, double* D) {
+  if (n <= 0) return hipSuccess;
+  if (rule == 5)
+    hipLaunchKernelGGL(k_diag<5>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
+  else
+    hipLaunchKernelGGL(k_diag<3>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
+  return hipGetLastError();
+}
+
+}  // namespace dmd

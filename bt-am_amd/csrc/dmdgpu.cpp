@@ -176,6 +176,7 @@ struct dmdgpu_ctx {
   DevBuf<int> d_spc, d_nnb, d_nbr, d_chunk_np, d_own;
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
+  DevBuf<double> d_diag;  // n x 9 diagonal Hessian blocks of the primary instance
   DevBuf<long long> d_pairs;
   std::vector<int> h_prec;
   std::vector<int> h_noxr;  // per instance, solver ticks: 1 = no occupancy residual needed
@@ -1336,7 +1337,7 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
                            &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
                            &c->d_slot,
                            &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart,
-                           &c->d_Xn,  &c->d_Xt};
+                           &c->d_Xn,  &c->d_Xt,    &c->d_diag};
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
                         &c->d_chunk_np,   &c->d_own,
@@ -1537,6 +1538,30 @@ dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* c, const double* X, const double* V, double
   }
   for (size_t k = 0; k < m; k++) JV[k] = (Rp[k] - Rm[k]) / (2.0 * eps);
   return upload_unknowns(c, I, X, X + 3 * n, nullptr, false);
+}
+
+dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
+  CHK(check_primary(c));
+  if (!h_D) return fail(DMDGPU_ERR_ARG, "null output");
+  if (exact != 0 && exact != 1) return fail(DMDGPU_ERR_ARG, "exact must be 0 or 1");
+  HIPCHK(hipSetDevice(c->device));
+  const int id = c->primary;
+  // phase 1 in FP64 (the blocks read the FP64 adjoint records)
+  std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+  c->h_prec[id] = dmd::kFP64;
+  CHK(run_eval(c, false));
+  const Instance& I = c->inst[id];
+  HIPCHK(c->d_diag.ensure((size_t)I.n * 9));
+  HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, exact, make_state(c), c->T, c->d_rec.p, I.base,
+                          I.n, c->d_diag.p));
+  std::vector<double> tmp((size_t)I.n * 9);
+  HIPCHK(hipMemcpyAsync(tmp.data(), c->d_diag.p, tmp.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < I.n; i++)
+    std::copy(tmp.begin() + 9 * (size_t)i, tmp.begin() + 9 * (size_t)(i + 1),
+              h_D + 9 * (size_t)I.perm[i]);
+  return DMDGPU_OK;
 }
 
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* c) {

@@ -36,7 +36,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_instance_residual", "dmdgpu_instance_state", "dmdgpu_instance_set_unknowns",
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
-           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule",
+           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks",
            "dmdgpu_set_tuning_precision"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
@@ -140,6 +140,7 @@ def load_library():
     L.dmdgpu_diffusion_rates.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts), _dp]
     L.dmdgpu_diffusion_neighbors.argtypes = [vp, C.c_int, _ip, _ip]
     L.dmdgpu_jvp.argtypes = [vp, _dp, _dp, _dp]
+    L.dmdgpu_diag_blocks.argtypes = [vp, C.c_int, _dp]
     L.dmdgpu_set_rule.argtypes = [vp, C.c_int]
     L.dmdgpu_set_tuning_precision.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     _lib = L
@@ -257,6 +258,12 @@ class Context:
         tot = np.zeros(2)
         self._chk(self.L.dmdgpu_residual(self.h, _d(Rq), _d(Rs), _d(Rx), _d(tot)))
         return {"Rq": Rq, "Rs": Rs, "Rx": Rx, "S0_total": tot[0], "V0_total": tot[1]}
+
+    def diag_blocks(self, exact=False):
+        """Diagonal 3x3 blocks of dR_q/dq, (n, 3, 3), FP64 (see dmdgpu_diag_blocks)."""
+        D = np.zeros((self.n, 9))
+        self._chk(self.L.dmdgpu_diag_blocks(self.h, 1 if exact else 0, _d(D)))
+        return D.reshape(self.n, 3, 3)
 
     def residual_resident(self):
         self._chk(self.L.dmdgpu_residual_resident(self.h))

@@ -138,6 +138,19 @@ dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
  * displacement 1e-5 A).  Occupancies fixed; the state is left at X. */
 dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* ctx, const double* h_X, const double* h_V, double* h_JV);
 
+/* Diagonal 3x3 blocks of the force residual's Jacobian, for a block-Jacobi
+ * preconditioner: h_D[9 u + 3 a + b] = D_u[a][b] with
+ *   D_u = sum_{i in {u} U N(u)} beta_i d2V0_i/dq_u2,
+ * the per-(u, i) Hessians of evaluate_D2V0_i_Dmeanq2_u_MgHx
+ * (MgHx-mf-V-bulk.cpp:1366-1599) summed with the weights of R_q(u).
+ * exact = 0: as that function assembles them (on the coincident triplet
+ * (i, u, u) it sums the codes 4 and 8 of u, not the mixed codes 5 and 7);
+ * exact = 1: the exact d R_q(u) / d q_u, those mixed codes included.  The
+ * pair rule is the context's (dmdgpu_set_rule); the reference defines the
+ * blocks for the 3rd-order rule.  Evaluated in FP64 whatever the precision
+ * setting; inactive sites (x < 5e-4) get zeros. */
+dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* ctx, int exact, double* h_D);
+
 /* Kernel tuning, for both precisions: k_phase1 = lanes of a wavefront that
  * share one site in phase 1 (4, 8 or 16); k_phase2 = blocks per 256-site chunk
  * of the pair kernels (4, 8 or 16: on average that many lanes per site);
