@@ -150,6 +150,27 @@ def split_leg(ctx, args, pqe):
             "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
 
 
+def diag_leg(ctx, reps=5):
+    """dmdgpu_diag_blocks on the same workload (reference assembly, FP64): the
+    k_diag kernel time from HIP events, the FP64 phase 1 it reads, and the wall
+    time of the whole call (phase 1 + blocks + copy-out of n x 9 doubles)."""
+    ctx.set_precision("fp64")
+    ctx.diag_blocks()
+    ctx.sync()
+    ctx.reset_kernel_times()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.diag_blocks()
+    dt = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    kt = ctx.kernel_times()
+    ctx.reset_kernel_times()
+    per = lambda k: kt[k][1] / max(kt[k][0], 1)
+    return {"kernel_ms": per("diag_blocks"), "phase1_ms": per("phase1_site_sums"),
+            "call_ms": dt * 1e3}
+
+
 def rule5_leg(ctx, args, pqe3):
     """The same workload under the 27-point tensor Gauss-Hermite pair rule
     (pair and coincident terms; distinct triplets keep the 3rd-order rule):
@@ -321,6 +342,7 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     ap.add_argument("--no-solves", action="store_true", help="skip the solves/hr leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-diag", action="store_true", help="skip the diagonal-Hessian leg")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests in the serving leg")
     ap.add_argument("--instances", type=int, default=64,
@@ -392,6 +414,11 @@ def main():
     rule5 = rule5_leg(ctx, args, pqe)
     log(f"rule 5: {rule5}")
 
+    diag = None
+    if not args.no_diag:
+        diag = diag_leg(ctx)
+        log(f"diag blocks: {diag}")
+
     serving = None
     if not args.no_serve:
         serving = serving_leg(((k1, k2, un), cfg32), args.serve_requests)
@@ -422,6 +449,7 @@ def main():
         "roofline": roof,
         "split32": split,
         "fifth_order_rule": rule5,
+        "diag_blocks": diag,
         "solves": solves,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -22,9 +22,11 @@
 //
 // One wavefront per site u; the lanes take the neighbours k = lane, lane + 64,
 // ... and every sum over k closes with a butterfly reduction (the same value
-// on every lane, independent of the order the lanes ran).  The Jacobian sums
-// are built one block at a time (27 values each) so that the per-lane state
-// stays in registers.
+// on every lane, independent of the order the lanes ran).  The work is split
+// into passes over the neighbours (adjoint points, pair-rule points, then the
+// Jacobian sums one 27-value block at a time) so that little state is live at
+// once: two wavefronts per SIMD fit with 36 B of spill (one wavefront and
+// 7.3 ms at 54k sites before the split, 2 wavefronts now).
 #include "dmd_device.h"
 
 namespace dmd {
@@ -85,34 +87,45 @@ DEV void jac_psi(const double* p, double ir, const double* uv, double* J) {
   }
 }
 
-// H += k (A^T M B + B^T M A) for 9 x 3 Jacobians.
+// A^T M B for 9 x 3 Jacobians, M the form above written out: weights 1/2 on
+// the dipole and diagonal-quadrupole rows, 1 on the off-diagonal rows, and
+// -1/6 trA trB from the trace coupling.
+DEV double mdot(const double* A, const double* B, int b, int g) {
+  const double tA = A[9 + b] + A[12 + b] + A[15 + b], tB = B[9 + g] + B[12 + g] + B[15 + g];
+  double h = A[b] * B[g];
+#pragma unroll
+  for (int a = 1; a < 6; a++) h += A[3 * a + b] * B[3 * a + g];
+  double f = A[18 + b] * B[18 + g];
+  f += A[21 + b] * B[21 + g];
+  f += A[24 + b] * B[24 + g];
+  return 0.5 * h + f - (1.0 / 6.0) * tA * tB;
+}
+// H += k (A^T M A) (symmetric).
+DEV void jtmj_self(const double* A, double k, double* H) {
+  H[0] += k * mdot(A, A, 0, 0); H[1] += k * mdot(A, A, 1, 1); H[2] += k * mdot(A, A, 2, 2);
+  H[3] += k * mdot(A, A, 0, 1); H[4] += k * mdot(A, A, 0, 2); H[5] += k * mdot(A, A, 1, 2);
+}
+// H += k (A^T M B + B^T M A).
 DEV void jtmj_sym(const double* A, const double* B, double k, double* H) {
-  double X[9];
-#pragma unroll
-  for (int g = 0; g < 3; g++) {
-    double col[9], mb[9];
-#pragma unroll
-    for (int a = 0; a < 9; a++) col[a] = B[3 * a + g];
-    applyM(1.0, col, mb);
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int a = 0; a < 9; a++) acc += A[3 * a + b] * mb[a];
-      X[3 * b + g] = acc;
-    }
-  }
-  H[0] += 2.0 * k * X[0]; H[1] += 2.0 * k * X[4]; H[2] += 2.0 * k * X[8];
-  H[3] += k * (X[1] + X[3]); H[4] += k * (X[2] + X[6]); H[5] += k * (X[5] + X[7]);
+  H[0] += 2.0 * k * mdot(A, B, 0, 0); H[1] += 2.0 * k * mdot(A, B, 1, 1);
+  H[2] += 2.0 * k * mdot(A, B, 2, 2);
+  H[3] += k * (mdot(A, B, 0, 1) + mdot(A, B, 1, 0));
+  H[4] += k * (mdot(A, B, 0, 2) + mdot(A, B, 2, 0));
+  H[5] += k * (mdot(A, B, 1, 2) + mdot(A, B, 2, 1));
 }
 
-// Block b (9 components) of a phase-1 record, dipole part negated if flip.
-DEV void rec_blk(const double* r, int b, bool flip, double* o) {
-  const double sgn = flip ? -1.0 : 1.0;
-  o[0] = sgn * r[8 * b]; o[1] = sgn * r[8 * b + 1]; o[2] = sgn * r[8 * b + 2];
+// L = wa Ru[bu] + wb flip(Rk[bk]) + M v: the combined adjoint of one point
+// (centre u's block, neighbour k's block with the dipole part negated since
+// psi(-p) flips it, and the j1 = j2 corrections v).
+DEV void adjoint(const double* Ru, int bu, double wa, const double* Rk, int bk, double wb,
+                 const double* v, double* L) {
+  applyM(1.0, v, L);
+  L[0] += wa * Ru[8 * bu] - wb * Rk[8 * bk];
+  L[1] += wa * Ru[8 * bu + 1] - wb * Rk[8 * bk + 1];
+  L[2] += wa * Ru[8 * bu + 2] - wb * Rk[8 * bk + 2];
 #pragma unroll
-  for (int c = 3; c < 8; c++) o[c] = r[8 * b + c];
-  o[8] = r[kRecTail + b];
+  for (int c = 3; c < 8; c++) L[c] += wa * Ru[8 * bu + c] + wb * Rk[8 * bk + c];
+  L[8] += wa * Ru[kRecTail + bu] + wb * Rk[kRecTail + bk];
 }
 
 DEV double emb_d2(const Tables& T, int sp, double rho) {
@@ -122,7 +135,7 @@ DEV double emb_d2(const Tables& T, int sp, double rho) {
 }
 
 template <int RULE>
-__global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __restrict__ rec,
+__global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double* __restrict__ rec,
                                              int base, int exact, double* __restrict__ D) {
   const int u = base + (int)blockIdx.x;
   const int lane = (int)threadIdx.x;
@@ -140,8 +153,9 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
   const int* row = s.nbr + (size_t)u * kCap;
   const double* Ru = rec + (size_t)u * kRec;
 
-  // ---- pass A: adjoint-contracted Hessians, pair, density, embedding ----
   double H[6] = {0, 0, 0, 0, 0, 0}, HJ[6] = {0, 0, 0, 0, 0, 0}, gu[3] = {0, 0, 0};
+
+  // ---- pass A: adjoint-contracted Hessians at the 13 shifted / unshifted points ----
 #pragma unroll 1
   for (int kk = lane; kk < nn; kk += 64) {
     const int k = row[kk];
@@ -149,21 +163,20 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
     if (!(nk > 0.0)) continue;
     const double4 qk = s.qs[k];
     const double sk = qk.w, bk = s.beta[k];
-    const int spk = s.spc[k];
-    const int t = pair_slot(spu, spk);
+    const int t = pair_slot(spu, s.spc[k]);
     const double* Rk = rec + (size_t)k * kRec;
     const double wa = bu * nu * nu * nk, wb = bk * nk * nk * nu;
+    const double ca = -wa * nk / 9.0, cb = -wb * nu / 9.0;
     double rd[3];
     min_image(box, qu, qk, rd);
     PairGeo<double> geo;
     geo.init(rd);
-    double p[3], ir, off, uv[6], ps[9], psi0[9], la[9], lb[9], L[9], tmp[9];
+    double p[3], ir, off, uv[6], ps[9], psi0[9], L[9], v[9];
     double Tuk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, SA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int m;
     geo.point(T, 0, 0.0, p, m, off, ir);
-    double uv0[6], ir0 = ir, p0[3] = {p[0], p[1], p[2]};
-    uw_at(T, t, m, off, uv0);
-    psi_from(uv0[0], uv0[3], p0, psi0);
+    uw_at(T, t, m, off, uv);
+    psi_from(uv[0], uv[3], p, psi0);
     // j-shift points of centre u (shift sigma_k) = i-shift points (l, -s) of k
 #pragma unroll 1
     for (int ls = 0; ls < 6; ls++) {
@@ -172,15 +185,11 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
       uw_at(T, t, m, off, uv);
       psi_from(uv[0], uv[3], p, ps);
 #pragma unroll
-      for (int c = 0; c < 9; c++) Tuk[c] += ps[c];
-      rec_blk(Ru, kBlkLG0, false, la);
-      applyM(-nk / 9.0, psi0, tmp);
-#pragma unroll
-      for (int c = 0; c < 9; c++) la[c] += tmp[c];
-      rec_blk(Rk, 2 * l + (1 - sg), true, lb);
-      applyM(-nu / 9.0, ps, tmp);
-#pragma unroll
-      for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * (lb[c] + tmp[c]);
+      for (int c = 0; c < 9; c++) {
+        Tuk[c] += ps[c];
+        v[c] = ca * psi0[c] + cb * ps[c];
+      }
+      adjoint(Ru, kBlkLG0, wa, Rk, 2 * l + (1 - sg), wb, v, L);
       hess_psi(L, p, ir, uv, 1.0, H);
     }
     // i-shift points of centre u (shift sigma_u) = j-shift points of k
@@ -191,45 +200,55 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
       uw_at(T, t, m, off, uv);
       psi_from(uv[0], uv[3], p, ps);
 #pragma unroll
-      for (int c = 0; c < 9; c++) SA[c] += ps[c];
-      rec_blk(Ru, ls, false, la);
-      applyM(-nk / 9.0, ps, tmp);
-#pragma unroll
-      for (int c = 0; c < 9; c++) la[c] += tmp[c];
-      rec_blk(Rk, kBlkLG0, true, lb);
-      applyM(-nu / 9.0, psi0, tmp);
-#pragma unroll
-      for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * (lb[c] + tmp[c]);
+      for (int c = 0; c < 9; c++) {
+        SA[c] += ps[c];
+        v[c] = ca * ps[c] + cb * psi0[c];
+      }
+      adjoint(Ru, ls, wa, Rk, kBlkLG0, wb, v, L);
       hess_psi(L, p, ir, uv, 1.0, H);
     }
     // unshifted point
-    rec_blk(Ru, kBlkLU, false, la);
-    applyM(-nk / 9.0, Tuk, tmp);
 #pragma unroll
-    for (int c = 0; c < 9; c++) la[c] += tmp[c];
-    rec_blk(Rk, kBlkLU, true, lb);
-    applyM(-nu / 9.0, SA, tmp);
-#pragma unroll
-    for (int c = 0; c < 9; c++) L[c] = wa * la[c] + wb * (lb[c] + tmp[c]);
-    hess_psi(L, p0, ir0, uv0, 1.0, H);
-    // pair-rule points: coincident angular term, pair and density
+    for (int c = 0; c < 9; c++) v[c] = ca * Tuk[c] + cb * SA[c];
+    adjoint(Ru, kBlkLU, wa, Rk, kBlkLU, wb, v, L);
+    geo.point(T, 0, 0.0, p, m, off, ir);
+    uw_at(T, t, m, off, uv);
+    hess_psi(L, p, ir, uv, 1.0, H);
+  }
+
+  // ---- pass B: pair-rule points: coincident angular term, pair and density ----
+#pragma unroll 1
+  for (int kk = lane; kk < nn; kk += 64) {
+    const int k = row[kk];
+    const double nk = s.ne[k];
+    if (!(nk > 0.0)) continue;
+    const double4 qk = s.qs[k];
+    const double sk = qk.w, bk = s.beta[k];
+    const int spk = s.spc[k];
+    const int t = pair_slot(spu, spk);
+    const double* Rk = rec + (size_t)k * kRec;
+    double rd[3];
+    min_image(box, qu, qk, rd);
+    PairGeo<double> geo;
+    geo.init(rd);
     const double cphi = 0.5 * nu * nk * (bu + bk);
     const double cra = bu * nu * Ru[kRecFP] * nk;
     const double crb = bk * nk * Rk[kRecFP] * nu;
-    const double cang = 2.0 * (wa * nk + wb * nu);
+    const double cang = 2.0 * (bu * nu * nu * nk * nk + bk * nk * nk * nu * nu);
     const double cjj = exact ? 2.0 * bk * nk * nk * nu * nu : 0.0;
     double gk[3] = {0, 0, 0};
 #pragma unroll 1
     for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      double dsi[3], dsj[3], J[27];
+      double p[3], ir, off, uv[6], ps[9], L[9], dsi[3], dsj[3], J[27];
+      int m;
       const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
       uw_at(T, t, m, off, uv);
       psi_from(uv[0], uv[3], p, ps);
       applyM(wt * cang, ps, L);
       hess_psi(L, p, ir, uv, 1.0, H);
       jac_psi(p, ir, uv, J);
-      jtmj_sym(J, J, wt * nk * nk, HJ);                 // 2 wt n_k^2 J^T M J
-      if (exact) jtmj_sym(J, J, 0.5 * wt * cjj, H);     // codes 5 + 7 of (k, u, u)
+      jtmj_self(J, 2.0 * wt * nk * nk, HJ);  // 2 wt n_k^2 J^T M J
+      jtmj_self(J, wt * cjj, H);              // codes 5 + 7 of (k, u, u); cjj = 0 unless exact
       const double* f = T.pr + ((size_t)t * T.nr + m) * 16;
       const double* fk = f + 4 + 4 * spk;
       const double* fu = f + 4 + 4 * spu;
@@ -249,12 +268,12 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
   }
 
   // ---- Jacobian sums of centre u, one block at a time ----
-  // blk 0..5: JG_ls; blk 6: JU (and the j1 = j2 part of JU^T M JG0);
-  // blk 7: JG0, closing JU^T M JG0 + JG0^T M JU.
+  // blk 0..5: JG_ls (with their j1 = j2 parts); blk 6: JU; blk 7: JG0, closing
+  // JU^T M JG0 + JG0^T M JU; blk 8: the j1 = j2 part of that product.
   __shared__ double sJU[27];
   double Hw[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform
 #pragma unroll 1
-  for (int blk = 0; blk < 8; blk++) {
+  for (int blk = 0; blk < 9; blk++) {
     double S[27];
 #pragma unroll
     for (int c = 0; c < 27; c++) S[c] = 0.0;
@@ -276,12 +295,8 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
         jac_psi(p, ir, uv, J);
 #pragma unroll
         for (int c = 0; c < 27; c++) S[c] += nk * J[c];
-        jtmj_sym(J, J, -nk * nk / 18.0, HJ);  // -n_k^2/9 J^T M J
+        jtmj_self(J, -nk * nk / 9.0, HJ);
       } else if (blk == 6) {
-        double J0[27];
-        geo.point(T, 0, 0.0, p, m, off, ir);
-        uw_at(T, t, m, off, uv);
-        jac_psi(p, ir, uv, J0);
 #pragma unroll 1
         for (int ls = 0; ls < 6; ls++) {
           geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
@@ -289,20 +304,36 @@ __global__ __launch_bounds__(64) void k_diag(State s, Tables T, const double* __
           jac_psi(p, ir, uv, J);
 #pragma unroll
           for (int c = 0; c < 27; c++) S[c] += nk * J[c];
-          jtmj_sym(J, J0, -nk * nk / 9.0, HJ);
         }
-      } else {
+      } else if (blk == 7) {
         geo.point(T, 0, 0.0, p, m, off, ir);
         uw_at(T, t, m, off, uv);
         jac_psi(p, ir, uv, J);
 #pragma unroll
         for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+      } else {
+        // S holds this pair's sum over the six j-shift points, J then J0
+#pragma unroll 1
+        for (int ls = 0; ls < 6; ls++) {
+          geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
+          uw_at(T, t, m, off, uv);
+          jac_psi(p, ir, uv, J);
+#pragma unroll
+          for (int c = 0; c < 27; c++) S[c] += J[c];
+        }
+        geo.point(T, 0, 0.0, p, m, off, ir);
+        uw_at(T, t, m, off, uv);
+        jac_psi(p, ir, uv, J);
+        jtmj_sym(S, J, -nk * nk / 9.0, HJ);
+#pragma unroll
+        for (int c = 0; c < 27; c++) S[c] = 0.0;
       }
     }
+    if (blk == 8) break;
 #pragma unroll
     for (int c = 0; c < 27; c++) S[c] = group_sum<64>(S[c]);
     if (blk < 6) {
-      jtmj_sym(S, S, 1.0 / 18.0, Hw);
+      jtmj_self(S, 1.0 / 9.0, Hw);
     } else if (blk == 6) {
       if (lane == 0)  // S is uniform after the reduction
 #pragma unroll

@@ -57,9 +57,9 @@ struct Spline {
   std::vector<double> c;  // n x 9 columns a b c d db dc dd ddc ddd
 };
 
-enum KernelId { KID_NEIGH = 0, KID_PHASE1, KID_PHASE2, KID_TOTALS, KID_COUNT };
+enum KernelId { KID_NEIGH = 0, KID_PHASE1, KID_PHASE2, KID_TOTALS, KID_DIAG, KID_COUNT };
 const char* kKernelNames[KID_COUNT] = {"neighbour_build", "phase1_site_sums", "phase2_gradient",
-                                       "totals"};
+                                       "totals", "diag_blocks"};
 
 template <typename T>
 struct DevBuf {
@@ -1552,8 +1552,11 @@ dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
   CHK(run_eval(c, false));
   const Instance& I = c->inst[id];
   HIPCHK(c->d_diag.ensure((size_t)I.n * 9));
+  hipEvent_t a{};
+  CHK(begin_timed(c, &a));
   HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, exact, make_state(c), c->T, c->d_rec.p, I.base,
                           I.n, c->d_diag.p));
+  CHK(end_timed(c, KID_DIAG, a));
   std::vector<double> tmp((size_t)I.n * 9);
   HIPCHK(hipMemcpyAsync(tmp.data(), c->d_diag.p, tmp.size() * sizeof(double),
                         hipMemcpyDeviceToHost, c->stream));
