@@ -863,6 +863,129 @@ void or_ref_diag_blocks(const or_system* s, const or_tables* T, int nsites, cons
   }
 }
 
+/* Zero-temperature gradient and Hessian: the functions of the mean-field
+ * versions evaluated at the mean positions (no measure).
+ * evaluate_DV_i_Dq_u_adp_MgHx (:147-329): H-only masks, "<" throughout
+ * (:163, :185, :200, :254); evaluate_D2V_i_Dq2_u_MgHx (:333-526): masks on x
+ * for any species, "<" for u and i (:350, :374), "<=" for j1, j2 (:387, :450),
+ * diagonal codes only as in a-21.  rho = the zero-temperature density. */
+static void pair_at(const or_system* s, int i, int j, double* q, double* n, int* spc) {
+  for (int a = 0; a < 3; a++) q[a] = s->q[3 * i + a];
+  img(s, i, j, q + 3);
+  n[0] = s->x[i]; n[1] = s->x[j];
+  spc[0] = s->spc[i]; spc[1] = s->spc[j];
+}
+static void trip_at(const or_system* s, int i, int j1, int j2, double* q, double* n, int* spc) {
+  for (int a = 0; a < 3; a++) q[a] = s->q[3 * i + a];
+  img(s, i, j1, q + 3);
+  img(s, i, j2, q + 6);
+  n[0] = s->x[i]; n[1] = s->x[j1]; n[2] = s->x[j2];
+  spc[0] = s->spc[i]; spc[1] = s->spc[j1]; spc[2] = s->spc[j2];
+}
+
+void or_ref_DV_i_Dq_u(const or_system* s, const or_tables* T, int u, int i, const double* rho,
+                      double out[3]) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (s->spc[u] == SPC_H && s->x[u] < OR_MIN_OCCUPANCY) return;
+  if (s->spc[i] == SPC_H && s->x[i] < OR_MIN_OCCUPANCY) return;
+  double drho[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, g[3], q[9], n[3];
+  int spc[3];
+  for (int k1 = 0; k1 < s->nnb[i]; k1++) {
+    int j1 = NB(s, i, k1);
+    if (s->spc[j1] == SPC_H && s->x[j1] < OR_MIN_OCCUPANCY) continue;
+    pair_at(s, i, j1, q, n, spc);
+    int sites2[2] = {i, j1};
+    for (int dir = 0; dir < 2; dir++) {
+      if (sites2[dir] != u) continue;
+      f_rho_dq(T, dir, g, n, q, spc);
+      for (int a = 0; a < 3; a++) drho[a] += g[a];
+      f_pair_dq(T, dir, g, n, q, spc);
+      for (int a = 0; a < 3; a++) dv[a] += g[a];
+    }
+    for (int k2 = k1; k2 < s->nnb[i]; k2++) {
+      int j2 = NB(s, i, k2);
+      if (s->spc[j2] == SPC_H && s->x[j2] < OR_MIN_OCCUPANCY) continue;
+      int sites3[3] = {i, j1, j2};
+      double f = (k1 == k2) ? 1.0 : 2.0;
+      trip_at(s, i, j1, j2, q, n, spc);
+      for (int dir = 0; dir < 3; dir++) {
+        if (sites3[dir] != u) continue;
+        f_dip_dq(T, dir, g, n, q, spc);
+        for (int a = 0; a < 3; a++) dv[a] += f * g[a];
+        f_quad_dq(T, dir, g, n, q, spc);
+        for (int a = 0; a < 3; a++) dv[a] += f * g[a];
+      }
+    }
+  }
+  double Fp = sp_d1(&T->embed[sslot(s->spc[i])], rho[i]);
+  for (int a = 0; a < 3; a++) out[a] = s->x[i] * Fp * drho[a] + dv[a];
+}
+
+void or_ref_D2V_i_Dq2_u(const or_system* s, const or_tables* T, int u, int i, const double* rho,
+                        double out[9]) {
+  for (int a = 0; a < 9; a++) out[a] = 0.0;
+  if (s->x[u] < OR_MIN_OCCUPANCY) return;
+  if (s->x[i] < OR_MIN_OCCUPANCY) return;
+  double drho[3] = {0, 0, 0}, d2rho[9] = {0}, hv[9] = {0}, g[3], h[9], q[9], n[3];
+  int spc[3];
+  for (int k1 = 0; k1 < s->nnb[i]; k1++) {
+    int j1 = NB(s, i, k1);
+    if (s->x[j1] <= OR_MIN_OCCUPANCY) continue;
+    pair_at(s, i, j1, q, n, spc);
+    int sites2[2] = {i, j1};
+    for (int dir = 0; dir < 2; dir++) {
+      if (sites2[dir] != u) continue;
+      f_rho_dq(T, dir, g, n, q, spc);
+      for (int a = 0; a < 3; a++) drho[a] += g[a];
+      f_rho_d2q(T, 3 * dir, h, n, q, spc);
+      for (int a = 0; a < 9; a++) d2rho[a] += h[a];
+      f_pair_d2q(T, 3 * dir, h, n, q, spc);
+      for (int a = 0; a < 9; a++) hv[a] += h[a];
+    }
+    for (int k2 = k1; k2 < s->nnb[i]; k2++) {
+      int j2 = NB(s, i, k2);
+      if (s->x[j2] <= OR_MIN_OCCUPANCY) continue;
+      int sites3[3] = {i, j1, j2};
+      if (sites3[0] != u && sites3[1] != u && sites3[2] != u) continue;
+      double f = (k1 == k2) ? 1.0 : 2.0;
+      trip_at(s, i, j1, j2, q, n, spc);
+      for (int dir = 0; dir < 3; dir++) {
+        if (sites3[dir] != u) continue;
+        f_dip_d2q(T, 4 * dir, h, n, q, spc);
+        for (int a = 0; a < 9; a++) hv[a] += f * h[a];
+        f_quad_d2q(T, 4 * dir, h, n, q, spc);
+        for (int a = 0; a < 9; a++) hv[a] += f * h[a];
+      }
+    }
+  }
+  const or_spline* emb = &T->embed[sslot(s->spc[i])];
+  double xi = s->x[i], Fp = sp_d1(emb, rho[i]), Fpp = sp_d2(emb, rho[i]);
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      out[3 * a + c] = xi * Fpp * drho[a] * drho[c] + xi * Fp * d2rho[3 * a + c] + hv[3 * a + c];
+}
+
+/* Zero-temperature force and diagonal blocks with the residual's weights:
+ * F_u = sum_{i in {u} U N(u)} beta_i dV_i/dq_u (n x 3), D_u likewise (n x 9). */
+void or_ref_zero_t(const or_system* s, const or_tables* T, int nsites, const int* sites,
+                   const double* rho, double* F, double* D) {
+  int cnt = sites ? nsites : s->n;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < cnt; k++) {
+    int u = sites ? sites[k] : k;
+    double f[3] = {0, 0, 0}, d[9] = {0}, g[3], h[9];
+    for (int c = -1; c < s->nnb[u]; c++) {
+      int i = c < 0 ? u : NB(s, u, c);
+      or_ref_DV_i_Dq_u(s, T, u, i, rho, g);
+      or_ref_D2V_i_Dq2_u(s, T, u, i, rho, h);
+      for (int a = 0; a < 3; a++) f[a] += s->beta[i] * g[a];
+      for (int a = 0; a < 9; a++) d[a] += s->beta[i] * h[a];
+    }
+    for (int a = 0; a < 3; a++) F[3 * (long)k + a] = f[a];
+    for (int a = 0; a < 9; a++) D[9 * (long)k + a] = d[a];
+  }
+}
+
 /* d(-log Z0_u)/d sigma_u and d(-log Z0_u - gamma min(x,0.9999))/dx_u for the
  * site functional S0 (:1124-1139). */
 static inline double ent_dsig(const or_system* s, int u) { return -3.0 / s->sig[u]; }

@@ -101,6 +101,15 @@ void or_ref_D2V0_i_Dq2_u(const or_system* s, const or_tables* t, int u, int i,
 /* D_u = sum_{i in {u} U N(u)} beta_i d2V0_i/dq_u2 for the listed sites (n x 9). */
 void or_ref_diag_blocks(const or_system* s, const or_tables* t, int nsites, const int* sites,
                         const double* mf_rho, double* D);
+/* Zero temperature (src/ADP/MgHx-mf-V-bulk.cpp:147-329 and :333-526): dV_i/dq_u
+ * and d2V_i/dq_u2 at the mean positions, rho = or_ref_rho0_i of every site. */
+void or_ref_DV_i_Dq_u(const or_system* s, const or_tables* t, int u, int i, const double* rho,
+                      double out[3]);
+void or_ref_D2V_i_Dq2_u(const or_system* s, const or_tables* t, int u, int i, const double* rho,
+                        double out[9]);
+/* F_u = sum_i beta_i dV_i/dq_u (cnt x 3) and D_u = sum_i beta_i d2V_i/dq_u2 (cnt x 9). */
+void or_ref_zero_t(const or_system* s, const or_tables* t, int nsites, const int* sites,
+                   const double* rho, double* F, double* D);
 /* Residual of sum_i S0_i for the listed sites u (all sites when sites==NULL). */
 void or_ref_residual(const or_system* s, const or_tables* t, int nsites, const int* sites,
                      const double* mf_rho, double* Rq, double* Rs, double* Rx);

@@ -79,6 +79,9 @@ class Oracle:
         L.or_ref_diag_blocks.argtypes = [sp, C.c_void_p, C.c_int, _ip, _dp, _dp]
         L.or_fact_diag_blocks.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp]
         L.or_fn_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _dp]
+        L.or_ref_DV_i_Dq_u.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.or_ref_D2V_i_Dq2_u.argtypes = [sp, C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.or_ref_zero_t.argtypes = [sp, C.c_void_p, C.c_int, _ip, _dp, _dp, _dp]
         h = C.c_void_p()
         rc = L.or_load_tables(C.byref(h), table_dir.encode())
         if rc != 0:
@@ -197,6 +200,32 @@ class Oracle:
         D = np.zeros((cnt, 9))
         self.lib.or_ref_diag_blocks(C.byref(s), self.tab, cnt, sp, _d(mf_rho), _d(D))
         return D.reshape(cnt, 3, 3)
+
+    def ref_zero_t(self, s, rho0, sites=None):
+        """Zero-temperature force and diagonal blocks with the residual's
+        weights: (F (cnt, 3), D (cnt, 3, 3)); rho0 = ref_rho0(s)."""
+        rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
+        if sites is None:
+            cnt, sp = s.n, None
+        else:
+            sarr = np.ascontiguousarray(sites, dtype=np.int32)
+            cnt, sp = len(sarr), _i(sarr)
+        F = np.zeros((cnt, 3))
+        D = np.zeros((cnt, 9))
+        self.lib.or_ref_zero_t(C.byref(s), self.tab, cnt, sp, _d(rho0), _d(F), _d(D))
+        return F, D.reshape(cnt, 3, 3)
+
+    def ref_DV_Dq_zero_t(self, s, u, i, rho0):
+        rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
+        out = np.zeros(3)
+        self.lib.or_ref_DV_i_Dq_u(C.byref(s), self.tab, u, i, _d(rho0), _d(out))
+        return out
+
+    def ref_D2V_Dq2_zero_t(self, s, u, i, rho0):
+        rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
+        out = np.zeros(9)
+        self.lib.or_ref_D2V_i_Dq2_u(C.byref(s), self.tab, u, i, _d(rho0), _d(out))
+        return out.reshape(3, 3)
 
     def fn_eval(self, which, deriv, code, n, q, slots):
         """One ADP pair/triplet function (which: 0 rho, 1 pair, 2 dipole,

@@ -138,3 +138,24 @@ def test_exact_blocks_are_residual_derivatives_at_full_size(tmp_path):
             assert err_r > 1e-5
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("maker", [lambda: lat.cluster(3, 2, 2, x_h=(0.0, 0.4), seed=3),
+                                   lambda: lat.hcp_mgh(5, 3, 3, x_h=(0.0, 0.3), seed=9)])
+def test_zero_temperature_force_and_blocks(gpu, oracle, maker):
+    """sigma -> 0: the GPU force residual R_q and the diagonal blocks equal the
+    reference's zero-temperature gradient and Hessian (MgHx-mf-V-bulk.cpp:
+    147-329, 333-526) summed with the residual's weights.  sigma = 1e-12 A:
+    the quadrature points sit 2.4e-12 A from the mean, so no point pair
+    straddles a spline knot (where the reference tables' derivatives jump)
+    and the O(sigma^2) rule error is below rounding."""
+    sysd = maker()
+    sysd["sigma"][:] = 1e-12
+    say(f"[gpu] zero-temperature force and blocks, {sysd['n']} sites")
+    s = _setup(gpu, oracle, sysd)
+    F, D = oracle.ref_zero_t(s, oracle.ref_rho0(s))
+    Rq = gpu.residual()["Rq"]
+    G = gpu.diag_blocks()
+    ef, ed = rel(Rq, F), rel(G, D)
+    say(f"    force {ef:.2e}, blocks {ed:.2e}")
+    assert ef < TOL and ed < TOL

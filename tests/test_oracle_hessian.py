@@ -173,3 +173,55 @@ def test_masked_sites_have_zero_blocks(oracle):
     assert (F[hs[:3]] == 0).all()
     R = oracle.ref_diag_blocks(s, oracle.ref_mf_rho(s))
     assert rel(F, R) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Zero temperature (evaluate_DV_i_Dq_u_adp_MgHx :147-329 and
+# evaluate_D2V_i_Dq2_u_MgHx :333-526): the functions of the mean-field forms
+# evaluated at the mean positions, so the sigma -> 0 limits of the pinned
+# mean-field residual and blocks; the Hessian also against differences of the
+# zero-temperature gradient.
+# ---------------------------------------------------------------------------
+
+def test_zero_temperature_force_and_blocks_are_width_limits(oracle):
+    sysd = lat.cluster(2, 1, 1, x_h=(0.0, 0.4), seed=3)
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    errs = []
+    for sg in (1e-5, 1e-6):
+        sysd["sigma"][:] = sg
+        s = oracle.system(sysd, nnb, nbr)
+        F, D = oracle.ref_zero_t(s, oracle.ref_rho0(s))
+        mf = oracle.ref_mf_rho(s)
+        Rq, _, _ = oracle.ref_residual(s, mf)
+        errs.append((rel(Rq, F), rel(oracle.ref_diag_blocks(s, mf), D)))
+    assert errs[1][0] < 2e-2 * errs[0][0] + 1e-14  # O(sigma^2)
+    assert errs[1][0] < 1e-9 and errs[1][1] < 1e-9
+
+
+def test_zero_temperature_hessian_against_gradient_differences(synth):
+    sysd = lat.cluster(2, 1, 1, x_h=(0.05, 0.5), seed=4, sigma_jitter=0.3)
+    st, nnb, nbr = synth.neighbors(sysd, 6.5)
+    s = synth.system(sysd, nnb, nbr)
+    rho0 = synth.ref_rho0(s)
+    rng = np.random.default_rng(8)
+    h = 1e-5
+    for u in rng.choice(sysd["n"], 3, replace=False):
+        for i in [u] + list(nbr[u, :2]):
+            H = synth.ref_D2V_Dq2_zero_t(s, u, i, rho0)
+            fd = np.zeros((3, 3))
+            for c in range(3):
+                out = []
+                for sg in (1, -1):
+                    sd = _copy(sysd)
+                    sd["q"][u, c] += sg * h
+                    ss = synth.system(sd, nnb, nbr)
+                    out.append(synth.ref_DV_Dq_zero_t(ss, u, i, synth.ref_rho0(ss)))
+                fd[:, c] = (out[0] - out[1]) / (2 * h)
+            if i != u:
+                # codes 5 + 7 of the coincident triplet (i, u, u) at the mean point
+                q = np.stack([sysd["q"][i], sysd["q"][u], sysd["q"][u]])
+                n = np.array([sysd["x"][i], sysd["x"][u], sysd["x"][u]])
+                sl = np.array([int(sysd["species"][k] == lat.H) for k in (i, u, u)])
+                for which in (2, 3):
+                    H = H + synth.fn_eval(which, 2, 5, n, q, sl) + synth.fn_eval(which, 2, 7, n, q, sl)
+            assert rel(H, fd) < 1e-7
