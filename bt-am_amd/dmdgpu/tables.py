@@ -57,6 +57,157 @@ def write_tables(folder, fn, nr, dxr, ne, dxe, rcut=None):
             _block(fh, hermite_rows(*fn[k + "MgH"], nr, dxr), dxr)
 
 
+# ---------------------------------------------------------------------------
+# setfl (LAMMPS pair_style adp) -> reference tables.  A re-derivation of the
+# reference's converter external/nist2solera/nist2solera.cpp: natural cubic
+# splines through the tabulated points (:53-103), r*phi -> phi with a cubic
+# extrapolation to r = 0 (:214-229), a zero appended at r_c when the grid
+# stops short of it by 1e-4 or more (:151-159), and its fixed-point writer
+# (:311-435).  Files are named as the reference's loader opens them
+# (adpmg.dat, adph.dat, adpmgh.dat: lower case).
+# ---------------------------------------------------------------------------
+
+def read_setfl_adp(path):
+    """Parse an ADP setfl file: 3 comment lines; N and the element names;
+    ``Nrho drho Nr dr rc``; per element ``Z mass a lattice``, F(rho) (Nrho
+    values), rho(r) (Nr values); then r*phi(r), u(r) and w(r) for the pairs
+    i <= j in j-major order (nist2solera.cpp:120-211)."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    tok = " ".join(lines[3:]).split()
+    pos = 0
+
+    def take(k):
+        nonlocal pos
+        out = tok[pos:pos + k]
+        pos += k
+        return out
+
+    n_el = int(take(1)[0])
+    names = take(n_el)
+    n_rho, d_rho, n_r, d_r, rc = take(5)
+    n_rho, n_r, d_rho, d_r, rc = int(n_rho), int(n_r), float(d_rho), float(d_r), float(rc)
+    el = []
+    for _ in range(n_el):
+        z, mass, a, lat = take(4)
+        F = np.array(take(n_rho), dtype=np.float64)
+        rho = np.array(take(n_r), dtype=np.float64)
+        el.append({"Z": int(z), "mass": float(mass), "a": float(a), "lattice": lat, "F": F,
+                   "rho": rho})
+    pairs = [(i, j) for j in range(n_el) for i in range(j + 1)]
+    tabs = {}
+    for key in ("rphi", "u", "w"):
+        for ij in pairs:
+            tabs[(key,) + ij] = np.array(take(n_r), dtype=np.float64)
+    return {"names": names, "n_rho": n_rho, "d_rho": d_rho, "n_r": n_r, "d_r": d_r, "rc": rc,
+            "elements": el, "pairs": pairs, "tables": tabs}
+
+
+def write_setfl_adp(path, names, masses, n_rho, d_rho, n_r, d_r, rc, F, rho, rphi, u, w):
+    """Write an ADP setfl file in the layout read_setfl_adp parses (synthetic
+    potentials for tests and examples).  F[i], rho[i]: per element; rphi, u, w:
+    dicts keyed by (i, j), i <= j."""
+    out = ["synthetic ADP potential", "written by dmdgpu.tables", "setfl layout",
+           f" {len(names)} " + " ".join(names),
+           f"{n_rho} {d_rho!r} {n_r} {d_r!r} {rc!r}"]
+    for i in range(len(names)):
+        out.append(f"{i + 1} {masses[i]!r} 1.0 hcp")
+        out += [repr(float(v)) for v in F[i]]
+        out += [repr(float(v)) for v in rho[i]]
+    for tab in (rphi, u, w):
+        for j in range(len(names)):
+            for i in range(j + 1):
+                out += [repr(float(v)) for v in tab[(i, j)]]
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+def natural_cubic_spline(x, y):
+    """Coefficients (a, b, c, d) per segment of the natural cubic spline
+    through (x_k, y_k), by the tridiagonal sweep of nist2solera.cpp:53-103
+    (the same operations in the same order, so the same doubles)."""
+    n = len(x) - 1
+    a = [float(v) for v in y]
+    x = [float(v) for v in x]
+    h = [x[i + 1] - x[i] for i in range(n)]
+    alpha = [0.0] * n
+    for i in range(1, n):
+        alpha[i] = 3 * (a[i + 1] - a[i]) / h[i] - 3 * (a[i] - a[i - 1]) / h[i - 1]
+    c = [0.0] * (n + 1)
+    l = [0.0] * (n + 1)
+    mu = [0.0] * (n + 1)
+    z = [0.0] * (n + 1)
+    l[0] = 1.0
+    for i in range(1, n):
+        l[i] = 2 * (x[i + 1] - x[i - 1]) - h[i - 1] * mu[i - 1]
+        mu[i] = h[i] / l[i]
+        z[i] = (alpha[i] - h[i - 1] * z[i - 1]) / l[i]
+    b = [0.0] * n
+    d = [0.0] * n
+    for j in range(n - 1, -1, -1):
+        c[j] = z[j] - mu[j] * c[j + 1]
+        b[j] = (a[j + 1] - a[j]) / h[j] - h[j] * (c[j + 1] + 2 * c[j]) / 3
+        d[j] = (c[j + 1] - c[j]) / 3 / h[j]
+    return a[:n], b, c[:n], d
+
+
+def _spline_block(x, y, dx):
+    a, b, c, d = natural_cubic_spline(x, y)
+    out = [f"{len(a)} {dx:.16f}\n"]
+    for k in range(len(a)):
+        vals = (a[k], b[k], c[k], d[k], b[k], 2 * c[k], 3 * d[k], 2 * c[k], 6 * d[k])
+        out.append(" ".join(f"{v:.16f}" for v in vals) + "\n")
+    return "".join(out)
+
+
+def setfl_to_tables(path, folder, tol=1e-4):
+    """Convert an ADP setfl file into the reference's tables in ``folder``:
+    one ``adp<el>.dat`` per element (rho, F, phi, u, w of the like pair) and
+    one ``adp<el_i><el_j>.dat`` per unlike pair (phi, u, w).  Returns the
+    written paths."""
+    S = read_setfl_adp(path)
+    n_r, d_r, rc = S["n_r"], S["d_r"], S["rc"]
+    extend = not (rc - (n_r - 1) * d_r < tol)
+    cols = n_r + 1 if extend else n_r
+    r = [k * d_r for k in range(cols)]
+    rho_x = [k * S["d_rho"] for k in range(S["n_rho"])]
+
+    def padded(v):
+        v = [float(t) for t in v]
+        return v + [0.0] if extend else v
+
+    def phi_of(rphi):
+        v = padded(rphi)
+        phi = [0.0] * cols
+        for k in range(1, cols):
+            phi[k] = v[k] / r[k]
+        phi[0] = 4 * phi[1] - 6 * phi[2] + 4 * phi[3] - phi[4]
+        return phi
+
+    os.makedirs(folder, exist_ok=True)
+    names = [nm.lower() for nm in S["names"]]
+    written = []
+    for (i, j) in S["pairs"]:
+        T = S["tables"]
+        body = [_spline_block(r, phi_of(T[("rphi", i, j)]), d_r),
+                _spline_block(r, padded(T[("u", i, j)]), d_r),
+                _spline_block(r, padded(T[("w", i, j)]), d_r)]
+        if i == j:
+            e = S["elements"][i]
+            head = f"{e['mass']:.5f} {1.0:.1f} {rc:.16f} \n"
+            body = [_spline_block(r, padded(e["rho"]), d_r),
+                    _spline_block(rho_x, e["F"], S["d_rho"])] + body
+            fname = f"adp{names[i]}.dat"
+        else:
+            head = f"{rc:.16f}\n"
+            fname = f"adp{names[i]}{names[j]}.dat"
+        dst = os.path.join(folder, fname)
+        with open(dst, "w") as fh:
+            fh.write(head + "".join(body))
+        written.append(dst)
+    return written
+
+
 def poly(*c):
     """(f, df) of the polynomial c0 + c1 x + c2 x^2 + c3 x^3."""
     c = list(c) + [0.0] * (4 - len(c))
