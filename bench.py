@@ -212,6 +212,46 @@ def solve_instances(count):
     return lat.mixed_batch(count, seed=2024)
 
 
+def _relax_plain(ctx, insts, opts):
+    """One instance at a time; returns (seconds, [(status, q, sigma)], per-instance)."""
+    t = 0.0
+    res, per = [], []
+    for sd in insts:
+        ctx.set_state(sd)
+        ctx.neighbors()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.relax(opts)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        t += dt
+        st = ctx.relax_status(0)
+        q, sg, _ = ctx.instance_state(0)
+        res.append((st, q, sg))
+        per.append({"sites": int(sd["n"]), "evals": st["evals"],
+                    "ms_per_eval": dt * 1e3 / max(st["evals"], 1)})
+    ctx.remove_instance(0)
+    return t, res, per
+
+
+def _relax_batched(ctx, insts, opts):
+    """All instances resident, one launch per tick; returns (seconds, ticks, [(status, q)])."""
+    ids = [ctx.add_instance(sd) for sd in insts]
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in ids:
+        ctx.relax_start(i, opts)
+    ticks = 0
+    while ctx.relax_tick():
+        ticks += 1
+    ctx.sync()
+    t = time.perf_counter() - t0
+    res = [(ctx.relax_status(i), ctx.instance_state(i)[0]) for i in ids]
+    for i in ids:
+        ctx.remove_instance(i)
+    return t, ticks, res
+
+
 def solves_leg(tuning, count, graph=True):
     # tuning = ((k1, k2, unroll2) FP64, (k1, k2, unroll2) split)
     """Solves per hour: the plain path (one instance at a time, FP64 Hessian
@@ -219,65 +259,48 @@ def solves_leg(tuning, count, graph=True):
     tick, split-precision probes, FP64 residuals and stopping test).  Both
     relax Mg positions and widths to |R|_2 <= 1e-8, each with the inner
     tolerance that minimises its own residual evaluations (CG eta 0.05 for
-    FP64 probes, 0.1 for split-precision probes)."""
+    FP64 probes, 0.1 for split-precision probes).  Run with plain CG and with
+    the block-Jacobi preconditioner from the diagonal Hessian blocks, the
+    same solver options on both sides of each comparison."""
     import dmdgpu
     insts = solve_instances(count)
     sizes = [int(s["n"]) for s in insts]
-    out = {"instances": len(insts), "sites_total": sum(sizes), "sites_min": min(sizes),
+    n = len(insts)
+    out = {"instances": n, "sites_total": sum(sizes), "sites_min": min(sizes),
            "sites_max": max(sizes), "atol": 1e-8}
     ctx = dmdgpu.Context(0)
     ctx.set_tuning_precision("fp64", *tuning[0])
     ctx.set_tuning_precision("split32", *tuning[1])
     ctx.set_graph(graph)
-    plain_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, mixed=False)
-    t_plain = 0.0
-    plain = []
-    per_inst = []
-    for sd in insts:
-        ctx.set_state(sd)
-        ctx.neighbors()
-        ctx.sync()
-        t0 = time.perf_counter()
-        ctx.relax(plain_opts)
-        ctx.sync()
-        dt = time.perf_counter() - t0
-        t_plain += dt
-        st = ctx.relax_status(0)
-        q, sg, _ = ctx.instance_state(0)
-        plain.append((st, q, sg))
-        per_inst.append({"sites": int(sd["n"]), "evals": st["evals"],
-                         "ms_per_eval": dt * 1e3 / max(st["evals"], 1)})
-    ctx.remove_instance(0)
-    ids = [ctx.add_instance(sd) for sd in insts]
-    mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.1, mixed=True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for i in ids:
-        ctx.relax_start(i, mixed_opts)
-    ticks = 0
-    while ctx.relax_tick():
-        ticks += 1
-    ctx.sync()
-    t_batch = time.perf_counter() - t0
-    batch = [ctx.relax_status(i) for i in ids]
-    dq = 0.0
-    for k, i in enumerate(ids):
-        q, sg, _ = ctx.instance_state(i)
-        dq = max(dq, float(np.abs(q - plain[k][1]).max()))
+    for pc in (False, True):
+        plain_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05,
+                                       mixed=False, precond=pc)
+        mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.1, mixed=True,
+                                       precond=pc)
+        t_plain, plain, per_inst = _relax_plain(ctx, insts, plain_opts)
+        t_batch, ticks, batch = _relax_batched(ctx, insts, mixed_opts)
+        dq = max(float(np.abs(b[1] - p[1]).max()) for b, p in zip(batch, plain))
+        rec = {
+            "plain_fp64_per_instance": {
+                "solves_per_hour": n * 3600.0 / t_plain, "seconds": t_plain,
+                "converged": sum(p[0]["state"] == "converged" for p in plain),
+                "residual_evals": sum(p[0]["evals"] for p in plain)},
+            "batched_split32_probes": {
+                "solves_per_hour": n * 3600.0 / t_batch, "seconds": t_batch,
+                "converged": sum(b[0]["state"] == "converged" for b in batch),
+                "residual_evals": sum(b[0]["evals"] for b in batch), "ticks": ticks,
+                "ms_per_tick": t_batch * 1e3 / max(ticks, 1)},
+            "speedup": t_plain / t_batch, "max_position_diff_A": dq}
+        log(f"plain per instance (precond={pc}): {per_inst}")
+        if pc:
+            out["block_jacobi_preconditioned"] = rec
+        else:
+            out.update(rec)
     ctx.close()
-    n = len(insts)
-    out["plain_fp64_per_instance"] = {
-        "solves_per_hour": n * 3600.0 / t_plain, "seconds": t_plain,
-        "converged": sum(p[0]["state"] == "converged" for p in plain),
-        "residual_evals": sum(p[0]["evals"] for p in plain)}
-    log(f"plain per instance: {per_inst}")
-    out["batched_split32_probes"] = {
-        "solves_per_hour": n * 3600.0 / t_batch, "seconds": t_batch,
-        "converged": sum(s["state"] == "converged" for s in batch),
-        "residual_evals": sum(s["evals"] for s in batch), "ticks": ticks,
-        "ms_per_tick": t_batch * 1e3 / max(ticks, 1)}
-    out["speedup"] = t_plain / t_batch
-    out["max_position_diff_A"] = dq
+    pcr = out["block_jacobi_preconditioned"]
+    out["preconditioner_gain"] = {
+        "plain": out["plain_fp64_per_instance"]["seconds"] / pcr["plain_fp64_per_instance"]["seconds"],
+        "batched": out["batched_split32_probes"]["seconds"] / pcr["batched_split32_probes"]["seconds"]}
     return out
 
 

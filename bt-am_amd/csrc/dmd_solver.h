@@ -16,7 +16,9 @@ enum : int {
   kUpdSdDir = 32      // D = c_sd P
 };
 enum : int { kMakeNone = 0, kMakeX = 1, kMakeXP = 2, kMakeXD = 3 };
-enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4 };
+enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4, kDotPc = 5 };
+// Dots per instance: [0..2] sums, [3..4] maxima, [5..13] sums (preconditioned CG).
+constexpr int kSolveDots = 14;
 
 // Diffusion-step modes (SolveOp.diff): one Picard sweep of the implicit
 // occupancy update, or the bare rates dx/dt written into Xt.
@@ -34,6 +36,7 @@ enum : int { kDofMgQ = 1, kDofMgS = 2, kDofHQ = 4, kDofHS = 8 };
 struct SolveOp {
   int upd, make, dot, mixed;
   int dof;          // kDof* mask; residual components outside it are zero
+  int pc;           // 1: block-Jacobi preconditioned CG (SolveVecs::minv)
   double eps_prev;  // probe step of the previous tick (kUpdCgStep)
   double alpha, beta, c_sd;
   double eps;       // probe step of this tick (kMakeXP, kDotCg)
@@ -53,8 +56,16 @@ struct SolveVecs {
   double4* Pp;   // CG direction
   double* Xn;    // occupancy at the start of a diffusion step
   double* Xt;    // occupancy of the next Picard iterate (or rates)
+  const double* minv;  // n x 10: inverse of the site's q block (3x3 row-major), sigma factor
 };
 
+// Block-Jacobi preconditioner of an instance's sites [base, base + cap) from
+// the diagonal Hessian blocks D (n x 9) of its n sites: minv = D_u^-1 on
+// relaxed q of active sites (a scaled identity where D_u is not positive
+// definite), 1 / (3/sigma^2 + max(tr D_u, 0)) on relaxed sigma, identity
+// elsewhere (also on the padding sites [base + n, base + cap)).
+hipError_t launch_pc_invert(hipStream_t st, int base, int n, int cap, int dof, const int* spc,
+                            const double* ne, const double4* qs, const double* D, double* minv);
 hipError_t launch_solver_step(hipStream_t st, int n, const int* inst, const int* spc,
                               const double* ne, const SolveOp* ops, const SolveVecs& v,
                               double4* qs, const Outputs& o);

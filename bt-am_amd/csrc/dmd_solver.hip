@@ -29,6 +29,11 @@ DEV double4 axpy4(double a, const double4& x, const double4& y) {
 DEV double dot4(const double4& a, const double4& b) {
   return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
 }
+// z = M^-1 r of one site (3x3 block on q, scalar on sigma).
+DEV double4 apply_minv(const double* m, const double4& r) {
+  return make_double4(m[0] * r.x + m[1] * r.y + m[2] * r.z, m[3] * r.x + m[4] * r.y + m[5] * r.z,
+                      m[6] * r.x + m[7] * r.y + m[8] * r.z, m[9] * r.w);
+}
 
 // Applies the requested updates of each instance, then writes its evaluation
 // point into the residual kernels' input qs.
@@ -57,7 +62,7 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
       const double4 Rr = axpy4(-op.alpha, Ap, v.Rr[i]);
       v.D[i] = D;
       v.Rr[i] = Rr;
-      v.Pp[i] = axpy4(op.beta, Pp, Rr);
+      v.Pp[i] = axpy4(op.beta, Pp, op.pc ? apply_minv(v.minv + 10 * (size_t)i, Rr) : Rr);
     }
     if (op.upd & kUpdR0FromRv) v.R0[i] = Rv;
     if (op.upd & kUpdR0fFromRv) v.R0f[i] = Rv;
@@ -67,7 +72,7 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
     const double4 m = make_double4(-R0.x, -R0.y, -R0.z, -R0.w);
     v.D[i] = make_double4(0.0, 0.0, 0.0, 0.0);
     v.Rr[i] = m;
-    v.Pp[i] = m;
+    v.Pp[i] = op.pc ? apply_minv(v.minv + 10 * (size_t)i, m) : m;
   }
   if (op.upd & kUpdSdDir) v.D[i] = make_double4(op.c_sd * v.Pp[i].x, op.c_sd * v.Pp[i].y,
                                                 op.c_sd * v.Pp[i].z, op.c_sd * v.Pp[i].w);
@@ -78,9 +83,14 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
   if (op.make != kMakeNone) qs[i] = Yn;
 }
 
-// Per-chunk partials: [0] sum a, [1] sum b, [2] sum c, [3] max d, [4] max e.
+// Per-chunk partials: [0] sum a, [1] sum b, [2] sum c, [3] max d, [4] max e,
+// [5..13] sums g[0..8].
 //   kDotNorm : a = |Rv|^2
-//   kDotCg   : Ap = (Rv - Rbase)/eps; a = P.Ap, b = Rr.Ap, c = Ap.Ap
+//   kDotCg   : Ap = (Rv - Rbase)/eps; a = P.Ap, b = Rr.Ap, c = Ap.Ap; with the
+//              preconditioner (z = M^-1 Rr, y = M^-1 Ap) also g = z.Ap, Ap.y,
+//              Rr.z, z.z, z.y, y.y, z.P, y.P, P.P -- what the host needs to
+//              advance r.z and |P|^2 of the next direction without a new pass
+//   kDotPc   : after CgInit (P = M^-1 Rr): g[2] = Rr.P, g[8] = P.P
 //   kDotLs   : d = max |D_q| (component), e = max(-D_sigma / X_sigma)
 //   kDotDiff : over H sites, a = sum Xt (hydrogen content of the new iterate),
 //              b = sum x (of the current one), d = max |Xt - x|,
@@ -89,10 +99,11 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
                                                     const double* ne, const SolveOp* ops,
                                                     SolveVecs v, Outputs o, const double* xr,
                                                     double* partials) {
-  __shared__ double s[5][256];
+  __shared__ double s[kSolveDots][256];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k = inst[blockIdx.x * 256];
   double a = 0, b = 0, c = 0, d = 0, e = -1e300;
+  double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (i < n && k >= 0 && ops[k].dot == kDotDiff) {
     if (spc[i] == 1) {
       const double xt = v.Xt[i], x = xr[i];
@@ -112,9 +123,21 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
       const double ie = 1.0 / op.eps;
       const double4 Ap = make_double4((Rv.x - Rb.x) * ie, (Rv.y - Rb.y) * ie, (Rv.z - Rb.z) * ie,
                                       (Rv.w - Rb.w) * ie);
-      a = dot4(v.Pp[i], Ap);
-      b = dot4(v.Rr[i], Ap);
+      const double4 P = v.Pp[i], Rr = v.Rr[i];
+      a = dot4(P, Ap);
+      b = dot4(Rr, Ap);
       c = dot4(Ap, Ap);
+      if (op.pc) {
+        const double* m = v.minv + 10 * (size_t)i;
+        const double4 z = apply_minv(m, Rr), y = apply_minv(m, Ap);
+        g[0] = dot4(z, Ap); g[1] = dot4(Ap, y); g[2] = dot4(Rr, z);
+        g[3] = dot4(z, z); g[4] = dot4(z, y); g[5] = dot4(y, y);
+        g[6] = dot4(z, P); g[7] = dot4(y, P); g[8] = dot4(P, P);
+      }
+    } else if (op.dot == kDotPc) {
+      const double4 P = v.Pp[i];
+      g[2] = dot4(v.Rr[i], P);
+      g[8] = dot4(P, P);
     } else if (op.dot == kDotLs) {
       const double4 D = v.D[i];
       d = fmax(fabs(D.x), fmax(fabs(D.y), fabs(D.z)));
@@ -123,6 +146,8 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
   }
   s[0][threadIdx.x] = a; s[1][threadIdx.x] = b; s[2][threadIdx.x] = c;
   s[3][threadIdx.x] = d; s[4][threadIdx.x] = e;
+#pragma unroll
+  for (int t = 0; t < 9; t++) s[5 + t][threadIdx.x] = g[t];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
@@ -131,10 +156,13 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
       s[2][threadIdx.x] += s[2][threadIdx.x + w];
       s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + w]);
       s[4][threadIdx.x] = fmax(s[4][threadIdx.x], s[4][threadIdx.x + w]);
+#pragma unroll
+      for (int t = 5; t < kSolveDots; t++) s[t][threadIdx.x] += s[t][threadIdx.x + w];
     }
     __syncthreads();
   }
-  if (threadIdx.x < 5) partials[5 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < kSolveDots)
+    partials[kSolveDots * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
 // One block per instance over its chunk range, fixed order.
@@ -145,14 +173,72 @@ __global__ __launch_bounds__(64) void k_solver_reduce(const int* chunk_lo, const
   if (threadIdx.x != 0) return;
   if (ops[k].dot == kDotNone) return;
   double a = 0, b = 0, c = 0, d = 0, e = -1e300;
+  double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int ch = chunk_lo[k]; ch < chunk_hi[k]; ch++) {
-    a += partials[5 * ch];
-    b += partials[5 * ch + 1];
-    c += partials[5 * ch + 2];
-    d = fmax(d, partials[5 * ch + 3]);
-    e = fmax(e, partials[5 * ch + 4]);
+    const double* p = partials + (size_t)kSolveDots * ch;
+    a += p[0];
+    b += p[1];
+    c += p[2];
+    d = fmax(d, p[3]);
+    e = fmax(e, p[4]);
+#pragma unroll
+    for (int t = 0; t < 9; t++) g[t] += p[5 + t];
   }
-  out[5 * k] = a; out[5 * k + 1] = b; out[5 * k + 2] = c; out[5 * k + 3] = d; out[5 * k + 4] = e;
+  double* o = out + (size_t)kSolveDots * k;
+  o[0] = a; o[1] = b; o[2] = c; o[3] = d; o[4] = e;
+#pragma unroll
+  for (int t = 0; t < 9; t++) o[5 + t] = g[t];
+}
+
+__global__ __launch_bounds__(256) void k_pc_invert(int base, int n, int cap, int dof,
+                                                  const int* spc, const double* ne,
+                                                  const double4* qs, const double* D,
+                                                  double* minv) {
+  const int li = blockIdx.x * 256 + threadIdx.x;
+  if (li >= cap) return;
+  const int i = base + li;
+  double* m = minv + 10 * (size_t)i;
+  if (li >= n) {  // padding sites of the instance's range: identity
+#pragma unroll
+    for (int t = 0; t < 10; t++) m[t] = (t % 4 == 0 || t == 9) ? 1.0 : 0.0;
+    return;
+  }
+  const double* d = D + 9 * (size_t)li;
+  const bool act = ne[i] != 0.0;
+  const int slot = spc[i];
+  const bool rq = act && (dof & (slot == 0 ? kDofMgQ : kDofHQ));
+  const bool rs = act && (dof & (slot == 0 ? kDofMgS : kDofHS));
+  const double tr = d[0] + d[4] + d[8];
+  // q block: inverse by cofactors when all leading minors are positive
+  const double c00 = d[4] * d[8] - d[5] * d[7], c01 = d[5] * d[6] - d[3] * d[8],
+               c02 = d[3] * d[7] - d[4] * d[6];
+  const double det = d[0] * c00 + d[1] * c01 + d[2] * c02;
+  const bool spd = d[0] > 0.0 && d[0] * d[4] - d[1] * d[3] > 0.0 && det > 0.0;
+  if (rq && spd) {
+    const double id = 1.0 / det;
+    m[0] = c00 * id;
+    m[1] = (d[2] * d[7] - d[1] * d[8]) * id;
+    m[2] = (d[1] * d[5] - d[2] * d[4]) * id;
+    m[3] = c01 * id;
+    m[4] = (d[0] * d[8] - d[2] * d[6]) * id;
+    m[5] = (d[2] * d[3] - d[0] * d[5]) * id;
+    m[6] = c02 * id;
+    m[7] = (d[1] * d[6] - d[0] * d[7]) * id;
+    m[8] = (d[0] * d[4] - d[1] * d[3]) * id;
+  } else {
+    const double sc = rq && tr > 0.0 ? 3.0 / tr : 1.0;
+#pragma unroll
+    for (int t = 0; t < 9; t++) m[t] = (t % 4 == 0) ? sc : 0.0;
+  }
+  const double sg = qs[i].w;
+  m[9] = rs ? 1.0 / (3.0 / (sg * sg) + fmax(tr, 0.0)) : 1.0;
+}
+
+hipError_t launch_pc_invert(hipStream_t st, int base, int n, int cap, int dof, const int* spc,
+                            const double* ne, const double4* qs, const double* D, double* minv) {
+  if (cap <= 0) return hipSuccess;
+  k_pc_invert<<<(cap + 255) / 256, 256, 0, st>>>(base, n, cap, dof, spc, ne, qs, D, minv);
+  return hipGetLastError();
 }
 
 hipError_t launch_solver_step(hipStream_t st, int n, const int* inst, const int* spc,

@@ -87,7 +87,7 @@ struct DevBuf {
   }
 };
 
-enum SolveStage { ST_IDLE = 0, ST_START, ST_BASE32, ST_CG, ST_LSPREP, ST_LS, ST_DIFF, ST_DONE };
+enum SolveStage { ST_IDLE = 0, ST_START, ST_BASE32, ST_PC, ST_CG, ST_LSPREP, ST_LS, ST_DIFF, ST_DONE };
 
 struct Solve {
   int stage = ST_IDLE;
@@ -100,6 +100,8 @@ struct Solve {
   double mass0 = 0, mass = 0;  // hydrogen content at x_n and at the iterate
   int newton = 0, cg = 0, ls = 0, evals = 0;
   double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
+  double rz = 0;          // r . M^-1 r of the preconditioned CG
+  bool pc_setup = false;  // build the preconditioner at X before the next tick
   int ndof = 0;
   dmd::SolveOp pending{};  // updates to apply at the start of the next tick
 };
@@ -176,7 +178,8 @@ struct dmdgpu_ctx {
   DevBuf<int> d_spc, d_nnb, d_nbr, d_chunk_np, d_own;
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
-  DevBuf<double> d_diag;  // n x 9 diagonal Hessian blocks of the primary instance
+  DevBuf<double> d_diag;  // diagonal Hessian blocks (n x 9)
+  DevBuf<double> d_minv;  // arena x 10: block-Jacobi preconditioner of the solver
   DevBuf<long long> d_pairs;
   std::vector<int> h_prec;
   std::vector<int> h_noxr;  // per instance, solver ticks: 1 = no occupancy residual needed
@@ -388,8 +391,8 @@ dmd::Outputs make_outputs(dmdgpu_ctx* c) {
 }
 
 dmd::SolveVecs make_vecs(dmdgpu_ctx* c) {
-  return dmd::SolveVecs{c->d_X.p, c->d_R0.p, c->d_R0f.p, c->d_D.p,
-                        c->d_Rr.p, c->d_Pp.p, c->d_Xn.p, c->d_Xt.p};
+  return dmd::SolveVecs{c->d_X.p,  c->d_R0.p, c->d_R0f.p, c->d_D.p,   c->d_Rr.p,
+                        c->d_Pp.p, c->d_Xn.p, c->d_Xt.p,  c->d_minv.p};
 }
 
 void drop_graph(dmdgpu_ctx* c) {
@@ -431,7 +434,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_S0.ensure(cap, keep));
   HIPCHK(c->d_partials.ensure(cap / kChunk * 2));
   HIPCHK(c->d_pairs.ensure(cap / kChunk));
-  HIPCHK(c->d_spart.ensure(cap / kChunk * 5));
+  HIPCHK(c->d_spart.ensure(cap / kChunk * dmd::kSolveDots));
   HIPCHK(c->d_X.ensure(cap, keep));
   HIPCHK(c->d_R0.ensure(cap, keep));
   HIPCHK(c->d_R0f.ensure(cap, keep));
@@ -440,6 +443,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_Pp.ensure(cap, keep));
   HIPCHK(c->d_Xn.ensure(cap, keep));
   HIPCHK(c->d_Xt.ensure(cap, keep));
+  HIPCHK(c->d_minv.ensure(cap * 10, keep * 10));
   HIPCHK(c->d_dnn.ensure(cap, keep));
   HIPCHK(c->d_dnbr.ensure(cap * dmd::kDCap, keep * dmd::kDCap));
   HIPCHK(c->d_act.ensure(2 * (cap / kChunk + 1)));
@@ -460,7 +464,7 @@ dmdgpu_status upload_meta(dmdgpu_ctx* c) {
   HIPCHK(c->d_chunk_hi.ensure(cap));
   HIPCHK(c->d_totals.ensure(cap * 2));
   HIPCHK(c->d_ops.ensure(cap));
-  HIPCHK(c->d_sdots.ensure(cap * 5));
+  HIPCHK(c->d_sdots.ensure(cap * dmd::kSolveDots));
   std::vector<dmd::Box> bx(cap);
   std::vector<int> lo(cap, 0), hi(cap, 0);
   if (c->h_prec.size() < cap) c->h_prec.resize(cap, -1);
@@ -925,6 +929,7 @@ void plan_tick(dmdgpu_ctx* c) {
     op.dot = dmd::kDotNone;
     op.mixed = I.sv.o.mixed;
     op.dof = I.sv.o.dof_mask;
+    op.pc = I.sv.o.precond;
     c->h_prec[k] = -1;
     if (I.used) {
       Solve& S = I.sv;
@@ -937,6 +942,9 @@ void plan_tick(dmdgpu_ctx* c) {
         case ST_BASE32:
           op.make = dmd::kMakeX;
           c->h_prec[k] = dmd::kSplit32;
+          break;
+        case ST_PC:
+          op.dot = dmd::kDotPc;  // CgInit with P = M^-1 Rr; no evaluation
           break;
         case ST_CG:
           S.eps = (S.o.probe > 0 ? S.o.probe : (S.o.mixed ? kProbe32 : kProbe64)) *
@@ -975,6 +983,21 @@ void finish(Solve& S, int result) {
   S.result = result;
 }
 
+// Start conjugate gradients at X (|R|^2 = rr): D = 0, Rr = -R0, P = Rr.  With
+// the preconditioner the host first builds M at X (between ticks), and the
+// next tick forms P = M^-1 Rr and its dots (ST_PC) before the first probe.
+void begin_cg(Solve& S, dmd::SolveOp& nx, double rr) {
+  nx.upd |= dmd::kUpdCgInit;
+  S.rr = S.pp = rr;
+  S.cg = 0;
+  if (S.o.precond) {
+    S.stage = ST_PC;
+    S.pc_setup = true;
+  } else {
+    S.stage = ST_CG;
+  }
+}
+
 // Consumes the tick's dots: advances each instance's state machine and queues
 // the vector updates for the start of the next tick.
 void digest_tick(dmdgpu_ctx* c) {
@@ -983,7 +1006,7 @@ void digest_tick(dmdgpu_ctx* c) {
     Instance& I = c->inst[k];
     if (!I.used) continue;
     Solve& S = I.sv;
-    const double* d = &c->h_sdots[5 * k];
+    const double* d = &c->h_sdots[(size_t)dmd::kSolveDots * k];
     dmd::SolveOp& nx = S.pending;
     switch (S.stage) {
       case ST_START: {
@@ -998,10 +1021,7 @@ void digest_tick(dmdgpu_ctx* c) {
             S.have_r0f = true;  // FP64 residual at X next, then CG
             break;
           }
-          nx.upd |= dmd::kUpdCgInit;
-          S.rr = S.pp = S.rnorm * S.rnorm;
-          S.cg = 0;
-          S.stage = ST_CG;
+          begin_cg(S, nx, S.rnorm * S.rnorm);
           break;
         }
         if (S.rnorm <= S.o.atol) {
@@ -1010,26 +1030,29 @@ void digest_tick(dmdgpu_ctx* c) {
         }
         if (S.o.mixed && S.have_r0f) {
           S.have_r0f = false;
-          nx.upd |= dmd::kUpdCgInit;
-          S.rr = S.pp = S.rnorm * S.rnorm;
-          S.cg = 0;
-          S.stage = ST_CG;
+          begin_cg(S, nx, S.rnorm * S.rnorm);
           break;
         }
         if (S.o.mixed) {
           S.stage = ST_BASE32;
           break;
         }
-        nx.upd |= dmd::kUpdCgInit;
-        S.rr = S.pp = S.rnorm * S.rnorm;
-        S.cg = 0;
-        S.stage = ST_CG;
+        begin_cg(S, nx, S.rnorm * S.rnorm);
         break;
       }
       case ST_BASE32: {
-        nx.upd |= dmd::kUpdR0fFromRv | dmd::kUpdCgInit;
-        S.rr = S.pp = S.rnorm * S.rnorm;
-        S.cg = 0;
+        nx.upd |= dmd::kUpdR0fFromRv;
+        begin_cg(S, nx, S.rnorm * S.rnorm);
+        break;
+      }
+      case ST_PC: {
+        // d[7] = Rr . M^-1 Rr, d[13] = |P|^2 with P = M^-1 Rr
+        S.rz = d[7];
+        S.pp = d[13];
+        if (!(S.rz > 0.0) || !std::isfinite(S.rz)) {
+          finish(S, DMDGPU_RELAX_STALLED);
+          break;
+        }
         S.stage = ST_CG;
         break;
       }
@@ -1044,18 +1067,34 @@ void digest_tick(dmdgpu_ctx* c) {
           S.stage = ST_LSPREP;
           break;
         }
-        const double a = S.rr / pAp;
-        double rr1 = S.rr - 2.0 * a * rAp + a * a * ApAp;
-        rr1 = std::max(rr1, 0.0);
-        const double b = rr1 / S.rr;
+        double a, b, rr1;
+        bool stop = false;
+        if (S.o.precond) {
+          // r1 = r - a Ap, z1 = z - a y (y = M^-1 Ap), P1 = z1 + b P
+          const double zAp = d[5], ApMAp = d[6], zz = d[8], zy = d[9], yy = d[10], zP = d[11],
+                       yP = d[12], PP = d[13];
+          a = S.rz / pAp;
+          rr1 = std::max(S.rr - 2.0 * a * rAp + a * a * ApAp, 0.0);
+          const double rz1 = std::max(S.rz - 2.0 * a * zAp + a * a * ApMAp, 0.0);
+          b = rz1 / S.rz;
+          const double z1z1 = std::max(zz - 2.0 * a * zy + a * a * yy, 0.0);
+          const double z1P = zP - a * yP;
+          S.pp = std::max(z1z1 + 2.0 * b * z1P + b * b * PP, 0.0);
+          S.rz = rz1;
+          stop = !(rz1 > 0.0);
+        } else {
+          a = S.rr / pAp;
+          rr1 = std::max(S.rr - 2.0 * a * rAp + a * a * ApAp, 0.0);
+          b = rr1 / S.rr;
+          S.pp = rr1 + b * b * S.pp;
+        }
         nx.upd |= dmd::kUpdCgStep;
         nx.eps_prev = S.eps;
         nx.alpha = a;
         nx.beta = b;
-        S.pp = rr1 + b * b * S.pp;
         S.rr = rr1;
         S.cg++;
-        if (std::sqrt(rr1) <= S.o.eta * S.rnorm || S.cg >= S.o.max_cg || !(rr1 > 0.0))
+        if (std::sqrt(rr1) <= S.o.eta * S.rnorm || S.cg >= S.o.max_cg || !(rr1 > 0.0) || stop)
           S.stage = ST_LSPREP;
         break;
       }
@@ -1087,10 +1126,7 @@ void digest_tick(dmdgpu_ctx* c) {
               S.stage = ST_START;  // FP64 residual at the accepted iterate
               break;
             }
-            nx.upd |= dmd::kUpdCgInit;
-            S.rr = S.pp = rn * rn;
-            S.cg = 0;
-            S.stage = ST_CG;
+            begin_cg(S, nx, rn * rn);
             break;
           }
           if (rn <= S.o.atol) {
@@ -1105,10 +1141,7 @@ void digest_tick(dmdgpu_ctx* c) {
             S.stage = ST_BASE32;
             break;
           }
-          nx.upd |= dmd::kUpdCgInit;
-          S.rr = S.pp = rn * rn;
-          S.cg = 0;
-          S.stage = ST_CG;
+          begin_cg(S, nx, rn * rn);
         } else {
           S.alpha_ls *= 0.5;
           if (++S.ls > 12) finish(S, DMDGPU_RELAX_STALLED);
@@ -1194,6 +1227,33 @@ dmdgpu_status end_diffusion(dmdgpu_ctx* c, Instance& I) {
   return refresh_activity(c, I);
 }
 
+// Block-Jacobi preconditioners of the instances about to start CG, outside
+// the captured tick: FP64 phase 1 at their evaluation point (their iterate X:
+// the last tick evaluated them there), the exact diagonal Hessian blocks, and
+// their inverses into d_minv.
+dmdgpu_status pc_setup(dmdgpu_ctx* c) {
+  const int ni = (int)c->inst.size();
+  bool any = false;
+  for (int k = 0; k < ni; k++) any |= c->inst[k].used && c->inst[k].sv.pc_setup;
+  if (!any) return DMDGPU_OK;
+  std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+  for (int k = 0; k < ni; k++)
+    if (c->inst[k].used && c->inst[k].sv.pc_setup) c->h_prec[k] = dmd::kFP64;
+  CHK(run_eval(c, false));
+  HIPCHK(c->d_diag.ensure((size_t)c->extent * 9));
+  const dmd::State s = make_state(c);
+  for (int k = 0; k < ni; k++) {
+    Instance& I = c->inst[k];
+    if (!I.used || !I.sv.pc_setup) continue;
+    double* D = c->d_diag.p + 9 * (size_t)I.base;
+    HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, 1, s, c->T, c->d_rec.p, I.base, I.n, D));
+    HIPCHK(dmd::launch_pc_invert(c->stream, I.base, I.n, I.cap, I.sv.o.dof_mask, c->d_spc.p,
+                                 c->d_ne.p, c->d_qs.p, D, c->d_minv.p));
+    I.sv.pc_setup = false;
+  }
+  return DMDGPU_OK;
+}
+
 dmdgpu_status run_tick(dmdgpu_ctx* c) {
   CHK(upload_meta(c));
   plan_tick(c);
@@ -1210,7 +1270,7 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   const size_t ob = sizeof(dmd::SolveOp) * ni, pb = sizeof(int) * ni;
   const size_t ab = sizeof(int) * c->h_chunks.size();
   HIPCHK(c->pin_up.ensure(ob + 2 * pb + ab));
-  HIPCHK(c->pin_down.ensure(sizeof(double) * 5 * ni));
+  HIPCHK(c->pin_down.ensure(sizeof(double) * dmd::kSolveDots * ni));
   std::memcpy(c->pin_up.p, c->h_ops.data(), ob);
   std::memcpy((char*)c->pin_up.p + ob, c->h_prec.data(), pb);
   std::memcpy((char*)c->pin_up.p + ob + pb, c->h_noxr.data(), pb);
@@ -1243,11 +1303,12 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   } else {
     CHK(tick_device(c));
   }
-  c->h_sdots.resize((size_t)5 * ni);
-  HIPCHK(hipMemcpyAsync(c->pin_down.p, c->d_sdots.p, sizeof(double) * 5 * ni,
-                        hipMemcpyDeviceToHost, c->stream));
+  const size_t nd = (size_t)dmd::kSolveDots * ni;
+  c->h_sdots.resize(nd);
+  HIPCHK(hipMemcpyAsync(c->pin_down.p, c->d_sdots.p, sizeof(double) * nd, hipMemcpyDeviceToHost,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  std::memcpy(c->h_sdots.data(), c->pin_down.p, sizeof(double) * 5 * ni);
+  std::memcpy(c->h_sdots.data(), c->pin_down.p, sizeof(double) * nd);
   std::vector<int> before(ni);
   for (int k = 0; k < ni; k++) before[k] = c->inst[k].sv.stage;
   digest_tick(c);
@@ -1277,7 +1338,7 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
                                    c->d_ops.p, make_vecs(c), c->d_qs.p, make_outputs(c)));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  return DMDGPU_OK;
+  return pc_setup(c);
 }
 
 int running_count(const dmdgpu_ctx* c) {
@@ -1337,7 +1398,7 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
                            &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
                            &c->d_slot,
                            &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart,
-                           &c->d_Xn,  &c->d_Xt,    &c->d_diag};
+                           &c->d_Xn,  &c->d_Xt,    &c->d_diag, &c->d_minv};
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
                         &c->d_chunk_np,   &c->d_own,
@@ -1662,7 +1723,8 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   Solve S;
   S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0};
   if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
-      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0))
+      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0 ||
+      S.o.precond > 1)
     return fail(DMDGPU_ERR_ARG, "bad relaxation options");
   if (S.o.max_evals <= 0) S.o.max_evals = 100000;
   if (S.o.dof_mask == 0) S.o.dof_mask = DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA;
