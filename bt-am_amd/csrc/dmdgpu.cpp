@@ -991,8 +991,9 @@ void begin_cg(Solve& S, dmd::SolveOp& nx, double rr) {
   S.rr = S.pp = rr;
   S.cg = 0;
   if (S.o.precond) {
+    // rebuilt every `precond` Newton steps (lagged in between: still SPD)
     S.stage = ST_PC;
-    S.pc_setup = true;
+    S.pc_setup = S.newton % S.o.precond == 0;
   } else {
     S.stage = ST_CG;
   }
@@ -1723,8 +1724,7 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   Solve S;
   S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0};
   if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
-      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0 ||
-      S.o.precond > 1)
+      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0)
     return fail(DMDGPU_ERR_ARG, "bad relaxation options");
   if (S.o.max_evals <= 0) S.o.max_evals = 100000;
   if (S.o.dof_mask == 0) S.o.dof_mask = DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA;

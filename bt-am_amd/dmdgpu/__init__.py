@@ -58,7 +58,7 @@ DOF_MG_Q, DOF_MG_SIGMA, DOF_H_Q, DOF_H_SIGMA = 1, 2, 4, 8
 def relax_opts(atol=1e-8, max_newton=50, max_cg=60, eta=0.05, mixed=False, max_evals=0,
                dof_mask=DOF_MG_Q | DOF_MG_SIGMA, probe=0.0, precond=False):
     return RelaxOpts(atol, max_newton, max_cg, eta, 1 if mixed else 0, max_evals, dof_mask,
-                     probe, 1 if precond else 0)
+                     probe, int(precond))
 
 
 class DiffuseOpts(C.Structure):

@@ -239,10 +239,11 @@ typedef struct {
   int dof_mask;    /* relaxed unknowns, DMDGPU_DOF_* bits; 0 = Mg q + sigma */
   double probe;    /* RMS displacement of a Hessian probe (A); 0 = default
                       (1e-7 with FP64 probes, 5e-5 with split-precision ones) */
-  int precond;     /* 1: block-Jacobi preconditioned CG: at the start of every
-                      Newton step the exact diagonal blocks of
-                      dmdgpu_diag_blocks are inverted per site (3x3 on q;
-                      1/(3/sigma^2 + tr D_u) on sigma); 0: plain CG */
+  int precond;     /* k > 0: block-Jacobi preconditioned CG: the exact
+                      diagonal blocks of dmdgpu_diag_blocks are inverted per
+                      site (3x3 on q; 1/(3/sigma^2 + tr D_u) on sigma) at
+                      Newton steps 0, k, 2k, ... and kept in between;
+                      0: plain CG */
 } dmdgpu_relax_opts;
 /* Relaxed unknowns.  Interstitial H at fixed occupancy has a soft (and at low
  * occupancy width-unbounded: sigma_eq^2 ~ 1/(beta k x)) landscape, so the
