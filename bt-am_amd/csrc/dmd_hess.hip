@@ -247,8 +247,18 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
       applyM(wt * cang, ps, L);
       hess_psi(L, p, ir, uv, 1.0, H);
       jac_psi(p, ir, uv, J);
-      jtmj_self(J, 2.0 * wt * nk * nk, HJ);  // 2 wt n_k^2 J^T M J
-      jtmj_self(J, wt * cjj, H);              // codes 5 + 7 of (k, u, u); cjj = 0 unless exact
+      {
+        // J^T M J once: 2 wt n_k^2 of it to the centre's terms, wt cjj to the
+        // mixed codes 5 + 7 of (k, u, u) (cjj = 0 unless exact)
+        double X[6] = {0, 0, 0, 0, 0, 0};
+        jtmj_self(J, 1.0, X);
+        const double c1 = 2.0 * wt * nk * nk, c2 = wt * cjj;
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+          HJ[t] += c1 * X[t];
+          H[t] += c2 * X[t];
+        }
+      }
       const double* f = T.pr + ((size_t)t * T.nr + m) * 16;
       const double* fk = f + 4 + 4 * spk;
       const double* fu = f + 4 + 4 * spu;

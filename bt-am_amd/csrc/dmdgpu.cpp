@@ -46,10 +46,11 @@ dmdgpu_status fail(dmdgpu_status st, const std::string& msg) {
 constexpr double kMinOcc = 5e-4;            // src/Macros.hpp:871
 constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
 constexpr int kChunk = dmd::kChunk;
-// Device bytes per arena site (state, lists, records, outputs, solver vectors).
+// Device bytes per arena site (state, lists, records, outputs, solver vectors,
+// preconditioner: 10 doubles of M^-1 and 9 of the diagonal blocks).
 constexpr size_t kBytesPerSite = 32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 +
                                  6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32 + 1 +
-                                 4 * dmd::kCap + 8 * dmd::kSlotW * dmd::kCap;
+                                 4 * dmd::kCap + 8 * dmd::kSlotW * dmd::kCap + 8 * (10 + 9);
 
 struct Spline {
   int n = 0;
