@@ -186,3 +186,34 @@ def test_relaxation_batch_equals_alone(ctx, precond):
         assert st["state"] == "converged"
     for iid in ids:
         ctx.remove_instance(iid)
+
+
+def test_checkpoint_resume(ctx, tmp_path):
+    """A relaxation stopped after 40 evaluations, written with
+    dmdgpu.checkpoint and resumed from the files, reaches the state of the
+    uninterrupted relaxation."""
+    import dmdgpu.checkpoint as ck
+    say("[gpu] checkpoint resume")
+    sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=7)
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=0.05, precond=40)
+    ctx.set_state(sd)
+    ctx.neighbors()
+    ctx.relax(opts)
+    assert ctx.relax_status(0)["state"] == "converged"
+    q_ref = ctx.instance_state(0)[0]
+    ctx.set_state(sd)
+    ctx.neighbors()
+    part = dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=0.05, precond=40,
+                             max_evals=40)
+    ctx.relax(part)
+    assert ctx.relax_status(0)["state"] == "maxiter"
+    q, sg, x = ctx.instance_state(0)
+    mid = dict(sd)
+    mid["q"], mid["sigma"], mid["x"] = q, sg, x
+    ck.write_checkpoint(str(tmp_path), mid, step=1)
+    back, _ = ck.read_checkpoint(str(tmp_path))
+    ctx.set_state(back)
+    ctx.neighbors()
+    ctx.relax(opts)
+    assert ctx.relax_status(0)["state"] == "converged"
+    assert np.abs(ctx.instance_state(0)[0] - q_ref).max() < 1e-7
