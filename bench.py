@@ -376,11 +376,22 @@ def main():
                     help="requests in the serving leg")
     ap.add_argument("--instances", type=int, default=64,
                     help="relaxations in the solves/hr leg (half nanowires, half spheres)")
+    ap.add_argument("--only-solves", action="store_true",
+                    help="run only the solves/hr leg (with --tuning k1,k2,u/k1,k2,u) and print it")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="solves leg: launch the tick kernels directly instead of as a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+
+    if args.only_solves:
+        parts = args.tuning.split("/") if args.tuning != "auto" else ["8,16,0", "8,16,1"]
+        tun = tuple(tuple(int(v) for v in p.split(",")) for p in (parts[0], parts[-1]))
+        print(json.dumps({"solves": solves_leg(tun, args.instances, graph=not args.no_graph)}),
+              flush=True)
+        return
 
     import dmdgpu
     sysd = workload()
