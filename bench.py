@@ -258,8 +258,8 @@ def solves_leg(tuning, count, graph=True):
     probes) against the batched path (all instances resident, one launch per
     tick, split-precision probes, FP64 residuals and stopping test).  Both
     relax Mg positions and widths to |R|_2 <= 1e-8, each with the inner
-    tolerance that minimises its own residual evaluations (CG eta 0.05 for
-    FP64 probes, 0.1 for split-precision probes).  Run with plain CG and with
+    tolerance that minimises its own time (CG eta 0.05 for FP64 probes, 0.1
+    for split-precision probes with plain CG, 0.05 with the preconditioner).  Run with plain CG and with
     the block-Jacobi preconditioner from the diagonal Hessian blocks (built
     once per solve), the same solver options on both sides of each
     comparison."""
@@ -278,8 +278,10 @@ def solves_leg(tuning, count, graph=True):
     for pc in (0, 60):
         plain_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05,
                                        mixed=False, precond=pc)
-        mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.1, mixed=True,
-                                       precond=pc)
+        # split probes: eta 0.1 with plain CG, 0.05 with the preconditioner (sweeps of
+        # eta in 0.01..0.3 on this workload)
+        mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100,
+                                       eta=0.05 if pc else 0.1, mixed=True, precond=pc)
         t_plain, plain, per_inst = _relax_plain(ctx, insts, plain_opts)
         t_batch, ticks, batch = _relax_batched(ctx, insts, mixed_opts)
         dq = max(float(np.abs(b[1] - p[1]).max()) for b, p in zip(batch, plain))

@@ -53,12 +53,13 @@ class Result:
 
 
 def default_opts(kind, mixed):
-    """Relaxations: Newton-CG to |R|_2 <= 1e-8 with the block-Jacobi
-    preconditioner built once per solve (rebuild interval 60 = max_newton);
+    """Relaxations: Newton-CG to |R|_2 <= 1e-8 (forcing eta 0.05) with the
+    block-Jacobi preconditioner built once per solve (rebuild interval 60 =
+    max_newton);
     diffusion steps: Picard sweeps to max |dx| <= 1e-10."""
     if kind == "relax":
-        return relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.1 if mixed else 0.05,
-                          mixed=mixed, precond=60)
+        return relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, mixed=mixed,
+                          precond=60)
     return diffuse_opts(dt=1e-9, D0=1e-7, Eb=0.4, tol=1e-10, max_iter=60, mixed=mixed)
 
 
