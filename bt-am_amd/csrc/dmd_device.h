@@ -9,6 +9,7 @@ namespace dmd {
 #define DEV __device__ __forceinline__
 
 constexpr double kSqrt6 = 2.449489742783178;  // src/Macros.hpp:42
+constexpr double kSqrt2 = 1.4142135623730951;  // src/Macros.hpp:40
 constexpr double kUma = 103.4993333E-6;        // src/Macros.hpp:851
 constexpr double kHbar = 6.5821192815E-4;      // src/Macros.hpp:847
 
@@ -191,6 +192,58 @@ template <> struct PairRule<5> {
     const double w1 = a1 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
     const double w2 = a2 == 0 ? 2.0 / 3.0 : 1.0 / 6.0;
     return w0 * w1 * w2;
+  }
+};
+
+// PairRule<44>: the reference's 6D degree-5 rule for a pair
+// (src/Numerical/Quadrature-Hermitian-5th.cpp:31-157; defined there, not used
+// by its evaluators): 22 generators zeta in R^6 and their negatives, site
+// displacement sqrt(2) sigma_site zeta_site -- zeta = (1,...,1) with weight
+// 1/128, sqrt(2) e_k with 1/16 (k = 0..5), and (1,...,1) with -1 at two
+// positions a < b (lexicographic) with 1/128.  The relative point is
+// p = r + sqrt(2) (sigma_i zeta_i - sigma_j zeta_j).  The set maps to itself
+// when the two sites' halves are swapped, so one evaluation serves both ends.
+DEV double rule44_gen(int t, double* z) {
+  const int g = t % 22;
+  const double sg = t < 22 ? 1.0 : -1.0;
+  double w;
+  if (g == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) z[k] = sg;
+    w = 1.0 / 128.0;
+  } else if (g <= 6) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) z[k] = (k == g - 1) ? sg * kSqrt2 : 0.0;
+    w = 1.0 / 16.0;
+  } else {
+    int idx = g - 7, a = 0;
+    while (idx >= 5 - a) {
+      idx -= 5 - a;
+      a++;
+    }
+    const int b = a + 1 + idx;
+#pragma unroll
+    for (int k = 0; k < 6; k++) z[k] = (k == a || k == b) ? -sg : sg;
+    w = 1.0 / 128.0;
+  }
+  return w;
+}
+template <> struct PairRule<44> {
+  static constexpr int NP = 44;
+  template <typename P>
+  DEV static double point(const PairGeo<P>& geo, const Tables& T, int t, P si, P sj, P* p, int& m,
+                          P& off, P& ir, P* dsi, P* dsj) {
+    double z[6];
+    const double w = rule44_gen(t, z);
+    P d[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      dsi[k] = (P)(kSqrt2 * z[k]);
+      dsj[k] = (P)(-kSqrt2 * z[3 + k]);
+      d[k] = dsi[k] * si + dsj[k] * sj;
+    }
+    geo.point3(T, d, p, m, off, ir);
+    return w;
   }
 };
 

@@ -386,6 +386,8 @@ hipError_t launch_diag(hipStream_t st, int rule, int exact, const State& s, cons
   if (n <= 0) return hipSuccess;
   if (rule == 5)
     hipLaunchKernelGGL(k_diag<5>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
+  else if (rule == 44)
+    hipLaunchKernelGGL(k_diag<44>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
   else
     hipLaunchKernelGGL(k_diag<3>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
   return hipGetLastError();

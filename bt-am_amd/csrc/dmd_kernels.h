@@ -137,7 +137,7 @@ struct PhaseConfig {
   // per Precision (index kFP64 / kSplit32):
   int k1[2], k2[2];  // lanes per site in phase 1 / phase 2 (1, 2, 4, 8, 16)
   int unroll2[2];    // 1: fully unrolled point loops in phase 2, 0: rolled
-  int rule = 3;      // pair / coincident rule: 3 (reference, 12 points) or 5
+  int rule = 3;      // pair / coincident rule: 3 (reference, 12 points), 5 or 44
                      // (27-point tensor Gauss-Hermite on the relative coordinate)
   bool operator==(const PhaseConfig& o) const {
     return k1[0] == o.k1[0] && k1[1] == o.k1[1] && k2[0] == o.k2[0] && k2[1] == o.k2[1] &&

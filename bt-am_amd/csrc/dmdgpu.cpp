@@ -1498,7 +1498,8 @@ dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* c, int precision) {
 }
 
 dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* c, int order) {
-  if (!c || (order != 3 && order != 5)) return fail(DMDGPU_ERR_ARG, "rule order must be 3 or 5");
+  if (!c || (order != 3 && order != 5 && order != 44))
+    return fail(DMDGPU_ERR_ARG, "rule must be 3, 5 or 44");
   c->cfg.rule = order;
   return DMDGPU_OK;
 }

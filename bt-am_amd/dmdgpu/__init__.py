@@ -215,8 +215,9 @@ class Context:
         self._chk(self.L.dmdgpu_set_tuning(self.h, k1, k2, unroll2))
 
     def set_rule(self, order):
-        """Pair/coincident quadrature: 3 (the reference's rule) or 5 (27-point
-        tensor Gauss-Hermite on the relative coordinate)."""
+        """Pair/coincident quadrature: 3 (the reference's rule), 5 (27-point
+        tensor Gauss-Hermite on the relative coordinate) or 44 (the reference's
+        44-point 6D degree-5 pair rule)."""
         self._chk(self.L.dmdgpu_set_rule(self.h, int(order)))
 
     def set_tuning_precision(self, precision, k1, k2, unroll2):

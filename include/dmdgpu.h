@@ -176,9 +176,10 @@ dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* ctx, int precision);
  * rule (src/Numerical/Quadrature-Hermitian-3th.cpp:36-110).  5: 27-point tensor
  * Gauss-Hermite rule on the relative coordinate r ~ N(r_ij, (s_i^2+s_j^2) I)
  * (nodes 0, +-sqrt(3) s_rel; weights 2/3, 1/6 per axis; exact to degree 5).
- * Distinct angular triplets keep the 3rd-order 9D rule under both: the
- * reference has no higher-order 9D rule (its 5th-order rule,
- * Quadrature-Hermitian-5th.cpp:31-157, is a pair rule). */
+ * 44: the reference's own 44-point 6D degree-5 pair rule
+ * (src/Numerical/Quadrature-Hermitian-5th.cpp:31-157, which its evaluators do
+ * not call).  Distinct angular triplets keep the 3rd-order 9D rule under all
+ * three: the reference has no higher-order 9D rule. */
 dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* ctx, int order);
 
 /* Resident instances -------------------------------------------------------

@@ -1099,12 +1099,48 @@ static inline int act(const or_system* s, int j) { return s->x[j] >= OR_MIN_OCCU
  *           one axis at a time, weight 1/12 each (12 points);
  *   rule 5: 3-point Gauss-Hermite per axis on the relative coordinate
  *           r ~ N(r_ij, (sigma_i^2 + sigma_j^2) I): nodes 0, +-sqrt(3)
- *           sigma_rel with weights 2/3, 1/6 (27 points, exact to degree 5). */
+ *           sigma_rel with weights 2/3, 1/6 (27 points, exact to degree 5);
+ *   rule 44: the reference's 44-point 6D degree-5 rule (below). */
 #define OR_SQRT3 1.7320508075688772
-static inline int pair_rule_points(int rule) { return rule == 5 ? 27 : 12; }
+/* rule 44: moment_1th_Vij_gaussian_measure_5th (Quadrature-Hermitian-5th.cpp:
+ * 31-157), the 6D degree-5 rule with 22 generators in zeta units and their
+ * negatives: (1,...,1) weight 1/128; sqrt(2) e_k weight 1/16; (1,...,1) with
+ * -1 at positions a < b (lexicographic) weight 1/128; site displacement
+ * sqrt(2) sigma zeta.  The reference scales the weights by pi^(3/2) and the
+ * sum by pi^(-3/2) (:50, :156); here they cancel exactly. */
+static inline double rule44_gen(int t, double* z) {
+  const int g = t % 22;
+  const double sg = t < 22 ? 1.0 : -1.0;
+  if (g == 0) {
+    for (int k = 0; k < 6; k++) z[k] = sg;
+    return 1.0 / 128.0;
+  }
+  if (g <= 6) {
+    for (int k = 0; k < 6; k++) z[k] = (k == g - 1) ? sg * OR_SQRT2 : 0.0;
+    return 1.0 / 16.0;
+  }
+  int idx = g - 7, a = 0;
+  while (idx >= 5 - a) {
+    idx -= 5 - a;
+    a++;
+  }
+  const int b = a + 1 + idx;
+  for (int k = 0; k < 6; k++) z[k] = (k == a || k == b) ? -sg : sg;
+  return 1.0 / 128.0;
+}
+static inline int pair_rule_points(int rule) { return rule == 5 ? 27 : rule == 44 ? 44 : 12; }
 static inline double pair_rule_point(int rule, int t, const double* r, double si, double sj,
                                      double* p, double* dsi) {
   p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
+  if (rule == 44) {
+    double z[6];
+    const double w = rule44_gen(t, z);
+    for (int k = 0; k < 3; k++) {
+      p[k] += OR_SQRT2 * (si * z[k] - sj * z[3 + k]);
+      dsi[k] = OR_SQRT2 * z[k];
+    }
+    return w;
+  }
   if (rule == 5) {
     static const double node[3] = {0.0, OR_SQRT3, -OR_SQRT3};
     static const double wn[3] = {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0};

@@ -65,7 +65,8 @@ typedef struct {
   int rule;             /* pair/coincident rule of the factorised form:
                            3 (default, 0 too) = the reference's 12-point 6D
                            Stroud rule; 5 = 27-point tensor Gauss-Hermite on
-                           the relative coordinate (degree 5)             */
+                           the relative coordinate (degree 5); 44 = the
+                           reference's 44-point 6D degree-5 pair rule      */
 } or_system;
 
 /* Tables ------------------------------------------------------------------ */

@@ -215,20 +215,21 @@ def test_error_codes(gpu):
     assert e.value.code == 1
 
 
-# Fifth-order pair rule (27-point tensor Gauss-Hermite on the relative
-# coordinate) for the pair and coincident terms: FP64 GPU against the oracle's
-# restatement of the same rule at the FP64 bar, and the split evaluation
-# against FP64 at the split bars.
+# Fifth-order pair rules for the pair and coincident terms -- 5: 27-point
+# tensor Gauss-Hermite on the relative coordinate; 44: the reference's 44-point
+# 6D rule -- FP64 GPU against the oracle's restatement of the same rule at the
+# FP64 bar, and the split evaluation against FP64 at the split bars.
+@pytest.mark.parametrize("rule", [5, 44])
 @pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "cluster_mgh"])
-def test_fifth_order_rule(gpu, oracle, case):
-    say(f"[gpu] rule 5 {case}")
+def test_fifth_order_rule(gpu, oracle, case, rule):
+    say(f"[gpu] rule {rule} {case}")
     sysd = CASES[case]()
     st, nnb, nbr = oracle.neighbors(sysd, RC)
-    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr, 5))
+    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr, rule))
     gpu.set_state(sysd)
     gpu.set_tuning(8, 8, 0)
     gpu.neighbors()
-    gpu.set_rule(5)
+    gpu.set_rule(rule)
     try:
         gpu.set_precision("fp64")
         out = gpu.residual()
@@ -241,7 +242,7 @@ def test_fifth_order_rule(gpu, oracle, case):
         err = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
         err["mf_rho"] = rel(gpu.mf_rho(), ref["mf_rho"])
         err["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"])
-        say(f"    rule-5 split32 errors {err}")
+        say(f"    rule-{rule} split32 errors {err}")
         for k, tol in SPLIT_TOL.items():
             assert err[k] < tol, (k, err[k])
     finally:

@@ -195,8 +195,9 @@ def test_fifth_order_pair_rule_closed_form(tmp_path):
     zero): V0_i = n_i^2 n_j^2 c^2/3 E|r|^4 over r ~ N(r_ij, s^2 I),
     s^2 = sigma_i^2 + sigma_j^2, whose Gaussian moment is
     |r_ij|^4 + 10 |r_ij|^2 s^2 + 15 s^4.  The 27-point tensor rule (degree 5)
-    is exact; the reference's 12-point rule (degree 3) gives 18 (sigma_i^4 +
-    sigma_j^4) in place of 15 s^4, which the test also pins."""
+    is exact, and so is the reference's 44-point 6D degree-5 rule (rule 44);
+    the reference's 12-point rule (degree 3) gives 18 (sigma_i^4 + sigma_j^4)
+    in place of 15 s^4, which the test also pins."""
     c = 0.7
     O = _poly_tables(tmp_path, rho=(0,), F=(0,), phi=(0,), u=(0,), w=(c,))
     sysd = lat.cluster(1, 1, 1, with_h=False, seed=3, sigma_jitter=0.3)
@@ -213,7 +214,7 @@ def test_fifth_order_pair_rule_closed_form(tmp_path):
     pre = (sysd["x"][0] * sysd["x"][1]) ** 2 * c * c / 3.0
     exact = pre * (A * A + 10 * A * s2 + 15 * s2 * s2)
     third = pre * (A * A + 10 * A * s2 + 18 * (si ** 4 + sj ** 4))
-    for rule, want in ((5, exact), (3, third)):
+    for rule, want in ((5, exact), (44, exact), (3, third)):
         V0 = O.fact_residual(O.system(sysd, nnb, nbr, rule))["V0"]
         assert V0 == pytest.approx([want, want], rel=1e-12)
 
@@ -349,7 +350,7 @@ def synth(tmp_path_factory):
     return Oracle(str(d))
 
 
-@pytest.mark.parametrize("rule", [3, 5])
+@pytest.mark.parametrize("rule", [3, 5, 44])
 def test_finite_differences_of_free_entropy(synth, rule):
     """R = d(sum_i S0_i)/d(q, sigma, x) on smooth tables (consistent clamp)."""
     sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21)
