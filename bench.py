@@ -171,12 +171,14 @@ def diag_leg(ctx, reps=5):
             "call_ms": dt * 1e3}
 
 
-def rule5_leg(ctx, args, pqe3):
-    """The same workload under the 27-point tensor Gauss-Hermite pair rule
-    (pair and coincident terms; distinct triplets keep the 3rd-order rule):
+def rule5_leg(ctx, args, pqe3, rule=5):
+    """The same workload under a degree-5 pair rule (pair and coincident
+    terms; distinct triplets keep the 3rd-order rule): rule 5, the 27-point
+    tensor Gauss-Hermite rule, or rule 44, the reference's 44-point 6D rule.
     FP64 and split-precision time per residual and the split error against
-    FP64.  PQE counts use 40 points per ordered pair per phase (13 + 27)."""
-    ctx.set_rule(5)
+    FP64.  PQE counts use 13 + (27 or 44) points per ordered pair per phase."""
+    npt = 13 + (27 if rule == 5 else 44)
+    ctx.set_rule(rule)
     out = {}
     res = {}
     try:
@@ -191,7 +193,8 @@ def rule5_leg(ctx, args, pqe3):
             ctx.sync()
             dt = (time.perf_counter() - t0) / args.steps
             res[prec] = ctx.residual()
-            out[prec] = {"ms_per_step": dt * 1e3, "value": pqe3 / 50 * 80 / dt, "unit": "PQE/s"}
+            out[prec] = {"ms_per_step": dt * 1e3, "value": pqe3 / 50 * 2 * npt / dt,
+                         "unit": "PQE/s"}
     finally:
         ctx.set_precision("fp64")
         ctx.set_rule(3)
@@ -455,6 +458,8 @@ def main():
 
     rule5 = rule5_leg(ctx, args, pqe)
     log(f"rule 5: {rule5}")
+    rule44 = rule5_leg(ctx, args, pqe, rule=44)
+    log(f"rule 44: {rule44}")
 
     diag = None
     if not args.no_diag:
@@ -491,6 +496,7 @@ def main():
         "roofline": roof,
         "split32": split,
         "fifth_order_rule": rule5,
+        "reference_44_point_rule": rule44,
         "diag_blocks": diag,
         "solves": solves,
         "serving": serving,
