@@ -658,7 +658,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
           xk += wt * (2.0 * xbk * nk * pp + bu * nu * (FPu * (double)cval(ck, o2) + hp));
         }
       };
-      if (UNR) {
+      if (UNR && PairRule<RULE>::NP <= 27) {  // 44 points unrolled by three spill
 #pragma unroll 3
         for (int pt = 0; pt < PairRule<RULE>::NP; pt++) cpt(pt);
       } else {
