@@ -1724,7 +1724,7 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
   if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
   Solve S;
-  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0};
+  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0, 0};
   if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
       S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0)
     return fail(DMDGPU_ERR_ARG, "bad relaxation options");
