@@ -1604,13 +1604,11 @@ dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* c, const double* X, const double* V, double
   return upload_unknowns(c, I, X, X + 3 * n, nullptr, false);
 }
 
-dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
-  CHK(check_primary(c));
+dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {
   if (!h_D) return fail(DMDGPU_ERR_ARG, "null output");
   if (exact != 0 && exact != 1) return fail(DMDGPU_ERR_ARG, "exact must be 0 or 1");
   HIPCHK(hipSetDevice(c->device));
-  const int id = c->primary;
-  // phase 1 in FP64 (the blocks read the FP64 adjoint records)
+  // phase 1 in FP64 at the instance's state (the blocks read the FP64 adjoint records)
   std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
   c->h_prec[id] = dmd::kFP64;
   CHK(run_eval(c, false));
@@ -1629,6 +1627,20 @@ dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
     std::copy(tmp.begin() + 9 * (size_t)i, tmp.begin() + 9 * (size_t)(i + 1),
               h_D + 9 * (size_t)I.perm[i]);
   return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
+  CHK(check_primary(c));
+  return diag_blocks_of(c, c->primary, exact, h_D);
+}
+
+dmdgpu_status dmdgpu_instance_diag_blocks(dmdgpu_ctx* c, int id, int exact, double* h_D) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
+  if (I->sv.stage != ST_IDLE && I->sv.stage != ST_DONE)
+    return fail(DMDGPU_ERR_STATE, "instance has a solve running");
+  return diag_blocks_of(c, id, exact, h_D);
 }
 
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* c) {

@@ -36,7 +36,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_instance_residual", "dmdgpu_instance_state", "dmdgpu_instance_set_unknowns",
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
-           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks",
+           "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
            "dmdgpu_set_tuning_precision"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
@@ -141,6 +141,7 @@ def load_library():
     L.dmdgpu_diffusion_neighbors.argtypes = [vp, C.c_int, _ip, _ip]
     L.dmdgpu_jvp.argtypes = [vp, _dp, _dp, _dp]
     L.dmdgpu_diag_blocks.argtypes = [vp, C.c_int, _dp]
+    L.dmdgpu_instance_diag_blocks.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.dmdgpu_set_rule.argtypes = [vp, C.c_int]
     L.dmdgpu_set_tuning_precision.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     _lib = L
@@ -265,6 +266,13 @@ class Context:
         D = np.zeros((self.n, 9))
         self._chk(self.L.dmdgpu_diag_blocks(self.h, 1 if exact else 0, _d(D)))
         return D.reshape(self.n, 3, 3)
+
+    def instance_diag_blocks(self, iid, exact=False):
+        """Diagonal blocks of a resident instance at its current state, (n, 3, 3)."""
+        n = self.instance_info(iid)["n"]
+        D = np.zeros((n, 9))
+        self._chk(self.L.dmdgpu_instance_diag_blocks(self.h, iid, 1 if exact else 0, _d(D)))
+        return D.reshape(n, 3, 3)
 
     def residual_resident(self):
         self._chk(self.L.dmdgpu_residual_resident(self.h))

@@ -150,6 +150,9 @@ dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* ctx, const double* h_X, const double* h_V, 
  * blocks for the 3rd-order rule.  Evaluated in FP64 whatever the precision
  * setting; inactive sites (x < 5e-4) get zeros. */
 dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* ctx, int exact, double* h_D);
+/* The same blocks for resident instance `id` at its current state (caller's
+ * site order); DMDGPU_ERR_STATE while a solve of that instance is running. */
+dmdgpu_status dmdgpu_instance_diag_blocks(dmdgpu_ctx* ctx, int id, int exact, double* h_D);
 
 /* Kernel tuning, for both precisions: k_phase1 = lanes of a wavefront that
  * share one site in phase 1 (4, 8 or 16); k_phase2 = blocks per 256-site chunk

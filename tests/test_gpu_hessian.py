@@ -159,3 +159,24 @@ def test_zero_temperature_force_and_blocks(gpu, oracle, maker):
     ef, ed = rel(Rq, F), rel(G, D)
     say(f"    force {ef:.2e}, blocks {ed:.2e}")
     assert ef < TOL and ed < TOL
+
+
+def test_instance_blocks_in_a_batch_equal_alone(gpu, oracle):
+    """dmdgpu_instance_diag_blocks on resident instances of a batch: each equals
+    the blocks of the same system alone (bit for bit) and the oracle."""
+    syss = [lat.cluster(2, 1, 1, x_h=(0.0, 0.4), seed=s) for s in (3, 4)]
+    say("[gpu] instance diag blocks in a batch")
+    alone = []
+    for sd in syss:
+        _setup(gpu, oracle, sd)
+        alone.append(gpu.diag_blocks(exact=True))
+    gpu.remove_instance(0)
+    ids = [gpu.add_instance(sd) for sd in syss]
+    for k, iid in enumerate(ids):
+        D = gpu.instance_diag_blocks(iid, exact=True)
+        assert np.array_equal(D, alone[k])
+        st, nnb, nbr = oracle.neighbors(syss[k], RC)
+        F = oracle.fact_diag_blocks(oracle.system(syss[k], nnb, nbr), exact=True)
+        assert rel(D, F) < TOL
+    for iid in ids:
+        gpu.remove_instance(iid)
