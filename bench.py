@@ -150,6 +150,22 @@ def split_leg(ctx, args, pqe):
             "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
 
 
+def host_leg(ctx, sysd, pqe, reps=10):
+    """The same FP64 residual through the host-buffer boundary (the SNES
+    FormFunction form): upload of q and sigma, the residual, copy-out of R_q,
+    R_sigma, R_x and the totals, per call; wall clock, PCIe included."""
+    ctx.set_precision("fp64")
+    q, sg = sysd["q"].copy(), sysd["sigma"].copy()
+    ctx.update_state(q=q, sigma=sg)
+    ctx.residual()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.update_state(q=q, sigma=sg)
+        ctx.residual()
+    dt = (time.perf_counter() - t0) / reps
+    return {"ms_per_call": dt * 1e3, "value": pqe / dt, "unit": "PQE/s"}
+
+
 def diag_leg(ctx, reps=5):
     """dmdgpu_diag_blocks on the same workload (reference assembly, FP64): the
     k_diag kernel time from HIP events, the FP64 phase 1 it reads, and the wall
@@ -461,6 +477,9 @@ def main():
     rule44 = rule5_leg(ctx, args, pqe, rule=44)
     log(f"rule 44: {rule44}")
 
+    host = host_leg(ctx, sysd, pqe)
+    log(f"host buffers: {host}")
+
     diag = None
     if not args.no_diag:
         diag = diag_leg(ctx)
@@ -497,6 +516,7 @@ def main():
         "split32": split,
         "fifth_order_rule": rule5,
         "reference_44_point_rule": rule44,
+        "host_buffers_pcie_inclusive": host,
         "diag_blocks": diag,
         "solves": solves,
         "serving": serving,
