@@ -888,8 +888,11 @@ dmdgpu_status check_primary(dmdgpu_ctx* c) {
 // Per instance (unknowns: q and sigma of its active sites; occupancies fixed):
 //   START  : FP64 residual at X; stop when |R|_2 <= atol.
 //   BASE32 : (mixed) split-precision residual at X, the base of FP32 probes.
-//   CG     : conjugate gradients on H d = -R with H v = (R(X + eps v) - R(X))/eps;
-//            stop at |r| <= eta |R|, max_cg iterations or negative curvature.
+//   PC     : (precond) P = M^-1 r with the block-Jacobi M built at X between
+//            ticks (pc_setup); no evaluation in this tick.
+//   CG     : conjugate gradients (preconditioned with precond) on H d = -R with
+//            H v = (R(X + eps v) - R(X))/eps; stop at |r| <= eta |R|, max_cg
+//            iterations or negative curvature.
 //   LSPREP : step cap so that max |dq| <= 0.2 A and sigma stays >= sigma/2.
 //   LS     : backtracking on |R(X + a d)| (sufficient decrease), FP64.
 // Every stage that needs a residual gets exactly one per tick.
