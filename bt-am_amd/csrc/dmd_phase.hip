@@ -77,9 +77,11 @@ constexpr int kShiftsPerWalk = 3;
 template <typename P> constexpr bool kCoop1 = true;
 template <typename P> constexpr bool kCoop2 = sizeof(P) == 8;
 // Phase-1 pair-rule points in their own pass, once per unordered pair
-// (k_phase1_pairs): faster in FP64; in FP32 the extra pass costs more than
-// the halved point count saves (measured on MI355X), so k_phase1 keeps them.
-template <typename P> constexpr bool kPairs1 = sizeof(P) == 8;
+// (k_phase1_pairs): faster in FP64; in FP32 with the 12-point rule the extra
+// pass costs more than the halved point count saves (measured on MI355X), so
+// k_phase1 keeps them there; the 27- and 44-point rules take the pass.
+template <typename P, int RULE>
+constexpr bool kPairs1 = sizeof(P) == 8 || PairRule<RULE>::NP > 12;
 // Fused pair-rule rows (phi | rho_Mg | rho_H) through the cooperative gather
 // in phase 2, in both precisions (FP32 phase 2: 0.79 -> 0.75 ms).
 template <typename P> constexpr bool kCoopF2 = true;
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
     TPsi += nj2 * (double)dotM(Tj, psi0);
     // pair and coincident points of the pair rule
-    if constexpr (kPairs1<P>) {  // per-pair sums from k_phase1_pairs
+    if constexpr (kPairs1<P, RULE>) {  // per-pair sums from k_phase1_pairs
       if (inb) {
         const double* e = o.slot + ((size_t)i * kCap + kk) * kSlotW;
         Cc += nj2 * e[0];
@@ -765,7 +767,7 @@ static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
   constexpr int kB = 16;  // blocks per chunk of the pair-rule kernel
-  if (kPairs1<P>)
+  if (kPairs1<P, RULE>)
     k_phase1_pairs<P, kB, RULE><<<(int)((long long)s.n * kB / 256), 256, 0, st>>>(s, T, o);
   k_phase1<P, K, RULE><<<nb, 256, 0, st>>>(s, T, o);
 }
