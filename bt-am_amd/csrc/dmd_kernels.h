@@ -69,7 +69,10 @@ struct State {
   const int* inst;     // instance of each site, -1 for a free slot
   const Box* boxes;    // per instance
   const int* prec;     // per instance: Precision to evaluate with, -1 = skip
-  const int* noxr;     // per instance: 1 = skip the occupancy residual (nullptr: none)
+  // per instance (nullptr: none): 1 = skip the occupancy residual; 2 = also
+  // skip the pairs with two H ends and the H sites' sums (only Mg positions
+  // and widths are needed: an H-H pair adds to nothing else)
+  const int* noxr;
   const double4* qs;   // (x, y, z, sigma)
   const double* ne;    // effective occupancy: x if active (x >= 5e-4) else 0
   const double* xr;    // raw occupancy
@@ -79,11 +82,14 @@ struct State {
   const int* nnb;
   const int* nbr;      // n x kCap
   // Pairs owned by the sites of each chunk (phase 2 evaluates each unordered
-  // pair once, at its owner), in site order: chunk c holds chunk_np[c] entries
+  // pair once, at its owner): chunk c holds chunk_np[c] entries, first the
+  // chunk_npx[c] pairs with an Mg end, then those with two H ends, each part in
+  // site order:
   // own[c * kChunk * kCap + j] = li | kk << 8 | pos << 16 -- owner site
   // c * kChunk + li, kk the slot of the neighbour k in its row, pos the slot of
   // the owner in k's row.
   const int* chunk_np;
+  const int* chunk_npx;
   const int* own;
   // Chunks to evaluate, per precision: act[p * act_stride] = count, then the
   // chunk indices in arena order.  The phase kernels map their blocks onto
@@ -126,7 +132,7 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
 // Owned-pair lists of one instance (after launch_neighbors on the same
 // stream): flags[2] is set if a neighbour relation is not symmetric.
 hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, const int* nbr,
-                            int* chunk_np, int* own, int* flags);
+                            const int* spc, int* chunk_np, int* chunk_npx, int* own, int* flags);
 
 // Precision of the quadrature-point evaluation: FP64 throughout, or the split
 // form (FP32 points and splines, FP64 distances, sums, records and totals).

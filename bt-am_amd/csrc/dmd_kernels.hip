@@ -223,8 +223,9 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n, const int* nnb, int* n
   if (t < cnt) row[t] = key[t];
 }
 
-// Owned pairs of one chunk (block = chunk, thread = site), in site order and
-// row order within a site: li | kk << 8 | pos << 16 (owner li of the chunk, kk
+// Owned pairs of one chunk (block = chunk, thread = site): first the pairs with
+// an Mg end, then those with two H ends (species 1), each part in site order
+// and row order within a site: li | kk << 8 | pos << 16 (owner li of the chunk, kk
 // the slot of the neighbour k in its row, pos the slot of the owner in k's
 // row).  Rows are sorted, so the reverse slot is a binary search.
 DEV int rev_slot(const int* nnb, const int* nbr, int k, int i) {
@@ -240,29 +241,42 @@ DEV int rev_slot(const int* nnb, const int* nbr, int k, int i) {
 }
 
 __global__ __launch_bounds__(256) void k_own_lists(int base, int n, const int* nnb, const int* nbr,
-                                                   int* chunk_np, int* own, int* flags) {
-  __shared__ int scan[256];
+                                                   const int* spc, int* chunk_np, int* chunk_npx,
+                                                   int* own, int* flags) {
+  __shared__ int scan[2][256];
   const int li = blockIdx.x * 256 + threadIdx.x;
   const int i = base + li;
   const int cnt = li < n ? nnb[i] : 0;
+  const bool hi = li < n && spc[i] == 1;
   const int* row = nbr + (size_t)i * kCap;
-  int no = 0;
-  for (int kk = 0; kk < cnt; kk++) no += pair_owner(i, row[kk]);
-  // exclusive scan over the chunk's sites (Hillis-Steele in LDS)
-  scan[threadIdx.x] = no;
-  __syncthreads();
-  for (int d = 1; d < 256; d <<= 1) {
-    const int v = threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
-    __syncthreads();
-    scan[threadIdx.x] += v;
-    __syncthreads();
-  }
-  const int ch = (base >> 8) + (int)blockIdx.x;
-  int* out = own + (size_t)ch * kChunk * kCap;
-  int j = scan[threadIdx.x] - no;
+  int nx = 0, nh = 0;  // owned pairs with an Mg end, with two H ends
   for (int kk = 0; kk < cnt; kk++) {
     const int k = row[kk];
     if (!pair_owner(i, k)) continue;
+    if (hi && spc[k] == 1) nh++;
+    else nx++;
+  }
+  // exclusive scans over the chunk's sites (Hillis-Steele in LDS)
+  scan[0][threadIdx.x] = nx;
+  scan[1][threadIdx.x] = nh;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const int vx = threadIdx.x >= d ? scan[0][threadIdx.x - d] : 0;
+    const int vh = threadIdx.x >= d ? scan[1][threadIdx.x - d] : 0;
+    __syncthreads();
+    scan[0][threadIdx.x] += vx;
+    scan[1][threadIdx.x] += vh;
+    __syncthreads();
+  }
+  const int ch = (base >> 8) + (int)blockIdx.x;
+  const int totx = scan[0][255];
+  int* out = own + (size_t)ch * kChunk * kCap;
+  int jx = scan[0][threadIdx.x] - nx;         // pairs with an Mg end first
+  int jh = totx + scan[1][threadIdx.x] - nh;  // then the H-H pairs
+  for (int kk = 0; kk < cnt; kk++) {
+    const int k = row[kk];
+    if (!pair_owner(i, k)) continue;
+    int& j = hi && spc[k] == 1 ? jh : jx;
     const int pos = rev_slot(nnb, nbr, k, i);
     if (pos < 0) {
       flags[2] = 1;
@@ -271,7 +285,10 @@ __global__ __launch_bounds__(256) void k_own_lists(int base, int n, const int* n
     }
     out[j++] = (int)threadIdx.x | (kk << 8) | (pos << 16);
   }
-  if (threadIdx.x == 255) chunk_np[ch] = scan[255];
+  if (threadIdx.x == 255) {
+    chunk_np[ch] = totx + scan[1][255];
+    chunk_npx[ch] = totx;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -300,8 +317,9 @@ hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, d
 }
 
 hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, const int* nbr,
-                            int* chunk_np, int* own, int* flags) {
-  k_own_lists<<<(n + 255) / 256, 256, 0, st>>>(base, n, nnb, nbr, chunk_np, own, flags);
+                            const int* spc, int* chunk_np, int* chunk_npx, int* own, int* flags) {
+  k_own_lists<<<(n + 255) / 256, 256, 0, st>>>(base, n, nnb, nbr, spc, chunk_np, chunk_npx, own,
+                                               flags);
   return hipGetLastError();
 }
 

@@ -448,7 +448,8 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
   const P* prt = tab_pr<P>(T);
   const int nr = T.nr;
   const int c0 = ch * kChunk;
-  const int np = s.chunk_np[ch];
+  // Mg positions and widths only: the list's H-H tail adds to nothing needed
+  const int np = s.noxr && s.noxr[inst] == 2 ? s.chunk_npx[ch] : s.chunk_np[ch];
   const int* own = s.own + (size_t)ch * kChunk * kCap;
   const int tq = (int)threadIdx.x & (kPairGroup - 1);
   for (int p0 = part * 256 + ((int)threadIdx.x & ~(kPairGroup - 1)); p0 < np; p0 += B * 256) {
@@ -714,6 +715,9 @@ __global__ __launch_bounds__(256) void k_phase2_sum(State s, Tables T, Outputs o
   const int inst = s.inst[u];
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
   const bool xr = !(s.noxr && s.noxr[inst]);
+  // Mg positions and widths only: an H site's slots are partly unwritten
+  // (its H-H pairs were skipped) and its R_q, R_sigma are not needed
+  const bool hq = !(s.noxr && s.noxr[inst] == 2 && s.spc[u] == 1);
   const double nu = s.ne[u];
   if (nu == 0.0) {
     if (g == 0) {
@@ -722,7 +726,7 @@ __global__ __launch_bounds__(256) void k_phase2_sum(State s, Tables T, Outputs o
     }
     return;
   }
-  const int nn = s.nnb[u];
+  const int nn = hq ? s.nnb[u] : 0;
   const double* sl = o.slot + (size_t)u * kCap * kSlotW;
   double q0 = 0, q1 = 0, q2 = 0, sacc = 0, xacc = 0;
   for (int kk = g; kk < nn; kk += K) {  // every slot is written each evaluation
@@ -754,7 +758,7 @@ __global__ __launch_bounds__(256) void k_phase2_sum(State s, Tables T, Outputs o
     const double V0 = o.V0[u];
     o.S0[u] = -logZ + bu * (1.0 / (2.0 * bu) + V0) - gx;
     o.Rq[3 * u] = q0; o.Rq[3 * u + 1] = q1; o.Rq[3 * u + 2] = q2;
-    o.Rs[u] = sacc - 3.0 / su;
+    o.Rs[u] = hq ? sacc - 3.0 / su : 0.0;
     o.Rx[u] = xr ? bu * o.rec[(size_t)u * kRec + kRecDVDN] + xacc + dx : 0.0;
   }
 }

@@ -120,8 +120,11 @@ struct Instance {
   int nactive = 0;
   Solve sv;
   // The residual buffers hold a relaxation tick's evaluation, which leaves
-  // R_x out; rx_prec is the precision to re-evaluate in for a full residual.
+  // R_x out (and, with only Mg unknowns relaxed, R_q and R_sigma of the H
+  // sites: hq_stale); rx_prec is the precision to re-evaluate in for a full
+  // residual.
   bool rx_stale = false;
+  bool hq_stale = false;
   int rx_prec = 0;
 };
 
@@ -176,14 +179,16 @@ struct dmdgpu_ctx {
   DevBuf<dmd::Box> d_boxes;
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
-  DevBuf<int> d_spc, d_nnb, d_nbr, d_chunk_np, d_own;
+  DevBuf<int> d_spc, d_nnb, d_nbr, d_chunk_np, d_chunk_npx, d_own;
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<double> d_diag;  // diagonal Hessian blocks (n x 9)
   DevBuf<double> d_minv;  // arena x 10: block-Jacobi preconditioner of the solver
   DevBuf<long long> d_pairs;
   std::vector<int> h_prec;
-  std::vector<int> h_noxr;  // per instance, solver ticks: 1 = no occupancy residual needed
+  // per instance, solver ticks: 1 = no occupancy residual needed, 2 = Mg
+  // positions and widths only (State::noxr)
+  std::vector<int> h_noxr;
   bool meta_dirty = true;
   // solver
   DevBuf<double4> d_X, d_R0, d_R0f, d_D, d_Rr, d_Pp;
@@ -369,6 +374,7 @@ dmd::State make_state(dmdgpu_ctx* c) {
   s.nnb = c->d_nnb.p;
   s.nbr = c->d_nbr.p;
   s.chunk_np = c->d_chunk_np.p;
+  s.chunk_npx = c->d_chunk_npx.p;
   s.own = c->d_own.p;
   s.act = c->d_act.p;
   s.act_stride = (int)(c->d_act.n / 2);
@@ -421,6 +427,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   HIPCHK(c->d_nnb.ensure(cap, keep));
   HIPCHK(c->d_nbr.ensure(cap * dmd::kCap, keep * dmd::kCap));
   HIPCHK(c->d_chunk_np.ensure(cap / kChunk, keep / kChunk));
+  HIPCHK(c->d_chunk_npx.ensure(cap / kChunk, keep / kChunk));
   HIPCHK(c->d_own.ensure(cap * dmd::kCap, keep * dmd::kCap));
   HIPCHK(c->d_slot.ensure(cap * dmd::kCap * dmd::kSlotW));
   HIPCHK(c->d_cell_of.ensure(cap));
@@ -640,8 +647,8 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   HIPCHK(dmd::launch_neighbors(c->stream, I.box, g, c->rc, I.base, I.n, c->d_qs.p, c->d_cell_of.p,
                                c->d_cell_count.p, c->d_cell_start.p, c->d_cell_cursor.p,
                                c->d_cell_atoms.p, c->d_nnb.p, c->d_nbr.p, c->d_flags.p));
-  HIPCHK(dmd::launch_own_lists(c->stream, I.base, I.n, c->d_nnb.p, c->d_nbr.p, c->d_chunk_np.p,
-                               c->d_own.p, c->d_flags.p));
+  HIPCHK(dmd::launch_own_lists(c->stream, I.base, I.n, c->d_nnb.p, c->d_nbr.p, c->d_spc.p,
+                               c->d_chunk_np.p, c->d_chunk_npx.p, c->d_own.p, c->d_flags.p));
   CHK(end_timed(c, KID_NEIGH, ev));
   int flags[3];
   HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
@@ -738,6 +745,8 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   HIPCHK(hipMemcpy(c->d_nnb.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_chunk_np.p + b / kChunk, zi.data(), sizeof(int) * (cp / kChunk),
                    hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_chunk_npx.p + b / kChunk, zi.data(), sizeof(int) * (cp / kChunk),
+                   hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_dnn.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_Xn.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_Xt.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
@@ -799,7 +808,7 @@ dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
   for (int k = 0; k < (int)c->inst.size(); k++)
     if (c->inst[k].used && c->h_prec[k] >= 0) {
       any[c->h_prec[k]] = true;
-      if (phase2) c->inst[k].rx_stale = false;
+      if (phase2) c->inst[k].rx_stale = c->inst[k].hq_stale = false;
     }
   const dmd::State s = make_state(c);
   const dmd::Outputs o = make_outputs(c);
@@ -857,9 +866,9 @@ dmdgpu_status d2h_sites(dmdgpu_ctx* c, const Instance& I, double* h, const doubl
 dmdgpu_status copy_residual(dmdgpu_ctx* c, int id, double* Rq, double* Rs, double* Rx,
                             double* tot) {
   Instance& I = c->inst[id];
-  if (Rx && I.rx_stale) {
+  if ((Rx && I.rx_stale) || ((Rq || Rs) && I.hq_stale)) {
     // Last evaluated by a relaxation tick: evaluate again, in the same
-    // precision, at the same point, with R_x.
+    // precision, at the same point, with R_x and the H sites' sums.
     std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
     c->h_prec[id] = I.rx_prec;
     CHK(run_eval(c, true));
@@ -1268,8 +1277,10 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
   for (int k = 0; k < ni; k++) {
     Instance& I = c->inst[k];
     if (!I.used || c->h_prec[k] < 0) continue;
-    c->h_noxr[k] = I.sv.stage != ST_DIFF;
+    const bool h_dof = (I.sv.o.dof_mask & (DMDGPU_DOF_H_Q | DMDGPU_DOF_H_SIGMA)) != 0;
+    c->h_noxr[k] = I.sv.stage == ST_DIFF ? 0 : h_dof ? 1 : 2;
     I.rx_stale = c->h_noxr[k] != 0;
+    I.hq_stale = c->h_noxr[k] == 2;
     I.rx_prec = c->h_prec[k];
   }
   const size_t ob = sizeof(dmd::SolveOp) * ni, pb = sizeof(int) * ni;
@@ -1406,7 +1417,7 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
                            &c->d_Xn,  &c->d_Xt,    &c->d_diag, &c->d_minv};
   for (auto* b : dbs) b->release();
   DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
-                        &c->d_chunk_np,   &c->d_own,
+                        &c->d_chunk_np,   &c->d_chunk_npx,  &c->d_own,
                         &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
                         &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_noxr,
                         &c->d_chunk_lo,
