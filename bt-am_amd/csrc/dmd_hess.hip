@@ -137,12 +137,14 @@ DEV double emb_d2(const Tables& T, int sp, double rho) {
 
 template <int RULE>
 __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double* __restrict__ rec,
-                                             int base, int exact, double* __restrict__ D) {
+                                             int base, int exact, int spc_mask,
+                                             double* __restrict__ D) {
   const int u = base + (int)blockIdx.x;
   const int lane = (int)threadIdx.x;
   double* Du = D + (size_t)blockIdx.x * 9;
   const double nu = s.ne[u];
-  if (!(nu > 0.0)) {  // inactive site (wave-uniform)
+  // inactive site, or a species whose blocks are not asked for (wave-uniform)
+  if (!(nu > 0.0) || !((spc_mask >> s.spc[u]) & 1)) {
     if (lane < 9) Du[lane] = 0.0;
     return;
   }
@@ -382,15 +384,15 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
 
 }  // namespace
 
-hipError_t launch_diag(hipStream_t st, int rule, int exact, const State& s, const Tables& T,
-                       const double* rec, int base, int n, double* D) {
+hipError_t launch_diag(hipStream_t st, int rule, int exact, int spc_mask, const State& s,
+                       const Tables& T, const double* rec, int base, int n, double* D) {
   if (n <= 0) return hipSuccess;
   if (rule == 5)
-    hipLaunchKernelGGL(k_diag<5>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
+    hipLaunchKernelGGL(k_diag<5>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, spc_mask, D);
   else if (rule == 44)
-    hipLaunchKernelGGL(k_diag<44>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
+    hipLaunchKernelGGL(k_diag<44>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, spc_mask, D);
   else
-    hipLaunchKernelGGL(k_diag<3>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, D);
+    hipLaunchKernelGGL(k_diag<3>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, spc_mask, D);
   return hipGetLastError();
 }
 

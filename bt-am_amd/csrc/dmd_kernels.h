@@ -160,9 +160,10 @@ hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, int precision, 
 // Per-instance totals: chunk sums, then per instance over its chunk range
 // [chunk_lo[k], chunk_hi[k]) in a fixed order.
 // Diagonal 3x3 blocks D[(u - base) * 9] of dR_q/dq for sites [base, base + n)
-// from the FP64 phase-1 records (rule 3 or 5; exact: see dmd_hess.hip).
-hipError_t launch_diag(hipStream_t st, int rule, int exact, const State& s, const Tables& T,
-                       const double* rec, int base, int n, double* D);
+// from the FP64 phase-1 records (rules 3, 5 and 44; exact: see dmd_hess.hip);
+// sites of a species s with bit s of spc_mask clear get zero blocks.
+hipError_t launch_diag(hipStream_t st, int rule, int exact, int spc_mask, const State& s,
+                       const Tables& T, const double* rec, int base, int n, double* D);
 hipError_t launch_totals(hipStream_t st, const State& s, const Outputs& o, int ninst,
                          const int* chunk_lo, const int* chunk_hi);
 hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_per_chunk);

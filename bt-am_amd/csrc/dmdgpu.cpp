@@ -1260,7 +1260,12 @@ dmdgpu_status pc_setup(dmdgpu_ctx* c) {
     Instance& I = c->inst[k];
     if (!I.used || !I.sv.pc_setup) continue;
     double* D = c->d_diag.p + 9 * (size_t)I.base;
-    HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, 1, s, c->T, c->d_rec.p, I.base, I.n, D));
+    // blocks of the species with relaxed unknowns only: the others are not read
+    const int dof = I.sv.o.dof_mask;
+    const int mask = ((dof & (DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA)) ? 1 : 0) |
+                     ((dof & (DMDGPU_DOF_H_Q | DMDGPU_DOF_H_SIGMA)) ? 2 : 0);
+    HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, 1, mask, s, c->T, c->d_rec.p, I.base, I.n,
+                            D));
     HIPCHK(dmd::launch_pc_invert(c->stream, I.base, I.n, I.cap, I.sv.o.dof_mask, c->d_spc.p,
                                  c->d_ne.p, c->d_qs.p, D, c->d_minv.p));
     I.sv.pc_setup = false;
@@ -1630,7 +1635,7 @@ dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {
   HIPCHK(c->d_diag.ensure((size_t)I.n * 9));
   hipEvent_t a{};
   CHK(begin_timed(c, &a));
-  HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, exact, make_state(c), c->T, c->d_rec.p, I.base,
+  HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, exact, 3, make_state(c), c->T, c->d_rec.p, I.base,
                           I.n, c->d_diag.p));
   CHK(end_timed(c, KID_DIAG, a));
   std::vector<double> tmp((size_t)I.n * 9);
