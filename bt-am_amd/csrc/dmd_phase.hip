@@ -770,7 +770,8 @@ template <typename P, int K, int RULE>
 static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
-  constexpr int kB = 16;  // blocks per chunk of the pair-rule kernel
+  constexpr int kB = 32;  // blocks per chunk of the pair-rule kernel: 1-2 % faster
+                          // than 16 and 8 at 54k sites (measured on MI355X)
   if (kPairs1<P, RULE>)
     k_phase1_pairs<P, kB, RULE><<<(int)((long long)s.n * kB / 256), 256, 0, st>>>(s, T, o);
   k_phase1<P, K, RULE><<<nb, 256, 0, st>>>(s, T, o);
