@@ -232,7 +232,12 @@ dmdgpu_status dmdgpu_instance_set_unknowns(dmdgpu_ctx* ctx, int id, const double
  * residual's noise) the Newton and line-search residuals are split
  * evaluations too, and below it every accepted residual and the stopping test
  * are FP64.  Instances advance independently: one tick gives every running
- * instance exactly one residual evaluation (all instances in one launch). */
+ * instance exactly one residual evaluation (all instances in one launch).
+ * A relaxation tick's evaluation forms only what the selected unknowns need:
+ * no R_x and, when dof_mask selects no H unknowns, no pairs with two H ends
+ * and no R_q, R_sigma of the H sites (every Mg row is the same bit for bit);
+ * dmdgpu_instance_residual after such a tick evaluates again, at the same
+ * point and in the same precision, for the whole residual. */
 typedef struct {
   double atol;     /* stop when |(R_q, R_sigma)|_2 <= atol                */
   int max_newton;  /* Newton steps                                        */

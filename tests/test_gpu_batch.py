@@ -130,6 +130,37 @@ def test_relaxation_converges(ctx, oracle, mixed):
     assert rel(out["Rq"], ref["Rq"]) < 1e-10
 
 
+@pytest.mark.parametrize("dof", [dmdgpu.DOF_MG_Q | dmdgpu.DOF_MG_SIGMA,
+                                 dmdgpu.DOF_MG_Q | dmdgpu.DOF_MG_SIGMA | dmdgpu.DOF_H_SIGMA])
+def test_relaxation_tick_residual_equals_full_evaluation(ctx, dof):
+    """The first tick of a relaxation evaluates the FP64 residual at the start
+    point the way relaxation ticks do: without R_x and, when only Mg unknowns
+    are relaxed, without the pairs with two H ends and the H sites' sums.  The
+    norm it reports over the relaxed unknowns equals the norm of the same rows
+    of a full evaluation (to the summation order's rounding), and reading the
+    residual afterwards gives the full evaluation bit for bit."""
+    say(f"[gpu] relaxation tick residual vs full evaluation, dof {dof}")
+    sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=7)
+    ctx.set_state(sd)
+    ctx.neighbors()
+    full = ctx.residual()
+    act = sd["x"] >= 5e-4
+    mg = (sd["species"] == lat.MG) & act
+    h = (sd["species"] == lat.H) & act
+    n2 = (full["Rq"][mg] ** 2).sum() + (full["Rs"][mg] ** 2).sum()
+    if dof & dmdgpu.DOF_H_SIGMA:
+        n2 += (full["Rs"][h] ** 2).sum()
+    ctx.relax_start(0, dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=0.05,
+                                         mixed=False, dof_mask=dof))
+    ctx.relax_tick()
+    st = ctx.relax_status(0)
+    say(f"    tick norm {st['rnorm']:.15e}, full {np.sqrt(n2):.15e}")
+    assert st["rnorm"] == pytest.approx(np.sqrt(n2), rel=1e-13)
+    again = ctx.instance_residual(0)
+    for k in ("Rq", "Rs", "Rx"):
+        assert np.array_equal(again[k], full[k]), k
+
+
 @pytest.mark.parametrize("mixed,every", [(False, 1), (True, 1), (True, 40)])
 def test_preconditioned_relaxation(ctx, oracle, mixed, every):
     """Block-Jacobi preconditioned CG (diagonal Hessian blocks inverted per
