@@ -804,6 +804,7 @@ static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Tables&
     case 4: p2<P, 4, UNR, RULE>(st, s, T, o); break;
     case 8: p2<P, 8, UNR, RULE>(st, s, T, o); break;
     case 16: p2<P, 16, UNR, RULE>(st, s, T, o); break;
+    case 32: p2<P, 32, UNR, RULE>(st, s, T, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -8,9 +8,10 @@ state and lists; phase 1 and phase 2 are independent kernels, so each picks its
 own best setting.
 """
 LANES = (4, 8, 16)
+BLOCKS = (4, 8, 16, 32)  # phase-2 blocks per chunk
 
 
-def autotune(ctx, precision="fp64", reps=3, lanes=LANES, unrolls=(0, 1)):
+def autotune(ctx, precision="fp64", reps=3, lanes=LANES, blocks=BLOCKS, unrolls=(0, 1)):
     """Tunes one precision ('fp64' or 'split32') on the context's resident
     state; returns ((k1, k2, unroll2), table) and leaves that precision tuned
     and selected.  table maps (k1, k2, unroll2) -> (phase1 ms, phase2 ms)."""
@@ -18,8 +19,8 @@ def autotune(ctx, precision="fp64", reps=3, lanes=LANES, unrolls=(0, 1)):
     ctx.set_precision(precision)
     ctx.set_timing(True)
     # phase 1 depends on k1 only and phase 2 on (k2, unroll2): one sweep each
-    cands = [(k1, lanes[-2], 1) for k1 in lanes] + [(lanes[-2], k2, un) for k2 in lanes
-                                                    for un in unrolls]
+    cands = [(k1, blocks[-2], 1) for k1 in lanes] + [(lanes[-2], k2, un) for k2 in blocks
+                                                     for un in unrolls]
     for k1, k2, un in dict.fromkeys(cands):
         ctx.set_tuning_precision(precision, k1, k2, un)
         ctx.residual_resident()  # first launch of this instantiation, untimed
