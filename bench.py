@@ -408,7 +408,7 @@ def main():
         sys.exit(2)
 
     if args.only_solves:
-        parts = args.tuning.split("/") if args.tuning != "auto" else ["8,32,0", "8,32,1"]
+        parts = args.tuning.split("/") if args.tuning != "auto" else ["8,64,0", "8,64,1"]
         tun = tuple(tuple(int(v) for v in p.split(",")) for p in (parts[0], parts[-1]))
         print(json.dumps({"solves": solves_leg(tun, args.instances, graph=not args.no_graph)}),
               flush=True)

@@ -770,8 +770,10 @@ template <typename P, int K, int RULE>
 static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o) {
   const long long threads = (long long)s.n * K;
   const int nb = (int)((threads + 255) / 256);
-  constexpr int kB = 32;  // blocks per chunk of the pair-rule kernel: 1-2 % faster
-                          // than 16 and 8 at 54k sites (measured on MI355X)
+  // blocks per chunk of the pair-rule kernel: 64 (about one owned pair per
+  // lane at the 54k-site workload) is 2-4 % faster than 32, 16 or 8 there
+  // (measured on MI355X)
+  constexpr int kB = 64;
   if (kPairs1<P, RULE>)
     k_phase1_pairs<P, kB, RULE><<<(int)((long long)s.n * kB / 256), 256, 0, st>>>(s, T, o);
   k_phase1<P, K, RULE><<<nb, 256, 0, st>>>(s, T, o);
@@ -805,6 +807,7 @@ static hipError_t phase2_pu(hipStream_t st, int K, const State& s, const Tables&
     case 8: p2<P, 8, UNR, RULE>(st, s, T, o); break;
     case 16: p2<P, 16, UNR, RULE>(st, s, T, o); break;
     case 32: p2<P, 32, UNR, RULE>(st, s, T, o); break;
+    case 64: p2<P, 64, UNR, RULE>(st, s, T, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

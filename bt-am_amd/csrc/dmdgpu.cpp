@@ -1487,8 +1487,8 @@ dmdgpu_status dmdgpu_set_cutoff(dmdgpu_ctx* c, double rc) {
 
 dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
   auto ok = [](int k) { return k == 4 || k == 8 || k == 16; };
-  if (!c || !ok(k1) || !(ok(k2) || k2 == 32))
-    return fail(DMDGPU_ERR_ARG, "k_phase1 must be 4, 8 or 16 and k_phase2 4, 8, 16 or 32");
+  if (!c || !ok(k1) || !(ok(k2) || k2 == 32 || k2 == 64))
+    return fail(DMDGPU_ERR_ARG, "k_phase1 must be 4, 8 or 16 and k_phase2 4, 8, 16, 32 or 64");
   for (int p = 0; p < 2; p++) {
     c->cfg.k1[p] = k1;
     c->cfg.k2[p] = k2;
@@ -1500,7 +1500,7 @@ dmdgpu_status dmdgpu_set_tuning(dmdgpu_ctx* c, int k1, int k2, int unroll2) {
 dmdgpu_status dmdgpu_set_tuning_precision(dmdgpu_ctx* c, int precision, int k1, int k2,
                                           int unroll2) {
   auto ok = [](int k) { return k == 4 || k == 8 || k == 16; };
-  if (!c || !ok(k1) || !(ok(k2) || k2 == 32) ||
+  if (!c || !ok(k1) || !(ok(k2) || k2 == 32 || k2 == 64) ||
       (precision != DMDGPU_FP64 && precision != DMDGPU_SPLIT32))
     return fail(DMDGPU_ERR_ARG, "bad tuning");
   const int p = precision == DMDGPU_SPLIT32 ? dmd::kSplit32 : dmd::kFP64;

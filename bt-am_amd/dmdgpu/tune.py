@@ -8,7 +8,7 @@ state and lists; phase 1 and phase 2 are independent kernels, so each picks its
 own best setting.
 """
 LANES = (4, 8, 16)
-BLOCKS = (4, 8, 16, 32)  # phase-2 blocks per chunk
+BLOCKS = (4, 8, 16, 32, 64)  # phase-2 blocks per chunk
 
 
 def autotune(ctx, precision="fp64", reps=3, lanes=LANES, blocks=BLOCKS, unrolls=(0, 1)):

@@ -156,7 +156,7 @@ dmdgpu_status dmdgpu_instance_diag_blocks(dmdgpu_ctx* ctx, int id, int exact, do
 
 /* Kernel tuning, for both precisions: k_phase1 = lanes of a wavefront that
  * share one site in phase 1 (4, 8 or 16); k_phase2 = blocks per 256-site chunk
- * of the pair kernels (4, 8, 16 or 32: on average that many lanes per site);
+ * of the pair kernels (4, 8, 16, 32 or 64: on average that many lanes per site);
  * unroll2 = phase 2 advances two shifted or three pair-rule points per loop
  * iteration (0: one).
  * The tuner picks them per workload.  No counterpart in the reference. */

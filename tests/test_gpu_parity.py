@@ -77,7 +77,7 @@ def test_residual_parity_fp64(gpu, oracle, case):
 
 
 @pytest.mark.parametrize("k1,k2,un", [(4, 16, 1), (16, 4, 0), (8, 8, 1), (4, 4, 0), (8, 32, 0),
-                                       (16, 32, 1)])
+                                       (16, 32, 1), (8, 64, 0), (4, 64, 1)])
 def test_tuning_variants_agree(gpu, oracle, k1, k2, un):
     say(f"[gpu] tuning {k1} {k2} {un}")
     sysd = CASES["hcp_mgh_5x3x3"]()
@@ -90,10 +90,10 @@ def test_tuning_variants_agree(gpu, oracle, k1, k2, un):
 
 
 def test_tuning_rejects_unsupported_values(gpu):
-    """k_phase1 in {4, 8, 16}, k_phase2 in {4, 8, 16, 32}; anything else is an
+    """k_phase1 in {4, 8, 16}, k_phase2 in {4, 8, 16, 32, 64}; anything else is an
     argument error and leaves the tuning unchanged."""
     import dmdgpu
-    for bad in ((32, 8, 0), (8, 64, 0), (2, 8, 0), (8, 12, 1)):
+    for bad in ((32, 8, 0), (8, 128, 0), (2, 8, 0), (8, 12, 1)):
         with pytest.raises(dmdgpu.DmdError):
             gpu.set_tuning(*bad)
     gpu.set_tuning(4, 4, 1)
