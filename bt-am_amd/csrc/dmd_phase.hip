@@ -77,11 +77,12 @@ constexpr int kShiftsPerWalk = 3;
 template <typename P> constexpr bool kCoop1 = true;
 template <typename P> constexpr bool kCoop2 = sizeof(P) == 8;
 // Phase-1 pair-rule points in their own pass, once per unordered pair
-// (k_phase1_pairs): faster in FP64; in FP32 with the 12-point rule the extra
-// pass costs more than the halved point count saves (measured on MI355X), so
-// k_phase1 keeps them there; the 27- and 44-point rules take the pass.
+// (k_phase1_pairs), in both precisions and for every rule: with 64 blocks per
+// chunk it is faster than evaluating them from both ends inside k_phase1
+// (split residual, 12-point rule: 1.48 -> 1.43 ms at 54k sites, measured on
+// MI355X; at 16 blocks per chunk the FP32 pass had not paid for this rule).
 template <typename P, int RULE>
-constexpr bool kPairs1 = sizeof(P) == 8 || PairRule<RULE>::NP > 12;
+constexpr bool kPairs1 = true;
 // Fused pair-rule rows (phi | rho_Mg | rho_H) through the cooperative gather
 // in phase 2, in both precisions (FP32 phase 2: 0.79 -> 0.75 ms).
 template <typename P> constexpr bool kCoopF2 = true;
