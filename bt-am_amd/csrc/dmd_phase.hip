@@ -76,13 +76,6 @@ constexpr int kShiftsPerWalk = 3;
 // gains in FP64 only (its FP32 body is already VALU-heavy).
 template <typename P> constexpr bool kCoop1 = true;
 template <typename P> constexpr bool kCoop2 = sizeof(P) == 8;
-// Phase-1 pair-rule points in their own pass, once per unordered pair
-// (k_phase1_pairs), in both precisions and for every rule: with 64 blocks per
-// chunk it is faster than evaluating them from both ends inside k_phase1
-// (split residual, 12-point rule: 1.48 -> 1.43 ms at 54k sites, measured on
-// MI355X; at 16 blocks per chunk the FP32 pass had not paid for this rule).
-template <typename P, int RULE>
-constexpr bool kPairs1 = true;
 // Fused pair-rule rows (phi | rho_Mg | rho_H) through the cooperative gather
 // in phase 2, in both precisions (FP32 phase 2: 0.79 -> 0.75 ms).
 template <typename P> constexpr bool kCoopF2 = true;
@@ -159,10 +152,8 @@ template <typename P, int K, int RULE>
 __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
   static_assert(K >= 4, "a site's lanes must cover whole gather groups");
   constexpr bool CO = kCoop1<P>;
-  // largest gathers: the rows of one i-shift walk; a u/w and a fused row
-  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, CO>::v > CptRounds<P, CO, CO>::v
-                              ? StageRounds<P, kShiftsPerWalk, CO>::v
-                              : CptRounds<P, CO, CO>::v;
+  // largest gather: the rows of one i-shift walk
+  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, CO>::v;
   __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
   u32x4* stage = stage_all[threadIdx.x >> 6];
   int i, g;
@@ -305,31 +296,12 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll
     for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
     TPsi += nj2 * (double)dotM(Tj, psi0);
-    // pair and coincident points of the pair rule
-    if constexpr (kPairs1<P, RULE>) {  // per-pair sums from k_phase1_pairs
-      if (inb) {
-        const double* e = o.slot + ((size_t)i * kCap + kk) * kSlotW;
-        Cc += nj2 * e[0];
-        Ep += nj * e[1];
-        rho += nj * e[2];
-      }
-    } else {
-      const P* prt = tab_pr<P>(T);
-      double Phi = 0, R = 0, C6 = 0;  // per-pair sums over the rule's points in FP64
-#pragma unroll 1
-      for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-        P dsi[3], dsj[3], f[16], cr[4];
-        const double wt = PairRule<RULE>::point(geo, T, pt, si, sj, p, m, off, ir, dsi, dsj);
-        gather_cpt<P, CO, CO>(uwt, rb + m, prt, rb + m, stage, cw, f);
-        psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
-        C6 += wt * (double)dotM(ps, ps);
-        Phi += wt * (double)cval(f, off);
-        fused_rho(f, spj, cr);
-        R += wt * (double)cval(cr, off);
-      }
-      Cc += nj2 * C6;
-      rho += nj * R;
-      Ep += nj * Phi;
+    // pair and coincident points of the pair rule: per-pair sums from k_phase1_pairs
+    if (inb) {
+      const double* e = o.slot + ((size_t)i * kCap + kk) * kSlotW;
+      Cc += nj2 * e[0];
+      Ep += nj * e[1];
+      rho += nj * e[2];
     }
   }
   if (K > 1) {
@@ -775,8 +747,11 @@ static void p1(hipStream_t st, const State& s, const Tables& T, const Outputs& o
   // lane at the 54k-site workload) is 2-4 % faster than 32, 16 or 8 there
   // (measured on MI355X)
   constexpr int kB = 64;
-  if (kPairs1<P, RULE>)
-    k_phase1_pairs<P, kB, RULE><<<(int)((long long)s.n * kB / 256), 256, 0, st>>>(s, T, o);
+  // The pair-rule points in their own pass, once per unordered pair, in both
+  // precisions and for every rule: faster than evaluating them from both ends
+  // inside k_phase1 (split residual, 12-point rule: 1.48 -> 1.43 ms at 54k
+  // sites, measured on MI355X).
+  k_phase1_pairs<P, kB, RULE><<<(int)((long long)s.n * kB / 256), 256, 0, st>>>(s, T, o);
   k_phase1<P, K, RULE><<<nb, 256, 0, st>>>(s, T, o);
 }
 template <typename P, int K, bool UNR, int RULE>
