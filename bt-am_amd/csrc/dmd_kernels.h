@@ -112,8 +112,9 @@ struct Outputs {
   double* S0;
   double* slot;        // n x kCap x kSlotW: a pair's phase-2 contribution to the site
                        // of that row (dR_q x3, dR_sigma, dR_x), written by the owner
-  double* partials;    // per-chunk [S0, V0] partial sums
-  double* totals;      // per instance [sum S0, sum V0]
+  double* partials;    // per-chunk [S0, V0, domain] partial sums
+  double* totals;      // per instance [sum S0, sum V0, sites whose density left the
+                       // embedding table's domain]
 };
 
 // Neighbour build (cell list).  Returns via flags[0] = overflow, flags[1] = empty row.
@@ -164,8 +165,8 @@ hipError_t launch_phase2(hipStream_t st, const PhaseConfig& cfg, int precision, 
 // sites of a species s with bit s of spc_mask clear get zero blocks.
 hipError_t launch_diag(hipStream_t st, int rule, int exact, int spc_mask, const State& s,
                        const Tables& T, const double* rec, int base, int n, double* D);
-hipError_t launch_totals(hipStream_t st, const State& s, const Outputs& o, int ninst,
-                         const int* chunk_lo, const int* chunk_hi);
+hipError_t launch_totals(hipStream_t st, const State& s, const Tables& T, const Outputs& o,
+                         int ninst, const int* chunk_lo, const int* chunk_hi);
 hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_per_chunk);
 
 }  // namespace dmd

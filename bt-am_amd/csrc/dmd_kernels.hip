@@ -4,28 +4,36 @@
 namespace dmd {
 
 // ---------------------------------------------------------------------------
-// Deterministic totals: per-block sums in a fixed tree, then one block.
+// Deterministic totals: per-block sums in a fixed tree, then one block.  The
+// third sum counts the sites whose density left the embedding table's domain:
+// the reference's spline takes segment trunc(x/dx), a negative index for
+// x <= -dx (src/Numerical/cubic-spline.cpp:84-93), so such a density has no
+// reference value; a non-finite density counts too.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_block_sums(int n, const int* inst, const int* prec,
-                                                   const double* S0, const double* V0,
-                                                   double* partials) {
-  __shared__ double sa[256], sb[256];
+                                                   const int* spc, const double* rho,
+                                                   double dxe0, double dxe1, const double* S0,
+                                                   const double* V0, double* partials) {
+  __shared__ double sa[256], sb[256], sc[256];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k = inst[blockIdx.x * 256];  // one instance per 256-site chunk
   const bool on = i < n && k >= 0 && prec[k] >= 0;
   sa[threadIdx.x] = on ? S0[i] : 0.0;
   sb[threadIdx.x] = on ? V0[i] : 0.0;
+  sc[threadIdx.x] = on && !(rho[i] > -(spc[i] ? dxe1 : dxe0) && rho[i] < 1e300) ? 1.0 : 0.0;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       sa[threadIdx.x] += sa[threadIdx.x + w];
       sb[threadIdx.x] += sb[threadIdx.x + w];
+      sc[threadIdx.x] += sc[threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = sa[0];
-    partials[2 * blockIdx.x + 1] = sb[0];
+    partials[3 * blockIdx.x] = sa[0];
+    partials[3 * blockIdx.x + 1] = sb[0];
+    partials[3 * blockIdx.x + 2] = sc[0];
   }
 }
 
@@ -33,27 +41,31 @@ __global__ __launch_bounds__(256) void k_block_sums(int n, const int* inst, cons
 __global__ __launch_bounds__(256) void k_inst_sums(const int* prec, const int* chunk_lo,
                                                   const int* chunk_hi, const double* partials,
                                                   double* totals) {
-  __shared__ double sa[256], sb[256];
+  __shared__ double sa[256], sb[256], sc[256];
   const int k = blockIdx.x;
   if (prec[k] < 0) return;
-  double a = 0.0, b = 0.0;
+  double a = 0.0, b = 0.0, d = 0.0;
   for (int c = chunk_lo[k] + threadIdx.x; c < chunk_hi[k]; c += 256) {
-    a += partials[2 * c];
-    b += partials[2 * c + 1];
+    a += partials[3 * c];
+    b += partials[3 * c + 1];
+    d += partials[3 * c + 2];
   }
   sa[threadIdx.x] = a;
   sb[threadIdx.x] = b;
+  sc[threadIdx.x] = d;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       sa[threadIdx.x] += sa[threadIdx.x + w];
       sb[threadIdx.x] += sb[threadIdx.x + w];
+      sc[threadIdx.x] += sc[threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    totals[2 * k] = sa[0];
-    totals[2 * k + 1] = sb[0];
+    totals[3 * k] = sa[0];
+    totals[3 * k + 1] = sb[0];
+    totals[3 * k + 2] = sc[0];
   }
 }
 
@@ -323,10 +335,11 @@ hipError_t launch_own_lists(hipStream_t st, int base, int n, const int* nnb, con
   return hipGetLastError();
 }
 
-hipError_t launch_totals(hipStream_t st, const State& s, const Outputs& o, int ninst,
-                         const int* chunk_lo, const int* chunk_hi) {
+hipError_t launch_totals(hipStream_t st, const State& s, const Tables& T, const Outputs& o,
+                         int ninst, const int* chunk_lo, const int* chunk_hi) {
   const int nb = (s.n + 255) / 256;
-  k_block_sums<<<nb, 256, 0, st>>>(s.n, s.inst, s.prec, o.S0, o.V0, o.partials);
+  k_block_sums<<<nb, 256, 0, st>>>(s.n, s.inst, s.prec, s.spc, o.rho, T.dxe[0], T.dxe[1], o.S0,
+                                   o.V0, o.partials);
   k_inst_sums<<<ninst, 256, 0, st>>>(s.prec, chunk_lo, chunk_hi, o.partials, o.totals);
   return hipGetLastError();
 }

@@ -17,8 +17,11 @@ enum : int {
 };
 enum : int { kMakeNone = 0, kMakeX = 1, kMakeXP = 2, kMakeXD = 3 };
 enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4, kDotPc = 5 };
-// Dots per instance: [0..2] sums, [3..4] maxima, [5..13] sums (preconditioned CG).
-constexpr int kSolveDots = 14;
+// Dots per instance: [0..2] sums, [3..4] maxima, [5..13] sums (preconditioned CG),
+// [14] sites of the tick's evaluation whose density left the embedding
+// table's domain.
+constexpr int kSolveDots = 15;
+constexpr int kDotDomain = 14;
 
 // Diffusion-step modes (SolveOp.diff): one Picard sweep of the implicit
 // occupancy update, or the bare rates dx/dt written into Xt.
@@ -81,8 +84,9 @@ hipError_t launch_diff_flux(hipStream_t st, const State& s, const SolveOp* ops, 
 hipError_t launch_diff_apply(hipStream_t st, int n, const int* inst, const int* spc,
                              const SolveOp* ops, const SolveVecs& v, double* xr, double* ne);
 hipError_t launch_solver_dots(hipStream_t st, int n, int ninst, const int* inst, const int* spc,
-                              const double* ne, const SolveOp* ops, const SolveVecs& v,
-                              const Outputs& o, const double* xr, double* partials,
+                              const double* ne, const SolveOp* ops, const int* prec,
+                              const SolveVecs& v, const Outputs& o, const double* xr,
+                              double* partials,
                               const int* chunk_lo, const int* chunk_hi, double* out);
 
 }  // namespace dmd

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
                                                     const double* ne, const SolveOp* ops,
                                                     SolveVecs v, Outputs o, const double* xr,
                                                     double* partials) {
-  __shared__ double s[kSolveDots][256];
+  __shared__ double s[kDotDomain][256];  // the reduced slots 0..13
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k = inst[blockIdx.x * 256];
   double a = 0, b = 0, c = 0, d = 0, e = -1e300;
@@ -157,20 +157,25 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
       s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + w]);
       s[4][threadIdx.x] = fmax(s[4][threadIdx.x], s[4][threadIdx.x + w]);
 #pragma unroll
-      for (int t = 5; t < kSolveDots; t++) s[t][threadIdx.x] += s[t][threadIdx.x + w];
+      for (int t = 5; t < kDotDomain; t++) s[t][threadIdx.x] += s[t][threadIdx.x + w];
     }
     __syncthreads();
   }
-  if (threadIdx.x < kSolveDots)
+  if (threadIdx.x < kDotDomain)
     partials[kSolveDots * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// One block per instance over its chunk range, fixed order.
+// One block per instance over its chunk range, fixed order.  Slot 14 carries
+// the table-domain count of the instance's evaluation in this tick (the third
+// per-instance total; 0 when it was not evaluated).
 __global__ __launch_bounds__(64) void k_solver_reduce(const int* chunk_lo, const int* chunk_hi,
-                                                     const SolveOp* ops, const double* partials,
+                                                     const SolveOp* ops, const int* prec,
+                                                     const double* totals, const double* partials,
                                                      double* out) {
   const int k = blockIdx.x;
   if (threadIdx.x != 0) return;
+  double* o = out + (size_t)kSolveDots * k;
+  o[kDotDomain] = prec[k] >= 0 ? totals[3 * k + 2] : 0.0;
   if (ops[k].dot == kDotNone) return;
   double a = 0, b = 0, c = 0, d = 0, e = -1e300;
   double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -184,7 +189,6 @@ __global__ __launch_bounds__(64) void k_solver_reduce(const int* chunk_lo, const
 #pragma unroll
     for (int t = 0; t < 9; t++) g[t] += p[5 + t];
   }
-  double* o = out + (size_t)kSolveDots * k;
   o[0] = a; o[1] = b; o[2] = c; o[3] = d; o[4] = e;
 #pragma unroll
   for (int t = 0; t < 9; t++) o[5 + t] = g[t];
@@ -249,11 +253,12 @@ hipError_t launch_solver_step(hipStream_t st, int n, const int* inst, const int*
 }
 
 hipError_t launch_solver_dots(hipStream_t st, int n, int ninst, const int* inst, const int* spc,
-                              const double* ne, const SolveOp* ops, const SolveVecs& v,
-                              const Outputs& o, const double* xr, double* partials,
+                              const double* ne, const SolveOp* ops, const int* prec,
+                              const SolveVecs& v, const Outputs& o, const double* xr,
+                              double* partials,
                               const int* chunk_lo, const int* chunk_hi, double* out) {
   k_solver_dots<<<(n + 255) / 256, 256, 0, st>>>(n, inst, spc, ne, ops, v, o, xr, partials);
-  k_solver_reduce<<<ninst, 64, 0, st>>>(chunk_lo, chunk_hi, ops, partials, out);
+  k_solver_reduce<<<ninst, 64, 0, st>>>(chunk_lo, chunk_hi, ops, prec, o.totals, partials, out);
   return hipGetLastError();
 }
 

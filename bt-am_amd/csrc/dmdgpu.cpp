@@ -46,12 +46,6 @@ dmdgpu_status fail(dmdgpu_status st, const std::string& msg) {
 constexpr double kMinOcc = 5e-4;            // src/Macros.hpp:871
 constexpr double kDefaultRc = 6.2934034034;  // src/Macros.hpp:857
 constexpr int kChunk = dmd::kChunk;
-// Device bytes per arena site (state, lists, records, outputs, solver vectors,
-// preconditioner: 10 doubles of M^-1 and 9 of the diagonal blocks).
-constexpr size_t kBytesPerSite = 32 + 8 * 5 + 4 * 4 + 4 * dmd::kCap + 8 * dmd::kRec + 8 * 8 +
-                                 6 * 32 + 2 * 8 + 4 + 4 * dmd::kDCap + 4 * dmd::kRec32 + 1 +
-                                 4 * dmd::kCap + 8 * dmd::kSlotW * dmd::kCap + 8 * (10 + 9);
-
 struct Spline {
   int n = 0;
   double dx = 0.0;
@@ -66,9 +60,13 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
-  // Grows to at least cnt elements, keeping the first `keep` elements.
+  size_t bytes() const { return p ? sizeof(T) * std::max<size_t>(n, 1) : 0; }
+  // Grows to at least cnt elements, keeping the first `keep` elements.  With
+  // nothing to keep the old allocation is released first, so a growth holds
+  // both copies only for buffers whose contents survive it.
   hipError_t ensure(size_t cnt, size_t keep = 0) {
     if (cnt <= n && p) return hipSuccess;
+    if (!keep) release();
     T* q = nullptr;
     hipError_t e = hipMalloc((void**)&q, sizeof(T) * std::max<size_t>(cnt, 1));
     if (e != hipSuccess) return e;
@@ -180,6 +178,7 @@ struct dmdgpu_ctx {
   DevBuf<double4> d_qs;
   DevBuf<double> d_ne, d_xr, d_beta, d_gamma;
   DevBuf<int> d_spc, d_nnb, d_nbr, d_chunk_np, d_chunk_npx, d_own;
+  DevBuf<int> d_perm;  // arena site -> the caller's index within its instance
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<double> d_diag;  // diagonal Hessian blocks (n x 9)
@@ -410,51 +409,121 @@ void drop_graph(dmdgpu_ctx* c) {
   c->graph_extent = -1;
 }
 
+// The arena's per-site arrays at a capacity of `cap` sites: f(buffer,
+// elements, keep) for each, keep = the contents of the sites in use survive a
+// growth.  The same list sizes the arena, counts its bytes and bounds the
+// memory held while it grows.
+template <typename F>
+void for_site_bufs(dmdgpu_ctx* c, size_t cap, F&& f) {
+  const size_t ch = cap / kChunk;
+  f(c->d_qs, cap, true);
+  f(c->d_ne, cap, true);
+  f(c->d_xr, cap, true);
+  f(c->d_beta, cap, true);
+  f(c->d_gamma, cap, true);
+  f(c->d_spc, cap, true);
+  f(c->d_inst, cap, true);
+  f(c->d_perm, cap, true);
+  f(c->d_nnb, cap, true);
+  f(c->d_nbr, cap * dmd::kCap, true);
+  f(c->d_chunk_np, ch, true);
+  f(c->d_chunk_npx, ch, true);
+  f(c->d_own, cap * dmd::kCap, true);
+  f(c->d_slot, cap * dmd::kCap * dmd::kSlotW, false);
+  f(c->d_cell_of, cap, false);
+  f(c->d_cell_atoms, cap, false);
+  f(c->d_rec, cap * dmd::kRec, true);
+  f(c->d_rec32, cap * dmd::kRec32, true);
+  f(c->d_V0, cap, true);
+  f(c->d_rho, cap, true);
+  f(c->d_Rq, cap * 3, true);
+  f(c->d_Rs, cap, true);
+  f(c->d_Rx, cap, true);
+  f(c->d_S0, cap, true);
+  f(c->d_partials, ch * 3, false);
+  f(c->d_pairs, ch, false);
+  f(c->d_spart, ch * dmd::kSolveDots, false);
+  f(c->d_X, cap, true);
+  f(c->d_R0, cap, true);
+  f(c->d_R0f, cap, true);
+  f(c->d_D, cap, true);
+  f(c->d_Rr, cap, true);
+  f(c->d_Pp, cap, true);
+  f(c->d_Xn, cap, true);
+  f(c->d_Xt, cap, true);
+  f(c->d_minv, cap * 10, true);
+  f(c->d_diag, cap * 9, false);
+  f(c->d_dnn, cap, true);
+  f(c->d_dnbr, cap * dmd::kDCap, true);
+  f(c->d_act, 2 * (ch + 1), false);
+}
+
+// Device bytes of the per-site arrays at a capacity of `cap` sites, and the
+// largest of those whose contents a growth keeps (both copies of one such
+// array are held at a time while the arena grows).
+size_t site_bytes(dmdgpu_ctx* c, size_t cap, size_t* max_kept = nullptr) {
+  size_t b = 0, mk = 0;
+  for_site_bufs(c, cap, [&](auto& buf, size_t cnt, bool keep) {
+    const size_t v = cnt * sizeof(*buf.p);
+    b += v;
+    if (keep) mk = std::max(mk, v);
+  });
+  if (max_kept) *max_kept = mk;
+  return b;
+}
+
+// The context's device arrays outside the arena: tables, cell grid,
+// per-instance metadata and solver descriptors.
+template <typename F>
+void for_other_bufs(dmdgpu_ctx* c, F&& f) {
+  f(c->d_uw); f(c->d_ph); f(c->d_rh); f(c->d_em); f(c->d_pr);
+  f(c->d_uw32); f(c->d_ph32); f(c->d_rh32); f(c->d_pr32);
+  f(c->d_cell_count); f(c->d_cell_start); f(c->d_cell_cursor); f(c->d_flags);
+  f(c->d_boxes); f(c->d_prec); f(c->d_noxr); f(c->d_chunk_lo); f(c->d_chunk_hi);
+  f(c->d_totals); f(c->d_ops); f(c->d_sdots);
+}
+
+// Every device allocation of the context.
+size_t device_bytes(dmdgpu_ctx* c) {
+  size_t b = 0;
+  for_site_bufs(c, 0, [&](auto& buf, size_t, bool) { b += buf.bytes(); });
+  for_other_bufs(c, [&](auto& buf) { b += buf.bytes(); });
+  return b;
+}
+
 // Makes every per-site array hold at least `sites` sites (keeping contents).
+// The capacity grows by half again when it must grow at all, but never past
+// what the budget allows: the context's other allocations, the new arrays and
+// the one old array held during its copy must fit.
 dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   const size_t keep = c->extent;
   if (sites <= c->d_qs.n && c->d_qs.p) return DMDGPU_OK;
-  size_t cap = std::max(sites, (size_t)(c->d_qs.n * 3 / 2));
-  cap = (cap + kChunk - 1) / kChunk * kChunk;
+  auto round_up = [](size_t v) { return (v + kChunk - 1) / kChunk * kChunk; };
+  size_t cap = round_up(std::max(sites, (size_t)(c->d_qs.n * 3 / 2)));
+  if (c->budget) {
+    size_t kept_old = 0;
+    const size_t old = site_bytes(c, c->d_qs.p ? c->d_qs.n : 0, &kept_old);
+    const size_t other = device_bytes(c) - std::min(device_bytes(c), old);
+    auto fits = [&](size_t cp) { return other + site_bytes(c, cp) + kept_old <= c->budget; };
+    const size_t need = round_up(sites);
+    if (!fits(need))
+      return fail(DMDGPU_ERR_BUDGET, "instance does not fit the device-memory budget");
+    if (!fits(cap)) {  // largest chunk multiple in [need, cap) that fits
+      size_t lo = need / kChunk, hi = cap / kChunk;
+      while (hi - lo > 1) {
+        const size_t mid = (lo + hi) / 2;
+        (fits(mid * kChunk) ? lo : hi) = mid;
+      }
+      cap = lo * kChunk;
+    }
+  }
   drop_graph(c);
-  HIPCHK(c->d_qs.ensure(cap, keep));
-  HIPCHK(c->d_ne.ensure(cap, keep));
-  HIPCHK(c->d_xr.ensure(cap, keep));
-  HIPCHK(c->d_beta.ensure(cap, keep));
-  HIPCHK(c->d_gamma.ensure(cap, keep));
-  HIPCHK(c->d_spc.ensure(cap, keep));
-  HIPCHK(c->d_inst.ensure(cap, keep));
-  HIPCHK(c->d_nnb.ensure(cap, keep));
-  HIPCHK(c->d_nbr.ensure(cap * dmd::kCap, keep * dmd::kCap));
-  HIPCHK(c->d_chunk_np.ensure(cap / kChunk, keep / kChunk));
-  HIPCHK(c->d_chunk_npx.ensure(cap / kChunk, keep / kChunk));
-  HIPCHK(c->d_own.ensure(cap * dmd::kCap, keep * dmd::kCap));
-  HIPCHK(c->d_slot.ensure(cap * dmd::kCap * dmd::kSlotW));
-  HIPCHK(c->d_cell_of.ensure(cap));
-  HIPCHK(c->d_cell_atoms.ensure(cap));
-  HIPCHK(c->d_rec.ensure(cap * dmd::kRec, keep * dmd::kRec));
-  HIPCHK(c->d_rec32.ensure(cap * dmd::kRec32, keep * dmd::kRec32));
-  HIPCHK(c->d_V0.ensure(cap, keep));
-  HIPCHK(c->d_rho.ensure(cap, keep));
-  HIPCHK(c->d_Rq.ensure(cap * 3, keep * 3));
-  HIPCHK(c->d_Rs.ensure(cap, keep));
-  HIPCHK(c->d_Rx.ensure(cap, keep));
-  HIPCHK(c->d_S0.ensure(cap, keep));
-  HIPCHK(c->d_partials.ensure(cap / kChunk * 2));
-  HIPCHK(c->d_pairs.ensure(cap / kChunk));
-  HIPCHK(c->d_spart.ensure(cap / kChunk * dmd::kSolveDots));
-  HIPCHK(c->d_X.ensure(cap, keep));
-  HIPCHK(c->d_R0.ensure(cap, keep));
-  HIPCHK(c->d_R0f.ensure(cap, keep));
-  HIPCHK(c->d_D.ensure(cap, keep));
-  HIPCHK(c->d_Rr.ensure(cap, keep));
-  HIPCHK(c->d_Pp.ensure(cap, keep));
-  HIPCHK(c->d_Xn.ensure(cap, keep));
-  HIPCHK(c->d_Xt.ensure(cap, keep));
-  HIPCHK(c->d_minv.ensure(cap * 10, keep * 10));
-  HIPCHK(c->d_dnn.ensure(cap, keep));
-  HIPCHK(c->d_dnbr.ensure(cap * dmd::kDCap, keep * dmd::kDCap));
-  HIPCHK(c->d_act.ensure(2 * (cap / kChunk + 1)));
+  hipError_t err = hipSuccess;
+  for_site_bufs(c, cap, [&](auto& buf, size_t cnt, bool kp) {
+    if (err != hipSuccess) return;
+    err = buf.ensure(cnt, kp ? cnt * keep / cap : 0);
+  });
+  HIPCHK(err);
   HIPCHK(c->d_flags.ensure(3));
   return DMDGPU_OK;
 }
@@ -470,7 +539,7 @@ dmdgpu_status upload_meta(dmdgpu_ctx* c) {
   HIPCHK(c->d_noxr.ensure(cap));
   HIPCHK(c->d_chunk_lo.ensure(cap));
   HIPCHK(c->d_chunk_hi.ensure(cap));
-  HIPCHK(c->d_totals.ensure(cap * 2));
+  HIPCHK(c->d_totals.ensure(cap * 3));
   HIPCHK(c->d_ops.ensure(cap));
   HIPCHK(c->d_sdots.ensure(cap * dmd::kSolveDots));
   std::vector<dmd::Box> bx(cap);
@@ -573,6 +642,10 @@ void make_order(const dmdgpu_ctx* c, Instance& I) {
 dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const double* sg,
                               const double* x, bool first) {
   const int n = I.n;
+  for (size_t k = 0; q && k < 3 * (size_t)n; k++)
+    if (!std::isfinite(q[k])) return fail(DMDGPU_ERR_ARG, "positions must be finite");
+  for (int k = 0; x && k < n; k++)
+    if (!std::isfinite(x[k])) return fail(DMDGPU_ERR_ARG, "occupancies must be finite");
   if (q) I.h_q.assign(q, q + 3 * (size_t)n);
   if (q || sg) {
     std::vector<double4> qs(n);
@@ -586,7 +659,8 @@ dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const
         qs[i].z = q[3 * o + 2];
       }
       if (sg) {
-        if (!(sg[o] > 0)) return fail(DMDGPU_ERR_ARG, "widths must be positive");
+        if (!(sg[o] > 0) || !std::isfinite(sg[o]))
+          return fail(DMDGPU_ERR_ARG, "widths must be positive and finite");
         qs[i].w = sg[o];
       }
     }
@@ -637,8 +711,19 @@ void recompute_extent(dmdgpu_ctx* c) {
 }
 
 dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
+  // The lists are rebuilt in place: until this build succeeds they are not
+  // valid, and no evaluation may read them.
+  I.have_neigh = false;
+  I.pairs = 0;
   const dmd::CellGrid g = make_grid(c, I);
   const size_t ncell = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
+  if (c->budget) {
+    const size_t grow = sizeof(int) * ((ncell > c->d_cell_count.n ? ncell : 0) +
+                                       (ncell + 1 > c->d_cell_start.n ? ncell + 1 : 0) +
+                                       (ncell > c->d_cell_cursor.n ? ncell : 0));
+    if (device_bytes(c) + grow > c->budget)
+      return fail(DMDGPU_ERR_BUDGET, "the cell grid does not fit the device-memory budget");
+  }
   HIPCHK(c->d_cell_count.ensure(ncell));
   HIPCHK(c->d_cell_start.ensure(ncell + 1));
   HIPCHK(c->d_cell_cursor.ensure(ncell));
@@ -695,6 +780,10 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
     if (spc[i] != 1 && spc[i] != 12)
       return fail(DMDGPU_ERR_SPECIES,
                   "species other than H (1) or Mg (12) at site " + std::to_string(i));
+  for (size_t k = 0; k < 3 * (size_t)n; k++)
+    if (!std::isfinite(q[k])) return fail(DMDGPU_ERR_ARG, "positions must be finite");
+  for (int k = 0; k < 6; k++)
+    if (!std::isfinite(box[k])) return fail(DMDGPU_ERR_ARG, "box bounds must be finite");
   Instance I;
   for (int a = 0; a < 3; a++) {
     I.box.lo[a] = box[a];
@@ -709,19 +798,26 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   I.cap = (n + kChunk - 1) / kChunk * kChunk;
   I.base = place(c, I.cap);
   const size_t need = (size_t)I.base + I.cap;
-  if (c->budget && need * kBytesPerSite > c->budget)
-    return fail(DMDGPU_ERR_BUDGET, "instance does not fit the device-memory budget");
+  // The arena in use after placement, with the context's other allocations,
+  // must fit the budget (ensure_sites also bounds what a growth allocates).
+  if (c->budget) {
+    const size_t held = site_bytes(c, c->d_qs.p ? c->d_qs.n : 0);
+    const size_t other = device_bytes(c) - std::min(device_bytes(c), held);
+    if (other + site_bytes(c, need) > c->budget)
+      return fail(DMDGPU_ERR_BUDGET, "instance does not fit the device-memory budget");
+  }
   CHK(ensure_sites(c, need));
   I.used = true;
   I.h_q.assign(q, q + 3 * (size_t)n);
   make_order(c, I);
   I.h_slot.resize(n);
   std::vector<double> bp(I.cap, 0.0), gp(I.cap, 0.0), z(I.cap, 0.0);
-  std::vector<int> sl(I.cap, 0), zi(I.cap, 0);
+  std::vector<int> sl(I.cap, 0), zi(I.cap, 0), pm(I.cap, -1);
   for (int i = 0; i < n; i++) {
     const int o = I.perm[i];
     I.h_slot[i] = spc[o] == 1 ? 1 : 0;
     sl[i] = I.h_slot[i];
+    pm[i] = o;
     bp[i] = beta[o];
     gp[i] = gamma[o];
   }
@@ -737,6 +833,7 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   }
   const int b = I.base, cp = I.cap;
   HIPCHK(hipMemcpy(c->d_spc.p + b, sl.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_perm.p + b, pm.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_beta.p + b, bp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_gamma.p + b, gp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
   // padding sites of the range: inactive (x = 0), no neighbours, unit width
@@ -816,22 +913,24 @@ dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
     // Both precisions present: the two chains touch disjoint instances, so
     // they run concurrently and fill the device together.
     CHK(eval_chains(c, s, o, phase2));
-    if (!phase2) return DMDGPU_OK;
   } else {
     hipEvent_t a{};
     CHK(begin_timed(c, &a));
     for (int p = 0; p < 2; p++)
       if (any[p]) HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
     CHK(end_timed(c, KID_PHASE1, a));
-    if (!phase2) return DMDGPU_OK;
-    CHK(begin_timed(c, &a));
-    for (int p = 0; p < 2; p++)
-      if (any[p]) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
-    CHK(end_timed(c, KID_PHASE2, a));
+    if (phase2) {
+      CHK(begin_timed(c, &a));
+      for (int p = 0; p < 2; p++)
+        if (any[p]) HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
+      CHK(end_timed(c, KID_PHASE2, a));
+    }
   }
+  // Totals (after phase 1 alone: the table-domain count of the new densities;
+  // the energy sums then still hold the last full evaluation's S0)
   hipEvent_t a{};
   CHK(begin_timed(c, &a));
-  HIPCHK(dmd::launch_totals(c->stream, s, o, (int)c->inst.size(), c->d_chunk_lo.p,
+  HIPCHK(dmd::launch_totals(c->stream, s, c->T, o, (int)c->inst.size(), c->d_chunk_lo.p,
                             c->d_chunk_hi.p));
   CHK(end_timed(c, KID_TOTALS, a));
   return DMDGPU_OK;
@@ -848,6 +947,25 @@ dmdgpu_status eval_all(dmdgpu_ctx* c) {
   for (int k = 0; k < (int)c->inst.size(); k++)
     c->h_prec[k] = c->inst[k].used && c->inst[k].have_neigh ? c->cfg.precision : -1;
   return run_eval(c, true);
+}
+
+// Status of an evaluation from its count of sites whose density left the
+// embedding table's domain (the third per-instance total).
+dmdgpu_status domain_status(double count) {
+  if (count > 0.0)
+    return fail(DMDGPU_ERR_DOMAIN, std::to_string((long long)count) +
+                                       " site(s) have a density at or below -drho of the "
+                                       "embedding table (outside the spline's domain)");
+  return DMDGPU_OK;
+}
+
+// The table-domain status of the last evaluation of instance id.
+dmdgpu_status check_domain(dmdgpu_ctx* c, int id) {
+  double v = 0.0;
+  HIPCHK(hipMemcpyAsync(&v, c->d_totals.p + 3 * id + 2, sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return domain_status(v);
 }
 
 // Copies `width` doubles per site of instance I back to the caller's order.
@@ -876,12 +994,14 @@ dmdgpu_status copy_residual(dmdgpu_ctx* c, int id, double* Rq, double* Rs, doubl
   CHK(d2h_sites(c, I, Rq, c->d_Rq.p, 3));
   CHK(d2h_sites(c, I, Rs, c->d_Rs.p, 1));
   CHK(d2h_sites(c, I, Rx, c->d_Rx.p, 1));
+  double t3[3];
+  HIPCHK(hipMemcpyAsync(t3, c->d_totals.p + 3 * id, sizeof(t3), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (tot) {
-    HIPCHK(hipMemcpyAsync(tot, c->d_totals.p + 2 * id, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    tot[0] = t3[0];
+    tot[1] = t3[1];
   }
-  return DMDGPU_OK;
+  return domain_status(t3[2]);
 }
 
 dmdgpu_status check_primary(dmdgpu_ctx* c) {
@@ -1022,6 +1142,11 @@ void digest_tick(dmdgpu_ctx* c) {
     Solve& S = I.sv;
     const double* d = &c->h_sdots[(size_t)dmd::kSolveDots * k];
     dmd::SolveOp& nx = S.pending;
+    if (d[dmd::kDotDomain] > 0.0 && S.stage != ST_DONE && S.stage != ST_IDLE) {
+      // a density of this tick's evaluation left the embedding table's domain
+      finish(S, DMDGPU_RELAX_DOMAIN);
+      continue;
+    }
     switch (S.stage) {
       case ST_START: {
         S.rnorm = std::sqrt(d[0]);
@@ -1193,11 +1318,11 @@ dmdgpu_status tick_device(dmdgpu_ctx* c) {
   HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
                                  c->d_ops.p, make_vecs(c), c->d_qs.p, o));
   CHK(eval_chains(c, s, o, true));
-  HIPCHK(dmd::launch_totals(c->stream, s, o, ni, c->d_chunk_lo.p, c->d_chunk_hi.p));
+  HIPCHK(dmd::launch_totals(c->stream, s, c->T, o, ni, c->d_chunk_lo.p, c->d_chunk_hi.p));
   HIPCHK(dmd::launch_diff_flux(c->stream, s, c->d_ops.p, make_vecs(c), o, c->d_dnn.p,
                                c->d_dnbr.p));
   HIPCHK(dmd::launch_solver_dots(c->stream, c->extent, ni, c->d_inst.p, c->d_spc.p, c->d_ne.p,
-                                 c->d_ops.p, make_vecs(c), o, c->d_xr.p, c->d_spart.p,
+                                 c->d_ops.p, c->d_prec.p, make_vecs(c), o, c->d_xr.p, c->d_spart.p,
                                  c->d_chunk_lo.p, c->d_chunk_hi.p, c->d_sdots.p));
   HIPCHK(dmd::launch_diff_apply(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ops.p,
                                 make_vecs(c), c->d_xr.p, c->d_ne.p));
@@ -1254,7 +1379,6 @@ dmdgpu_status pc_setup(dmdgpu_ctx* c) {
   for (int k = 0; k < ni; k++)
     if (c->inst[k].used && c->inst[k].sv.pc_setup) c->h_prec[k] = dmd::kFP64;
   CHK(run_eval(c, false));
-  HIPCHK(c->d_diag.ensure((size_t)c->extent * 9));
   const dmd::State s = make_state(c);
   for (int k = 0; k < ni; k++) {
     Instance& I = c->inst[k];
@@ -1408,31 +1532,8 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   drop_graph(c);
-  c->d_uw32.release();
-  c->d_ph32.release();
-  c->d_rh32.release();
-  c->d_pr32.release();
-  c->d_rec32.release();
-  DevBuf<double>* dbs[] = {&c->d_uw,  &c->d_ph,    &c->d_rh,  &c->d_em,       &c->d_ne,
-                           &c->d_pr,
-                           &c->d_xr,  &c->d_beta,  &c->d_gamma, &c->d_rec,    &c->d_V0,
-                           &c->d_rho, &c->d_Rq,    &c->d_Rs,  &c->d_Rx,       &c->d_S0,
-                           &c->d_slot,
-                           &c->d_partials, &c->d_totals, &c->d_sdots, &c->d_spart,
-                           &c->d_Xn,  &c->d_Xt,    &c->d_diag, &c->d_minv};
-  for (auto* b : dbs) b->release();
-  DevBuf<int>* ibs[] = {&c->d_spc,        &c->d_nnb,        &c->d_nbr,        &c->d_cell_of,
-                        &c->d_chunk_np,   &c->d_chunk_npx,  &c->d_own,
-                        &c->d_cell_count, &c->d_cell_start, &c->d_cell_cursor, &c->d_cell_atoms,
-                        &c->d_flags,      &c->d_inst,       &c->d_prec,       &c->d_noxr,
-                        &c->d_chunk_lo,
-                        &c->d_chunk_hi,   &c->d_dnn,        &c->d_dnbr};
-  for (auto* b : ibs) b->release();
-  DevBuf<double4>* vbs[] = {&c->d_qs, &c->d_X, &c->d_R0, &c->d_R0f, &c->d_D, &c->d_Rr, &c->d_Pp};
-  for (auto* b : vbs) b->release();
-  c->d_boxes.release();
-  c->d_ops.release();
-  c->d_pairs.release();
+  for_site_bufs(c, 0, [](auto& buf, size_t, bool) { buf.release(); });
+  for_other_bufs(c, [](auto& buf) { buf.release(); });
   (void)hipStreamSynchronize(c->side);
   c->pin_up.release();
   c->pin_down.release();
@@ -1577,7 +1678,8 @@ dmdgpu_status dmdgpu_mf_rho(dmdgpu_ctx* c, double* h_rho) {
   CHK(check_primary(c));
   HIPCHK(hipSetDevice(c->device));
   CHK(eval_one(c, c->primary, false));
-  return d2h_sites(c, c->inst[c->primary], h_rho, c->d_rho.p, 1);
+  CHK(d2h_sites(c, c->inst[c->primary], h_rho, c->d_rho.p, 1));
+  return check_domain(c, c->primary);
 }
 
 dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* c, double* h_V0, double* h_S0) {
@@ -1585,7 +1687,8 @@ dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* c, double* h_V0, double* h_S0) {
   HIPCHK(hipSetDevice(c->device));
   CHK(eval_one(c, c->primary));
   CHK(d2h_sites(c, c->inst[c->primary], h_V0, c->d_V0.p, 1));
-  return d2h_sites(c, c->inst[c->primary], h_S0, c->d_S0.p, 1);
+  CHK(d2h_sites(c, c->inst[c->primary], h_S0, c->d_S0.p, 1));
+  return check_domain(c, c->primary);
 }
 
 dmdgpu_status dmdgpu_residual(dmdgpu_ctx* c, double* h_Rq, double* h_Rs, double* h_Rx,
@@ -1633,7 +1736,6 @@ dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {
   c->h_prec[id] = dmd::kFP64;
   CHK(run_eval(c, false));
   const Instance& I = c->inst[id];
-  HIPCHK(c->d_diag.ensure((size_t)I.n * 9));
   hipEvent_t a{};
   CHK(begin_timed(c, &a));
   HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, exact, 3, make_state(c), c->T, c->d_rec.p, I.base,
@@ -1646,7 +1748,7 @@ dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {
   for (int i = 0; i < I.n; i++)
     std::copy(tmp.begin() + 9 * (size_t)i, tmp.begin() + 9 * (size_t)(i + 1),
               h_D + 9 * (size_t)I.perm[i]);
-  return DMDGPU_OK;
+  return check_domain(c, id);
 }
 
 dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
@@ -1707,7 +1809,16 @@ dmdgpu_status dmdgpu_set_budget(dmdgpu_ctx* c, int64_t bytes) {
   return DMDGPU_OK;
 }
 
-int64_t dmdgpu_bytes_per_site(void) { return (int64_t)kBytesPerSite; }
+int64_t dmdgpu_bytes_per_site(void) {
+  static dmdgpu_ctx* proto = new dmdgpu_ctx();  // types only: nothing is allocated
+  return (int64_t)((site_bytes(proto, (size_t)1 << 20) + ((size_t)1 << 20) - 1) >> 20);
+}
+
+dmdgpu_status dmdgpu_device_bytes(dmdgpu_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return fail(DMDGPU_ERR_ARG, "null argument");
+  *bytes = (int64_t)device_bytes(c);
+  return DMDGPU_OK;
+}
 
 dmdgpu_status dmdgpu_evaluate(dmdgpu_ctx* c) {
   if (!c) return fail(DMDGPU_ERR_ARG, "null context");

@@ -21,8 +21,10 @@ MAXNEIGH = 250           # src/Macros.hpp:862
 FP64, SPLIT32 = 0, 1     # dmdgpu_set_precision codes
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_IO", 4: "ERR_SPECIES", 5: "ERR_MAXNEIGH",
-          6: "ERR_NONEIGH", 7: "ERR_BOX", 8: "ERR_STATE", 9: "ERR_TABLES", 10: "ERR_BUDGET"}
-RELAX_STATE = {0: "idle", 1: "running", 2: "converged", 3: "maxiter", 4: "stalled"}
+          6: "ERR_NONEIGH", 7: "ERR_BOX", 8: "ERR_STATE", 9: "ERR_TABLES", 10: "ERR_BUDGET",
+          11: "ERR_DOMAIN"}
+RELAX_STATE = {0: "idle", 1: "running", 2: "converged", 3: "maxiter", 4: "stalled",
+               5: "domain"}
 
 # Every symbol include/dmdgpu.h declares.
 EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_stream", "dmdgpu_sync",
@@ -37,7 +39,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
-           "dmdgpu_set_tuning_precision"]
+           "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -126,6 +128,7 @@ def load_library():
     L.dmdgpu_instance_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _lp, _lp]
     L.dmdgpu_set_budget.argtypes = [vp, C.c_int64]
     L.dmdgpu_bytes_per_site.restype = C.c_int64
+    L.dmdgpu_device_bytes.argtypes = [vp, _lp]
     L.dmdgpu_evaluate.argtypes = [vp]
     L.dmdgpu_instance_residual.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp]
     L.dmdgpu_instance_state.argtypes = [vp, C.c_int, _dp, _dp, _dp]
@@ -335,6 +338,12 @@ class Context:
 
     def bytes_per_site(self):
         return int(self.L.dmdgpu_bytes_per_site())
+
+    def device_bytes(self):
+        """Bytes of every device allocation the context holds now."""
+        b = C.c_int64()
+        self._chk(self.L.dmdgpu_device_bytes(self.h, C.byref(b)))
+        return b.value
 
     def evaluate(self):
         self._chk(self.L.dmdgpu_evaluate(self.h))

@@ -45,7 +45,13 @@ typedef enum {
   DMDGPU_ERR_BOX = 7,          /* periodic box edge < 2 r_c (Atom.cpp:230-255) */
   DMDGPU_ERR_STATE = 8,        /* call order: tables / state / neighbours unset */
   DMDGPU_ERR_TABLES = 9,       /* r-tables do not share one (n, dx) grid        */
-  DMDGPU_ERR_BUDGET = 10       /* instance does not fit the device-memory budget */
+  DMDGPU_ERR_BUDGET = 10,      /* instance does not fit the device-memory budget */
+  DMDGPU_ERR_DOMAIN = 11       /* a spline argument left the table's domain: a
+                                  density rho_bar <= -drho of the embedding table,
+                                  where the reference's cubic_spline indexes a
+                                  negative segment (src/Numerical/cubic-spline.cpp:
+                                  84-93), or a non-finite density.  The outputs
+                                  of that evaluation are still written.          */
 } dmdgpu_status;
 
 typedef struct dmdgpu_ctx dmdgpu_ctx;
@@ -121,7 +127,10 @@ dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* ctx, double* h_V0, double* h_S0);
  *   R_x(u)     = sum_i beta_i dV0_i/dx_u + d(-log Z0_u)/dx_u - gamma_u [H] (n)
  * The per-(u, i) terms are evaluate_DV0_i_Dq_u_MgHx (:1823-2098) and
  * evaluate_DV0_i_Dxi_u_adp_MgHx (:2102-2325).  Inactive sites (x < 5e-4) get
- * zeros.  totals[2] = {sum_i S0_i, sum_i V0_i}.  Any output may be NULL. */
+ * zeros.  totals[2] = {sum_i S0_i, sum_i V0_i}.  Any output may be NULL.
+ * DMDGPU_ERR_DOMAIN when a density left the embedding table's domain (this
+ * and dmdgpu_mf_rho, dmdgpu_site_energies, dmdgpu_diag_blocks,
+ * dmdgpu_instance_residual). */
 dmdgpu_status dmdgpu_residual(dmdgpu_ctx* ctx, double* h_Rq, double* h_Rsigma, double* h_Rx,
                               double* h_totals);
 
@@ -204,10 +213,15 @@ dmdgpu_status dmdgpu_remove_instance(dmdgpu_ctx* ctx, int id);
 /* Sites, active ordered pairs of an instance, and the arena extent (sites). */
 dmdgpu_status dmdgpu_instance_info(dmdgpu_ctx* ctx, int id, int* n, int64_t* pairs,
                                    int64_t* arena_sites);
-/* Device-memory budget of the arena in bytes (0 = unlimited), and the bytes one
- * arena site needs (state, lists, records, outputs, solver vectors). */
+/* Device-memory budget in bytes (0 = unlimited): every allocation of the
+ * context -- tables, arena, cell grid, metadata -- must stay within it, also
+ * while the arena grows (an instance that would take it past the budget is
+ * refused with DMDGPU_ERR_BUDGET).  dmdgpu_bytes_per_site: bytes of one arena
+ * site (state, lists, records, outputs, solver vectors); dmdgpu_device_bytes:
+ * the context's device allocations now. */
 dmdgpu_status dmdgpu_set_budget(dmdgpu_ctx* ctx, int64_t bytes);
 int64_t dmdgpu_bytes_per_site(void);
+dmdgpu_status dmdgpu_device_bytes(dmdgpu_ctx* ctx, int64_t* bytes);
 /* One residual evaluation of every instance (default precision). */
 dmdgpu_status dmdgpu_evaluate(dmdgpu_ctx* ctx);
 /* Results of the last evaluation of an instance, in its caller's site order. */
@@ -256,7 +270,9 @@ typedef struct {
 } dmdgpu_relax_opts;
 /* Relaxed unknowns.  Interstitial H at fixed occupancy has a soft (and at low
  * occupancy width-unbounded: sigma_eq^2 ~ 1/(beta k x)) landscape, so the
- * default relaxes the Mg lattice and keeps H sites fixed. */
+ * default relaxes the Mg lattice and keeps H sites fixed.  DMDGPU_DOF_H_Q is
+ * accepted, but the Newton-CG iteration with the residual-norm line search
+ * stalls on it (the H position landscape at fixed occupancy is not convex). */
 enum {
   DMDGPU_DOF_MG_Q = 1,
   DMDGPU_DOF_MG_SIGMA = 2,
@@ -268,7 +284,9 @@ enum {
   DMDGPU_RELAX_RUNNING = 1,
   DMDGPU_RELAX_CONVERGED = 2,
   DMDGPU_RELAX_MAXITER = 3,
-  DMDGPU_RELAX_STALLED = 4  /* line search found no decrease */
+  DMDGPU_RELAX_STALLED = 4,  /* line search found no decrease */
+  DMDGPU_RELAX_DOMAIN = 5    /* an evaluation left the embedding table's domain
+                                (see DMDGPU_ERR_DOMAIN) */
 };
 dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* ctx, int id, const dmdgpu_relax_opts* opts);
 /* One tick of every running instance; *running = instances still running. */
@@ -288,12 +306,17 @@ dmdgpu_status dmdgpu_set_graph(dmdgpu_ctx* ctx, int on);
  * sites are the H sites; their neighbours are the H sites within 2.6 A
  * (list_of_diffusive_neighs, src/Atoms/Neighbors.cpp:139-277; r_cutoff_Eb and
  * max_chemical_neighs = 50, src/Macros.hpp:860,863; parameters declared in
- * src/Variables.hpp:66-90).  The reference has no rate law; this one is the
- * mean-field master equation
- *   xdot_i = sum_j G_ij [x_j (1-x_i) e^{(mu_j-mu_i)/2} - x_i (1-x_j) e^{(mu_i-mu_j)/2}],
+ * src/Variables.hpp:66-90).  The reference has no rate law; this one is a
+ * lattice-gas gradient flow in Onsager form with the ideal site entropy
+ * l(x) = ln(x/(1-x)) and a logarithmic-mean mobility:
+ *   xdot_i = sum_j G_ij [(x_j - x_i) + L(x_i, x_j) (mu_j - mu_i)],
+ *   L(a, b) = (b - a) / (l(b) - l(a))   (a (1 - a) when a = b),
  *   G_ij = D0 / r_ij^2 exp(-Eb (beta_i + beta_j)/2),
  * mu_i = the occupancy residual R_x(i) without the site's own entropy terms
- * (0 on masked sites): Fick's law with D = D0 exp(-Eb/k_B T) when mu = 0.
+ * (in k_B T; 0 on masked sites): Fick's law with D = D0 exp(-Eb/k_B T) when
+ * mu = 0, antisymmetric pair fluxes (mass conservation), and stationary states
+ * with mu + l(x) uniform.  A masked site (x < 5e-4) exchanges by the Fick term
+ * only; which sites are masked is frozen at the start of the step.
  * Advanced by dmdgpu_relax_tick together with relaxations; dmdgpu_relax_status
  * reports the sweeps (newton_steps) and the last max |dx| (rnorm).
  * Errors: DMDGPU_ERR_MAXNEIGH (> 50 diffusive neighbours),

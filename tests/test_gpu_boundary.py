@@ -1,0 +1,160 @@
+"""GPU tests of the C-ABI boundary's guards.
+
+* A neighbour build that fails leaves the instance without lists: no
+  evaluation reads truncated rows (DMDGPU_ERR_STATE until a build succeeds).
+* The device-memory budget bounds every allocation of the context, also while
+  the arena grows: the context's own count and the device's free memory
+  (hipMemGetInfo) both stay within it across admissions.
+* A density at or below -drho of the embedding table (the reference's spline
+  indexes a negative segment there, src/Numerical/cubic-spline.cpp:84-93) is
+  reported as DMDGPU_ERR_DOMAIN exactly when the oracle's densities put a
+  site there; a relaxation that meets it ends in the "domain" state.
+* Non-finite positions, widths or occupancies are argument errors.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import RC
+
+import dmdgpu
+import dmdgpu.lattice as lat
+import dmdgpu.tables as tab
+
+pytestmark = pytest.mark.gpu
+
+ERR_ARG, ERR_MAXNEIGH, ERR_STATE, ERR_BUDGET, ERR_DOMAIN = 1, 5, 8, 10, 11
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def hip_free_bytes():
+    """Free device memory from the HIP runtime the library itself uses."""
+    dmdgpu.load_library()
+    hip = C.CDLL("libamdhip64.so.7")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_failed_neighbor_build_invalidates_lists(gpu):
+    say("[gpu] failed neighbour build -> ERR_STATE")
+    sd = lat.cluster(6, 4, 4, x_h=(0.05, 0.3), seed=3)  # 1152 sites, free
+    gpu.set_state(sd)
+    gpu.neighbors()
+    gpu.residual()  # a good build evaluates
+    ctr = sd["q"].mean(0)
+    dense = ctr + 0.45 * (sd["q"] - ctr)  # ~11x the density: > 250 neighbours
+    gpu.update_state(q=dense)
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.neighbors()
+    assert e.value.code == ERR_MAXNEIGH
+    for call in (gpu.residual, gpu.mf_rho, gpu.site_energies):
+        with pytest.raises(dmdgpu.DmdError) as e:
+            call()
+        assert e.value.code == ERR_STATE
+    gpu.update_state(q=sd["q"])
+    gpu.neighbors()  # lists valid again
+    gpu.residual()
+
+
+def test_budget_bounds_device_memory():
+    say("[gpu] budget bounds every device allocation")
+    budget = 512 * 2 ** 20
+    ctx = dmdgpu.Context(0)
+    try:
+        free0 = hip_free_bytes()
+        ctx.set_budget(budget)
+        ids, refused = [], 0
+        for k in range(40):
+            sd = lat.box_of_size(6000 + 500 * k, seed=k)
+            try:
+                ids.append(ctx.add_instance(sd))
+            except dmdgpu.DmdError as e:
+                assert e.code == ERR_BUDGET
+                refused += 1
+                if refused >= 3:
+                    break
+            held = ctx.device_bytes()
+            used = free0 - hip_free_bytes()
+            say(f"    {len(ids)} instances, {held / 2**20:.1f} MiB held, "
+                f"{used / 2**20:.1f} MiB taken from the device")
+            assert held <= budget
+            # allocator rounding: at most 2 MiB per allocation (about 60 arrays)
+            assert used <= budget + 128 * 2 ** 20
+        assert refused > 0 and len(ids) >= 2
+        ctx.evaluate()  # what was admitted still evaluates
+        for i in ids[:2]:
+            ctx.instance_residual(i)
+    finally:
+        ctx.close()
+
+
+def test_table_domain_status(oracle, tmp_path):
+    """Tables whose Mg density is negative drive rho_bar below -drho of the Mg
+    embedding table: every call that reads an evaluation reports it, as the
+    oracle's densities say it should; the same system with the positive
+    synthetic tables evaluates normally."""
+    from oracle.oracle import Oracle
+    say("[gpu] table-domain status")
+    sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=5)
+    for sign in (1.0, -1.0):
+        folder = str(tmp_path / f"tables{int(sign)}")
+        tab.synthetic_mgh(folder, seed=2)
+        if sign < 0:  # rewrite adpmg.dat with the Mg density negated
+            rng = np.random.default_rng(2)
+            jit = lambda: 1.0 + 0.1 * (2 * rng.random() - 1)
+            fn = {"rho_Mg": tab.smooth(-0.05 * jit(), 1.0 * jit(), 3.2, 6.5)}
+            path = folder + "/adpmg.dat"
+            lines = open(path).read().splitlines()
+            nr = int(lines[1].split()[0])
+            dx = float(lines[1].split()[1])
+            rows = tab.hermite_rows(*fn["rho_Mg"], nr, dx)
+            lines[2:2 + nr] = [" ".join(f"{v:.17e}" for v in r) for r in rows]
+            open(path, "w").write("\n".join(lines) + "\n")
+        O = Oracle(folder)
+        st, nnb, nbr = O.neighbors(sd, 6.5)
+        ref = O.fact_residual(O.system(sd, nnb, nbr))
+        # embedding-table step of each site's species (slot 0 Mg, 1 H)
+        dxe = np.where(sd["species"] == lat.H, O.spline_size(1, 1)[1], O.spline_size(1, 0)[1])
+        act = sd["x"] >= 5e-4
+        expect = bool((act & (ref["mf_rho"] <= -dxe)).any())
+        assert expect == (sign < 0)
+        ctx = dmdgpu.Context(0, folder, 6.5)
+        try:
+            ctx.set_state(sd)
+            ctx.neighbors()
+            if not expect:
+                ctx.residual()
+                continue
+            for call in (ctx.residual, ctx.mf_rho, ctx.site_energies, ctx.diag_blocks):
+                with pytest.raises(dmdgpu.DmdError) as e:
+                    call()
+                assert e.value.code == ERR_DOMAIN
+            ctx.relax(dmdgpu.relax_opts(atol=1e-8, max_newton=5))
+            assert ctx.relax_status(0)["state"] == "domain"
+        finally:
+            ctx.close()
+
+
+def test_nonfinite_inputs_rejected(gpu):
+    say("[gpu] non-finite inputs")
+    sd = lat.cluster(2, 2, 1, x_h=(0.05, 0.3), seed=1)
+    for key in ("q", "sigma", "x"):
+        bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sd.items()}
+        bad[key].flat[3] = np.nan
+        with pytest.raises(dmdgpu.DmdError) as e:
+            gpu.set_state(bad)
+        assert e.value.code == ERR_ARG, key
+    gpu.set_state(sd)
+    gpu.neighbors()
+    q = sd["q"].copy()
+    q[0, 0] = np.inf
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.update_state(q=q)
+    assert e.value.code == ERR_ARG

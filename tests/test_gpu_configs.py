@@ -1,0 +1,222 @@
+"""GPU parity on the full workloads of the benchmark configurations.
+
+* C3: the 54,000-site Mg-H system (hcp 20 x 15 x 15, 36,000 tetrahedral H)
+  under the degree-5 pair rules (5: 27-point tensor Gauss-Hermite; 44: the
+  reference's 44-point 6D rule, src/Numerical/Quadrature-Hermitian-5th.cpp:
+  31-157): FP64 within 1e-10 of the oracle, split precision within the split
+  bars of the oracle.
+* C4: the 64 nanowires (periodic along c) and free spheres of the batched
+  workload, all resident and evaluated in one launch per kernel, in FP64 and
+  in split precision: every instance against the oracle, and a subset bitwise
+  equal to the same instances evaluated alone.
+* C5: a served mix with a request of more than 150,000 sites and nanowires,
+  under a budget that makes requests wait: every result checked against the
+  oracle (relaxations: the oracle's residual of the final state; diffusion
+  steps: the oracle's own backward-Euler step), two bitwise equal to the
+  request served alone.
+* C1: the 500-site pure-Mg crystal relaxed to |R|_2 <= 1e-8, confirmed by the
+  oracle's residual of the final state.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import RC
+
+import dmdgpu
+import dmdgpu.lattice as lat
+from dmdgpu.serve import Request, serve
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+SPLIT_TOL = {"Rq": 1e-6, "Rs": 1e-6, "Rx": 1e-7, "S0_total": 1e-8}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def errors(out, ref):
+    e = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
+    e["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"])
+    return e
+
+
+def oracle_of(oracle, sysd, rule=3):
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    assert st == 0
+    return oracle.fact_residual(oracle.system(sysd, nnb, nbr, rule))
+
+
+@pytest.fixture(scope="module")
+def c2():
+    sd = lat.hcp_mgh(20, 15, 15, x_h=(0.0, 0.1), seed=2024)
+    assert sd["n"] == 54000
+    return sd
+
+
+@pytest.mark.parametrize("rule", [5, 44])
+def test_c3_fifth_order_full_size(gpu, oracle, c2, rule):
+    say(f"[gpu] C3 rule {rule} at 54,000 sites")
+    ref = oracle_of(oracle, c2, rule)
+    gpu.set_state(c2)
+    gpu.neighbors()
+    gpu.set_rule(rule)
+    try:
+        for tun in ((8, 64, 0), (8, 64, 1)):
+            gpu.set_tuning(*tun)
+            gpu.set_precision("fp64")
+            e = errors(gpu.residual(), ref)
+            say(f"    fp64 {tun}: {e}")
+            assert max(e.values()) < TOL, e
+            gpu.set_precision("split32")
+            e = errors(gpu.residual(), ref)
+            say(f"    split32 {tun}: {e}")
+            for k, tol in SPLIT_TOL.items():
+                assert e[k] < tol, (k, e[k])
+    finally:
+        gpu.set_precision("fp64")
+        gpu.set_rule(3)
+        gpu.set_tuning(4, 4, 1)
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return lat.mixed_batch(64, seed=2024)
+
+
+def test_c4_batch_of_64_against_oracle(oracle, c4):
+    sizes = [int(s["n"]) for s in c4]
+    say(f"[gpu] C4 64 instances, {min(sizes)}..{max(sizes)} sites, {sum(sizes)} total")
+    assert all(s["periodic"].tolist() == [0, 0, 1] for s in c4[0::2])  # nanowires
+    assert all(s["periodic"].tolist() == [0, 0, 0] for s in c4[1::2])  # spheres
+    ctx = dmdgpu.Context(0)
+    try:
+        ctx.set_tuning_precision("fp64", 8, 64, 0)
+        ctx.set_tuning_precision("split32", 8, 64, 1)
+        ids = [ctx.add_instance(sd) for sd in c4]
+        res = {}
+        for prec in ("fp64", "split32"):
+            ctx.set_precision(prec)
+            ctx.evaluate()  # every instance, one launch per kernel
+            res[prec] = [ctx.instance_residual(i) for i in ids]
+        worst = {"fp64": 0.0, "split32": {}}
+        for k, sd in enumerate(c4):
+            ref = oracle_of(oracle, sd)
+            e = errors(res["fp64"][k], ref)
+            worst["fp64"] = max(worst["fp64"], max(e.values()))
+            assert max(e.values()) < TOL, (k, e)
+            e = errors(res["split32"][k], ref)
+            for key, tol in SPLIT_TOL.items():
+                worst["split32"][key] = max(worst["split32"].get(key, 0.0), e[key])
+                assert e[key] < tol, (k, key, e[key])
+        say(f"    worst errors vs oracle: {worst}")
+        # bitwise: alone equals in the batch (both precisions)
+        sub = [0, 1, int(np.argmax(sizes)), int(np.argmin(sizes))]
+        for prec in ("fp64", "split32"):
+            ctx.set_precision(prec)
+            alone = dmdgpu.Context(0)
+            try:
+                alone.set_tuning_precision("fp64", 8, 64, 0)
+                alone.set_tuning_precision("split32", 8, 64, 1)
+                alone.set_precision(prec)
+                for k in sub:
+                    alone.set_state(c4[k])
+                    alone.neighbors()
+                    a = alone.residual()
+                    for key in ("Rq", "Rs", "Rx"):
+                        assert np.array_equal(a[key], res[prec][k][key]), (prec, k, key)
+                    assert a["S0_total"] == res[prec][k]["S0_total"]
+            finally:
+                alone.close()
+    finally:
+        ctx.close()
+
+
+def c5_requests():
+    mix = [("relax", lat.box_of_size(155_000, seed=21)),
+           ("diffuse", lat.wire_of_size(12_000, seed=22)),
+           ("relax", lat.wire_of_size(6_000, seed=23)),
+           ("diffuse", lat.box_of_size(40_000, seed=24)),
+           ("relax", lat.box_of_size(2_000, seed=25)),
+           ("diffuse", lat.wire_of_size(900, seed=26)),
+           ("relax", lat.box_of_size(600, seed=27))]
+    arrivals = [0, 0, 2, 3, 5, 8, 9]  # ticks
+    return [Request(k, kind, sd, float(a)) for k, ((kind, sd), a) in
+            enumerate(zip(mix, arrivals))]
+
+
+def test_c5_served_mix_against_oracle(oracle):
+    reqs = c5_requests()
+    sizes = [int(r.system["n"]) for r in reqs]
+    assert max(sizes) >= 150_000
+    say(f"[gpu] C5 serving {len(reqs)} requests, sizes {sizes}")
+    ctx = dmdgpu.Context(0)
+    try:
+        budget = ctx.bytes_per_site() * (max(sizes) + 30_000)
+        res, span = serve(ctx, reqs, budget_bytes=budget, mixed=True, clock="ticks")
+        say(f"    served in {span:.0f} ticks")
+        assert any(r.admitted > r.arrival for r in res)  # some request waited
+        for req, r in zip(reqs, res):
+            sd = req.system
+            say(f"    {req.kind} n={r.sites}: {r.state} it={r.iterations} evals={r.evals}")
+            assert r.state == "converged"
+            fin = dict(sd)
+            fin["q"], fin["sigma"], fin["x"] = r.q, r.sigma, r.x
+            if req.kind == "relax":
+                ref = oracle_of(oracle, fin)
+                mg = (sd["species"] == lat.MG) & (sd["x"] >= 5e-4)
+                rn = np.sqrt((ref["Rq"][mg] ** 2).sum() + (ref["Rs"][mg] ** 2).sum())
+                say(f"        oracle |R| of the final state {rn:.3e}")
+                assert rn < 1e-7, rn
+                assert np.array_equal(r.x, sd["x"])  # occupancies fixed
+            else:
+                st, nnb, nbr = oracle.neighbors(sd, RC)
+                x, sweeps, dmax = oracle.diffusion_step(sd, nnb, nbr, tol=1e-12)
+                d = float(np.abs(r.x - x).max())
+                say(f"        oracle step: {sweeps} sweeps, max |x_gpu - x_oracle| {d:.3e}")
+                assert d < 1e-9, d
+                assert abs(r.x.sum() - sd["x"].sum()) < 1e-9 * sd["n"]  # H conserved
+                assert np.array_equal(r.q, sd["q"])  # positions fixed
+        for k in (0, 1):  # the largest request and a nanowire, served alone
+            req = reqs[k]
+            alone, _ = serve(ctx, [Request(req.rid, req.kind, req.system, 0.0)], mixed=True,
+                             clock="ticks")
+            a = alone[0]
+            assert (a.state, a.iterations, a.evals) == (res[k].state, res[k].iterations,
+                                                        res[k].evals)
+            assert np.array_equal(a.q, res[k].q)
+            assert np.array_equal(a.sigma, res[k].sigma)
+            assert np.array_equal(a.x, res[k].x)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_c1_crystal_relaxation(gpu, oracle, mixed):
+    say(f"[gpu] C1 500-site Mg crystal relaxation, mixed={mixed}")
+    sd = lat.hcp_mgh(5, 5, 5, with_h=False, seed=1)
+    assert sd["n"] == 500 and sd["periodic"].tolist() == [1, 1, 1]
+    gpu.set_state(sd)
+    gpu.neighbors()
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, mixed=mixed,
+                             precond=60)
+    gpu.relax(opts)
+    st = gpu.relax_status(0)
+    say(f"    {st}")
+    assert st["state"] == "converged" and st["rnorm"] <= 1e-8
+    q, sg, _ = gpu.instance_state(0)
+    fin = dict(sd)
+    fin["q"], fin["sigma"] = q, sg
+    ref = oracle_of(oracle, fin)
+    rn = np.sqrt((ref["Rq"] ** 2).sum() + (ref["Rs"] ** 2).sum())
+    say(f"    oracle |R| of the final state {rn:.3e}")
+    assert rn < 1e-7, rn
+    assert np.abs(q - sd["q"]).max() > 1e-3  # the noise of 0.01 A relaxed away

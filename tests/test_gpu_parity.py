@@ -34,6 +34,9 @@ CASES = {
     "hcp_mgh_5x3x3": lambda: lat.hcp_mgh(5, 3, 3, x_h=(0.0, 0.3), seed=9),
     "mg_5x5x5_c1": lambda: lat.hcp_mgh(5, 5, 5, with_h=False, seed=1),
     "dump_x5": lambda: lat.read_dump(os.path.join(GOLDEN, "Mg-hcp-cube-x5-x5-x5.dump")),
+    # mixed periodicity: periodic along c only, and free on every axis
+    "nanowire_r15": lambda: lat.nanowire(15.0, seed=31),
+    "sphere_r12": lambda: lat.sphere(12.0, seed=32),
 }
 
 
@@ -106,18 +109,22 @@ def test_tuning_rejects_unsupported_values(gpu):
 SPLIT_TOL = {"Rq": 1e-6, "Rs": 1e-6, "Rx": 1e-7, "mf_rho": 1e-7, "S0_total": 1e-8}
 
 
-@pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "mg_5x5x5_c1", "cluster_mgh"])
-@pytest.mark.parametrize("un", [0, 1])
-def test_split32_against_fp64(gpu, oracle, case, un):
-    say(f"[gpu] split32 {case} unroll={un}")
+@pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "mg_5x5x5_c1", "cluster_mgh", "nanowire_r15",
+                                  "sphere_r12"])
+@pytest.mark.parametrize("tun", [(4, 4, 0), (4, 4, 1), (8, 64, 1), (8, 32, 0)])
+def test_split32_against_fp64(gpu, oracle, case, tun):
+    """Split precision against the FP64 oracle at the autotuner's candidates
+    (the bench picks 8, 64, 1 for the split path)."""
+    say(f"[gpu] split32 {case} tuning={tun}")
     sysd = CASES[case]()
-    _, _, ref = run_both(gpu, oracle, sysd, 4, 4, un, "split32")
+    _, _, ref = run_both(gpu, oracle, sysd, *tun, "split32")
     out = gpu.residual()
     err = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
     err["mf_rho"] = rel(gpu.mf_rho(), ref["mf_rho"])
     err["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"])
     say(f"    split32 errors {err}")
     gpu.set_precision("fp64")
+    gpu.set_tuning(4, 4, 1)
     for k, tol in SPLIT_TOL.items():
         assert err[k] < tol, (k, err[k])
 
