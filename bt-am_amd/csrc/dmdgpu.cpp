@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "dmd_kernels.h"
+#include "dmd_noalloc.h"
 #include "dmd_solver.h"
 
 namespace {
@@ -1506,6 +1507,8 @@ dmdgpu_status dmdgpu_create(int device, dmdgpu_ctx** out) {
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(DMDGPU_ERR_ARG, "no such HIP device");
   HIPCHK(hipSetDevice(device));
+  // device address of this device's scratch array (allocation-free entry point)
+  HIPCHK(dmd::noalloc_init(device));
   dmdgpu_ctx* c = new dmdgpu_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1754,6 +1757,36 @@ dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {
 dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {
   CHK(check_primary(c));
   return diag_blocks_of(c, c->primary, exact, h_D);
+}
+
+dmdgpu_status dmdgpu_diag_blocks_noalloc(dmdgpu_ctx* c, int exact, double* d_D) {
+  CHK(check_primary(c));
+  if (!d_D) return fail(DMDGPU_ERR_ARG, "null output");
+  if (exact != 0 && exact != 1) return fail(DMDGPU_ERR_ARG, "exact must be 0 or 1");
+  const int id = c->primary;
+  const Instance& I = c->inst[id];
+  if (I.n > dmd::kScratchSites)
+    return fail(DMDGPU_ERR_ARG, "dmdgpu_diag_blocks_noalloc takes at most " +
+                                    std::to_string(dmd::kScratchSites) + " sites (the instance has " +
+                                    std::to_string(I.n) + ")");
+  HIPCHK(hipSetDevice(c->device));
+  double* scr = dmd::noalloc_scratch(c->device);
+  if (!scr) return fail(DMDGPU_ERR_STATE, "device scratch not set up");
+  // Everything below runs on memory the context already holds: the metadata
+  // and per-site arrays exist since dmdgpu_neighbors, and no timing events
+  // are created.
+  if (c->meta_dirty) return fail(DMDGPU_ERR_STATE, "dmdgpu_neighbors has not been called");
+  const bool timing = c->timing;
+  c->timing = false;
+  std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+  c->h_prec[id] = dmd::kFP64;
+  dmdgpu_status st = run_eval(c, false);
+  c->timing = timing;
+  CHK(st);
+  HIPCHK(dmd::launch_diag(c->stream, c->cfg.rule, exact, 3, make_state(c), c->T, c->d_rec.p, I.base,
+                          I.n, scr));
+  HIPCHK(dmd::launch_scatter_blocks(c->stream, I.base, I.n, c->d_perm.p, scr, d_D));
+  return check_domain(c, id);
 }
 
 dmdgpu_status dmdgpu_instance_diag_blocks(dmdgpu_ctx* c, int id, int exact, double* h_D) {

@@ -39,7 +39,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
-           "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes"]
+           "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -145,6 +145,7 @@ def load_library():
     L.dmdgpu_jvp.argtypes = [vp, _dp, _dp, _dp]
     L.dmdgpu_diag_blocks.argtypes = [vp, C.c_int, _dp]
     L.dmdgpu_instance_diag_blocks.argtypes = [vp, C.c_int, C.c_int, _dp]
+    L.dmdgpu_diag_blocks_noalloc.argtypes = [vp, C.c_int, vp]
     L.dmdgpu_set_rule.argtypes = [vp, C.c_int]
     L.dmdgpu_set_tuning_precision.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     _lib = L
@@ -269,6 +270,13 @@ class Context:
         D = np.zeros((self.n, 9))
         self._chk(self.L.dmdgpu_diag_blocks(self.h, 1 if exact else 0, _d(D)))
         return D.reshape(self.n, 3, 3)
+
+    def diag_blocks_noalloc(self, exact, device_ptr):
+        """The diagonal blocks into device memory at device_ptr (n x 9 doubles,
+        allocated by the caller) without any device allocation by the
+        library (dmdgpu_diag_blocks_noalloc)."""
+        self._chk(self.L.dmdgpu_diag_blocks_noalloc(self.h, 1 if exact else 0,
+                                                    C.c_void_p(device_ptr)))
 
     def instance_diag_blocks(self, iid, exact=False):
         """Diagonal blocks of a resident instance at its current state, (n, 3, 3)."""

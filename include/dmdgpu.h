@@ -162,6 +162,18 @@ dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* ctx, int exact, double* h_D);
 /* The same blocks for resident instance `id` at its current state (caller's
  * site order); DMDGPU_ERR_STATE while a solve of that instance is running. */
 dmdgpu_status dmdgpu_instance_diag_blocks(dmdgpu_ctx* ctx, int id, int exact, double* h_D);
+/* The blocks of dmdgpu_diag_blocks written into DEVICE memory d_D (n x 9
+ * doubles, caller's site order), bit for bit the same values, with no device
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The blocks are staged in a fixed device array of 65,536
+ * sites' blocks (4.5 MiB, held by each device the library runs on until the
+ * process ends); an instance with more sites is refused with DMDGPU_ERR_ARG.
+ * The array is shared by every call on that device and keeps the last call's
+ * contents: calls that overlap on one device (from different contexts or
+ * threads) race.  Needs dmdgpu_neighbors first; returns after the stream has
+ * finished (with DMDGPU_ERR_DOMAIN as dmdgpu_diag_blocks). */
+dmdgpu_status dmdgpu_diag_blocks_noalloc(dmdgpu_ctx* ctx, int exact, double* d_D);
 
 /* Kernel tuning, for both precisions: k_phase1 = lanes of a wavefront that
  * share one site in phase 1 (4, 8 or 16); k_phase2 = blocks per 256-site chunk
