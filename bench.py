@@ -187,13 +187,19 @@ def diag_leg(ctx, reps=5):
             "call_ms": dt * 1e3}
 
 
+RULE_POINTS = {3: 12, 5: 27, 44: 44, 6: 6}
+
+
 def rule5_leg(ctx, args, pqe3, rule=5):
-    """The same workload under a degree-5 pair rule (pair and coincident
-    terms; distinct triplets keep the 3rd-order rule): rule 5, the 27-point
-    tensor Gauss-Hermite rule, or rule 44, the reference's 44-point 6D rule.
-    FP64 and split-precision time per residual and the split error against
-    FP64.  PQE counts use 13 + (27 or 44) points per ordered pair per phase."""
-    npt = 13 + (27 if rule == 5 else 44)
+    """The same workload under another pair rule (pair and coincident terms;
+    distinct triplets keep the 3rd-order rule): rule 5, the 27-point tensor
+    Gauss-Hermite rule, rule 44, the reference's 44-point 6D rule, or rule 6,
+    the 6-point Stroud rule on the relative coordinate.  FP64 and
+    split-precision time per residual and the split error against FP64; for
+    rule 6 also its difference from the reference's 12-point rule (FP64).  PQE
+    counts use 13 + (points of the rule) per ordered pair per phase."""
+    npt = 13 + RULE_POINTS[rule]
+    base = ctx.residual() if rule == 6 else None
     ctx.set_rule(rule)
     out = {}
     res = {}
@@ -220,6 +226,11 @@ def rule5_leg(ctx, args, pqe3, rule=5):
     err["S0_total"] = float(abs(res["split32"]["S0_total"] - ref["S0_total"]) /
                             abs(ref["S0_total"]))
     out["split32"]["max_rel_err_vs_fp64"] = err
+    if base is not None:
+        d = {k: float(np.abs(ref[k] - base[k]).max() / np.abs(base[k]).max())
+             for k in ("Rq", "Rs", "Rx")}
+        d["S0_total"] = float(abs(ref["S0_total"] - base["S0_total"]) / abs(base["S0_total"]))
+        out["max_rel_diff_vs_reference_12_point_rule"] = d
     return out
 
 
@@ -476,6 +487,8 @@ def main():
     log(f"rule 5: {rule5}")
     rule44 = rule5_leg(ctx, args, pqe, rule=44)
     log(f"rule 44: {rule44}")
+    rule6 = rule5_leg(ctx, args, pqe, rule=6)
+    log(f"rule 6: {rule6}")
 
     host = host_leg(ctx, sysd, pqe)
     log(f"host buffers: {host}")
@@ -516,6 +529,7 @@ def main():
         "split32": split,
         "fifth_order_rule": rule5,
         "reference_44_point_rule": rule44,
+        "relative_6_point_rule": rule6,
         "host_buffers_pcie_inclusive": host,
         "diag_blocks": diag,
         "solves": solves,

@@ -195,6 +195,28 @@ template <> struct PairRule<5> {
   }
 };
 
+// PairRule<6>: the 6-point Stroud rule on the relative coordinate
+// r ~ N(r_ij, (sigma_i^2 + sigma_j^2) I): r +- sqrt(3) sigma_rel e_l, weight
+// 1/6 (degree 3).  With sigma_i = sigma_j it is PairRule<3> (both put
+// r +- sqrt(6) sigma e_l); with unequal widths it matches the exact fourth
+// moment 3 sigma_rel^4 along an axis, which PairRule<3> does not.  The point
+// set is symmetric, so one evaluation serves both ends.
+template <> struct PairRule<6> {
+  static constexpr int NP = 6;
+  template <typename P>
+  DEV static double point(const PairGeo<P>& geo, const Tables& T, int t, P si, P sj, P* p, int& m,
+                          P& off, P& ir, P* dsi, P* dsj) {
+    const int l = t >> 1;
+    const P x = (t & 1) ? P(-1.7320508075688772) : P(1.7320508075688772);
+    const P srel = sqrt(si * si + sj * sj);
+    geo.point(T, l, x * srel, p, m, off, ir);
+    const P f = x * si / srel, fj = x * sj / srel;
+    dsi[0] = l == 0 ? f : P(0); dsi[1] = l == 1 ? f : P(0); dsi[2] = l == 2 ? f : P(0);
+    dsj[0] = l == 0 ? fj : P(0); dsj[1] = l == 1 ? fj : P(0); dsj[2] = l == 2 ? fj : P(0);
+    return 1.0 / 6.0;
+  }
+};
+
 // PairRule<44>: the reference's 6D degree-5 rule for a pair
 // (src/Numerical/Quadrature-Hermitian-5th.cpp:31-157; defined there, not used
 // by its evaluators): 22 generators zeta in R^6 and their negatives, site

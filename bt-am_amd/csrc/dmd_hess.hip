@@ -387,7 +387,9 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
 hipError_t launch_diag(hipStream_t st, int rule, int exact, int spc_mask, const State& s,
                        const Tables& T, const double* rec, int base, int n, double* D) {
   if (n <= 0) return hipSuccess;
-  if (rule == 5)
+  if (rule == 6)
+    hipLaunchKernelGGL(k_diag<6>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, spc_mask, D);
+  else if (rule == 5)
     hipLaunchKernelGGL(k_diag<5>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, spc_mask, D);
   else if (rule == 44)
     hipLaunchKernelGGL(k_diag<44>, dim3(n), dim3(64), 0, st, s, T, rec, base, exact, spc_mask, D);

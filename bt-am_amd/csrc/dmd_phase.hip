@@ -793,6 +793,7 @@ template <typename P>
 static hipError_t phase1_r(hipStream_t st, const PhaseConfig& cfg, const State& s,
                            const Tables& T, const Outputs& o) {
   const int k1 = cfg.k1[PrecCode<P>::v];
+  if (cfg.rule == 6) return phase1_p<P, 6>(st, k1, s, T, o);
   if (cfg.rule == 5) return phase1_p<P, 5>(st, k1, s, T, o);
   if (cfg.rule == 44) return phase1_p<P, 44>(st, k1, s, T, o);
   return phase1_p<P, 3>(st, k1, s, T, o);
@@ -803,6 +804,8 @@ static hipError_t phase2_r(hipStream_t st, const PhaseConfig& cfg, const State& 
                            const Tables& T, const Outputs& o) {
   const int k2 = cfg.k2[PrecCode<P>::v];
   const bool un = cfg.unroll2[PrecCode<P>::v] != 0;
+  if (cfg.rule == 6)
+    return un ? phase2_pu<P, true, 6>(st, k2, s, T, o) : phase2_pu<P, false, 6>(st, k2, s, T, o);
   if (cfg.rule == 5)
     return un ? phase2_pu<P, true, 5>(st, k2, s, T, o) : phase2_pu<P, false, 5>(st, k2, s, T, o);
   if (cfg.rule == 44)

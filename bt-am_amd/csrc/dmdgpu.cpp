@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "dmd_hooks.h"
 #include "dmd_kernels.h"
 #include "dmd_noalloc.h"
 #include "dmd_solver.h"
@@ -183,6 +184,8 @@ struct dmdgpu_ctx {
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<double> d_diag;  // diagonal Hessian blocks (n x 9)
+  DevBuf<int> d_hookq;     // per-(u, i) queries (2 per query)
+  DevBuf<double> d_hookv;  // their results (5 per query)
   DevBuf<double> d_minv;  // arena x 10: block-Jacobi preconditioner of the solver
   DevBuf<long long> d_pairs;
   std::vector<int> h_prec;
@@ -481,7 +484,7 @@ void for_other_bufs(dmdgpu_ctx* c, F&& f) {
   f(c->d_uw32); f(c->d_ph32); f(c->d_rh32); f(c->d_pr32);
   f(c->d_cell_count); f(c->d_cell_start); f(c->d_cell_cursor); f(c->d_flags);
   f(c->d_boxes); f(c->d_prec); f(c->d_noxr); f(c->d_chunk_lo); f(c->d_chunk_hi);
-  f(c->d_totals); f(c->d_ops); f(c->d_sdots);
+  f(c->d_totals); f(c->d_ops); f(c->d_sdots); f(c->d_hookq); f(c->d_hookv);
 }
 
 // Every device allocation of the context.
@@ -1622,8 +1625,8 @@ dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* c, int precision) {
 }
 
 dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* c, int order) {
-  if (!c || (order != 3 && order != 5 && order != 44))
-    return fail(DMDGPU_ERR_ARG, "rule must be 3, 5 or 44");
+  if (!c || (order != 3 && order != 5 && order != 44 && order != 6))
+    return fail(DMDGPU_ERR_ARG, "rule must be 3, 5, 44 or 6");
   c->cfg.rule = order;
   return DMDGPU_OK;
 }
@@ -1752,6 +1755,56 @@ dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {
     std::copy(tmp.begin() + 9 * (size_t)i, tmp.begin() + 9 * (size_t)(i + 1),
               h_D + 9 * (size_t)I.perm[i]);
   return check_domain(c, id);
+}
+
+// Per-(u, i) derivatives of V0_i for m queries in the caller's site numbering
+// of the one-instance state: FP64 phase 1, then one wavefront per query.
+dmdgpu_status site_hooks(dmdgpu_ctx* c, int m, const int32_t* hu, const int32_t* hi,
+                         double* out4, double* outx) {
+  CHK(check_primary(c));
+  if (m < 0 || (m > 0 && (!hu || !hi))) return fail(DMDGPU_ERR_ARG, "bad query list");
+  if (m == 0) return DMDGPU_OK;
+  const int id = c->primary;
+  const Instance& I = c->inst[id];
+  std::vector<int> q(2 * (size_t)m);
+  for (int k = 0; k < m; k++) {
+    if (hu[k] < 0 || hu[k] >= I.n || hi[k] < 0 || hi[k] >= I.n)
+      return fail(DMDGPU_ERR_ARG, "site index out of range in query " + std::to_string(k));
+    q[k] = I.base + I.inv[hu[k]];
+    q[m + k] = I.base + I.inv[hi[k]];
+  }
+  HIPCHK(hipSetDevice(c->device));
+  std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+  c->h_prec[id] = dmd::kFP64;
+  CHK(run_eval(c, false));
+  HIPCHK(c->d_hookq.ensure(2 * (size_t)m));
+  HIPCHK(c->d_hookv.ensure(5 * (size_t)m));
+  HIPCHK(hipMemcpyAsync(c->d_hookq.p, q.data(), sizeof(int) * q.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(dmd::launch_site_dv0(c->stream, c->cfg.rule, make_state(c), c->T, c->d_rec.p, m,
+                              c->d_hookq.p, c->d_hookq.p + m, c->d_hookv.p));
+  std::vector<double> v(5 * (size_t)m);
+  HIPCHK(hipMemcpyAsync(v.data(), c->d_hookv.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < m; k++) {
+    if (out4)
+      for (int a = 0; a < 4; a++) out4[4 * (size_t)k + a] = v[5 * (size_t)k + a];
+    if (outx) outx[k] = v[5 * (size_t)k + 4];
+  }
+  return check_domain(c, id);
+}
+
+dmdgpu_status dmdgpu_site_DV0_Dq(dmdgpu_ctx* c, int m, const int32_t* h_u, const int32_t* h_i,
+                                 double* h_out) {
+  if (!h_out && m > 0) return fail(DMDGPU_ERR_ARG, "null output");
+  return site_hooks(c, m, h_u, h_i, h_out, nullptr);
+}
+
+dmdgpu_status dmdgpu_site_DV0_Dxi(dmdgpu_ctx* c, int m, const int32_t* h_u, const int32_t* h_i,
+                                  double* h_out) {
+  if (!h_out && m > 0) return fail(DMDGPU_ERR_ARG, "null output");
+  return site_hooks(c, m, h_u, h_i, nullptr, h_out);
 }
 
 dmdgpu_status dmdgpu_diag_blocks(dmdgpu_ctx* c, int exact, double* h_D) {

@@ -39,7 +39,8 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_relax_start", "dmdgpu_relax_tick", "dmdgpu_relax_status", "dmdgpu_relax",
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
-           "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc"]
+           "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc",
+           "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -146,6 +147,8 @@ def load_library():
     L.dmdgpu_diag_blocks.argtypes = [vp, C.c_int, _dp]
     L.dmdgpu_instance_diag_blocks.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.dmdgpu_diag_blocks_noalloc.argtypes = [vp, C.c_int, vp]
+    L.dmdgpu_site_DV0_Dq.argtypes = [vp, C.c_int, _ip, _ip, _dp]
+    L.dmdgpu_site_DV0_Dxi.argtypes = [vp, C.c_int, _ip, _ip, _dp]
     L.dmdgpu_set_rule.argtypes = [vp, C.c_int]
     L.dmdgpu_set_tuning_precision.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     _lib = L
@@ -264,6 +267,22 @@ class Context:
         tot = np.zeros(2)
         self._chk(self.L.dmdgpu_residual(self.h, _d(Rq), _d(Rs), _d(Rx), _d(tot)))
         return {"Rq": Rq, "Rs": Rs, "Rx": Rx, "S0_total": tot[0], "V0_total": tot[1]}
+
+    def site_DV0_Dq(self, u, i):
+        """[dV0_i/dq_u (3), dV0_i/dsigma_u] per query (u, i): (m, 4)."""
+        u = np.ascontiguousarray(np.atleast_1d(u), dtype=np.int32)
+        i = np.ascontiguousarray(np.atleast_1d(i), dtype=np.int32)
+        out = np.zeros((len(u), 4))
+        self._chk(self.L.dmdgpu_site_DV0_Dq(self.h, len(u), _i(u), _i(i), _d(out)))
+        return out
+
+    def site_DV0_Dxi(self, u, i):
+        """dV0_i/dx_u per query (u, i) in the reference's form: (m,)."""
+        u = np.ascontiguousarray(np.atleast_1d(u), dtype=np.int32)
+        i = np.ascontiguousarray(np.atleast_1d(i), dtype=np.int32)
+        out = np.zeros(len(u))
+        self._chk(self.L.dmdgpu_site_DV0_Dxi(self.h, len(u), _i(u), _i(i), _d(out)))
+        return out
 
     def diag_blocks(self, exact=False):
         """Diagonal 3x3 blocks of dR_q/dq, (n, 3, 3), FP64 (see dmdgpu_diag_blocks)."""

@@ -119,6 +119,23 @@ dmdgpu_status dmdgpu_mf_rho(dmdgpu_ctx* ctx, double* h_mf_rho);
  * evaluate_S0_i_adp_MgHx (:957-1142) with mf_rho from the same call. */
 dmdgpu_status dmdgpu_site_energies(dmdgpu_ctx* ctx, double* h_V0, double* h_S0);
 
+/* Per-(u, i) parity hooks at the reference's own granularity: for m queries
+ * (h_u[k], h_i[k]) in the caller's site numbering,
+ *   dmdgpu_site_DV0_Dq:  h_out[4 k .. 4 k + 3] = [dV0_i/dq_u (3), dV0_i/dsigma_u],
+ *                        evaluate_DV0_i_Dq_u_MgHx (MgHx-mf-V-bulk.cpp:1823-2098);
+ *   dmdgpu_site_DV0_Dxi: h_out[k] = dV0_i/dx_u,
+ *                        evaluate_DV0_i_Dxi_u_adp_MgHx (:2102-2325), with its
+ *                        embedding form for an Mg centre i (F'(rho_i) drho_i/dx_u
+ *                        without the factor x_i, and 0 for u = i, :2300-2315).
+ * Zeros when u or i is inactive or u is neither i nor a neighbour of i, as
+ * the reference's loops give.  FP64 at the current state, with the context's
+ * pair rule (the reference defines them for the 3rd-order rule).  Unweighted:
+ * R_q(u) = sum_{i in {u} U N(u)} beta_i dV0_i/dq_u. */
+dmdgpu_status dmdgpu_site_DV0_Dq(dmdgpu_ctx* ctx, int m, const int32_t* h_u, const int32_t* h_i,
+                                 double* h_out);
+dmdgpu_status dmdgpu_site_DV0_Dxi(dmdgpu_ctx* ctx, int m, const int32_t* h_u, const int32_t* h_i,
+                                  double* h_out);
+
 /* The SNES FormFunction body.  Gradient of the free-entropy functional
  * sum_i S0_i (MgHx-mf-V-bulk.cpp:957-1142) with respect to the mean positions,
  * widths and occupancies of every site:
@@ -202,7 +219,10 @@ dmdgpu_status dmdgpu_set_precision(dmdgpu_ctx* ctx, int precision);
  * (nodes 0, +-sqrt(3) s_rel; weights 2/3, 1/6 per axis; exact to degree 5).
  * 44: the reference's own 44-point 6D degree-5 pair rule
  * (src/Numerical/Quadrature-Hermitian-5th.cpp:31-157, which its evaluators do
- * not call).  Distinct angular triplets keep the 3rd-order 9D rule under all
+ * not call).  6: the 6-point Stroud rule on the relative coordinate
+ * (r +- sqrt(3) s_rel e_l, weight 1/6, degree 3): the reference's 12-point
+ * rule when the two widths are equal, and otherwise exact in the fourth
+ * moment along an axis where that rule is not.  Distinct angular triplets keep the 3rd-order 9D rule under all
  * three: the reference has no higher-order 9D rule. */
 dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* ctx, int order);
 

@@ -1100,7 +1100,11 @@ static inline int act(const or_system* s, int j) { return s->x[j] >= OR_MIN_OCCU
  *   rule 5: 3-point Gauss-Hermite per axis on the relative coordinate
  *           r ~ N(r_ij, (sigma_i^2 + sigma_j^2) I): nodes 0, +-sqrt(3)
  *           sigma_rel with weights 2/3, 1/6 (27 points, exact to degree 5);
- *   rule 44: the reference's 44-point 6D degree-5 rule (below). */
+ *   rule 44: the reference's 44-point 6D degree-5 rule (below);
+ *   rule 6: the 6-point 3D Stroud rule on the relative coordinate
+ *           r ~ N(r_ij, (sigma_i^2 + sigma_j^2) I): r +- sqrt(3) sigma_rel e_l,
+ *           weight 1/6 (degree 3).  With sigma_i = sigma_j it is the
+ *           reference's 12-point rule (both put r +- sqrt(6) sigma e_l). */
 #define OR_SQRT3 1.7320508075688772
 /* rule 44: moment_1th_Vij_gaussian_measure_5th (Quadrature-Hermitian-5th.cpp:
  * 31-157), the 6D degree-5 rule with 22 generators in zeta units and their
@@ -1128,7 +1132,9 @@ static inline double rule44_gen(int t, double* z) {
   for (int k = 0; k < 6; k++) z[k] = (k == a || k == b) ? -sg : sg;
   return 1.0 / 128.0;
 }
-static inline int pair_rule_points(int rule) { return rule == 5 ? 27 : rule == 44 ? 44 : 12; }
+static inline int pair_rule_points(int rule) {
+  return rule == 5 ? 27 : rule == 44 ? 44 : rule == 6 ? 6 : 12;
+}
 static inline double pair_rule_point(int rule, int t, const double* r, double si, double sj,
                                      double* p, double* dsi) {
   p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
@@ -1154,6 +1160,15 @@ static inline double pair_rule_point(int rule, int t, const double* r, double si
       w *= wn[a[k]];
     }
     return w;
+  }
+  if (rule == 6) {
+    const int l = t >> 1;
+    const double sg = (t & 1) ? -OR_SQRT3 : OR_SQRT3;
+    const double srel = sqrt(si * si + sj * sj);
+    p[l] += sg * srel;
+    dsi[0] = dsi[1] = dsi[2] = 0.0;
+    dsi[l] = sg * si / srel;
+    return 1.0 / 6.0;
   }
   const int wsel = t / 6, l = (t >> 1) % 3;
   const double d = (t & 1) ? -OR_SQRT6 : OR_SQRT6;

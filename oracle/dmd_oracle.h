@@ -66,7 +66,9 @@ typedef struct {
                            3 (default, 0 too) = the reference's 12-point 6D
                            Stroud rule; 5 = 27-point tensor Gauss-Hermite on
                            the relative coordinate (degree 5); 44 = the
-                           reference's 44-point 6D degree-5 pair rule      */
+                           reference's 44-point 6D degree-5 pair rule;
+                           6 = 6-point Stroud rule on the relative
+                           coordinate (degree 3)                          */
 } or_system;
 
 /* Tables ------------------------------------------------------------------ */

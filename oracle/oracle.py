@@ -119,8 +119,9 @@ class Oracle:
     def system(self, sysd, nnb, nbr, rule=3):
         """Pack a system dict (see dmdgpu.lattice) into the C struct.  The
         returned object keeps the arrays alive.  rule: 3 (the reference's
-        pair rule) or 5 (27-point tensor Gauss-Hermite pair rule) for the
-        factorised functions."""
+        12-point pair rule), 5 (27-point tensor Gauss-Hermite), 44 (the
+        reference's 44-point 6D rule) or 6 (6-point Stroud rule on the
+        relative coordinate) for the factorised functions."""
         keep = {}
         for k in ("q", "sigma", "x", "beta", "gamma"):
             keep[k] = np.ascontiguousarray(sysd[k], dtype=np.float64)

@@ -58,7 +58,7 @@ def test_diag_blocks_match_reference_order(gpu, oracle, maker):
     assert err < TOL
 
 
-@pytest.mark.parametrize("rule", [3, 5, 44])
+@pytest.mark.parametrize("rule", [3, 5, 44, 6])
 @pytest.mark.parametrize("exact", [False, True])
 def test_diag_blocks_match_factorised(gpu, oracle, rule, exact):
     sysd = lat.hcp_mgh(5, 3, 3, x_h=(0.0, 0.3), seed=9)

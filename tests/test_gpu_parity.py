@@ -233,11 +233,12 @@ def test_error_codes(gpu):
     assert e.value.code == 1
 
 
-# Fifth-order pair rules for the pair and coincident terms -- 5: 27-point
-# tensor Gauss-Hermite on the relative coordinate; 44: the reference's 44-point
-# 6D rule -- FP64 GPU against the oracle's restatement of the same rule at the
-# FP64 bar, and the split evaluation against FP64 at the split bars.
-@pytest.mark.parametrize("rule", [5, 44])
+# Other pair rules for the pair and coincident terms -- 5: 27-point tensor
+# Gauss-Hermite on the relative coordinate; 44: the reference's 44-point 6D
+# rule; 6: the 6-point Stroud rule on the relative coordinate -- FP64 GPU
+# against the oracle's restatement of the same rule at the FP64 bar, and the
+# split evaluation against FP64 at the split bars.
+@pytest.mark.parametrize("rule", [5, 44, 6])
 @pytest.mark.parametrize("case", ["hcp_mgh_5x3x3", "cluster_mgh"])
 def test_fifth_order_rule(gpu, oracle, case, rule):
     say(f"[gpu] rule {rule} {case}")

@@ -197,7 +197,9 @@ def test_fifth_order_pair_rule_closed_form(tmp_path):
     |r_ij|^4 + 10 |r_ij|^2 s^2 + 15 s^4.  The 27-point tensor rule (degree 5)
     is exact, and so is the reference's 44-point 6D degree-5 rule (rule 44);
     the reference's 12-point rule (degree 3) gives 18 (sigma_i^4 + sigma_j^4)
-    in place of 15 s^4, which the test also pins."""
+    in place of 15 s^4, and the 6-point relative rule (rule 6, degree 3)
+    9 s^4 -- (|r|^2 + 3 s^2)^2 + 4 |r|^2 s^2 over its points -- which the
+    test also pins."""
     c = 0.7
     O = _poly_tables(tmp_path, rho=(0,), F=(0,), phi=(0,), u=(0,), w=(c,))
     sysd = lat.cluster(1, 1, 1, with_h=False, seed=3, sigma_jitter=0.3)
@@ -214,9 +216,30 @@ def test_fifth_order_pair_rule_closed_form(tmp_path):
     pre = (sysd["x"][0] * sysd["x"][1]) ** 2 * c * c / 3.0
     exact = pre * (A * A + 10 * A * s2 + 15 * s2 * s2)
     third = pre * (A * A + 10 * A * s2 + 18 * (si ** 4 + sj ** 4))
-    for rule, want in ((5, exact), (44, exact), (3, third)):
+    six = pre * (A * A + 10 * A * s2 + 9 * s2 * s2)
+    for rule, want in ((5, exact), (44, exact), (3, third), (6, six)):
         V0 = O.fact_residual(O.system(sysd, nnb, nbr, rule))["V0"]
         assert V0 == pytest.approx([want, want], rel=1e-12)
+
+
+def test_relative_six_point_rule_equals_reference_rule_at_equal_widths(oracle):
+    """With sigma_i = sigma_j on every pair the 6-point relative rule puts
+    the same points (r +- sqrt(6) sigma e_l) with the same weights as the
+    reference's 12-point rule, and its width derivatives agree too (sqrt(3/2)
+    on six points of weight 1/6 against sqrt(6) on the i-half of weight 1/12):
+    every output equal to rounding.  With unequal widths they differ."""
+    sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21)
+    sysd["sigma"][:] = 0.12
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    a = oracle.fact_residual(oracle.system(sysd, nnb, nbr, 3))
+    b = oracle.fact_residual(oracle.system(sysd, nnb, nbr, 6))
+    for k in ("Rq", "Rs", "Rx", "V0", "mf_rho"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * np.abs(a[k]).max(), k
+    sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21, sigma_jitter=0.2)
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    a = oracle.fact_residual(oracle.system(sysd, nnb, nbr, 3))
+    b = oracle.fact_residual(oracle.system(sysd, nnb, nbr, 6))
+    assert np.abs(a["V0"] - b["V0"]).max() > 1e-9 * np.abs(a["V0"]).max()
 
 
 def numpy_stroud_V0(sysd, nnb, nbr, rho, F, phi, u, w):
@@ -350,7 +373,7 @@ def synth(tmp_path_factory):
     return Oracle(str(d))
 
 
-@pytest.mark.parametrize("rule", [3, 5, 44])
+@pytest.mark.parametrize("rule", [3, 5, 44, 6])
 def test_finite_differences_of_free_entropy(synth, rule):
     """R = d(sum_i S0_i)/d(q, sigma, x) on smooth tables (consistent clamp)."""
     sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21)

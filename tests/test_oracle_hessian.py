@@ -144,7 +144,7 @@ def test_factorised_blocks_equal_reference_order(oracle, maker):
     assert rel(F[sites], R) < 1e-12
 
 
-@pytest.mark.parametrize("rule", [3, 5, 44])
+@pytest.mark.parametrize("rule", [3, 5, 44, 6])
 def test_exact_blocks_are_residual_derivatives(synth, rule):
     sysd = lat.hcp_mgh(4, 3, 3, x_h=(0.05, 0.5), seed=21)
     st, nnb, nbr = synth.neighbors(sysd, 6.5)
