@@ -168,5 +168,14 @@ hipError_t launch_diag(hipStream_t st, int rule, int exact, int spc_mask, const 
 hipError_t launch_totals(hipStream_t st, const State& s, const Tables& T, const Outputs& o,
                          int ninst, const int* chunk_lo, const int* chunk_hi);
 hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_per_chunk);
+// Packed vectors of one instance (sites [base, base + n) of the arena; perm =
+// the caller's index of each arena site): X = [q (3n) | sigma (n)] into qs and
+// the solver iterate Xs (bad[0] = 1 on a non-finite position or a width that
+// is not positive); R = [R_q (3n) | R_sigma (n) | R_x (n)] out.
+hipError_t launch_gather_unknowns(hipStream_t st, int base, int n, const int* perm, const double* X,
+                                  double4* qs, double4* Xs, int* bad);
+hipError_t launch_scatter_residual(hipStream_t st, int base, int n, const int* perm,
+                                   const double* Rq, const double* Rs, const double* Rx,
+                                   double* R);
 
 }  // namespace dmd

@@ -304,6 +304,43 @@ __global__ __launch_bounds__(256) void k_own_lists(int base, int n, const int* n
 }
 
 // ---------------------------------------------------------------------------
+// Packed SNES vectors in the caller's site order <-> the arena (internal
+// order, site i of the instance holds the caller's site perm[base + i]).
+// ---------------------------------------------------------------------------
+// X = [q (3n) | sigma (n)]: positions and widths of the n sites into the
+// evaluation state and the solver iterate; bad[0] is set for a non-finite
+// position or a width that is not positive and finite.
+__global__ __launch_bounds__(256) void k_gather_unknowns(int base, int n, const int* perm,
+                                                         const double* X, double4* qs,
+                                                         double4* Xs, int* bad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[base + i];
+  const double4 v = make_double4(X[3 * (size_t)o], X[3 * (size_t)o + 1], X[3 * (size_t)o + 2],
+                                 X[3 * (size_t)n + o]);
+  if (!(v.w > 0.0 && v.w < 1e300) || !(fabs(v.x) < 1e300 && fabs(v.y) < 1e300 &&
+                                        fabs(v.z) < 1e300))
+    bad[0] = 1;
+  qs[base + i] = v;
+  Xs[base + i] = v;
+}
+
+// R = [R_q (3n) | R_sigma (n) | R_x (n)] in the caller's order.
+__global__ __launch_bounds__(256) void k_scatter_residual(int base, int n, const int* perm,
+                                                          const double* Rq, const double* Rs,
+                                                          const double* Rx, double* R) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[base + i];
+  const size_t a = base + (size_t)i;
+  R[3 * (size_t)o] = Rq[3 * a];
+  R[3 * (size_t)o + 1] = Rq[3 * a + 1];
+  R[3 * (size_t)o + 2] = Rq[3 * a + 2];
+  R[3 * (size_t)n + o] = Rs[a];
+  R[4 * (size_t)n + o] = Rx[a];
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_neighbors(hipStream_t st, const Box& box, const CellGrid& g, double rc, int base,
@@ -341,6 +378,21 @@ hipError_t launch_totals(hipStream_t st, const State& s, const Tables& T, const 
   k_block_sums<<<nb, 256, 0, st>>>(s.n, s.inst, s.prec, s.spc, o.rho, T.dxe[0], T.dxe[1], o.S0,
                                    o.V0, o.partials);
   k_inst_sums<<<ninst, 256, 0, st>>>(s.prec, chunk_lo, chunk_hi, o.partials, o.totals);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_unknowns(hipStream_t st, int base, int n, const int* perm, const double* X,
+                                  double4* qs, double4* Xs, int* bad) {
+  if (n <= 0) return hipSuccess;
+  k_gather_unknowns<<<(n + 255) / 256, 256, 0, st>>>(base, n, perm, X, qs, Xs, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_residual(hipStream_t st, int base, int n, const int* perm,
+                                   const double* Rq, const double* Rs, const double* Rx,
+                                   double* R) {
+  if (n <= 0) return hipSuccess;
+  k_scatter_residual<<<(n + 255) / 256, 256, 0, st>>>(base, n, perm, Rq, Rs, Rx, R);
   return hipGetLastError();
 }
 

@@ -115,6 +115,7 @@ struct Instance {
   std::vector<int> h_slot;     // species slot, internal order
   std::vector<char> h_act;     // site active (x >= 5e-4), internal order
   std::vector<double> h_q;     // positions, caller's order
+  bool h_q_stale = false;      // positions changed on the device since h_q
   bool have_neigh = false;
   long long pairs = 0;
   int nactive = 0;
@@ -159,6 +160,7 @@ struct dmdgpu_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   PinnedBuf pin_up, pin_down;
+  PinnedBuf pin_pk;  // staging of the packed host-buffer residual
   double rc = kDefaultRc;
   dmd::PhaseConfig cfg{dmd::kFP64, {16, 16}, {8, 8}, {0, 1}, 3};
   // tables
@@ -184,6 +186,7 @@ struct dmdgpu_ctx {
   DevBuf<int> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_flags;
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<double> d_diag;  // diagonal Hessian blocks (n x 9)
+  DevBuf<double> d_pkin, d_pkout;  // packed X in / R out of the host-buffer residual
   DevBuf<int> d_hookq;     // per-(u, i) queries (2 per query)
   DevBuf<double> d_hookv;  // their results (5 per query)
   DevBuf<double> d_minv;  // arena x 10: block-Jacobi preconditioner of the solver
@@ -460,6 +463,8 @@ void for_site_bufs(dmdgpu_ctx* c, size_t cap, F&& f) {
   f(c->d_dnn, cap, true);
   f(c->d_dnbr, cap * dmd::kDCap, true);
   f(c->d_act, 2 * (ch + 1), false);
+  f(c->d_pkin, cap * 4, false);
+  f(c->d_pkout, cap * 5, false);
 }
 
 // Device bytes of the per-site arrays at a capacity of `cap` sites, and the
@@ -650,7 +655,10 @@ dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const
     if (!std::isfinite(q[k])) return fail(DMDGPU_ERR_ARG, "positions must be finite");
   for (int k = 0; x && k < n; k++)
     if (!std::isfinite(x[k])) return fail(DMDGPU_ERR_ARG, "occupancies must be finite");
-  if (q) I.h_q.assign(q, q + 3 * (size_t)n);
+  if (q) {
+    I.h_q.assign(q, q + 3 * (size_t)n);
+    I.h_q_stale = false;
+  }
   if (q || sg) {
     std::vector<double4> qs(n);
     if (!first)
@@ -719,6 +727,17 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   // valid, and no evaluation may read them.
   I.have_neigh = false;
   I.pairs = 0;
+  if (I.h_q_stale) {  // positions were last set on the device
+    std::vector<double4> qs(I.n);
+    HIPCHK(hipMemcpy(qs.data(), c->d_qs.p + I.base, sizeof(double4) * I.n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < I.n; i++) {
+      const int o = I.perm[i];
+      I.h_q[3 * o] = qs[i].x;
+      I.h_q[3 * o + 1] = qs[i].y;
+      I.h_q[3 * o + 2] = qs[i].z;
+    }
+    I.h_q_stale = false;
+  }
   const dmd::CellGrid g = make_grid(c, I);
   const size_t ncell = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
   if (c->budget) {
@@ -1543,6 +1562,7 @@ dmdgpu_status dmdgpu_destroy(dmdgpu_ctx* c) {
   (void)hipStreamSynchronize(c->side);
   c->pin_up.release();
   c->pin_down.release();
+  c->pin_pk.release();
   (void)hipEventDestroy(c->ev_fork);
   (void)hipEventDestroy(c->ev_join);
   (void)hipStreamDestroy(c->side);
@@ -1703,6 +1723,52 @@ dmdgpu_status dmdgpu_residual(dmdgpu_ctx* c, double* h_Rq, double* h_Rs, double*
   HIPCHK(hipSetDevice(c->device));
   CHK(eval_one(c, c->primary));
   return copy_residual(c, c->primary, h_Rq, h_Rs, h_Rx, h_tot);
+}
+
+dmdgpu_status dmdgpu_residual_packed(dmdgpu_ctx* c, const double* X, double* R, double* h_totals,
+                                     int on_device) {
+  CHK(check_primary(c));
+  if (on_device != 0 && on_device != 1) return fail(DMDGPU_ERR_ARG, "on_device must be 0 or 1");
+  HIPCHK(hipSetDevice(c->device));
+  const int id = c->primary;
+  Instance& I = c->inst[id];
+  const size_t n = (size_t)I.n;
+  if (!on_device) HIPCHK(c->pin_pk.ensure(sizeof(double) * 5 * n));
+  double* pin = static_cast<double*>(c->pin_pk.p);
+  if (X) {
+    const double* dX = X;
+    if (!on_device) {
+      std::memcpy(pin, X, sizeof(double) * 4 * n);
+      HIPCHK(hipMemcpyAsync(c->d_pkin.p, pin, sizeof(double) * 4 * n, hipMemcpyHostToDevice,
+                            c->stream));
+      dX = c->d_pkin.p;
+    }
+    HIPCHK(hipMemsetAsync(c->d_flags.p, 0, sizeof(int), c->stream));
+    HIPCHK(dmd::launch_gather_unknowns(c->stream, I.base, I.n, c->d_perm.p, dX, c->d_qs.p, c->d_X.p,
+                                       c->d_flags.p));
+    I.h_q_stale = true;
+  }
+  CHK(eval_one(c, id));
+  if (R) {
+    double* dR = on_device ? R : c->d_pkout.p;
+    HIPCHK(dmd::launch_scatter_residual(c->stream, I.base, I.n, c->d_perm.p, c->d_Rq.p, c->d_Rs.p,
+                                        c->d_Rx.p, dR));
+    if (!on_device)
+      HIPCHK(hipMemcpyAsync(pin, c->d_pkout.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost,
+                            c->stream));
+  }
+  double t3[3];
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(t3, c->d_totals.p + 3 * id, sizeof(t3), hipMemcpyDeviceToHost, c->stream));
+  if (X) HIPCHK(hipMemcpyAsync(&bad, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (R && !on_device) std::memcpy(R, pin, sizeof(double) * 5 * n);
+  if (h_totals) {
+    h_totals[0] = t3[0];
+    h_totals[1] = t3[1];
+  }
+  if (bad) return fail(DMDGPU_ERR_ARG, "positions must be finite and widths positive and finite");
+  return domain_status(t3[2]);
 }
 
 dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* c, const double* X, const double* V, double* JV) {
@@ -1971,6 +2037,7 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   S.ndof = nd;
   S.early = S.o.mixed != 0;
   I->sv = S;
+  I->h_q_stale = true;  // the solve moves the positions on the device
   // the iterate starts from the current evaluation state of the instance
   HIPCHK(hipMemcpyAsync(c->d_X.p + I->base, c->d_qs.p + I->base, sizeof(double4) * I->cap,
                         hipMemcpyDeviceToDevice, c->stream));

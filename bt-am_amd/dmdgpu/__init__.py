@@ -40,7 +40,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
            "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc",
-           "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi"]
+           "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi", "dmdgpu_residual_packed"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -148,6 +148,7 @@ def load_library():
     L.dmdgpu_instance_diag_blocks.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.dmdgpu_diag_blocks_noalloc.argtypes = [vp, C.c_int, vp]
     L.dmdgpu_site_DV0_Dq.argtypes = [vp, C.c_int, _ip, _ip, _dp]
+    L.dmdgpu_residual_packed.argtypes = [vp, vp, vp, _dp, C.c_int]
     L.dmdgpu_site_DV0_Dxi.argtypes = [vp, C.c_int, _ip, _ip, _dp]
     L.dmdgpu_set_rule.argtypes = [vp, C.c_int]
     L.dmdgpu_set_tuning_precision.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -283,6 +284,23 @@ class Context:
         out = np.zeros(len(u))
         self._chk(self.L.dmdgpu_site_DV0_Dxi(self.h, len(u), _i(u), _i(i), _d(out)))
         return out
+
+    def residual_packed(self, X=None, device=None):
+        """SNES FormFunction form: X = [q (3n) | sigma (n)] (None: current
+        state) -> R = [R_q (3n) | R_sigma (n) | R_x (n)], totals.  With
+        device=(dX, dR) device pointers are passed instead (dX may be 0 for
+        the current state) and R stays on the device."""
+        tot = np.zeros(2)
+        if device is not None:
+            dX, dR = device
+            self._chk(self.L.dmdgpu_residual_packed(self.h, C.c_void_p(dX or None),
+                                                    C.c_void_p(dR or None), _d(tot), 1))
+            return None, tot
+        R = np.zeros(5 * self.n)
+        Xc = None if X is None else np.ascontiguousarray(X, dtype=np.float64)
+        self._chk(self.L.dmdgpu_residual_packed(self.h, None if Xc is None else Xc.ctypes.data,
+                                                R.ctypes.data, _d(tot), 0))
+        return R, tot
 
     def diag_blocks(self, exact=False):
         """Diagonal 3x3 blocks of dR_q/dq, (n, 3, 3), FP64 (see dmdgpu_diag_blocks)."""

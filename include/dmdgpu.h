@@ -151,6 +151,18 @@ dmdgpu_status dmdgpu_site_DV0_Dxi(dmdgpu_ctx* ctx, int m, const int32_t* h_u, co
 dmdgpu_status dmdgpu_residual(dmdgpu_ctx* ctx, double* h_Rq, double* h_Rsigma, double* h_Rx,
                               double* h_totals);
 
+/* The same residual in the packed vector form of a SNES FormFunction
+ * (evaluate_RHS, Mechanical-Relaxation-bulk.cpp:441, with X and Y the
+ * solution and function Vecs): X = [q (3n) | sigma (n)] sets the positions
+ * and widths first (NULL: evaluate at the current state), R = [R_q (3n) |
+ * R_sigma (n) | R_x (n)] receives the residual (may be NULL); caller's site
+ * order.  on_device = 0: X and R are host memory (staged through page-locked
+ * buffers); 1: device memory of the context's device (no host copies; the
+ * reordering runs on the device).  totals[2] as dmdgpu_residual (host, may be
+ * NULL).  Occupancies are set with dmdgpu_update_state. */
+dmdgpu_status dmdgpu_residual_packed(dmdgpu_ctx* ctx, const double* X, double* R,
+                                     double* h_totals, int on_device);
+
 /* Same evaluation on the device-resident state, leaving the results in device
  * buffers (no host copies); used by resident solvers and for timing. */
 dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);

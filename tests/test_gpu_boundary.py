@@ -158,3 +158,52 @@ def test_nonfinite_inputs_rejected(gpu):
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.update_state(q=q)
     assert e.value.code == ERR_ARG
+
+
+def test_residual_packed_host_and_device(gpu, oracle):
+    """The packed SNES FormFunction form: host buffers and device buffers give
+    bit for bit the residual of dmdgpu_residual at X, in the caller's order,
+    within 1e-10 of the oracle; a width that is not positive is an argument
+    error."""
+    say("[gpu] packed residual, host and device buffers")
+    sd = lat.hcp_mgh(5, 3, 3, x_h=(0.0, 0.3), seed=9)
+    n = sd["n"]
+    gpu.set_state(sd)
+    gpu.neighbors()
+    rng = np.random.default_rng(3)
+    q = sd["q"] + rng.normal(0, 0.02, sd["q"].shape)
+    sg = sd["sigma"] * (1 + 0.05 * rng.random(n))
+    X = np.concatenate([q.ravel(), sg])
+    R, tot = gpu.residual_packed(X)
+    gpu.update_state(q=q, sigma=sg)
+    ref = gpu.residual()
+    assert np.array_equal(R[:3 * n].reshape(n, 3), ref["Rq"])
+    assert np.array_equal(R[3 * n:4 * n], ref["Rs"])
+    assert np.array_equal(R[4 * n:], ref["Rx"])
+    assert tot[0] == ref["S0_total"] and tot[1] == ref["V0_total"]
+    fin = dict(sd)
+    fin["q"], fin["sigma"] = q, sg
+    nnb, nbr = gpu.get_neighbors()  # the lists of the start state, kept by the update
+    o = oracle.fact_residual(oracle.system(fin, nnb, nbr))
+    assert np.abs(R[:3 * n] - o["Rq"].ravel()).max() < 1e-10 * np.abs(o["Rq"]).max()
+    # device buffers (same HIP runtime as the library)
+    hip = C.CDLL("libamdhip64.so.7")
+    dX, dR = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(dX), C.c_size_t(8 * 4 * n)) == 0
+    assert hip.hipMalloc(C.byref(dR), C.c_size_t(8 * 5 * n)) == 0
+    try:
+        X2 = np.concatenate([sd["q"].ravel(), sd["sigma"]])
+        assert hip.hipMemcpy(dX, X2.ctypes.data_as(C.c_void_p), C.c_size_t(8 * 4 * n), 1) == 0
+        _, tot2 = gpu.residual_packed(device=(dX.value, dR.value))
+        Rd = np.empty(5 * n)
+        assert hip.hipMemcpy(Rd.ctypes.data_as(C.c_void_p), dR, C.c_size_t(8 * 5 * n), 2) == 0
+        Rh, toth = gpu.residual_packed(X2)
+        assert np.array_equal(Rd, Rh) and np.array_equal(tot2, toth)
+    finally:
+        hip.hipFree(dX)
+        hip.hipFree(dR)
+    bad = X.copy()
+    bad[3 * n + 5] = -0.1
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.residual_packed(bad)
+    assert e.value.code == ERR_ARG
