@@ -19,9 +19,10 @@ enum : int { kMakeNone = 0, kMakeX = 1, kMakeXP = 2, kMakeXD = 3 };
 enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4, kDotPc = 5 };
 // Dots per instance: [0..2] sums, [3..4] maxima, [5..13] sums (preconditioned CG),
 // [14] sites of the tick's evaluation whose density left the embedding
-// table's domain.
-constexpr int kSolveDots = 15;
+// table's domain, [15] its free energy sum_i S0_i.
+constexpr int kSolveDots = 16;
 constexpr int kDotDomain = 14;
+constexpr int kDotEnergy = 15;
 
 // Diffusion-step modes (SolveOp.diff): one Picard sweep of the implicit
 // occupancy update, or the bare rates dx/dt written into Xt.

@@ -91,7 +91,8 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
 //              Rr.z, z.z, z.y, y.y, z.P, y.P, P.P -- what the host needs to
 //              advance r.z and |P|^2 of the next direction without a new pass
 //   kDotPc   : after CgInit (P = M^-1 Rr): g[2] = Rr.P, g[8] = P.P
-//   kDotLs   : d = max |D_q| (component), e = max(-D_sigma / X_sigma)
+//   kDotLs   : a = R0 . D (the slope of the energy along D), d = max |D_q|
+//              (component), e = max(-D_sigma / X_sigma)
 //   kDotDiff : over H sites, a = sum Xt (hydrogen content of the new iterate),
 //              b = sum x (of the current one), d = max |Xt - x|,
 //              e = max(-Xt, Xt - 1) (> 0 when an occupancy leaves [0, 1])
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
       g[8] = dot4(P, P);
     } else if (op.dot == kDotLs) {
       const double4 D = v.D[i];
+      a = dot4(v.R0[i], D);
       d = fmax(fabs(D.x), fmax(fabs(D.y), fabs(D.z)));
       e = -D.w / v.X[i].w;
     }
@@ -165,9 +167,9 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
     partials[kSolveDots * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// One block per instance over its chunk range, fixed order.  Slot 14 carries
-// the table-domain count of the instance's evaluation in this tick (the third
-// per-instance total; 0 when it was not evaluated).
+// One block per instance over its chunk range, fixed order.  Slots 14 and 15
+// carry the table-domain count and the free energy of the instance's
+// evaluation in this tick (per-instance totals; 0 when it was not evaluated).
 __global__ __launch_bounds__(64) void k_solver_reduce(const int* chunk_lo, const int* chunk_hi,
                                                      const SolveOp* ops, const int* prec,
                                                      const double* totals, const double* partials,
@@ -176,6 +178,7 @@ __global__ __launch_bounds__(64) void k_solver_reduce(const int* chunk_lo, const
   if (threadIdx.x != 0) return;
   double* o = out + (size_t)kSolveDots * k;
   o[kDotDomain] = prec[k] >= 0 ? totals[3 * k + 2] : 0.0;
+  o[kDotEnergy] = prec[k] >= 0 ? totals[3 * k] : 0.0;
   if (ops[k].dot == kDotNone) return;
   double a = 0, b = 0, c = 0, d = 0, e = -1e300;
   double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};

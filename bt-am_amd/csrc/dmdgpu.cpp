@@ -102,6 +102,8 @@ struct Solve {
   int newton = 0, cg = 0, ls = 0, evals = 0;
   double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
   double rz = 0;          // r . M^-1 r of the preconditioned CG
+  double energy = 0;      // free energy sum_i S0_i at the iterate X
+  double slope = 0;       // R . D, the energy's slope along the Newton step
   bool pc_setup = false;  // build the preconditioner at X before the next tick
   int ndof = 0;
   dmd::SolveOp pending{};  // updates to apply at the start of the next tick
@@ -1063,6 +1065,10 @@ constexpr double kMaxStep = 0.2;   // max |dq| of a Newton step (A)
 // the stopping test are FP64.
 constexpr double kSplitSwitch = 1e-3;
 
+// Relative rounding level of the free-energy sum below which the energy merit
+// cannot tell a decrease (sum of ~1e5 site terms in FP64).
+constexpr double kEnergyRound = 1e-11;
+
 constexpr double kA2PerM2 = 1e20;  // diffusivity m^2/s -> A^2/s
 // Mixed diffusion steps sweep with split-precision residuals until the
 // iterate moves by less than this, then finish with FP64 sweeps (the split
@@ -1173,6 +1179,7 @@ void digest_tick(dmdgpu_ctx* c) {
     switch (S.stage) {
       case ST_START: {
         S.rnorm = std::sqrt(d[0]);
+        S.energy = d[dmd::kDotEnergy];
         if (S.newton == 0) S.rnorm0 = S.rnorm;
         nx.upd |= dmd::kUpdR0FromRv;
         if (S.early) {
@@ -1262,6 +1269,7 @@ void digest_tick(dmdgpu_ctx* c) {
       }
       case ST_LSPREP: {
         const double dq = d[3], ds = d[4];
+        S.slope = d[0];
         double a = 1.0;
         if (dq > kMaxStep) a = std::min(a, kMaxStep / dq);
         if (ds > 0.5) a = std::min(a, 0.5 / ds);
@@ -1272,9 +1280,18 @@ void digest_tick(dmdgpu_ctx* c) {
       }
       case ST_LS: {
         const double rn = std::sqrt(d[0]);
-        if (std::isfinite(rn) && rn <= (1.0 - 1e-4 * S.alpha_ls) * S.rnorm) {
+        const double en = d[dmd::kDotEnergy];
+        const bool rdec = std::isfinite(rn) && rn <= (1.0 - 1e-4 * S.alpha_ls) * S.rnorm;
+        // energy merit: Armijo on sum S0; once the change is at the rounding
+        // level of the sum, a step that reduces |R| is taken
+        const bool edec =
+            std::isfinite(en) && std::isfinite(rn) &&
+            (en <= S.energy + 1e-4 * S.alpha_ls * std::min(S.slope, 0.0) ||
+             (std::fabs(en - S.energy) <= kEnergyRound * std::fabs(S.energy) && rdec));
+        if (S.o.merit ? edec : rdec) {
           nx.upd |= dmd::kUpdAcceptY | dmd::kUpdR0FromRv;
           S.rnorm = rn;
+          S.energy = en;
           S.newton++;
           if (S.early) {
             nx.upd |= dmd::kUpdR0fFromRv;
@@ -2019,9 +2036,10 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
   if (!I->have_neigh) return fail(DMDGPU_ERR_STATE, "instance has no neighbour lists");
   Solve S;
-  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0, 0};
+  S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0, 0, 0};
   if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
-      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0)
+      S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0 ||
+      (S.o.merit != 0 && S.o.merit != 1))
     return fail(DMDGPU_ERR_ARG, "bad relaxation options");
   if (S.o.max_evals <= 0) S.o.max_evals = 100000;
   if (S.o.dof_mask == 0) S.o.dof_mask = DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA;
@@ -2064,6 +2082,14 @@ dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* c, int id, int* state, double* rno
   if (rnorm) *rnorm = S.rnorm;
   if (newton) *newton = S.newton;
   if (evals) *evals = S.evals;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_relax_energy(dmdgpu_ctx* c, int id, double* energy) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (!energy) return fail(DMDGPU_ERR_ARG, "null output");
+  *energy = I->sv.energy;
   return DMDGPU_OK;
 }
 

@@ -40,7 +40,8 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_set_graph", "dmdgpu_diffuse_start", "dmdgpu_diffusion_rates",
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
            "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc",
-           "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi", "dmdgpu_residual_packed"]
+           "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi", "dmdgpu_residual_packed",
+           "dmdgpu_relax_energy"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -52,16 +53,18 @@ class RelaxOpts(C.Structure):
     """dmdgpu_relax_opts."""
     _fields_ = [("atol", C.c_double), ("max_newton", C.c_int), ("max_cg", C.c_int),
                 ("eta", C.c_double), ("mixed", C.c_int), ("max_evals", C.c_int),
-                ("dof_mask", C.c_int), ("probe", C.c_double), ("precond", C.c_int)]
+                ("dof_mask", C.c_int), ("probe", C.c_double), ("precond", C.c_int),
+                ("merit", C.c_int)]
 
 
 DOF_MG_Q, DOF_MG_SIGMA, DOF_H_Q, DOF_H_SIGMA = 1, 2, 4, 8
 
 
 def relax_opts(atol=1e-8, max_newton=50, max_cg=60, eta=0.05, mixed=False, max_evals=0,
-               dof_mask=DOF_MG_Q | DOF_MG_SIGMA, probe=0.0, precond=False):
+               dof_mask=DOF_MG_Q | DOF_MG_SIGMA, probe=0.0, precond=False, merit="residual"):
+    """merit: "residual" (line search on |R|_2) or "energy" (on sum_i S0_i)."""
     return RelaxOpts(atol, max_newton, max_cg, eta, 1 if mixed else 0, max_evals, dof_mask,
-                     probe, int(precond))
+                     probe, int(precond), {"residual": 0, "energy": 1}[merit])
 
 
 class DiffuseOpts(C.Structure):
@@ -139,6 +142,7 @@ def load_library():
     L.dmdgpu_relax_status.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]
     L.dmdgpu_relax.argtypes = [vp, C.POINTER(RelaxOpts), C.c_int]
+    L.dmdgpu_relax_energy.argtypes = [vp, C.c_int, _dp]
     L.dmdgpu_set_graph.argtypes = [vp, C.c_int]
     L.dmdgpu_diffuse_start.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts)]
     L.dmdgpu_diffusion_rates.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts), _dp]
@@ -434,6 +438,12 @@ class Context:
                                              C.byref(ev)))
         return {"state": RELAX_STATE[st.value], "rnorm": rn.value, "newton": nw.value,
                 "evals": ev.value}
+
+    def relax_energy(self, iid):
+        """Free energy sum_i S0_i at the solve's current iterate."""
+        e = C.c_double()
+        self._chk(self.L.dmdgpu_relax_energy(self.h, iid, C.byref(e)))
+        return e.value
 
     def relax(self, opts=None, max_ticks=100000):
         o = opts if opts is not None else relax_opts()

@@ -311,12 +311,20 @@ typedef struct {
                       site (3x3 on q; 1/(3/sigma^2 + tr D_u) on sigma) at
                       Newton steps 0, k, 2k, ... and kept in between;
                       0: plain CG */
+  int merit;       /* line-search merit: 0 = the residual norm |R|_2
+                      (sufficient decrease); 1 = the free energy sum_i S0_i,
+                      of which R is the gradient (Armijo on the energy, with a
+                      step also accepted when the energy change is below its
+                      rounding level and |R| decreases): monotone descent
+                      where the landscape is not convex (free surfaces,
+                      interstitial H positions) */
 } dmdgpu_relax_opts;
 /* Relaxed unknowns.  Interstitial H at fixed occupancy has a soft (and at low
  * occupancy width-unbounded: sigma_eq^2 ~ 1/(beta k x)) landscape, so the
- * default relaxes the Mg lattice and keeps H sites fixed.  DMDGPU_DOF_H_Q is
- * accepted, but the Newton-CG iteration with the residual-norm line search
- * stalls on it (the H position landscape at fixed occupancy is not convex). */
+ * default relaxes the Mg lattice and keeps H sites fixed.  With H positions
+ * (DMDGPU_DOF_H_Q) periodic bulk systems converge; with free surfaces the
+ * structure can reconstruct far from the start, where the residual-norm line
+ * search stalls and the energy merit (merit = 1) keeps descending. */
 enum {
   DMDGPU_DOF_MG_Q = 1,
   DMDGPU_DOF_MG_SIGMA = 2,
@@ -337,6 +345,8 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* ctx, int id, const dmdgpu_relax_opt
 dmdgpu_status dmdgpu_relax_tick(dmdgpu_ctx* ctx, int* running);
 dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* ctx, int id, int* state, double* rnorm,
                                   int* newton_steps, int* residual_evals);
+/* Free energy sum_i S0_i at the solve's current iterate (as last evaluated). */
+dmdgpu_status dmdgpu_relax_energy(dmdgpu_ctx* ctx, int id, double* energy);
 /* Starts every instance and ticks until all are done (or max_ticks). */
 dmdgpu_status dmdgpu_relax(dmdgpu_ctx* ctx, const dmdgpu_relax_opts* opts, int max_ticks);
 /* 1 (default): a tick's device work is captured once as a hipGraph and
