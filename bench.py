@@ -31,7 +31,13 @@ import numpy as np  # noqa: E402
 # once) plus the slot sums.  The PQE count per residual stays the workload's
 # fixed 50 per active ordered pair.
 FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2943}
+# The split-precision evaluation, counted the same way (SQ_INSTS_VALU_FLOPS_FP32
+# and _FP64 x 64 per active ordered pair, profiles/valu_mgh_20x15x15.md):
+# (FP32, FP64) FLOPs per pair of each phase.
+FLOP_PER_PAIR_SPLIT = {"phase1_site_sums": (821 + 532, 335 + 78),
+                       "phase2_gradient": (2644, 229 + 22)}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
+PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
 HBM_PEAK = 8.0e12          # B/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_mgh_20x15x15_fp64.md")
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_mgh_20x15x15.md")
@@ -78,6 +84,14 @@ def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
     import dmdgpu
     O = Oracle(dmdgpu.TABLE_DIR)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     s = O.system(sysd, nnb, nbr)
     O.fact_residual(s, threads)  # warm
     t0 = time.perf_counter()
@@ -89,6 +103,7 @@ def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
             break
     dt = time.perf_counter() - t0
     out = {"value": pqe_per_res * reps / dt, "unit": "PQE/s", "cores": threads, "kind": "port",
+           "cpu_model": model, "host_cpus": os.cpu_count(),
            "sample": f"{reps} full FP64 residuals of the same 54,000-site workload "
                      f"({dt:.1f} s, factorised oracle, OpenMP)"}
     # The reference's own summation order (O(nb^2) triplets per site), timed on
@@ -145,9 +160,20 @@ def split_leg(ctx, args, pqe):
     ctx.set_precision("fp64")
     err = {k: float(np.abs(out[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in ("Rq", "Rs", "Rx")}
     err["S0_total"] = float(abs(out["S0_total"] - ref["S0_total"]) / abs(ref["S0_total"]))
+    per = {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}
+    pairs = ctx.work_counts()[0]
+    f32, f64 = (c * pairs for c in FLOP_PER_PAIR_SPLIT["phase2_gradient"])
+    t2 = per["phase2_gradient"] / 1e3
+    roof = {"bound": "valu", "kernel": "phase2_gradient", "unit": "TFLOP/s",
+            "achieved": (f32 + f64) / t2 / 1e12, "achieved_fp32": f32 / t2 / 1e12,
+            "achieved_fp64": f64 / t2 / 1e12, "peak_fp32": PEAK_FP32_VALU / 1e12,
+            "peak_fp64": PEAK_FP64_VALU / 1e12,
+            # time-weighted: each precision's FLOPs at its own published peak
+            "frac": (f32 / PEAK_FP32_VALU + f64 / PEAK_FP64_VALU) / t2,
+            "avg_launch_ms": per["phase2_gradient"]}
     return {"value": pqe * args.steps / dt, "unit": "PQE/s", "ms_per_step": dt * 1e3 / args.steps,
             "tuning_k1_k2_unroll": list(cfg), "max_rel_err_vs_fp64": err,
-            "kernel_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in kt.items() if v[0]}}
+            "kernel_ms_per_launch": per, "roofline": roof}
 
 
 def host_leg(ctx, sysd, pqe, reps=10):
@@ -252,8 +278,10 @@ def solve_instances(count):
 
 
 def _relax_plain(ctx, insts, opts):
-    """One instance at a time; returns (seconds, [(status, q, sigma)], per-instance)."""
+    """One instance at a time; returns (seconds, [(status, q, sigma)], per-instance,
+    (FP64, FP32) FLOPs)."""
     t = 0.0
+    flops = [0.0, 0.0]
     res, per = [], []
     for sd in insts:
         ctx.set_state(sd)
@@ -265,12 +293,41 @@ def _relax_plain(ctx, insts, opts):
         dt = time.perf_counter() - t0
         t += dt
         st = ctx.relax_status(0)
+        fl = solve_flops(ctx, [0])
+        flops[0] += fl[0]
+        flops[1] += fl[1]
         q, sg, _ = ctx.instance_state(0)
         res.append((st, q, sg))
         per.append({"sites": int(sd["n"]), "evals": st["evals"],
                     "ms_per_eval": dt * 1e3 / max(st["evals"], 1)})
     ctx.remove_instance(0)
-    return t, res, per
+    return t, res, per, flops
+
+
+def solve_flops(ctx, ids):
+    """(FP64, FP32) hardware FLOPs of the residual evaluations of the listed
+    instances' solves: per evaluation, phase 1 over every active ordered pair
+    and phase 2 over those with an Mg end (relaxation ticks of the Mg lattice
+    skip the pairs with two H ends), at the per-pair counts of each precision.
+    The one-off preconditioner builds are not counted."""
+    f64 = f32 = 0.0
+    for i in ids:
+        e64, e32 = ctx.relax_evals(i)
+        p, pm = ctx.instance_pairs(i)
+        f64 += e64 * (FLOP_PER_PAIR["phase1_site_sums"] * p + FLOP_PER_PAIR["phase2_gradient"] * pm)
+        s1, s2 = FLOP_PER_PAIR_SPLIT["phase1_site_sums"], FLOP_PER_PAIR_SPLIT["phase2_gradient"]
+        f32 += e32 * (s1[0] * p + s2[0] * pm)
+        f64 += e32 * (s1[1] * p + s2[1] * pm)
+    return f64, f32
+
+
+def roofline_of(f64, f32, seconds):
+    """Time-weighted VALU roofline of a mixed FP64 / FP32 workload."""
+    return {"bound": "valu", "unit": "TFLOP/s", "achieved": (f64 + f32) / seconds / 1e12,
+            "achieved_fp64": f64 / seconds / 1e12, "achieved_fp32": f32 / seconds / 1e12,
+            "peak_fp64": PEAK_FP64_VALU / 1e12, "peak_fp32": PEAK_FP32_VALU / 1e12,
+            "frac": (f64 / PEAK_FP64_VALU + f32 / PEAK_FP32_VALU) / seconds,
+            "flops_fp64": f64, "flops_fp32": f32}
 
 
 def _relax_batched(ctx, insts, opts):
@@ -286,9 +343,10 @@ def _relax_batched(ctx, insts, opts):
     ctx.sync()
     t = time.perf_counter() - t0
     res = [(ctx.relax_status(i), ctx.instance_state(i)[0]) for i in ids]
+    flops = solve_flops(ctx, ids)
     for i in ids:
         ctx.remove_instance(i)
-    return t, ticks, res
+    return t, ticks, res, flops
 
 
 def solves_leg(tuning, count, graph=True):
@@ -321,19 +379,21 @@ def solves_leg(tuning, count, graph=True):
         # eta in 0.01..0.3 on this workload)
         mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100,
                                        eta=0.05 if pc else 0.1, mixed=True, precond=pc)
-        t_plain, plain, per_inst = _relax_plain(ctx, insts, plain_opts)
-        t_batch, ticks, batch = _relax_batched(ctx, insts, mixed_opts)
+        t_plain, plain, per_inst, fl_plain = _relax_plain(ctx, insts, plain_opts)
+        t_batch, ticks, batch, fl_batch = _relax_batched(ctx, insts, mixed_opts)
         dq = max(float(np.abs(b[1] - p[1]).max()) for b, p in zip(batch, plain))
         rec = {
             "plain_fp64_per_instance": {
                 "solves_per_hour": n * 3600.0 / t_plain, "seconds": t_plain,
                 "converged": sum(p[0]["state"] == "converged" for p in plain),
-                "residual_evals": sum(p[0]["evals"] for p in plain)},
+                "residual_evals": sum(p[0]["evals"] for p in plain),
+                "roofline": roofline_of(*fl_plain, t_plain)},
             "batched_split32_probes": {
                 "solves_per_hour": n * 3600.0 / t_batch, "seconds": t_batch,
                 "converged": sum(b[0]["state"] == "converged" for b in batch),
                 "residual_evals": sum(b[0]["evals"] for b in batch), "ticks": ticks,
-                "ms_per_tick": t_batch * 1e3 / max(ticks, 1)},
+                "ms_per_tick": t_batch * 1e3 / max(ticks, 1),
+                "roofline": roofline_of(fl_batch[0], fl_batch[1], t_batch)},
             "speedup": t_plain / t_batch, "max_position_diff_A": dq}
         log(f"plain per instance (precond={pc}): {per_inst}")
         if pc:
@@ -413,7 +473,7 @@ def main():
     ap.add_argument("--no-solves", action="store_true", help="skip the solves/hr leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-diag", action="store_true", help="skip the diagonal-Hessian leg")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests in the serving leg")
     ap.add_argument("--instances", type=int, default=64,
                     help="relaxations in the solves/hr leg (half nanowires, half spheres)")

@@ -167,6 +167,7 @@ hipError_t launch_diag(hipStream_t st, int rule, int exact, int spc_mask, const 
                        const Tables& T, const double* rec, int base, int n, double* D);
 hipError_t launch_totals(hipStream_t st, const State& s, const Tables& T, const Outputs& o,
                          int ninst, const int* chunk_lo, const int* chunk_hi);
+// out_per_chunk[2 c] = active ordered pairs of chunk c, [2 c + 1] = those with an Mg end.
 hipError_t launch_active_pairs(hipStream_t st, const State& s, long long* out_per_chunk);
 // Packed vectors of one instance (sites [base, base + n) of the arena; perm =
 // the caller's index of each arena site): X = [q (3n) | sigma (n)] into qs and

@@ -69,22 +69,36 @@ __global__ __launch_bounds__(256) void k_inst_sums(const int* prec, const int* c
   }
 }
 
-// Active ordered pairs per 256-site chunk (for the work count).
+// Active ordered pairs per 256-site chunk (for the work count): out[2 c] all
+// of them, out[2 c + 1] those with an Mg end (what a tick that relaxes the Mg
+// lattice only evaluates in phase 2).
 __global__ __launch_bounds__(256) void k_active_pairs(State s, long long* out) {
-  __shared__ long long sh[256];
+  __shared__ long long sh[2][256];
   int i = blockIdx.x * 256 + threadIdx.x;
-  long long c = 0;
+  long long c = 0, cm = 0;
   if (i < s.n && s.inst[i] >= 0 && s.ne[i] != 0.0) {
     const int* row = s.nbr + (size_t)i * kCap;
-    for (int k = 0; k < s.nnb[i]; k++) c += s.ne[row[k]] != 0.0;
+    const bool mi = s.spc[i] == 0;
+    for (int k = 0; k < s.nnb[i]; k++) {
+      const bool on = s.ne[row[k]] != 0.0;
+      c += on;
+      cm += on && (mi || s.spc[row[k]] == 0);
+    }
   }
-  sh[threadIdx.x] = c;
+  sh[0][threadIdx.x] = c;
+  sh[1][threadIdx.x] = cm;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = sh[0][0];
+    out[2 * blockIdx.x + 1] = sh[1][0];
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -100,6 +100,7 @@ struct Solve {
   bool have_r0f = false;       // R0f already holds the split residual at X
   double mass0 = 0, mass = 0;  // hydrogen content at x_n and at the iterate
   int newton = 0, cg = 0, ls = 0, evals = 0;
+  int evals32 = 0;  // of those, split-precision evaluations
   double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
   double rz = 0;          // r . M^-1 r of the preconditioned CG
   double energy = 0;      // free energy sum_i S0_i at the iterate X
@@ -119,7 +120,8 @@ struct Instance {
   std::vector<double> h_q;     // positions, caller's order
   bool h_q_stale = false;      // positions changed on the device since h_q
   bool have_neigh = false;
-  long long pairs = 0;
+  long long pairs = 0;     // active ordered pairs
+  long long pairs_mg = 0;  // those with an Mg end
   int nactive = 0;
   Solve sv;
   // The residual buffers hold a relaxation tick's evaluation, which leaves
@@ -450,7 +452,7 @@ void for_site_bufs(dmdgpu_ctx* c, size_t cap, F&& f) {
   f(c->d_Rx, cap, true);
   f(c->d_S0, cap, true);
   f(c->d_partials, ch * 3, false);
-  f(c->d_pairs, ch, false);
+  f(c->d_pairs, 2 * ch, false);
   f(c->d_spart, ch * dmd::kSolveDots, false);
   f(c->d_X, cap, true);
   f(c->d_R0, cap, true);
@@ -728,7 +730,7 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   // The lists are rebuilt in place: until this build succeeds they are not
   // valid, and no evaluation may read them.
   I.have_neigh = false;
-  I.pairs = 0;
+  I.pairs = I.pairs_mg = 0;
   if (I.h_q_stale) {  // positions were last set on the device
     std::vector<double4> qs(I.n);
     HIPCHK(hipMemcpy(qs.data(), c->d_qs.p + I.base, sizeof(double4) * I.n, hipMemcpyDeviceToHost));
@@ -770,12 +772,15 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   CHK(upload_meta(c));
   HIPCHK(dmd::launch_active_pairs(c->stream, make_state(c), c->d_pairs.p));
   const int c0 = I.base / kChunk, nch = I.cap / kChunk;
-  std::vector<long long> pp(nch);
-  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p + c0, sizeof(long long) * nch,
+  std::vector<long long> pp(2 * (size_t)nch);
+  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p + 2 * c0, sizeof(long long) * 2 * nch,
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  I.pairs = 0;
-  for (long long v : pp) I.pairs += v;
+  I.pairs = I.pairs_mg = 0;
+  for (int k = 0; k < nch; k++) {
+    I.pairs += pp[2 * k];
+    I.pairs_mg += pp[2 * k + 1];
+  }
   I.have_neigh = true;
   return DMDGPU_OK;
 }
@@ -1134,6 +1139,7 @@ void plan_tick(dmdgpu_ctx* c) {
           break;
       }
       if (c->h_prec[k] >= 0) S.evals++;
+      if (c->h_prec[k] == dmd::kSplit32) S.evals32++;
       S.pending = dmd::SolveOp{};
     }
     c->h_ops[k] = op;
@@ -1385,12 +1391,15 @@ dmdgpu_status refresh_activity(dmdgpu_ctx* c, Instance& I) {
   CHK(upload_meta(c));
   HIPCHK(dmd::launch_active_pairs(c->stream, make_state(c), c->d_pairs.p));
   const int c0 = I.base / kChunk, nch = I.cap / kChunk;
-  std::vector<long long> pp(nch);
-  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p + c0, sizeof(long long) * nch,
+  std::vector<long long> pp(2 * (size_t)nch);
+  HIPCHK(hipMemcpyAsync(pp.data(), c->d_pairs.p + 2 * c0, sizeof(long long) * 2 * nch,
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  I.pairs = 0;
-  for (long long v : pp) I.pairs += v;
+  I.pairs = I.pairs_mg = 0;
+  for (int k = 0; k < nch; k++) {
+    I.pairs += pp[2 * k];
+    I.pairs_mg += pp[2 * k + 1];
+  }
   return DMDGPU_OK;
 }
 
@@ -1475,9 +1484,11 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
       dmdgpu_status st = tick_device(c);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (st) return st;
-      if (e != hipSuccess)
+      if (st || e != hipSuccess) {  // a failed capture leaves no graph behind
+        if (g) (void)hipGraphDestroy(g);
+        if (st) return st;
         return fail(DMDGPU_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      }
       c->graph = g;
       HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
       c->graph_extent = c->extent;
@@ -2082,6 +2093,22 @@ dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* c, int id, int* state, double* rno
   if (rnorm) *rnorm = S.rnorm;
   if (newton) *newton = S.newton;
   if (evals) *evals = S.evals;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_relax_evals(dmdgpu_ctx* c, int id, int* evals_fp64, int* evals_split) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (evals_fp64) *evals_fp64 = I->sv.evals - I->sv.evals32;
+  if (evals_split) *evals_split = I->sv.evals32;
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_instance_pairs(dmdgpu_ctx* c, int id, int64_t* pairs, int64_t* pairs_mg) {
+  Instance* I = get_inst(c, id);
+  if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
+  if (pairs) *pairs = I->pairs;
+  if (pairs_mg) *pairs_mg = I->pairs_mg;
   return DMDGPU_OK;
 }
 

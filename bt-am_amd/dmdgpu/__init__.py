@@ -41,7 +41,7 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
            "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc",
            "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi", "dmdgpu_residual_packed",
-           "dmdgpu_relax_energy"]
+           "dmdgpu_relax_energy", "dmdgpu_relax_evals", "dmdgpu_instance_pairs"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -143,6 +143,8 @@ def load_library():
                                       C.POINTER(C.c_int)]
     L.dmdgpu_relax.argtypes = [vp, C.POINTER(RelaxOpts), C.c_int]
     L.dmdgpu_relax_energy.argtypes = [vp, C.c_int, _dp]
+    L.dmdgpu_relax_evals.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.dmdgpu_instance_pairs.argtypes = [vp, C.c_int, _lp, _lp]
     L.dmdgpu_set_graph.argtypes = [vp, C.c_int]
     L.dmdgpu_diffuse_start.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts)]
     L.dmdgpu_diffusion_rates.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts), _dp]
@@ -438,6 +440,18 @@ class Context:
                                              C.byref(ev)))
         return {"state": RELAX_STATE[st.value], "rnorm": rn.value, "newton": nw.value,
                 "evals": ev.value}
+
+    def relax_evals(self, iid):
+        """(FP64, split) residual evaluations of the instance's solve."""
+        a, b = C.c_int(), C.c_int()
+        self._chk(self.L.dmdgpu_relax_evals(self.h, iid, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def instance_pairs(self, iid):
+        """(active ordered pairs, those with an Mg end) of an instance."""
+        a, b = C.c_int64(), C.c_int64()
+        self._chk(self.L.dmdgpu_instance_pairs(self.h, iid, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def relax_energy(self, iid):
         """Free energy sum_i S0_i at the solve's current iterate."""

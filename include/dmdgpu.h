@@ -345,6 +345,12 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* ctx, int id, const dmdgpu_relax_opt
 dmdgpu_status dmdgpu_relax_tick(dmdgpu_ctx* ctx, int* running);
 dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* ctx, int id, int* state, double* rnorm,
                                   int* newton_steps, int* residual_evals);
+/* Residual evaluations of a solve by precision (FP64, split), and the active
+ * ordered pairs of an instance (all; those with an Mg end, the pairs a tick
+ * relaxing the Mg lattice evaluates in its gradient phase): work counts for
+ * throughput figures. */
+dmdgpu_status dmdgpu_relax_evals(dmdgpu_ctx* ctx, int id, int* evals_fp64, int* evals_split);
+dmdgpu_status dmdgpu_instance_pairs(dmdgpu_ctx* ctx, int id, int64_t* pairs, int64_t* pairs_mg);
 /* Free energy sum_i S0_i at the solve's current iterate (as last evaluated). */
 dmdgpu_status dmdgpu_relax_energy(dmdgpu_ctx* ctx, int id, double* energy);
 /* Starts every instance and ticks until all are done (or max_ticks). */

@@ -338,3 +338,30 @@ def test_checkpoint_resume(ctx, tmp_path):
     ctx.relax(opts)
     assert ctx.relax_status(0)["state"] == "converged"
     assert np.abs(ctx.instance_state(0)[0] - q_ref).max() < 1e-7
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graph_ticks_equal_direct_launches(mixed):
+    """The tick's device work captured once as a hipGraph (FP64 and split
+    chains on two streams, joined) and replayed gives bit for bit the
+    relaxations of the same ticks launched directly, for a batch whose
+    instances run in different precisions in the same ticks."""
+    say(f"[gpu] graph ticks vs direct launches, mixed={mixed}")
+    syss = [lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=s) for s in (21, 22)] + \
+        [lat.hcp_mgh(4, 3, 3, with_h=False, seed=23)]
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=0.05, mixed=mixed,
+                             precond=40)
+    res = {}
+    for graph in (False, True):
+        c = dmdgpu.Context(0)
+        try:
+            c.set_graph(graph)
+            ids = [c.add_instance(sd) for sd in syss]
+            c.relax(opts)
+            res[graph] = [(c.relax_status(i), c.instance_state(i)) for i in ids]
+        finally:
+            c.close()
+    for (sa, xa), (sb, xb) in zip(res[False], res[True]):
+        assert sa == sb and sa["state"] == "converged"
+        for a, b in zip(xa, xb):
+            assert np.array_equal(a, b)
