@@ -211,8 +211,10 @@ struct dmdgpu_ctx {
   std::vector<dmd::SolveOp> h_ops;
   std::vector<double> h_sdots;
   bool use_graph = true;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // one captured tick per set of precisions evaluated in it (bit 0 FP64,
+  // bit 1 split), so a tick launches no kernels of a precision it does not use
+  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   int graph_extent = -1, graph_ninst = -1;
   dmd::PhaseConfig graph_cfg{};
   long long ticks = 0;
@@ -413,10 +415,12 @@ dmd::SolveVecs make_vecs(dmdgpu_ctx* c) {
 }
 
 void drop_graph(dmdgpu_ctx* c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
+  for (int k = 0; k < 4; k++) {
+    if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    c->gexec[k] = nullptr;
+    c->graph[k] = nullptr;
+  }
   c->graph_extent = -1;
 }
 
@@ -1355,15 +1359,24 @@ void digest_tick(dmdgpu_ctx* c) {
 }
 
 // The device work of one tick: vector updates + evaluation points, residual
-// of every instance that needs one (both precisions), totals, dots.
-dmdgpu_status tick_device(dmdgpu_ctx* c) {
+// of every instance that needs one (the precisions of mask: bit 0 FP64, bit 1
+// split), totals, dots.
+dmdgpu_status tick_device(dmdgpu_ctx* c, int mask) {
   dmd::State s = make_state(c);
   s.noxr = c->d_noxr.p;  // relaxations need no occupancy residual
   const dmd::Outputs o = make_outputs(c);
   const int ni = (int)c->inst.size();
   HIPCHK(dmd::launch_solver_step(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ne.p,
                                  c->d_ops.p, make_vecs(c), c->d_qs.p, o));
-  CHK(eval_chains(c, s, o, true));
+  if (mask == 3) {
+    CHK(eval_chains(c, s, o, true));
+  } else {
+    for (int p = 0; p < 2; p++)
+      if (mask & (1 << p)) {
+        HIPCHK(dmd::launch_phase1(c->stream, c->cfg, p, s, c->T, o));
+        HIPCHK(dmd::launch_phase2(c->stream, c->cfg, p, s, c->T, o));
+      }
+  }
   HIPCHK(dmd::launch_totals(c->stream, s, c->T, o, ni, c->d_chunk_lo.p, c->d_chunk_hi.p));
   HIPCHK(dmd::launch_diff_flux(c->stream, s, c->d_ops.p, make_vecs(c), o, c->d_dnn.p,
                                c->d_dnbr.p));
@@ -1476,12 +1489,15 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
                         c->stream));
   HIPCHK(hipMemcpyAsync(c->d_act.p, (char*)c->pin_up.p + ob + 2 * pb, ab,
                         hipMemcpyHostToDevice, c->stream));
+  int mask = 0;
+  for (int k = 0; k < ni; k++)
+    if (c->h_prec[k] >= 0) mask |= 1 << c->h_prec[k];
   const bool same_cfg = c->graph_cfg == c->cfg;
   if (c->use_graph) {
-    if (!c->gexec || c->graph_extent != c->extent || c->graph_ninst != ni || !same_cfg) {
-      drop_graph(c);
+    if (c->graph_extent != c->extent || c->graph_ninst != ni || !same_cfg) drop_graph(c);
+    if (!c->gexec[mask]) {
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      dmdgpu_status st = tick_device(c);
+      dmdgpu_status st = tick_device(c, mask);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(c->stream, &g);
       if (st || e != hipSuccess) {  // a failed capture leaves no graph behind
@@ -1489,15 +1505,15 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
         if (st) return st;
         return fail(DMDGPU_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
       }
-      c->graph = g;
-      HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      c->graph[mask] = g;
+      HIPCHK(hipGraphInstantiate(&c->gexec[mask], c->graph[mask], nullptr, nullptr, 0));
       c->graph_extent = c->extent;
       c->graph_ninst = ni;
       c->graph_cfg = c->cfg;
     }
-    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    HIPCHK(hipGraphLaunch(c->gexec[mask], c->stream));
   } else {
-    CHK(tick_device(c));
+    CHK(tick_device(c, mask));
   }
   const size_t nd = (size_t)dmd::kSolveDots * ni;
   c->h_sdots.resize(nd);
