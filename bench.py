@@ -367,8 +367,20 @@ def solves_leg(tuning, count, graph=True):
     out = {"instances": n, "sites_total": sum(sizes), "sites_min": min(sizes),
            "sites_max": max(sizes), "atol": 1e-8}
     ctx = dmdgpu.Context(0)
+    if tuning is None:
+        # tuned on this workload: all instances resident, each precision in turn
+        from dmdgpu.tune import autotune
+        ids = [ctx.add_instance(sd) for sd in insts]
+        t32, _ = autotune(ctx, "split32")
+        t64, _ = autotune(ctx, "fp64")
+        for i in ids:
+            ctx.remove_instance(i)
+        tuning = (t64, t32)
+        log(f"solves: autotuned on the batch: fp64 {t64}, split32 {t32}")
     ctx.set_tuning_precision("fp64", *tuning[0])
     ctx.set_tuning_precision("split32", *tuning[1])
+    ctx.set_precision("fp64")
+    out["tuning_k1_k2_unroll"] = {"fp64": list(tuning[0]), "split32": list(tuning[1])}
     ctx.set_graph(graph)
     # preconditioner built once per solve (rebuild interval = max_newton): rebuilding at
     # every Newton step needs the same evaluations on this workload and costs setups
@@ -488,8 +500,10 @@ def main():
         sys.exit(2)
 
     if args.only_solves:
-        parts = args.tuning.split("/") if args.tuning != "auto" else ["8,64,0", "8,64,1"]
-        tun = tuple(tuple(int(v) for v in p.split(",")) for p in (parts[0], parts[-1]))
+        tun = None
+        if args.tuning != "auto":
+            parts = args.tuning.split("/")
+            tun = tuple(tuple(int(v) for v in p.split(",")) for p in (parts[0], parts[-1]))
         print(json.dumps({"solves": solves_leg(tun, args.instances, graph=not args.no_graph)}),
               flush=True)
         return
@@ -549,7 +563,8 @@ def main():
 
     solves = None
     if not args.no_solves:
-        solves = solves_leg(((k1, k2, un), cfg32), args.instances)
+        solves = solves_leg(None if args.tuning == "auto" else ((k1, k2, un), cfg32),
+                            args.instances)
         log(f"solves: {solves}")
 
     rule5 = rule5_leg(ctx, args, pqe)
