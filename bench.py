@@ -338,12 +338,26 @@ def _relax_batched(ctx, insts, opts):
     for i in ids:
         ctx.relax_start(i, opts)
     ticks = 0
-    while ctx.relax_tick():
+    live, tick_s = len(ids), []
+    while True:
+        ta = time.perf_counter()
+        running = ctx.relax_tick()
+        tick_s.append((live, time.perf_counter() - ta))
+        live = running
+        if not running:
+            break
         ticks += 1
     ctx.sync()
     t = time.perf_counter() - t0
     res = [(ctx.relax_status(i), ctx.instance_state(i)[0]) for i in ids]
     flops = solve_flops(ctx, ids)
+    # wall time per tick by the number of solves live in it
+    by_live = {}
+    for lo, hi in ((1, 4), (5, 16), (17, 48), (49, len(ids))):
+        d = [dt for lv, dt in tick_s if lo <= lv <= hi]
+        if d:
+            by_live[f"{lo}-{hi}"] = {"ticks": len(d), "ms_per_tick": 1e3 * sum(d) / len(d)}
+    flops = (flops, by_live)
     for i in ids:
         ctx.remove_instance(i)
     return t, ticks, res, flops
@@ -405,7 +419,8 @@ def solves_leg(tuning, count, graph=True):
                 "converged": sum(b[0]["state"] == "converged" for b in batch),
                 "residual_evals": sum(b[0]["evals"] for b in batch), "ticks": ticks,
                 "ms_per_tick": t_batch * 1e3 / max(ticks, 1),
-                "roofline": roofline_of(fl_batch[0], fl_batch[1], t_batch)},
+                "ms_per_tick_by_live_solves": fl_batch[1],
+                "roofline": roofline_of(fl_batch[0][0], fl_batch[0][1], t_batch)},
             "speedup": t_plain / t_batch, "max_position_diff_A": dq}
         log(f"plain per instance (precond={pc}): {per_inst}")
         if pc:
