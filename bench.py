@@ -487,6 +487,34 @@ def serving_leg(tuning, count, load=0.8, budget_gb=6.0):
         "max_position_diff_A": dq, "max_occupancy_diff": dx}
 
 
+def copy_calibration(mib, reps=3):
+    """Device-to-device copies of `mib` MiB (the HIP runtime's copy kernel):
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE they give the counters' scale
+    (known bytes read and written per dispatch)."""
+    import ctypes as C
+    import dmdgpu
+    dmdgpu.load_library()  # the HIP runtime the library itself uses
+    hip = C.CDLL("libamdhip64.so.7")
+    n = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), n) or hip.hipMalloc(C.byref(dst), n):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, n):
+            raise RuntimeError("hipMemset failed")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            if hip.hipMemcpy(dst, src, n, 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return {"copy_calibration": {"mib": mib, "copies": reps, "bytes_read": mib << 20,
+                                 "bytes_written": mib << 20, "ms_per_copy": dt * 1e3}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,11 +536,17 @@ def main():
                     help="run only the solves/hr leg (with --tuning k1,k2,u/k1,k2,u) and print it")
     ap.add_argument("--no-graph", action="store_true",
                     help="solves leg: launch the tick kernels directly instead of as a hipGraph")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device a few times (the HBM counter "
+                         "calibration of the traffic profile) and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
 
     if args.only_solves:
         tun = None
