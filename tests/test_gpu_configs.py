@@ -15,7 +15,8 @@
   steps: the oracle's own backward-Euler step), two bitwise equal to the
   request served alone.
 * C1: the 500-site pure-Mg crystal relaxed to |R|_2 <= 1e-8, confirmed by the
-  oracle's residual of the final state.
+  oracle's residual of the final state, and to 1e-11, the FP64 resolution of
+  its residual.
 """
 import sys
 
@@ -220,3 +221,38 @@ def test_c1_crystal_relaxation(gpu, oracle, mixed):
     say(f"    oracle |R| of the final state {rn:.3e}")
     assert rn < 1e-7, rn
     assert np.abs(q - sd["q"]).max() > 1e-3  # the noise of 0.01 A relaxed away
+
+
+def test_c1_crystal_tight_tolerance(gpu, oracle):
+    """C1 at the FP64 resolution of its 2,000-unknown residual.  The reference's
+    atol 1e-12 (main-BT-AM.cpp:63) applies to its 3-unknown box-stretch SNES
+    (Mechanical-Relaxation-bulk.cpp:128, 235); here it would apply to 2,000
+    components whose rounding alone is of that size: the GPU's residual and the
+    oracle's (the same FP64 formulas summed in another order) differ by about
+    1e-12 in 2-norm at the relaxed state.  The relaxation reaches 1e-11, and a
+    solve asked for 1e-12 ends converged or stalled on that rounding, below
+    1e-11."""
+    say("[gpu] C1 crystal to the FP64 resolution of the residual")
+    sd = lat.hcp_mgh(5, 5, 5, with_h=False, seed=1)
+    st0, nnb, nbr = oracle.neighbors(sd, RC)
+    out = {}
+    for atol in (1e-11, 1e-12):
+        gpu.set_state(sd)
+        gpu.neighbors()
+        gpu.relax(dmdgpu.relax_opts(atol=atol, max_newton=80, max_cg=200, eta=0.01, precond=80))
+        st = gpu.relax_status(0)
+        say(f"    atol {atol:g}: {st}")
+        out[atol] = st
+        if atol == 1e-11:
+            assert st["state"] == "converged" and st["rnorm"] <= 1e-11, st
+            q, sg, _ = gpu.instance_state(0)
+            fin = dict(sd)
+            fin["q"], fin["sigma"] = q, sg
+            gpu.update_state(q=q, sigma=sg)  # keeps the start state's lists
+            R = gpu.residual()
+            ref = oracle.fact_residual(oracle.system(fin, nnb, nbr))
+            order = np.sqrt(((R["Rq"] - ref["Rq"]) ** 2).sum() + ((R["Rs"] - ref["Rs"]) ** 2).sum())
+            rn = np.sqrt((ref["Rq"] ** 2).sum() + (ref["Rs"] ** 2).sum())
+            say(f"    |R_gpu - R_oracle| {order:.2e} (summation order), oracle |R| {rn:.2e}")
+            assert order < 1e-10 and rn < 1e-10
+    assert out[1e-12]["state"] in ("converged", "stalled") and out[1e-12]["rnorm"] < 1e-11
