@@ -404,6 +404,51 @@ def test_finite_differences_of_free_entropy(synth, rule):
             assert fd == pytest.approx(base["Rx"][u], rel=1e-6)
 
 
+def test_per_pair_derivatives_by_finite_differences(synth):
+    """The reference-order per-(u, i) derivatives (evaluate_DV0_i_Dq_u_MgHx,
+    src/ADP/MgHx-mf-V-bulk.cpp:1823-2098, and evaluate_DV0_i_Dxi_u_adp_MgHx,
+    :2102-2325) against central differences of the reference-order V0_i with
+    the densities re-evaluated, one (u, i) at a time: u = i and two neighbours
+    u of an Mg centre and of an H centre (x only for H centres, where the
+    reference's form is the exact derivative)."""
+    sysd = lat.cluster(2, 2, 2, x_h=(0.05, 0.5), seed=4)
+    st, nnb, nbr = synth.neighbors(sysd, 6.5)
+    assert st == 0
+    s0 = synth.system(sysd, nnb, nbr)
+    rho0 = synth.ref_mf_rho(s0)
+
+    def v0(i, key, u, a, h):
+        sd = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sysd.items()}
+        if key == "q":
+            sd["q"][u, a] += h
+        else:
+            sd[key][u] += h
+        s = synth.system(sd, nnb, nbr)
+        return synth.ref_V0(s, synth.ref_mf_rho(s), [i])[0]
+
+    def fd(i, key, u, a, h):
+        return (v0(i, key, u, a, h) - v0(i, key, u, a, -h)) / (2 * h)
+
+    rng = np.random.default_rng(2)
+    act = sysd["x"] >= 5e-4
+    centres = [int(rng.choice(np.flatnonzero(sysd["species"] == lat.MG))),
+               int(rng.choice(np.flatnonzero((sysd["species"] == lat.H) & act)))]
+    checked = 0
+    for i in centres:
+        row = [int(v) for v in nbr[i, :nnb[i]] if act[v]]
+        for u in [i, row[0], row[len(row) // 2]]:
+            d = synth.ref_DV0_Dq(s0, u, i, rho0)
+            scale = np.abs(d).max() * 1e-7 + 1e-9  # FD rounding: |V0| eps / h ~ 1e-11
+            for a in range(3):
+                assert fd(i, "q", u, a, 1e-5) == pytest.approx(d[a], rel=1e-6, abs=scale)
+            assert fd(i, "sigma", u, 0, 1e-6) == pytest.approx(d[3], rel=1e-6, abs=scale)
+            if sysd["species"][i] == lat.H:
+                dx = synth.ref_DV0_Dxi(s0, u, i, rho0)
+                assert fd(i, "x", u, 0, 1e-6) == pytest.approx(dx, rel=1e-6, abs=1e-9)
+            checked += 1
+    assert checked == 6
+
+
 def test_translation_invariance_free_cluster(oracle):
     sysd = lat.cluster(3, 2, 2, x_h=(0.0, 0.3), seed=5)
     st, nnb, nbr = oracle.neighbors(sysd, RC)
