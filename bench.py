@@ -134,7 +134,98 @@ def cpu_baseline(sysd, nnb, nbr, pqe_per_res, budget_s=12.0):
     d1 = time.perf_counter() - t0
     out["one_core_500_sites"] = {"value": 50 * pairs1 * k / d1, "unit": "PQE/s", "cores": 1,
                                  "sample": f"{k} residuals of 500-site hcp Mg 5x5x5 ({d1:.1f} s)"}
+    # one whole solve: the 500-site crystal relaxed to |R|_2 <= 1e-8 by the
+    # same Newton-CG as the GPU's plain path, on the host cores and on the GPU
+    t0 = time.perf_counter()
+    ev, rn = cpu_newton_cg(O, c1, nn1, nb1, threads)
+    dc = time.perf_counter() - t0
+    ctx = dmdgpu.Context(0)
+    try:
+        ctx.set_state(c1)
+        ctx.neighbors()
+        opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100, eta=0.05, precond=0)
+        ctx.relax(opts)  # first solve captures the tick graphs
+        ctx.set_state(c1)
+        ctx.neighbors()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.relax(opts)
+        dg = time.perf_counter() - t0
+        st = ctx.relax_status(0)
+    finally:
+        ctx.close()
+    out["c1_solve"] = {"cpu_seconds": dc, "cpu_residual_evals": ev, "cpu_rnorm": rn,
+                       "cores": threads, "gpu_seconds": dg, "gpu_residual_evals": st["evals"],
+                       "gpu_rnorm": st["rnorm"], "gpu_vs_cpu": dc / dg,
+                       "sample": "500-site hcp Mg 5x5x5 relaxed (positions and widths) to "
+                                 "|R|_2 <= 1e-8: Newton-CG, one-sided difference Hessian "
+                                 "probes, eta 0.05, no preconditioner, FP64 on both"}
     return out
+
+
+def cpu_newton_cg(O, sd, nnb, nbr, threads, atol=1e-8, eta=0.05, max_newton=60, max_cg=100,
+                  probe=1e-7, cap=0.2):
+    """The GPU relaxation's iteration on the oracle's FP64 residual, for the
+    CPU solve baseline: unknowns the positions and widths of every site;
+    Hessian actions (R(X + e p) - R(X)) / e with RMS displacement `probe`; CG
+    to |r| <= eta |R| (steepest step on negative curvature at the first
+    iteration); steps capped at `cap` A in position and half of each width;
+    backtracking until |R|_2 decreases by 1e-4 of the step.  Returns
+    (residual evaluations, final |R|_2)."""
+    n = int(sd["n"])
+    evals = [0]
+
+    def res(x):
+        evals[0] += 1
+        d = dict(sd)
+        d["q"] = x[:3 * n].reshape(n, 3)
+        d["sigma"] = x[3 * n:]
+        r = O.fact_residual(O.system(d, nnb, nbr), threads)
+        return np.concatenate([r["Rq"].ravel(), r["Rs"]])
+
+    x = np.concatenate([sd["q"].ravel(), sd["sigma"]])
+    g = res(x)
+    for _ in range(max_newton):
+        gn = float(np.linalg.norm(g))
+        if gn <= atol:
+            break
+        d = np.zeros_like(x)
+        r = -g
+        p = r.copy()
+        rr = r @ r
+        for it in range(max_cg):
+            e = probe * np.sqrt(len(x) / (p @ p))
+            ap = (res(x + e * p) - g) / e
+            pap = p @ ap
+            if pap <= 0:
+                if it == 0:
+                    d = -g
+                break
+            a = rr / pap
+            d += a * p
+            r -= a * ap
+            rr1 = r @ r
+            if np.sqrt(rr1) <= eta * gn:
+                break
+            p = r + (rr1 / rr) * p
+            rr = rr1
+        al = 1.0
+        dq = np.abs(d[:3 * n]).max()
+        ds = (-d[3 * n:] / x[3 * n:]).max()
+        if dq > cap:
+            al = cap / dq
+        if ds > 0.5:
+            al = min(al, 0.5 / ds)
+        for _ in range(30):
+            gnew = res(x + al * d)
+            if np.linalg.norm(gnew) <= (1 - 1e-4 * al) * gn:
+                break
+            al *= 0.5
+        else:
+            break
+        x = x + al * d
+        g = gnew
+    return evals[0], float(np.linalg.norm(g))
 
 
 def split_leg(ctx, args, pqe):
@@ -381,8 +472,11 @@ def solves_leg(tuning, count, graph=True):
     out = {"instances": n, "sites_total": sum(sizes), "sites_min": min(sizes),
            "sites_max": max(sizes), "atol": 1e-8}
     ctx = dmdgpu.Context(0)
+    t64_plain = None
     if tuning is None:
-        # tuned on this workload: all instances resident, each precision in turn
+        # each path tuned on its own work: the batched path with all instances
+        # resident, each precision in turn; the plain path's FP64 setting over
+        # single instances (the smallest, the median and the largest) timed alone
         from dmdgpu.tune import autotune
         ids = [ctx.add_instance(sd) for sd in insts]
         t32, _ = autotune(ctx, "split32")
@@ -390,11 +484,25 @@ def solves_leg(tuning, count, graph=True):
         for i in ids:
             ctx.remove_instance(i)
         tuning = (t64, t32)
-        log(f"solves: autotuned on the batch: fp64 {t64}, split32 {t32}")
+        order = np.argsort(sizes)
+        total = {}
+        for k in (order[0], order[len(order) // 2], order[-1]):
+            ctx.set_state(insts[k])
+            ctx.neighbors()
+            for cfg, (p1, p2) in autotune(ctx, "fp64")[1].items():
+                a, b = total.get(cfg, (0.0, 0.0))
+                total[cfg] = (a + p1, b + p2)
+        best1 = min((v[0], k[0]) for k, v in total.items())[1]
+        best2 = min((v[1], (k[1], k[2])) for k, v in total.items())[1]
+        t64_plain = (best1, best2[0], best2[1])
+        log(f"solves: autotuned on the batch: fp64 {t64}, split32 {t32}; "
+            f"single instances: fp64 {t64_plain}")
     ctx.set_tuning_precision("fp64", *tuning[0])
     ctx.set_tuning_precision("split32", *tuning[1])
     ctx.set_precision("fp64")
-    out["tuning_k1_k2_unroll"] = {"fp64": list(tuning[0]), "split32": list(tuning[1])}
+    t64_plain = t64_plain or tuning[0]
+    out["tuning_k1_k2_unroll"] = {"fp64": list(tuning[0]), "split32": list(tuning[1]),
+                                  "fp64_plain_path": list(t64_plain)}
     ctx.set_graph(graph)
     # preconditioner built once per solve (rebuild interval = max_newton): rebuilding at
     # every Newton step needs the same evaluations on this workload and costs setups
@@ -405,7 +513,9 @@ def solves_leg(tuning, count, graph=True):
         # eta in 0.01..0.3 on this workload)
         mixed_opts = dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100,
                                        eta=0.05 if pc else 0.1, mixed=True, precond=pc)
+        ctx.set_tuning_precision("fp64", *t64_plain)
         t_plain, plain, per_inst, fl_plain = _relax_plain(ctx, insts, plain_opts)
+        ctx.set_tuning_precision("fp64", *tuning[0])
         t_batch, ticks, batch, fl_batch = _relax_batched(ctx, insts, mixed_opts)
         dq = max(float(np.abs(b[1] - p[1]).max()) for b, p in zip(batch, plain))
         rec = {
@@ -515,6 +625,32 @@ def copy_calibration(mib, reps=3):
                                  "bytes_written": mib << 20, "ms_per_copy": dt * 1e3}}
 
 
+def tick_flops_leg(tuning, ticks=4):
+    """A few relaxation ticks of the 54,000-site workload (Mg positions and
+    widths, residual-norm line search), FP64 probes and then split-precision
+    probes, tick kernels launched directly: the run whose hardware FLOP
+    counters give the per-pair FLOPs of relaxation ticks (which leave out R_x,
+    the pairs with two H ends in phase 2 and the pair-rule values)."""
+    import dmdgpu
+    ctx = dmdgpu.Context(0)
+    try:
+        ctx.set_graph(False)
+        ctx.set_tuning_precision("fp64", *tuning[0])
+        ctx.set_tuning_precision("split32", *tuning[1])
+        ctx.set_state(workload())
+        ctx.neighbors()
+        pairs, pairs_mg = ctx.instance_pairs(0)
+        for mixed in (False, True):
+            ctx.relax_start(0, dmdgpu.relax_opts(atol=1e-8, max_newton=60, max_cg=100,
+                                                 eta=0.05, mixed=mixed))
+            for _ in range(ticks):
+                ctx.relax_tick()
+        ctx.sync()
+    finally:
+        ctx.close()
+    return {"tick_flops": {"pairs": int(pairs), "pairs_mg": int(pairs_mg), "ticks": ticks}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -536,6 +672,10 @@ def main():
                     help="run only the solves/hr leg (with --tuning k1,k2,u/k1,k2,u) and print it")
     ap.add_argument("--no-graph", action="store_true",
                     help="solves leg: launch the tick kernels directly instead of as a hipGraph")
+    ap.add_argument("--tick-flops", action="store_true",
+                    help="only run a few relaxation ticks of the headline workload (FP64, then "
+                         "split probes; for the FLOP counters of relaxation ticks) and print "
+                         "its pair counts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device a few times (the HBM counter "
                          "calibration of the traffic profile) and print it")
@@ -546,6 +686,11 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
+    if args.tick_flops:
+        parts = (args.tuning if args.tuning != "auto" else "8,64,0/8,64,1").split("/")
+        tun = tuple(tuple(int(v) for v in p.split(",")) for p in (parts[0], parts[-1]))
+        print(json.dumps(tick_flops_leg(tun)), flush=True)
         return
 
     if args.only_solves:

@@ -393,27 +393,39 @@ DEV int group_bcast(int v) {
   }
 }
 
+// Row loads address base + a 32-bit byte offset: with a wavefront-uniform
+// base the load takes the scalar-base + 32-bit vector-offset form, which saves
+// the 64-bit address arithmetic (sign extension, shift, add) of every gathered
+// row.  Callers keep offsets below 4 GiB: the tables are a few MB, and phase 2
+// addresses adjoint rows from the first record of their instance
+// (kMaxInstanceSites).
+template <typename T>
+DEV const T* at_off(const T* base, unsigned off) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+DEV u32x4 ld16(const u32x4* base, unsigned idx) { return *at_off(base, idx * 16u); }
+
 template <int NQ>
 DEV void coop_rounds(const u32x4* base, int row, int a, u32x4* v) {
   if constexpr (NQ == 1) {
-    v[0] = base[row];
+    v[0] = ld16(base, (unsigned)row);
   } else if constexpr (NQ == 8) {
-    v[0] = base[(size_t)group_bcast<8, 0>(row) * 8 + a];
-    v[1] = base[(size_t)group_bcast<8, 1>(row) * 8 + a];
-    v[2] = base[(size_t)group_bcast<8, 2>(row) * 8 + a];
-    v[3] = base[(size_t)group_bcast<8, 3>(row) * 8 + a];
-    v[4] = base[(size_t)group_bcast<8, 4>(row) * 8 + a];
-    v[5] = base[(size_t)group_bcast<8, 5>(row) * 8 + a];
-    v[6] = base[(size_t)group_bcast<8, 6>(row) * 8 + a];
-    v[7] = base[(size_t)group_bcast<8, 7>(row) * 8 + a];
+    v[0] = ld16(base, (unsigned)group_bcast<8, 0>(row) * 8u + (unsigned)a);
+    v[1] = ld16(base, (unsigned)group_bcast<8, 1>(row) * 8u + (unsigned)a);
+    v[2] = ld16(base, (unsigned)group_bcast<8, 2>(row) * 8u + (unsigned)a);
+    v[3] = ld16(base, (unsigned)group_bcast<8, 3>(row) * 8u + (unsigned)a);
+    v[4] = ld16(base, (unsigned)group_bcast<8, 4>(row) * 8u + (unsigned)a);
+    v[5] = ld16(base, (unsigned)group_bcast<8, 5>(row) * 8u + (unsigned)a);
+    v[6] = ld16(base, (unsigned)group_bcast<8, 6>(row) * 8u + (unsigned)a);
+    v[7] = ld16(base, (unsigned)group_bcast<8, 7>(row) * 8u + (unsigned)a);
   } else if constexpr (NQ == 4) {
-    v[0] = base[(size_t)group_bcast<4, 0>(row) * 4 + a];
-    v[1] = base[(size_t)group_bcast<4, 1>(row) * 4 + a];
-    v[2] = base[(size_t)group_bcast<4, 2>(row) * 4 + a];
-    v[3] = base[(size_t)group_bcast<4, 3>(row) * 4 + a];
+    v[0] = ld16(base, (unsigned)group_bcast<4, 0>(row) * 4u + (unsigned)a);
+    v[1] = ld16(base, (unsigned)group_bcast<4, 1>(row) * 4u + (unsigned)a);
+    v[2] = ld16(base, (unsigned)group_bcast<4, 2>(row) * 4u + (unsigned)a);
+    v[3] = ld16(base, (unsigned)group_bcast<4, 3>(row) * 4u + (unsigned)a);
   } else {
-    v[0] = base[(size_t)group_bcast<2, 0>(row) * 2 + a];
-    v[1] = base[(size_t)group_bcast<2, 1>(row) * 2 + a];
+    v[0] = ld16(base, (unsigned)group_bcast<2, 0>(row) * 2u + (unsigned)a);
+    v[1] = ld16(base, (unsigned)group_bcast<2, 1>(row) * 2u + (unsigned)a);
   }
 }
 
@@ -425,7 +437,7 @@ DEV void gather8(const P* const* t8, const int* r8, u32x4* stage, P (*o8)[8]) {
   if constexpr (!COOP) {
 #pragma unroll
     for (int q = 0; q < N; q++) {
-      const P* c = t8[q] + ((size_t)r8[q] << 3);
+      const P* c = at_off(t8[q], (unsigned)r8[q] * (unsigned)(8 * sizeof(P)));
 #pragma unroll
       for (int e = 0; e < 8; e++) o8[q][e] = c[e];
     }
@@ -455,14 +467,15 @@ DEV void gather8(const P* const* t8, const int* r8, u32x4* stage, P (*o8)[8]) {
 
 // A pair-rule point's rows: o8 = u/w row r8 of t8 (8 values), o16 = fused
 // row r16 of t16 (16 values: phi, rho_Mg, rho_H, 0), with one LDS round trip.
-// C8 / C16: cooperative or direct for each.  stage: CptRounds<P, C8, C16> *
-// kStageSlots slots.
+// C8 / C16: cooperative or direct for each; L8 = false loads the fused row
+// only (o8 untouched).  stage: CptRounds<P, C8, C16> * kStageSlots slots.
 template <typename P, bool C8, bool C16> struct CptRounds {
   static constexpr int v = (C8 ? RowQ<P, 8>::v : 0) + (C16 ? RowQ<P, 16>::v : 0) > 0
                                ? (C8 ? RowQ<P, 8>::v : 0) + (C16 ? RowQ<P, 16>::v : 0) : 1;
 };
-template <typename P, bool C8, bool C16>
+template <typename P, bool C8_, bool C16, bool L8 = true>
 DEV void gather_cpt(const P* t8, int r8, const P* t16, int r16, u32x4* stage, P* o8, P* o16) {
+  constexpr bool C8 = C8_ && L8;
   constexpr int Q8 = RowQ<P, 8>::v, Q16 = RowQ<P, 16>::v;
   const int lane = (int)__lane_id();
   const int a8 = lane & (Q8 - 1), a16 = lane & (Q16 - 1);
@@ -470,14 +483,14 @@ DEV void gather_cpt(const P* t8, int r8, const P* t16, int r16, u32x4* stage, P*
   const u32x4* b16 = reinterpret_cast<const u32x4*>(t16);
   u32x4 v8[Q8], v16[Q16];
   if constexpr (C8) coop_rounds<Q8>(b8, r8, a8, v8);
-  else {
+  else if constexpr (L8) {
 #pragma unroll
-    for (int c = 0; c < Q8; c++) v8[c] = b8[(size_t)r8 * Q8 + c];
+    for (int c = 0; c < Q8; c++) v8[c] = ld16(b8, (unsigned)r8 * Q8 + c);
   }
   if constexpr (C16) coop_rounds<Q16>(b16, r16, a16, v16);
   else {
 #pragma unroll
-    for (int c = 0; c < Q16 - 1; c++) v16[c] = b16[(size_t)r16 * Q16 + c];  // last chunk: padding
+    for (int c = 0; c < Q16 - 1; c++) v16[c] = ld16(b16, (unsigned)r16 * Q16 + c);  // last chunk: padding
     v16[Q16 - 1] = u32x4{0, 0, 0, 0};
   }
   if constexpr (C8 || C16) {
@@ -502,7 +515,7 @@ DEV void gather_cpt(const P* t8, int r8, const P* t16, int r16, u32x4* stage, P*
       for (int c = 0; c < Q16; c++) v16[c] = stage[(o + a16) * kStageSlots + (lane - a16) + c];
     }
   }
-  __builtin_memcpy(o8, v8, sizeof(v8));
+  if constexpr (L8) __builtin_memcpy(o8, v8, sizeof(v8));
   __builtin_memcpy(o16, v16, sizeof(v16));
 }
 

@@ -56,22 +56,36 @@ struct Tables {
 struct Box {
   double lo[3], L[3], invL[3];
   int per[3];
+  int base;  // first arena site of the instance
 };
 
 // Sites of all resident instances live in one arena.  Every instance starts
 // at a multiple of kChunk sites, so a kChunk-aligned block of sites belongs to
 // a single instance.
 constexpr int kChunk = 256;
+// Largest instance (sites, a multiple of kChunk): phase 2 addresses a
+// neighbour's adjoint rows by a 32-bit byte offset from its instance's first
+// record (kRec doubles per site), so an instance's records stay below 4 GiB.
+constexpr int kMaxInstanceSites = (int)((4294967296ull / (kRec * sizeof(double))) / kChunk * kChunk);
 constexpr int kSlotW = 5;
+
+// What a solver tick's evaluation may leave out, per instance (State::noxr):
+// the occupancy residual R_x (relaxations keep occupancies fixed); the pairs
+// with two H ends and the H sites' sums (only Mg positions and widths are
+// relaxed: an H-H pair adds to nothing else); the pair-rule values C6 and Phi
+// (the residual-norm line search needs no energies: V0, S0 and the energy
+// totals are then not formed, every gradient is).
+constexpr int kSkipRx = 1;
+constexpr int kSkipHH = 2;
+constexpr int kSkipValues = 4;
 
 struct State {
   int n;               // arena extent (sites)
   const int* inst;     // instance of each site, -1 for a free slot
   const Box* boxes;    // per instance
   const int* prec;     // per instance: Precision to evaluate with, -1 = skip
-  // per instance (nullptr: none): 1 = skip the occupancy residual; 2 = also
-  // skip the pairs with two H ends and the H sites' sums (only Mg positions
-  // and widths are needed: an H-H pair adds to nothing else)
+  // per instance (nullptr: none): kSkip* bits, what a solver tick's
+  // evaluation leaves out
   const int* noxr;
   const double4* qs;   // (x, y, z, sigma)
   const double* ne;    // effective occupancy: x if active (x >= 5e-4) else 0

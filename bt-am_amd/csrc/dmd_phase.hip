@@ -91,18 +91,12 @@ constexpr int kPairGroup = 8;
 // the same points.  The owner writes (C6, Phi, density at that end) into the
 // slots of both ends (the slots are free until phase 2); k_phase1 adds them up
 // in row order.  Work layout as in phase 2 (per-chunk owned-pair lists).
-template <typename P, int B, int RULE>
-__global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs o) {
+// VALUES = false (relaxation ticks with the residual-norm line search): the
+// densities only -- the fused row of each point, no u/w row, no C6 or Phi.
+template <typename P, int B, int RULE, bool VALUES>
+DEV void phase1_pairs(const State& s, const Tables& T, const Outputs& o, int inst, int ch,
+                      int part, u32x4* stage) {
   constexpr bool CO = kCoop1<P>;
-  __shared__ u32x4 stage_all[4][CptRounds<P, CO, CO>::v * kStageSlots];
-  u32x4* stage = stage_all[threadIdx.x >> 6];
-  const int* act = s.act + PrecCode<P>::v * s.act_stride;
-  const int nb = act[0] * B;
-  if ((int)blockIdx.x >= nb) return;  // uniform per block
-  const int lb = xcd_block(blockIdx.x, nb);
-  const int ch = act[1 + lb / B];
-  const int inst = s.inst[ch * kChunk];
-  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;
   const Box box = s.boxes[inst];
   const P* uwt = tab_uw<P>(T);
   const P* prt = tab_pr<P>(T);
@@ -111,7 +105,7 @@ __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs
   const int np = s.chunk_np[ch];
   const int* own = s.own + (size_t)ch * kChunk * kCap;
   const int tq = (int)threadIdx.x & (kPairGroup - 1);
-  for (int p0 = (lb % B) * 256 + ((int)threadIdx.x & ~(kPairGroup - 1)); p0 < np; p0 += B * 256) {
+  for (int p0 = part * 256 + ((int)threadIdx.x & ~(kPairGroup - 1)); p0 < np; p0 += B * 256) {
     const bool inb = p0 + tq < np;  // lanes past the end repeat the group's first pair
     const int ent = own[inb ? p0 + tq : p0];
     const int u = c0 + (ent & 255), kk = (ent >> 8) & 255, pos = ent >> 16;
@@ -130,10 +124,12 @@ __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs
       P p[3], off, ir, dsi[3], dsj[3], cw[8], f[16], cr[4], ps[9];
       int m;
       const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
-      gather_cpt<P, CO, CO>(uwt, t * nr + m, prt, t * nr + m, stage, cw, f);
-      psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
-      C6 += wt * (double)dotM(ps, ps);
-      Phi += wt * (double)cval(f, off);
+      gather_cpt<P, CO, CO, VALUES>(uwt, t * nr + m, prt, t * nr + m, stage, cw, f);
+      if constexpr (VALUES) {
+        psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
+        C6 += wt * (double)dotM(ps, ps);
+        Phi += wt * (double)cval(f, off);
+      }
       fused_rho(f, spk, cr);
       Ru += wt * (double)cval(cr, off);
       fused_rho(f, spu, cr);
@@ -146,6 +142,24 @@ __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs
       ek[0] = C6; ek[1] = Phi; ek[2] = Rk;
     }
   }
+}
+
+template <typename P, int B, int RULE>
+__global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs o) {
+  constexpr bool CO = kCoop1<P>;
+  __shared__ u32x4 stage_all[4][CptRounds<P, CO, CO>::v * kStageSlots];
+  u32x4* stage = stage_all[threadIdx.x >> 6];
+  const int* act = s.act + PrecCode<P>::v * s.act_stride;
+  const int nb = act[0] * B;
+  if ((int)blockIdx.x >= nb) return;  // uniform per block
+  const int lb = xcd_block(blockIdx.x, nb);
+  const int ch = act[1 + lb / B];
+  const int inst = s.inst[ch * kChunk];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;
+  if (s.noxr && (s.noxr[inst] & kSkipValues))
+    phase1_pairs<P, B, RULE, false>(s, T, o, inst, ch, lb % B, stage);
+  else
+    phase1_pairs<P, B, RULE, true>(s, T, o, inst, ch, lb % B, stage);
 }
 
 template <typename P, int K, int RULE>
@@ -372,15 +386,23 @@ DEV void blk_from(const P* row, P tail, bool flip, P* o) {
 }
 
 // Block b of neighbour k's record (k differs between lanes): a cooperative
-// gather of its row plus the tail value.
+// gather of its row plus the tail value.  ib: the adjoint rows of k's instance
+// (inst_adj), k0: that instance's first site.
 template <typename P, bool CO>
-DEV void nbr_blk(const Outputs& o, int k, int b, bool flip, u32x4* stage, P* out) {
-  const P* base = adj_base(o, P(0));
+DEV void nbr_blk(const Outputs& o, const P* ib, int k0, int k, int b, bool flip, u32x4* stage,
+                 P* out) {
   const P tail = adj_rec(o, k, P(0))[kRecTail + b];
-  const int r = k * adj_rows<P>() + b;
+  const int r = (k - k0) * adj_rows<P>() + b;
   P row[1][8];
-  gather8<P, 1, CO>(&base, &r, stage, row);
+  gather8<P, 1, CO>(&ib, &r, stage, row);
   blk_from(row[0], tail, flip, out);
+}
+
+// The adjoint rows of the instance whose first site is k0: row offsets from
+// here stay below 4 GiB (kMaxInstanceSites), as the row gathers need.
+template <typename P>
+DEV const P* inst_adj(const Outputs& o, int k0) {
+  return adj_base(o, P(0)) + (size_t)k0 * adj_rows<P>() * 8;
 }
 
 // ---------------------------------------------------------------------------
@@ -419,10 +441,11 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
   const Box box = s.boxes[inst];
   const P* uwt = tab_uw<P>(T);
   const P* prt = tab_pr<P>(T);
+  const P* adjI = inst_adj<P>(o, box.base);
   const int nr = T.nr;
   const int c0 = ch * kChunk;
   // Mg positions and widths only: the list's H-H tail adds to nothing needed
-  const int np = s.noxr && s.noxr[inst] == 2 ? s.chunk_npx[ch] : s.chunk_np[ch];
+  const int np = s.noxr && (s.noxr[inst] & kSkipHH) ? s.chunk_npx[ch] : s.chunk_np[ch];
   const int* own = s.own + (size_t)ch * kChunk * kCap;
   const int tq = (int)threadIdx.x & (kPairGroup - 1);
   for (int p0 = part * 256 + ((int)threadIdx.x & ~(kPairGroup - 1)); p0 < np; p0 += B * 256) {
@@ -490,8 +513,8 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
       geo.point(T, l, sh * skP, p, mm, o2, ir);
       const int blk = 2 * l + (1 - sg);
       const P tail = Ak[kRecTail + blk];
-      const int r8[2] = {rb + mm, k * adj_rows<P>() + blk};
-      const P* t8[2] = {uwt, adj_base(o, P(0))};
+      const int r8[2] = {rb + mm, (k - box.base) * adj_rows<P>() + blk};
+      const P* t8[2] = {uwt, adjI};
       gather8<P, 2, CO>(t8, r8, stage, c);
       blk_from(c[1], tail, true, lb);
       const P uu = cval(c[0], o2), du = cder(c[0], o2), ww = cval(c[0] + 4, o2),
@@ -543,7 +566,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
     {
       P psi0[9];
       psi_from(kept_out(u0), w0, pz, psi0);
-      nbr_blk<P, CO>(o, k, kBlkLG0, true, stage, lbA);
+      nbr_blk<P, CO>(o, adjI, box.base, k, kBlkLG0, true, stage, lbA);
       applyM(cnu, psi0, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) { lbA[c] += tmp[c]; SA[c] = P(0); }
@@ -587,7 +610,7 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
     // unshifted point, role (b): wbP * (LU_k - nu/9 M SA)
     {
       P lb[9], L[9], g3[3];
-      nbr_blk<P, CO>(o, k, kBlkLU, true, stage, lb);
+      nbr_blk<P, CO>(o, adjI, box.base, k, kBlkLU, true, stage, lb);
       applyM(cnu, SA, tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++) {
@@ -671,7 +694,7 @@ __global__ __launch_bounds__(256, 2) void k_phase2(State s, Tables T, Outputs o)
   const int ch = act[1 + lb / B];
   const int inst = s.inst[ch * kChunk];
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;
-  if (s.noxr && s.noxr[inst])
+  if (s.noxr && (s.noxr[inst] & kSkipRx))
     phase2_pairs<P, B, UNR, RULE, false>(s, T, o, inst, ch, lb % B, stage);
   else
     phase2_pairs<P, B, UNR, RULE, true>(s, T, o, inst, ch, lb % B, stage);
@@ -687,10 +710,10 @@ __global__ __launch_bounds__(256) void k_phase2_sum(State s, Tables T, Outputs o
   if (!site_of<P, K>(s, u, g)) return;
   const int inst = s.inst[u];
   if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
-  const bool xr = !(s.noxr && s.noxr[inst]);
+  const bool xr = !(s.noxr && (s.noxr[inst] & kSkipRx));
   // Mg positions and widths only: an H site's slots are partly unwritten
   // (its H-H pairs were skipped) and its R_q, R_sigma are not needed
-  const bool hq = !(s.noxr && s.noxr[inst] == 2 && s.spc[u] == 1);
+  const bool hq = !(s.noxr && (s.noxr[inst] & kSkipHH) && s.spc[u] == 1);
   const double nu = s.ne[u];
   if (nu == 0.0) {
     if (g == 0) {

@@ -104,6 +104,7 @@ struct Solve {
   double rnorm = 0, rnorm0 = 0, rr = 0, pp = 0, eps = 0, alpha_ls = 1;
   double rz = 0;          // r . M^-1 r of the preconditioned CG
   double energy = 0;      // free energy sum_i S0_i at the iterate X
+  bool energy_lazy = false;  // ticks formed no energies: evaluate at the end
   double slope = 0;       // R . D, the energy's slope along the Newton step
   bool pc_setup = false;  // build the preconditioner at X before the next tick
   int ndof = 0;
@@ -126,10 +127,11 @@ struct Instance {
   Solve sv;
   // The residual buffers hold a relaxation tick's evaluation, which leaves
   // R_x out (and, with only Mg unknowns relaxed, R_q and R_sigma of the H
-  // sites: hq_stale); rx_prec is the precision to re-evaluate in for a full
-  // residual.
+  // sites: hq_stale; with the residual-norm line search, the energies:
+  // e_stale); rx_prec is the precision to re-evaluate in for a full residual.
   bool rx_stale = false;
   bool hq_stale = false;
+  bool e_stale = false;
   int rx_prec = 0;
 };
 
@@ -809,6 +811,9 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
                            const double* box, const int32_t* periodic, int* out_id) {
   if (n <= 0 || !q || !sg || !x || !spc || !beta || !gamma || !box || !periodic)
     return fail(DMDGPU_ERR_ARG, "null argument or n <= 0");
+  if (n > dmd::kMaxInstanceSites)
+    return fail(DMDGPU_ERR_ARG, "an instance holds at most " +
+                                    std::to_string(dmd::kMaxInstanceSites) + " sites");
   HIPCHK(hipSetDevice(c->device));
   for (int i = 0; i < n; i++)
     if (spc[i] != 1 && spc[i] != 12)
@@ -831,6 +836,7 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   I.n = n;
   I.cap = (n + kChunk - 1) / kChunk * kChunk;
   I.base = place(c, I.cap);
+  I.box.base = I.base;
   const size_t need = (size_t)I.base + I.cap;
   // The arena in use after placement, with the context's other allocations,
   // must fit the budget (ensure_sites also bounds what a growth allocates).
@@ -939,7 +945,7 @@ dmdgpu_status run_eval(dmdgpu_ctx* c, bool phase2 = true) {
   for (int k = 0; k < (int)c->inst.size(); k++)
     if (c->inst[k].used && c->h_prec[k] >= 0) {
       any[c->h_prec[k]] = true;
-      if (phase2) c->inst[k].rx_stale = c->inst[k].hq_stale = false;
+      if (phase2) c->inst[k].rx_stale = c->inst[k].hq_stale = c->inst[k].e_stale = false;
     }
   const dmd::State s = make_state(c);
   const dmd::Outputs o = make_outputs(c);
@@ -1018,9 +1024,10 @@ dmdgpu_status d2h_sites(dmdgpu_ctx* c, const Instance& I, double* h, const doubl
 dmdgpu_status copy_residual(dmdgpu_ctx* c, int id, double* Rq, double* Rs, double* Rx,
                             double* tot) {
   Instance& I = c->inst[id];
-  if ((Rx && I.rx_stale) || ((Rq || Rs) && I.hq_stale)) {
+  if ((Rx && I.rx_stale) || ((Rq || Rs) && I.hq_stale) || (tot && I.e_stale)) {
     // Last evaluated by a relaxation tick: evaluate again, in the same
-    // precision, at the same point, with R_x and the H sites' sums.
+    // precision, at the same point, with R_x, the H sites' sums and the
+    // energies.
     std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
     c->h_prec[id] = I.rx_prec;
     CHK(run_eval(c, true));
@@ -1469,9 +1476,14 @@ dmdgpu_status run_tick(dmdgpu_ctx* c) {
     Instance& I = c->inst[k];
     if (!I.used || c->h_prec[k] < 0) continue;
     const bool h_dof = (I.sv.o.dof_mask & (DMDGPU_DOF_H_Q | DMDGPU_DOF_H_SIGMA)) != 0;
-    c->h_noxr[k] = I.sv.stage == ST_DIFF ? 0 : h_dof ? 1 : 2;
-    I.rx_stale = c->h_noxr[k] != 0;
-    I.hq_stale = c->h_noxr[k] == 2;
+    int skip = 0;  // a diffusion sweep needs the whole evaluation (R_x)
+    if (I.sv.stage != ST_DIFF)
+      skip = dmd::kSkipRx | (h_dof ? 0 : dmd::kSkipHH) | (I.sv.o.merit ? 0 : dmd::kSkipValues);
+    c->h_noxr[k] = skip;
+    I.rx_stale = skip != 0;
+    I.hq_stale = (skip & dmd::kSkipHH) != 0;
+    I.e_stale = (skip & dmd::kSkipValues) != 0;
+    if (I.e_stale) I.sv.energy_lazy = true;
     I.rx_prec = c->h_prec[k];
   }
   const size_t ob = sizeof(dmd::SolveOp) * ni, pb = sizeof(int) * ni;
@@ -2132,6 +2144,22 @@ dmdgpu_status dmdgpu_relax_energy(dmdgpu_ctx* c, int id, double* energy) {
   Instance* I = get_inst(c, id);
   if (!I) return fail(DMDGPU_ERR_ARG, "no such instance");
   if (!energy) return fail(DMDGPU_ERR_ARG, "null output");
+  if (I->sv.energy_lazy) {
+    // The solve's ticks formed no energies (residual-norm line search): the
+    // energy of its final iterate, evaluated once in FP64.
+    if (I->sv.stage != ST_DONE)
+      return fail(DMDGPU_ERR_STATE, "energy of a running residual-merit solve");
+    HIPCHK(hipSetDevice(c->device));
+    std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+    c->h_prec[id] = dmd::kFP64;
+    CHK(run_eval(c, true));
+    double t3[3];
+    HIPCHK(hipMemcpyAsync(t3, c->d_totals.p + 3 * id, sizeof(t3), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    I->sv.energy = t3[0];
+    I->sv.energy_lazy = false;
+  }
   *energy = I->sv.energy;
   return DMDGPU_OK;
 }

@@ -86,7 +86,10 @@ dmdgpu_status dmdgpu_set_cutoff(dmdgpu_ctx* ctx, double rc);
  * Copies the DMSwarm-equivalent fields (src/Atoms/Atom.cpp:149-203:
  * coor, stdv-q, molar-fraction, specie, thermalmultp = beta,
  * chemicalmultp = gamma) and the box (dump-input.cpp:76-125) to device SoA
- * buffers.  box = {xlo, ylo, zlo, xhi, yhi, zhi}; periodic[3] = 1 for "pp". */
+ * buffers.  box = {xlo, ylo, zlo, xhi, yhi, zhi}; periodic[3] = 1 for "pp".
+ * At most 6,710,784 sites per instance (DMDGPU_ERR_ARG beyond: the gradient
+ * kernel addresses an instance's adjoint records, 640 B per site, with 32-bit
+ * byte offsets). */
 dmdgpu_status dmdgpu_set_state(dmdgpu_ctx* ctx, int n, const double* h_mean_q,
                                const double* h_stdv_q, const double* h_xi,
                                const int32_t* h_specie, const double* h_beta,
@@ -247,7 +250,8 @@ dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* ctx, int order);
 
 /* Copies an instance in (same fields as dmdgpu_set_state), builds its neighbour
  * lists and returns its id.  DMDGPU_ERR_BUDGET when the arena would exceed the
- * budget set with dmdgpu_set_budget. */
+ * budget set with dmdgpu_set_budget; DMDGPU_ERR_ARG for more than 6,710,784
+ * sites (as dmdgpu_set_state). */
 dmdgpu_status dmdgpu_add_instance(dmdgpu_ctx* ctx, int n, const double* h_mean_q,
                                   const double* h_stdv_q, const double* h_xi,
                                   const int32_t* h_specie, const double* h_beta,
@@ -291,11 +295,13 @@ dmdgpu_status dmdgpu_instance_set_unknowns(dmdgpu_ctx* ctx, int id, const double
  * evaluations too, and below it every accepted residual and the stopping test
  * are FP64.  Instances advance independently: one tick gives every running
  * instance exactly one residual evaluation (all instances in one launch).
- * A relaxation tick's evaluation forms only what the selected unknowns need:
- * no R_x and, when dof_mask selects no H unknowns, no pairs with two H ends
- * and no R_q, R_sigma of the H sites (every Mg row is the same bit for bit);
- * dmdgpu_instance_residual after such a tick evaluates again, at the same
- * point and in the same precision, for the whole residual. */
+ * A relaxation tick's evaluation forms only what the solve needs: no R_x;
+ * when dof_mask selects no H unknowns, no pairs with two H ends and no R_q,
+ * R_sigma of the H sites; with the residual-norm line search (merit = 0), no
+ * energies (the pair-rule points give the densities only).  Every relaxed
+ * component is the same bit for bit.  dmdgpu_instance_residual after such a
+ * tick evaluates again, at the same point and in the same precision, for the
+ * whole residual and the totals. */
 typedef struct {
   double atol;     /* stop when |(R_q, R_sigma)|_2 <= atol                */
   int max_newton;  /* Newton steps                                        */
@@ -351,7 +357,10 @@ dmdgpu_status dmdgpu_relax_status(dmdgpu_ctx* ctx, int id, int* state, double* r
  * throughput figures. */
 dmdgpu_status dmdgpu_relax_evals(dmdgpu_ctx* ctx, int id, int* evals_fp64, int* evals_split);
 dmdgpu_status dmdgpu_instance_pairs(dmdgpu_ctx* ctx, int id, int64_t* pairs, int64_t* pairs_mg);
-/* Free energy sum_i S0_i at the solve's current iterate (as last evaluated). */
+/* Free energy sum_i S0_i at the solve's current iterate: with merit = 1 as
+ * the solve last evaluated it; with merit = 0 (whose ticks form no energies)
+ * evaluated in FP64 at the final iterate once the solve has ended
+ * (DMDGPU_ERR_STATE while it runs). */
 dmdgpu_status dmdgpu_relax_energy(dmdgpu_ctx* ctx, int id, double* energy);
 /* Starts every instance and ticks until all are done (or max_ticks). */
 dmdgpu_status dmdgpu_relax(dmdgpu_ctx* ctx, const dmdgpu_relax_opts* opts, int max_ticks);

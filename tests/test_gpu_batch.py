@@ -262,6 +262,29 @@ def test_energy_merit_descends_where_residual_merit_stalls(ctx):
     assert out["energy"][0]["rnorm"] < out["residual"][0]["rnorm"]
 
 
+def test_residual_merit_energy_of_final_iterate(ctx):
+    """Relaxation ticks with the residual-norm line search form no energies
+    (the pair-rule points give densities only); relax_energy then evaluates
+    the final iterate in FP64 once the solve has ended: bit for bit the total
+    of a full evaluation at that state, and the residual read after the solve
+    is the full one (its totals included)."""
+    say("[gpu] residual-merit ticks: energy of the final iterate")
+    sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=31)
+    ctx.set_state(sd)
+    ctx.neighbors()
+    ctx.relax(dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=0.05, precond=40))
+    st = ctx.relax_status(0)
+    assert st["state"] == "converged", st
+    after = ctx.instance_residual(0)  # evaluated again: whole residual and totals
+    e = ctx.relax_energy(0)
+    q, sg, _ = ctx.instance_state(0)
+    ctx.update_state(q=q, sigma=sg)
+    full = ctx.residual()
+    assert e == full["S0_total"] and after["S0_total"] == full["S0_total"]
+    for k in ("Rq", "Rs", "Rx"):
+        assert np.array_equal(after[k], full[k]), k
+
+
 @pytest.mark.skip(reason="the free 144-site cluster with H positions relaxed reconstructs: on the "
                          "CPU prototype the energy merit needs about 440 Newton steps and 32k "
                          "evaluations to reach |R| = 3e-5 (atoms move > 4 A); the residual merit "

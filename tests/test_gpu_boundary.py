@@ -9,7 +9,8 @@
   indexes a negative segment there, src/Numerical/cubic-spline.cpp:84-93) is
   reported as DMDGPU_ERR_DOMAIN exactly when the oracle's densities put a
   site there; a relaxation that meets it ends in the "domain" state.
-* Non-finite positions, widths or occupancies are argument errors.
+* Non-finite positions, widths or occupancies are argument errors, and so is
+  an instance above the 6,710,784-site limit.
 """
 import ctypes as C
 import sys
@@ -158,6 +159,26 @@ def test_nonfinite_inputs_rejected(gpu):
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.update_state(q=q)
     assert e.value.code == ERR_ARG
+
+
+def test_instance_size_limit(gpu):
+    """An instance above 6,710,784 sites (its adjoint records would pass the
+    4 GiB that the gradient kernel's 32-bit row offsets reach) is an argument
+    error, refused before anything is read or allocated."""
+    say("[gpu] instance size limit")
+    n = 6_710_784 + 1
+    big = {"n": n, "q": np.zeros((n, 3)), "sigma": np.full(n, 0.1), "x": np.ones(n),
+           "species": np.full(n, 12, dtype=np.int32), "beta": np.ones(n), "gamma": np.zeros(n),
+           "lo": np.zeros(3), "hi": np.full(3, 1e4), "periodic": np.zeros(3, dtype=np.int32)}
+    ctx = dmdgpu.Context(0)
+    try:
+        before = ctx.device_bytes()
+        with pytest.raises(dmdgpu.DmdError) as e:
+            ctx.add_instance(big)
+        assert e.value.code == ERR_ARG
+        assert ctx.device_bytes() == before
+    finally:
+        ctx.close()
 
 
 def test_residual_packed_host_and_device(gpu, oracle):
