@@ -162,18 +162,13 @@ __global__ __launch_bounds__(256) void k_phase1_pairs(State s, Tables T, Outputs
     phase1_pairs<P, B, RULE, true>(s, T, o, inst, ch, lb % B, stage);
 }
 
-template <typename P, int K, int RULE>
-__global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
-  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
+// VALUES = false (relaxation ticks with the residual-norm line search): the
+// adjoint records, F'(rho_bar) and rho_bar only -- no V0 (its coincident,
+// pair and self terms are left out).
+template <typename P, int K, int RULE, bool VALUES>
+DEV void phase1_site(const State& s, const Tables& T, const Outputs& o, int i, int g, int inst,
+                     u32x4* stage) {
   constexpr bool CO = kCoop1<P>;
-  // largest gather: the rows of one i-shift walk
-  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, CO>::v;
-  __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
-  u32x4* stage = stage_all[threadIdx.x >> 6];
-  int i, g;
-  if (!site_of<P, K>(s, i, g)) return;
-  const int inst = s.inst[i];
-  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
   const Box box = s.boxes[inst];
   double* rec = o.rec + (size_t)i * kRec;
   float* rec32 = o.rec32 + (size_t)i * kRec32;
@@ -235,7 +230,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
           psi_from(cval(cw[b], off[b]), cval(cw[b] + 4, off[b]), p[b], ps);
 #pragma unroll
           for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
-          selfA += nj * nj * (double)dotM(ps, ps);
+          if constexpr (VALUES) selfA += nj * nj * (double)dotM(ps, ps);
         }
       } else {
 #pragma unroll
@@ -249,7 +244,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
           psi_from(cval(c, off), cval(c + 4, off), p, ps);
 #pragma unroll
           for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
-          selfA += nj * nj * (double)dotM(ps, ps);
+          if constexpr (VALUES) selfA += nj * nj * (double)dotM(ps, ps);
         }
       }
     }
@@ -260,7 +255,7 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
 #pragma unroll
         for (int c = 0; c < 9; c++) G[b][c] = group_sum<K>(G[b][c]);
       }
-      GG += dotM(G[b], G[b]);
+      if constexpr (VALUES) GG += dotM(G[b], G[b]);
       applyM(1.0 / 9.0, G[b], tmp);
 #pragma unroll
       for (int c = 0; c < 9; c++)
@@ -309,18 +304,23 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     }
 #pragma unroll
     for (int c = 0; c < 9; c++) U[c] += nj * (double)Tj[c];
-    TPsi += nj2 * (double)dotM(Tj, psi0);
+    if constexpr (VALUES) TPsi += nj2 * (double)dotM(Tj, psi0);
     // pair and coincident points of the pair rule: per-pair sums from k_phase1_pairs
     if (inb) {
       const double* e = o.slot + ((size_t)i * kCap + kk) * kSlotW;
-      Cc += nj2 * e[0];
-      Ep += nj * e[1];
+      if constexpr (VALUES) {
+        Cc += nj2 * e[0];
+        Ep += nj * e[1];
+      }
       rho += nj * e[2];
     }
   }
   if (K > 1) {
-    rho = group_sum<K>(rho); Ep = group_sum<K>(Ep); Cc = group_sum<K>(Cc);
-    selfA = group_sum<K>(selfA); TPsi = group_sum<K>(TPsi);
+    rho = group_sum<K>(rho);
+    if constexpr (VALUES) {
+      Ep = group_sum<K>(Ep); Cc = group_sum<K>(Cc);
+      selfA = group_sum<K>(selfA); TPsi = group_sum<K>(TPsi);
+    }
 #pragma unroll
     for (int c = 0; c < 9; c++) {
       G0[c] = group_sum<K>(G0[c]);
@@ -353,6 +353,24 @@ __global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
     o.V0[i] = ni * F + 0.5 * ni * Ep + ni * ni * At;
     o.rho[i] = rho;
   }
+}
+
+template <typename P, int K, int RULE>
+__global__ __launch_bounds__(256) void k_phase1(State s, Tables T, Outputs o) {
+  static_assert(K >= 4, "a site's lanes must cover whole gather groups");
+  constexpr bool CO = kCoop1<P>;
+  // largest gather: the rows of one i-shift walk
+  constexpr int kRounds = StageRounds<P, kShiftsPerWalk, CO>::v;
+  __shared__ u32x4 stage_all[4][kRounds * kStageSlots];
+  u32x4* stage = stage_all[threadIdx.x >> 6];
+  int i, g;
+  if (!site_of<P, K>(s, i, g)) return;
+  const int inst = s.inst[i];
+  if (inst < 0 || s.prec[inst] != PrecCode<P>::v) return;  // uniform per block
+  if (s.noxr && (s.noxr[inst] & kSkipValues))
+    phase1_site<P, K, RULE, false>(s, T, o, i, g, inst, stage);
+  else
+    phase1_site<P, K, RULE, true>(s, T, o, i, g, inst, stage);
 }
 
 // Block b of a record as P, negating the dipole part when flip is set.
