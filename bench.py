@@ -30,12 +30,20 @@ import numpy as np  # noqa: E402
 # (once per unordered pair); phase 2: the pair kernel (each unordered pair
 # once) plus the slot sums.  The PQE count per residual stays the workload's
 # fixed 50 per active ordered pair.
-FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2943}
+FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2941}
 # The split-precision evaluation, counted the same way (SQ_INSTS_VALU_FLOPS_FP32
 # and _FP64 x 64 per active ordered pair, profiles/valu_mgh_20x15x15.md):
 # (FP32, FP64) FLOPs per pair of each phase.
 FLOP_PER_PAIR_SPLIT = {"phase1_site_sums": (821 + 532, 335 + 78),
                        "phase2_gradient": (2644, 229 + 22)}
+# Relaxation ticks (Mg positions and widths, residual-norm line search) form
+# less: the pair-rule pass the densities only, phase 1 no energy sums, phase 2
+# no occupancy terms and only the pairs with an Mg end.  Counted the same way
+# on ticks of this workload (profiles/valu_mgh_20x15x15.md, relaxation ticks):
+# per active ordered pair for phase 1, per pair with an Mg end for phase 2;
+# FP64 ticks, then split ticks as (FP32, FP64).
+FLOP_PER_PAIR_TICK = {"phase1_site_sums": 1203, "phase2_gradient": 2368}
+FLOP_PER_PAIR_TICK_SPLIT = {"phase1_site_sums": (839, 358), "phase2_gradient": (2186, 144)}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
 HBM_PEAK = 8.0e12          # B/s
@@ -397,16 +405,17 @@ def _relax_plain(ctx, insts, opts):
 
 def solve_flops(ctx, ids):
     """(FP64, FP32) hardware FLOPs of the residual evaluations of the listed
-    instances' solves: per evaluation, phase 1 over every active ordered pair
-    and phase 2 over those with an Mg end (relaxation ticks of the Mg lattice
-    skip the pairs with two H ends), at the per-pair counts of each precision.
+    instances' solves (relaxation ticks of the Mg lattice with the residual
+    merit): per evaluation, phase 1 over every active ordered pair and phase 2
+    over those with an Mg end, at the per-pair tick counts of each precision.
     The one-off preconditioner builds are not counted."""
     f64 = f32 = 0.0
+    t1, t2 = FLOP_PER_PAIR_TICK["phase1_site_sums"], FLOP_PER_PAIR_TICK["phase2_gradient"]
+    s1, s2 = FLOP_PER_PAIR_TICK_SPLIT["phase1_site_sums"], FLOP_PER_PAIR_TICK_SPLIT["phase2_gradient"]
     for i in ids:
         e64, e32 = ctx.relax_evals(i)
         p, pm = ctx.instance_pairs(i)
-        f64 += e64 * (FLOP_PER_PAIR["phase1_site_sums"] * p + FLOP_PER_PAIR["phase2_gradient"] * pm)
-        s1, s2 = FLOP_PER_PAIR_SPLIT["phase1_site_sums"], FLOP_PER_PAIR_SPLIT["phase2_gradient"]
+        f64 += e64 * (t1 * p + t2 * pm)
         f32 += e32 * (s1[0] * p + s2[0] * pm)
         f64 += e32 * (s1[1] * p + s2[1] * pm)
     return f64, f32
