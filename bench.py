@@ -296,8 +296,44 @@ def host_leg(ctx, sysd, pqe, reps=10):
         ctx.update_state(q=q, sigma=sg)
         ctx.residual()
     dt2 = (time.perf_counter() - t0) / reps
+    # one SNES iteration's device work of the drop-in (the FP64 workload of
+    # this configuration): FormFunction + one MatShell product, host buffers
+    # and device buffers
+    import ctypes as C
+    import dmdgpu
+    V = np.random.default_rng(1).normal(size=X.size)
+    V[3 * sysd["n"]:] *= 0.01
+    ctx.jvp_packed(X, V)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.residual_packed(X)
+        ctx.jvp_packed(X, V)
+    dt3 = (time.perf_counter() - t0) / reps
+    dmdgpu.load_library()
+    hip = C.CDLL("libamdhip64.so.7")
+    ptr = [C.c_void_p() for _ in range(4)]
+    sizes = [X.size, X.size, X.size, 5 * sysd["n"]]
+    try:
+        for p, b in zip(ptr, sizes):
+            if hip.hipMalloc(C.byref(p), C.c_size_t(8 * b)):
+                raise RuntimeError("hipMalloc failed")
+        for p, a in zip(ptr[:2], (X, V)):
+            hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(8 * a.size), 1)
+        dev = (ptr[0].value, ptr[1].value, ptr[2].value)
+        ctx.jvp_packed(device=dev)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.residual_packed(device=(ptr[0].value, ptr[3].value))
+            ctx.jvp_packed(device=dev)
+        dt4 = (time.perf_counter() - t0) / reps
+    finally:
+        for p in ptr:
+            hip.hipFree(p)
     return {"ms_per_call": dt * 1e3, "value": pqe / dt, "unit": "PQE/s",
-            "separate_arrays_ms_per_call": dt2 * 1e3}
+            "separate_arrays_ms_per_call": dt2 * 1e3,
+            "snes_iteration_residual_plus_jvp_ms": {"host_buffers": dt3 * 1e3,
+                                                    "device_buffers": dt4 * 1e3,
+                                                    "residual_evaluations": 3}}
 
 
 def diag_leg(ctx, reps=5):

@@ -192,5 +192,12 @@ hipError_t launch_gather_unknowns(hipStream_t st, int base, int n, const int* pe
 hipError_t launch_scatter_residual(hipStream_t st, int base, int n, const int* perm,
                                    const double* Rq, const double* Rs, const double* Rx,
                                    double* R);
+// Packed-vector helpers (caller-order vectors of length m): out[0] = |v|^2
+// (fixed-order sum), y = x + s v, d = (a - b) c.
+hipError_t launch_sumsq(hipStream_t st, const double* v, size_t m, double* out);
+hipError_t launch_axpy(hipStream_t st, const double* x, const double* v, double s, size_t m,
+                       double* y);
+hipError_t launch_diff_scale(hipStream_t st, const double* a, const double* b, double c, size_t m,
+                             double* d);
 
 }  // namespace dmd

@@ -354,6 +354,36 @@ __global__ __launch_bounds__(256) void k_scatter_residual(int base, int n, const
   R[4 * (size_t)n + o] = Rx[a];
 }
 
+// Packed-vector helpers of the matrix-free Jacobian-vector product.
+// out[0] = sum of v[k]^2 over [0, m): one block, lanes stride through v, then
+// a fixed tree in LDS (deterministic).
+__global__ __launch_bounds__(1024) void k_sumsq(const double* v, size_t m, double* out) {
+  __shared__ double sh[1024];
+  double a = 0.0;
+  for (size_t k = threadIdx.x; k < m; k += 1024) a += v[k] * v[k];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// y = x + s v over [0, m).
+__global__ __launch_bounds__(256) void k_axpy(const double* x, const double* v, double s, size_t m,
+                                              double* y) {
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < m) y[k] = x[k] + s * v[k];
+}
+
+// d[k] = (a[k] - b[k]) * c over [0, m).
+__global__ __launch_bounds__(256) void k_diff_scale(const double* a, const double* b, double c,
+                                                    size_t m, double* d) {
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < m) d[k] = (a[k] - b[k]) * c;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -407,6 +437,25 @@ hipError_t launch_scatter_residual(hipStream_t st, int base, int n, const int* p
                                    double* R) {
   if (n <= 0) return hipSuccess;
   k_scatter_residual<<<(n + 255) / 256, 256, 0, st>>>(base, n, perm, Rq, Rs, Rx, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_sumsq(hipStream_t st, const double* v, size_t m, double* out) {
+  k_sumsq<<<1, 1024, 0, st>>>(v, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(hipStream_t st, const double* x, const double* v, double s, size_t m,
+                       double* y) {
+  if (m == 0) return hipSuccess;
+  k_axpy<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(x, v, s, m, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_scale(hipStream_t st, const double* a, const double* b, double c, size_t m,
+                             double* d) {
+  if (m == 0) return hipSuccess;
+  k_diff_scale<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(a, b, c, m, d);
   return hipGetLastError();
 }
 

@@ -193,6 +193,9 @@ struct dmdgpu_ctx {
   DevBuf<double> d_rec, d_V0, d_rho, d_Rq, d_Rs, d_Rx, d_S0, d_partials, d_totals, d_slot;
   DevBuf<double> d_diag;  // diagonal Hessian blocks (n x 9)
   DevBuf<double> d_pkin, d_pkout;  // packed X in / R out of the host-buffer residual
+  // Jacobian-vector product: X and V staged from host memory (4 per site), the
+  // second residual (5 per site), |V|^2
+  DevBuf<double> d_jx, d_jv, d_jr, d_jscal;
   DevBuf<int> d_hookq;     // per-(u, i) queries (2 per query)
   DevBuf<double> d_hookv;  // their results (5 per query)
   DevBuf<double> d_minv;  // arena x 10: block-Jacobi preconditioner of the solver
@@ -475,6 +478,9 @@ void for_site_bufs(dmdgpu_ctx* c, size_t cap, F&& f) {
   f(c->d_act, 2 * (ch + 1), false);
   f(c->d_pkin, cap * 4, false);
   f(c->d_pkout, cap * 5, false);
+  f(c->d_jx, cap * 4, false);
+  f(c->d_jv, cap * 4, false);
+  f(c->d_jr, cap * 5, false);
 }
 
 // Device bytes of the per-site arrays at a capacity of `cap` sites, and the
@@ -499,7 +505,7 @@ void for_other_bufs(dmdgpu_ctx* c, F&& f) {
   f(c->d_uw32); f(c->d_ph32); f(c->d_rh32); f(c->d_pr32);
   f(c->d_cell_count); f(c->d_cell_start); f(c->d_cell_cursor); f(c->d_flags);
   f(c->d_boxes); f(c->d_prec); f(c->d_noxr); f(c->d_chunk_lo); f(c->d_chunk_hi);
-  f(c->d_totals); f(c->d_ops); f(c->d_sdots); f(c->d_hookq); f(c->d_hookv);
+  f(c->d_totals); f(c->d_ops); f(c->d_sdots); f(c->d_hookq); f(c->d_hookv); f(c->d_jscal);
 }
 
 // Every device allocation of the context.
@@ -1781,44 +1787,33 @@ dmdgpu_status dmdgpu_residual(dmdgpu_ctx* c, double* h_Rq, double* h_Rs, double*
   return copy_residual(c, c->primary, h_Rq, h_Rs, h_Rx, h_tot);
 }
 
-dmdgpu_status dmdgpu_residual_packed(dmdgpu_ctx* c, const double* X, double* R, double* h_totals,
-                                     int on_device) {
-  CHK(check_primary(c));
-  if (on_device != 0 && on_device != 1) return fail(DMDGPU_ERR_ARG, "on_device must be 0 or 1");
-  HIPCHK(hipSetDevice(c->device));
+// The packed residual of the primary instance on device buffers: dX (4n,
+// caller order) sets q and sigma first (nullptr: the current state); dR (5n,
+// caller order) receives R (nullptr: none), and pin (nullptr: none) a copy of
+// it.  Returns after the stream has finished.
+dmdgpu_status packed_eval(dmdgpu_ctx* c, const double* dX, double* dR, double* pin,
+                          double* h_totals) {
   const int id = c->primary;
   Instance& I = c->inst[id];
   const size_t n = (size_t)I.n;
-  if (!on_device) HIPCHK(c->pin_pk.ensure(sizeof(double) * 5 * n));
-  double* pin = static_cast<double*>(c->pin_pk.p);
-  if (X) {
-    const double* dX = X;
-    if (!on_device) {
-      std::memcpy(pin, X, sizeof(double) * 4 * n);
-      HIPCHK(hipMemcpyAsync(c->d_pkin.p, pin, sizeof(double) * 4 * n, hipMemcpyHostToDevice,
-                            c->stream));
-      dX = c->d_pkin.p;
-    }
+  if (dX) {
     HIPCHK(hipMemsetAsync(c->d_flags.p, 0, sizeof(int), c->stream));
     HIPCHK(dmd::launch_gather_unknowns(c->stream, I.base, I.n, c->d_perm.p, dX, c->d_qs.p, c->d_X.p,
                                        c->d_flags.p));
     I.h_q_stale = true;
   }
   CHK(eval_one(c, id));
-  if (R) {
-    double* dR = on_device ? R : c->d_pkout.p;
+  if (dR) {
     HIPCHK(dmd::launch_scatter_residual(c->stream, I.base, I.n, c->d_perm.p, c->d_Rq.p, c->d_Rs.p,
                                         c->d_Rx.p, dR));
-    if (!on_device)
-      HIPCHK(hipMemcpyAsync(pin, c->d_pkout.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost,
-                            c->stream));
+    if (pin)
+      HIPCHK(hipMemcpyAsync(pin, dR, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, c->stream));
   }
   double t3[3];
   int bad = 0;
   HIPCHK(hipMemcpyAsync(t3, c->d_totals.p + 3 * id, sizeof(t3), hipMemcpyDeviceToHost, c->stream));
-  if (X) HIPCHK(hipMemcpyAsync(&bad, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (dX) HIPCHK(hipMemcpyAsync(&bad, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (R && !on_device) std::memcpy(R, pin, sizeof(double) * 5 * n);
   if (h_totals) {
     h_totals[0] = t3[0];
     h_totals[1] = t3[1];
@@ -1827,32 +1822,88 @@ dmdgpu_status dmdgpu_residual_packed(dmdgpu_ctx* c, const double* X, double* R, 
   return domain_status(t3[2]);
 }
 
-dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* c, const double* X, const double* V, double* JV) {
+dmdgpu_status dmdgpu_residual_packed(dmdgpu_ctx* c, const double* X, double* R, double* h_totals,
+                                     int on_device) {
   CHK(check_primary(c));
-  if (!X || !V || !JV) return fail(DMDGPU_ERR_ARG, "null argument");
+  if (on_device != 0 && on_device != 1) return fail(DMDGPU_ERR_ARG, "on_device must be 0 or 1");
   HIPCHK(hipSetDevice(c->device));
+  const size_t n = (size_t)c->inst[c->primary].n;
+  if (on_device) return packed_eval(c, X, R, nullptr, h_totals);
+  HIPCHK(c->pin_pk.ensure(sizeof(double) * 5 * n));
+  double* pin = static_cast<double*>(c->pin_pk.p);
+  if (X) {
+    std::memcpy(pin, X, sizeof(double) * 4 * n);
+    HIPCHK(hipMemcpyAsync(c->d_pkin.p, pin, sizeof(double) * 4 * n, hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  const dmdgpu_status st =
+      packed_eval(c, X ? c->d_pkin.p : nullptr, R ? c->d_pkout.p : nullptr, R ? pin : nullptr,
+                  h_totals);
+  if (R && (st == DMDGPU_OK || st == DMDGPU_ERR_DOMAIN))
+    std::memcpy(R, pin, sizeof(double) * 5 * n);
+  return st;
+}
+
+// J V of the (q, sigma) residual at X by central differences, on device
+// buffers (caller order, 4n each): |V|^2 on the device, R(X + eps V) and
+// R(X - eps V) through the packed residual, JV = their difference / (2 eps);
+// the state is left at X.
+dmdgpu_status jvp_device(dmdgpu_ctx* c, const double* dX, const double* dV, double* dJ) {
   Instance& I = c->inst[c->primary];
   const size_t n = (size_t)I.n, m = 4 * n;
+  HIPCHK(c->d_jscal.ensure(1));
+  HIPCHK(dmd::launch_sumsq(c->stream, dV, m, c->d_jscal.p));
   double vv = 0.0;
-  for (size_t k = 0; k < m; k++) vv += V[k] * V[k];
+  HIPCHK(hipMemcpyAsync(&vv, c->d_jscal.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!std::isfinite(vv)) return fail(DMDGPU_ERR_ARG, "direction must be finite");
   if (!(vv > 0.0)) {
-    std::fill(JV, JV + m, 0.0);
+    HIPCHK(hipMemsetAsync(dJ, 0, sizeof(double) * m, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return DMDGPU_OK;
   }
-  // central difference with an RMS displacement of 1e-5 A: truncation
-  // ~1e-10 |d3 S| and rounding ~1e-8 relative
+  // RMS displacement of 1e-5 A: truncation ~1e-10 |d3 S| and rounding ~1e-8
+  // relative
   const double eps = 1e-5 * std::sqrt((double)m / vv);
-  std::vector<double> Y(m), Rp(m), Rm(m);
-  for (int side = 0; side < 2; side++) {
-    const double s = side == 0 ? eps : -eps;
-    for (size_t k = 0; k < m; k++) Y[k] = X[k] + s * V[k];
-    CHK(upload_unknowns(c, I, Y.data(), Y.data() + 3 * n, nullptr, false));
-    CHK(eval_one(c, c->primary));
-    double* R = side == 0 ? Rp.data() : Rm.data();
-    CHK(copy_residual(c, c->primary, R, R + 3 * n, nullptr, nullptr));
+  dmdgpu_status st = DMDGPU_OK;
+  for (int side = 0; side < 2 && st == DMDGPU_OK; side++) {
+    HIPCHK(dmd::launch_axpy(c->stream, dX, dV, side == 0 ? eps : -eps, m, c->d_pkin.p));
+    st = packed_eval(c, c->d_pkin.p, side == 0 ? c->d_pkout.p : c->d_jr.p, nullptr, nullptr);
   }
-  for (size_t k = 0; k < m; k++) JV[k] = (Rp[k] - Rm[k]) / (2.0 * eps);
-  return upload_unknowns(c, I, X, X + 3 * n, nullptr, false);
+  if (st != DMDGPU_OK) return st;
+  HIPCHK(dmd::launch_diff_scale(c->stream, c->d_pkout.p, c->d_jr.p, 0.5 / eps, m, dJ));
+  // back to X (no evaluation)
+  HIPCHK(hipMemsetAsync(c->d_flags.p, 0, sizeof(int), c->stream));
+  HIPCHK(dmd::launch_gather_unknowns(c->stream, I.base, I.n, c->d_perm.p, dX, c->d_qs.p, c->d_X.p,
+                                     c->d_flags.p));
+  I.h_q_stale = true;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_jvp_packed(dmdgpu_ctx* c, const double* X, const double* V, double* JV,
+                                int on_device) {
+  CHK(check_primary(c));
+  if (!X || !V || !JV) return fail(DMDGPU_ERR_ARG, "null argument");
+  if (on_device != 0 && on_device != 1) return fail(DMDGPU_ERR_ARG, "on_device must be 0 or 1");
+  HIPCHK(hipSetDevice(c->device));
+  if (on_device) return jvp_device(c, X, V, JV);
+  const size_t m = 4 * (size_t)c->inst[c->primary].n;
+  HIPCHK(c->pin_pk.ensure(sizeof(double) * 2 * m));
+  double* pin = static_cast<double*>(c->pin_pk.p);
+  std::memcpy(pin, X, sizeof(double) * m);
+  std::memcpy(pin + m, V, sizeof(double) * m);
+  HIPCHK(hipMemcpyAsync(c->d_jx.p, pin, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_jv.p, pin + m, sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+  CHK(jvp_device(c, c->d_jx.p, c->d_jv.p, c->d_jv.p));
+  HIPCHK(hipMemcpyAsync(pin, c->d_jv.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::memcpy(JV, pin, sizeof(double) * m);
+  return DMDGPU_OK;
+}
+
+dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* c, const double* X, const double* V, double* JV) {
+  return dmdgpu_jvp_packed(c, X, V, JV, 0);
 }
 
 dmdgpu_status diag_blocks_of(dmdgpu_ctx* c, int id, int exact, double* h_D) {

@@ -41,7 +41,8 @@ EXPORTS = ["dmdgpu_create", "dmdgpu_destroy", "dmdgpu_last_error", "dmdgpu_strea
            "dmdgpu_diffusion_neighbors", "dmdgpu_jvp", "dmdgpu_set_rule", "dmdgpu_diag_blocks", "dmdgpu_instance_diag_blocks",
            "dmdgpu_set_tuning_precision", "dmdgpu_device_bytes", "dmdgpu_diag_blocks_noalloc",
            "dmdgpu_site_DV0_Dq", "dmdgpu_site_DV0_Dxi", "dmdgpu_residual_packed",
-           "dmdgpu_relax_energy", "dmdgpu_relax_evals", "dmdgpu_instance_pairs"]
+           "dmdgpu_relax_energy", "dmdgpu_relax_evals", "dmdgpu_instance_pairs",
+           "dmdgpu_jvp_packed"]
 DCAP = 50  # diffusive neighbours per site, src/Macros.hpp:863
 
 _dp = C.POINTER(C.c_double)
@@ -150,6 +151,7 @@ def load_library():
     L.dmdgpu_diffusion_rates.argtypes = [vp, C.c_int, C.POINTER(DiffuseOpts), _dp]
     L.dmdgpu_diffusion_neighbors.argtypes = [vp, C.c_int, _ip, _ip]
     L.dmdgpu_jvp.argtypes = [vp, _dp, _dp, _dp]
+    L.dmdgpu_jvp_packed.argtypes = [vp, vp, vp, vp, C.c_int]
     L.dmdgpu_diag_blocks.argtypes = [vp, C.c_int, _dp]
     L.dmdgpu_instance_diag_blocks.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.dmdgpu_diag_blocks_noalloc.argtypes = [vp, C.c_int, vp]
@@ -474,6 +476,21 @@ class Context:
         JV = np.zeros(4 * n)
         self._chk(self.L.dmdgpu_jvp(self.h, _d(X), _d(V), _d(JV)))
         return JV[:3 * n].reshape(n, 3), JV[3 * n:]
+
+    def jvp_packed(self, X=None, V=None, device=None):
+        """dmdgpu_jvp_packed: JV = J V at X, packed [q (3n) | sigma (n)] vectors.
+        Host arrays X, V -> JV; or device=(dX, dV, dJV) device pointers."""
+        if device is not None:
+            dX, dV, dJ = device
+            self._chk(self.L.dmdgpu_jvp_packed(self.h, C.c_void_p(dX), C.c_void_p(dV),
+                                               C.c_void_p(dJ), 1))
+            return None
+        Xc = np.ascontiguousarray(X, dtype=np.float64)
+        Vc = np.ascontiguousarray(V, dtype=np.float64)
+        JV = np.zeros(4 * self.n)
+        self._chk(self.L.dmdgpu_jvp_packed(self.h, Xc.ctypes.data, Vc.ctypes.data, JV.ctypes.data,
+                                           0))
+        return JV
 
     # -- diffusion step -------------------------------------------------------
     def diffuse_start(self, iid, opts=None):

@@ -178,6 +178,13 @@ dmdgpu_status dmdgpu_residual_resident(dmdgpu_ctx* ctx);
  * central difference of two residual evaluations at X +- eps V (RMS
  * displacement 1e-5 A).  Occupancies fixed; the state is left at X. */
 dmdgpu_status dmdgpu_jvp(dmdgpu_ctx* ctx, const double* h_X, const double* h_V, double* h_JV);
+/* The same product with X, V and JV in host memory (on_device = 0, staged
+ * through page-locked buffers) or in device memory of the context's device
+ * (on_device = 1: a MatShell over device Vecs; no host copies of the vectors,
+ * |V| is reduced on the device).  dmdgpu_jvp is the on_device = 0 form;
+ * both give bit for bit the same JV. */
+dmdgpu_status dmdgpu_jvp_packed(dmdgpu_ctx* ctx, const double* X, const double* V, double* JV,
+                                int on_device);
 
 /* Diagonal 3x3 blocks of the force residual's Jacobian, for a block-Jacobi
  * preconditioner: h_D[9 u + 3 a + b] = D_u[a][b] with

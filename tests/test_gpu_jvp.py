@@ -73,3 +73,42 @@ def test_jvp_hessian_symmetry_on_smooth_tables(tmp_path):
         assert abs(a - b) <= 1e-7 * max(abs(a), abs(b))
     finally:
         ctx.close()
+
+
+def test_jvp_packed_host_and_device(gpu):
+    """dmdgpu_jvp_packed with device buffers (a MatShell over device Vecs)
+    gives bit for bit the host-buffer product, which is dmdgpu_jvp's; a zero
+    direction gives zeros and a non-finite one is an argument error."""
+    import ctypes as C
+    sd = lat.hcp_mgh(4, 3, 3, x_h=(0.0, 0.3), seed=19)
+    n = sd["n"]
+    print(f"[gpu] jvp host and device buffers n={n}", flush=True)
+    gpu.set_precision("fp64")
+    gpu.set_state(sd)
+    gpu.neighbors()
+    rng = np.random.default_rng(7)
+    X = np.concatenate([sd["q"].ravel(), sd["sigma"]])
+    V = np.concatenate([rng.normal(size=3 * n), 0.1 * rng.normal(size=n)])
+    host = gpu.jvp_packed(X, V)
+    jq, js = gpu.jvp(sd["q"], sd["sigma"], V[:3 * n].reshape(n, 3), V[3 * n:])
+    assert np.array_equal(host, np.concatenate([jq.ravel(), js]))
+    hip = C.CDLL("libamdhip64.so.7")
+    ptr = [C.c_void_p() for _ in range(3)]
+    for p in ptr:
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(8 * 4 * n)) == 0
+    try:
+        for p, a in zip(ptr[:2], (X, V)):
+            assert hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(8 * 4 * n), 1) == 0
+        gpu.jvp_packed(device=(ptr[0].value, ptr[1].value, ptr[2].value))
+        dev = np.empty(4 * n)
+        assert hip.hipMemcpy(dev.ctypes.data_as(C.c_void_p), ptr[2], C.c_size_t(8 * 4 * n), 2) == 0
+        assert np.array_equal(dev, host)
+    finally:
+        for p in ptr:
+            hip.hipFree(p)
+    assert not np.abs(gpu.jvp_packed(X, np.zeros(4 * n))).any()
+    bad = V.copy()
+    bad[5] = np.nan
+    with pytest.raises(dmdgpu.DmdError) as e:
+        gpu.jvp_packed(X, bad)
+    assert e.value.code == 1
