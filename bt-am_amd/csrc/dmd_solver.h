@@ -13,10 +13,17 @@ enum : int {
   kUpdR0FromRv = 4,   // R0 = Rv (FP64 residual at X)
   kUpdR0fFromRv = 8,  // R0f = Rv (split-precision residual at X, for FP32 probes)
   kUpdCgInit = 16,    // D = 0; Rr = P = -R0
-  kUpdSdDir = 32      // D = c_sd P
+  kUpdSdDir = 32,     // D = c_sd P
+  // NGMRES (D and Rr hold the line-search point x^M and its residual F^M)
+  kUpdNgKeepM = 64,     // D = Y, Rr = Rv
+  kUpdNgAcceptM = 128,  // X = D, R0 = Rr
+  kUpdNgPush = 256      // history slot `slot` = (X, R0)
 };
-enum : int { kMakeNone = 0, kMakeX = 1, kMakeXP = 2, kMakeXD = 3 };
-enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4, kDotPc = 5 };
+enum : int { kMakeNone = 0, kMakeX = 1, kMakeXP = 2, kMakeXD = 3, kMakeXR = 4, kMakeNgA = 5 };
+enum : int { kDotNone = 0, kDotNorm = 1, kDotCg = 2, kDotLs = 3, kDotDiff = 4, kDotPc = 5,
+             kDotNg = 6 };
+// NGMRES history depth (iterates and residuals kept per site).
+constexpr int kNgHist = 3;
 // Dots per instance: [0..2] sums, [3..4] maxima, [5..13] sums (preconditioned CG),
 // [14] sites of the tick's evaluation whose density left the embedding
 // table's domain, [15] its free energy sum_i S0_i.
@@ -44,7 +51,10 @@ struct SolveOp {
   double eps_prev;  // probe step of the previous tick (kUpdCgStep)
   double alpha, beta, c_sd;
   double eps;       // probe step of this tick (kMakeXP, kDotCg)
-  double alpha_ls;  // line-search step (kMakeXD)
+  double alpha_ls;  // line-search step (kMakeXD; kMakeXR: Y = X - alpha_ls R0)
+  int slot;         // NGMRES history slot written by kUpdNgPush
+  int nh;           // NGMRES history entries used by kMakeNgA
+  double b[kNgHist];  // kMakeNgA: Y = D + sum_s b[s] (HX_s - D)
   int diff;         // kDiff* mode of a diffusion-step solve
   double dt;        // time step (s)
   double d0;        // diffusivity prefactor (A^2/s)
@@ -61,6 +71,8 @@ struct SolveVecs {
   double* Xn;    // occupancy at the start of a diffusion step
   double* Xt;    // occupancy of the next Picard iterate (or rates)
   const double* minv;  // n x 10: inverse of the site's q block (3x3 row-major), sigma factor
+  double4* hx;   // n x kNgHist: NGMRES history of iterates (site-major)
+  double4* hf;   // n x kNgHist: their residuals
 };
 
 // Block-Jacobi preconditioner of an instance's sites [base, base + cap) from
@@ -87,7 +99,7 @@ hipError_t launch_diff_apply(hipStream_t st, int n, const int* inst, const int* 
 hipError_t launch_solver_dots(hipStream_t st, int n, int ninst, const int* inst, const int* spc,
                               const double* ne, const SolveOp* ops, const int* prec,
                               const SolveVecs& v, const Outputs& o, const double* xr,
-                              double* partials,
+                              const double4* qs, double* partials,
                               const int* chunk_lo, const int* chunk_hi, double* out);
 
 }  // namespace dmd

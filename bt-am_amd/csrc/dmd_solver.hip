@@ -49,7 +49,8 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
   double4 X = v.X[i];
   const double4 Y = qs[i];
   if (op.upd & kUpdAcceptY) { X = Y; v.X[i] = X; }
-  if (op.upd & (kUpdR0FromRv | kUpdR0fFromRv | kUpdCgStep)) {
+  double4 R0 = v.R0[i];
+  if (op.upd & (kUpdR0FromRv | kUpdR0fFromRv | kUpdCgStep | kUpdNgKeepM)) {
     const double4 Rv = ld_rv(o, i, op.dof, spc[i], ne[i] != 0.0);
     if (op.upd & kUpdCgStep) {
       // Ap = (R(X + eps P) - R(X)) / eps, the finite-difference Hessian action
@@ -64,8 +65,19 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
       v.Rr[i] = Rr;
       v.Pp[i] = axpy4(op.beta, Pp, op.pc ? apply_minv(v.minv + 10 * (size_t)i, Rr) : Rr);
     }
-    if (op.upd & kUpdR0FromRv) v.R0[i] = Rv;
+    if (op.upd & kUpdR0FromRv) { R0 = Rv; v.R0[i] = Rv; }
     if (op.upd & kUpdR0fFromRv) v.R0f[i] = Rv;
+    if (op.upd & kUpdNgKeepM) { v.D[i] = Y; v.Rr[i] = Rv; }
+  }
+  if (op.upd & kUpdNgAcceptM) {
+    X = v.D[i];
+    R0 = v.Rr[i];
+    v.X[i] = X;
+    v.R0[i] = R0;
+  }
+  if (op.upd & kUpdNgPush) {
+    v.hx[(size_t)kNgHist * i + op.slot] = X;
+    v.hf[(size_t)kNgHist * i + op.slot] = R0;
   }
   if (op.upd & kUpdCgInit) {
     const double4 R0 = v.R0[i];
@@ -80,6 +92,15 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
   if (op.make == kMakeX) Yn = X;
   else if (op.make == kMakeXP) Yn = axpy4(op.eps, v.Pp[i], X);
   else if (op.make == kMakeXD) Yn = axpy4(op.alpha_ls, v.D[i], X);
+  else if (op.make == kMakeXR) Yn = axpy4(-op.alpha_ls, R0, X);
+  else if (op.make == kMakeNgA) {
+    const double4 M = v.D[i];
+    Yn = M;
+    for (int t = 0; t < op.nh; t++) {
+      const double4 H = v.hx[(size_t)kNgHist * i + t];
+      Yn = axpy4(op.b[t], make_double4(H.x - M.x, H.y - M.y, H.z - M.z, H.w - M.w), Yn);
+    }
+  }
   if (op.make != kMakeNone) qs[i] = Yn;
 }
 
@@ -93,13 +114,19 @@ __global__ __launch_bounds__(256) void k_solver_step(int n, const int* inst, con
 //   kDotPc   : after CgInit (P = M^-1 Rr): g[2] = Rr.P, g[8] = P.P
 //   kDotLs   : a = R0 . D (the slope of the energy along D), d = max |D_q|
 //              (component), e = max(-D_sigma / X_sigma)
+//   kDotNg   : NGMRES, at the evaluation point Y: a = |Rv|^2, b = Rv . R0 (the
+//              line-search direction), c = relaxed widths <= 0, d = max |Rv_q|
+//              (component), e = max
+//              Rv_sigma / Y_sigma over relaxed widths (the step bounds if Y
+//              becomes the iterate), g[0..2] = Rv . HF_s, g[3] = |Y - D|^2,
+//              g[4..6] = |HX_s - Y|^2
 //   kDotDiff : over H sites, a = sum Xt (hydrogen content of the new iterate),
 //              b = sum x (of the current one), d = max |Xt - x|,
 //              e = max(-Xt, Xt - 1) (> 0 when an occupancy leaves [0, 1])
 __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, const int* spc,
                                                     const double* ne, const SolveOp* ops,
                                                     SolveVecs v, Outputs o, const double* xr,
-                                                    double* partials) {
+                                                    const double4* qs, double* partials) {
   __shared__ double s[kDotDomain][256];  // the reduced slots 0..13
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k = inst[blockIdx.x * 256];
@@ -134,6 +161,25 @@ __global__ __launch_bounds__(256) void k_solver_dots(int n, const int* inst, con
         g[0] = dot4(z, Ap); g[1] = dot4(Ap, y); g[2] = dot4(Rr, z);
         g[3] = dot4(z, z); g[4] = dot4(z, y); g[5] = dot4(y, y);
         g[6] = dot4(z, P); g[7] = dot4(y, P); g[8] = dot4(P, P);
+      }
+    } else if (op.dot == kDotNg) {
+      const int sl = spc[i];
+      const double4 Rv = ld_rv(o, i, op.dof, sl, true);
+      const double4 Y = qs[i], M = v.D[i];
+      a = dot4(Rv, Rv);
+      b = dot4(Rv, v.R0[i]);
+      d = fmax(fabs(Rv.x), fmax(fabs(Rv.y), fabs(Rv.z)));
+      if (op.dof & (sl == 0 ? kDofMgS : kDofHS)) {
+        e = Rv.w / Y.w;
+        c = Y.w > 0.0 ? 0.0 : 1.0;  // a relaxed width that is not positive
+      }
+      const double4 dm = make_double4(Y.x - M.x, Y.y - M.y, Y.z - M.z, Y.w - M.w);
+      g[3] = dot4(dm, dm);
+      for (int t = 0; t < op.nh; t++) {  // the history entries in use
+        const double4 H = v.hx[(size_t)kNgHist * i + t];
+        const double4 dh = make_double4(H.x - Y.x, H.y - Y.y, H.z - Y.z, H.w - Y.w);
+        g[t] = dot4(Rv, v.hf[(size_t)kNgHist * i + t]);
+        g[4 + t] = dot4(dh, dh);
       }
     } else if (op.dot == kDotPc) {
       const double4 P = v.Pp[i];
@@ -258,9 +304,9 @@ hipError_t launch_solver_step(hipStream_t st, int n, const int* inst, const int*
 hipError_t launch_solver_dots(hipStream_t st, int n, int ninst, const int* inst, const int* spc,
                               const double* ne, const SolveOp* ops, const int* prec,
                               const SolveVecs& v, const Outputs& o, const double* xr,
-                              double* partials,
+                              const double4* qs, double* partials,
                               const int* chunk_lo, const int* chunk_hi, double* out) {
-  k_solver_dots<<<(n + 255) / 256, 256, 0, st>>>(n, inst, spc, ne, ops, v, o, xr, partials);
+  k_solver_dots<<<(n + 255) / 256, 256, 0, st>>>(n, inst, spc, ne, ops, v, o, xr, qs, partials);
   k_solver_reduce<<<ninst, 64, 0, st>>>(chunk_lo, chunk_hi, ops, prec, o.totals, partials, out);
   return hipGetLastError();
 }

@@ -88,7 +88,11 @@ struct DevBuf {
   }
 };
 
-enum SolveStage { ST_IDLE = 0, ST_START, ST_BASE32, ST_PC, ST_CG, ST_LSPREP, ST_LS, ST_DIFF, ST_DONE };
+enum SolveStage {
+  ST_IDLE = 0, ST_START, ST_BASE32, ST_PC, ST_CG, ST_LSPREP, ST_LS, ST_DIFF,
+  ST_NG_LS, ST_NG_LSF, ST_NG_A,  // NGMRES: line-search trials, x^M, x^A
+  ST_DONE
+};
 
 struct Solve {
   int stage = ST_IDLE;
@@ -108,6 +112,15 @@ struct Solve {
   double slope = 0;       // R . D, the energy's slope along the Newton step
   bool pc_setup = false;  // build the preconditioner at X before the next tick
   int ndof = 0;
+  // NGMRES: history size, next slot, Gram matrix F_s . F_t of the history,
+  // the smallest |F| since the last restart, failed-iteration count; line
+  // search step, its previous trial and slope; step bound of the iterate;
+  // x^M's |F|, F^M . F_s and step bound; the combination's coefficients.
+  int nh = 0, ng_next = 0, ng_fail = 0;
+  double ff[dmd::kNgHist][dmd::kNgHist] = {};
+  double fmin = 0, lam = 0, lam_old = 0, dphi_old = 0, cap = 0;
+  double fm = 0, fmf[dmd::kNgHist] = {}, cap_m = 0, em = 0;
+  double beta[dmd::kNgHist] = {};
   dmd::SolveOp pending{};  // updates to apply at the start of the next tick
 };
 
@@ -207,6 +220,7 @@ struct dmdgpu_ctx {
   bool meta_dirty = true;
   // solver
   DevBuf<double4> d_X, d_R0, d_R0f, d_D, d_Rr, d_Pp;
+  DevBuf<double4> d_hx, d_hf;  // NGMRES history (kNgHist per site)
   DevBuf<double> d_Xn, d_Xt;    // diffusion step: x_n and the next iterate
   DevBuf<int> d_dnn, d_dnbr;    // diffusive neighbour lists (n x kDCap)
   DevBuf<int> d_act;            // chunks to evaluate per precision (State::act)
@@ -415,8 +429,8 @@ dmd::Outputs make_outputs(dmdgpu_ctx* c) {
 }
 
 dmd::SolveVecs make_vecs(dmdgpu_ctx* c) {
-  return dmd::SolveVecs{c->d_X.p,  c->d_R0.p, c->d_R0f.p, c->d_D.p,   c->d_Rr.p,
-                        c->d_Pp.p, c->d_Xn.p, c->d_Xt.p,  c->d_minv.p};
+  return dmd::SolveVecs{c->d_X.p,  c->d_R0.p, c->d_R0f.p,  c->d_D.p,  c->d_Rr.p, c->d_Pp.p,
+                        c->d_Xn.p, c->d_Xt.p, c->d_minv.p, c->d_hx.p, c->d_hf.p};
 }
 
 void drop_graph(dmdgpu_ctx* c) {
@@ -469,6 +483,8 @@ void for_site_bufs(dmdgpu_ctx* c, size_t cap, F&& f) {
   f(c->d_D, cap, true);
   f(c->d_Rr, cap, true);
   f(c->d_Pp, cap, true);
+  f(c->d_hx, cap * dmd::kNgHist, true);
+  f(c->d_hf, cap * dmd::kNgHist, true);
   f(c->d_Xn, cap, true);
   f(c->d_Xt, cap, true);
   f(c->d_minv, cap * 10, true);
@@ -1079,6 +1095,17 @@ dmdgpu_status check_primary(dmdgpu_ctx* c) {
 constexpr double kProbe64 = 1e-7;
 constexpr double kProbe32 = 5e-5;
 constexpr double kMaxStep = 0.2;   // max |dq| of a Newton step (A)
+// NGMRES (Oosterlee and Washio, SIAM J. Sci. Comput. 21 (2000) 1670; the
+// selection and restart tests with the constants of PETSc's SNESNGMRES
+// defaults): x^A is kept when |F^A| < kNgGammaA min|F| and either
+// kNgEpsB |x^A - x^M| < min_s |x^A - x_s| or |F^A|^2 < kNgDeltaB min|F|^2;
+// the history restarts after kNgRestart iterations in a row with
+// |F| > kNgGammaC min|F|.  The critical-point line search takes at most
+// kNgLsIts secant trials from lambda = min(2 lambda_prev, bound), lambda_0 =
+// kNgLam0 (A^2/eV).
+constexpr double kNgGammaA = 2.0, kNgGammaC = 2.0, kNgEpsB = 0.1, kNgDeltaB = 0.9;
+constexpr int kNgRestart = 2, kNgLsIts = 3;
+constexpr double kNgLam0 = 1e-3;
 // Mixed relaxation far from convergence: while |R|_2 > kSplitSwitch sqrt(ndof)
 // -- a thousand times the split residual's RMS noise of ~1e-6 per component --
 // the residuals of the Newton iteration and of its line search are split
@@ -1120,7 +1147,7 @@ void plan_tick(dmdgpu_ctx* c) {
       switch (S.stage) {
         case ST_START:
           op.make = dmd::kMakeX;
-          op.dot = dmd::kDotNorm;
+          op.dot = S.o.solver ? dmd::kDotNg : dmd::kDotNorm;
           c->h_prec[k] = S.early ? dmd::kSplit32 : dmd::kFP64;
           break;
         case ST_BASE32:
@@ -1146,6 +1173,21 @@ void plan_tick(dmdgpu_ctx* c) {
           op.alpha_ls = S.alpha_ls;
           op.dot = dmd::kDotNorm;
           c->h_prec[k] = S.early ? dmd::kSplit32 : dmd::kFP64;
+          break;
+        case ST_NG_LS:
+        case ST_NG_LSF:
+          op.make = dmd::kMakeXR;  // X - lambda R0
+          op.alpha_ls = S.lam;
+          op.dot = dmd::kDotNg;
+          op.nh = S.nh;
+          c->h_prec[k] = dmd::kFP64;
+          break;
+        case ST_NG_A:
+          op.make = dmd::kMakeNgA;
+          for (int t = 0; t < dmd::kNgHist; t++) op.b[t] = S.beta[t];
+          op.dot = dmd::kDotNg;
+          op.nh = S.nh;
+          c->h_prec[k] = dmd::kFP64;
           break;
         case ST_DIFF:
           set_diff_op(op, S.dop, dmd::kDiffStep);
@@ -1184,6 +1226,184 @@ void begin_cg(Solve& S, dmd::SolveOp& nx, double rr) {
   }
 }
 
+// ---- NGMRES ------------------------------------------------------------------
+// Step bound of an iterate whose residual has max |F_q| = dq (component) and
+// max F_sigma / sigma = ds over relaxed widths: the step -lambda F moves no
+// position by more than kMaxStep and no width below half its value.
+double ng_bound(double dq, double ds) {
+  double b = 1e300;
+  if (dq > 0.0) b = std::min(b, kMaxStep / dq);
+  if (ds > 0.0) b = std::min(b, 0.5 / ds);
+  return b;
+}
+
+// Line search from the iterate along -F (|F|^2 = f2): the first trial at S.lam.
+void ng_ls_begin(Solve& S, double f2) {
+  S.lam_old = 0.0;
+  S.dphi_old = -f2;  // d/dlambda of the energy at lambda = 0: -F . F
+  S.ls = 0;
+  S.stage = ST_NG_LS;
+}
+
+// Coefficients beta of x^A = x^M + sum_s beta_s (x_s - x^M) minimising the
+// linearised residual |F^M + sum_s beta_s (F_s - F^M)|: normal equations from
+// the history's Gram matrix and F^M . F_s, solved by Cholesky with a diagonal
+// shift of 1e-13 of the largest diagonal entry (a Gram matrix is only
+// semi-definite when residuals repeat).
+void ng_combine(Solve& S) {
+  const int n = S.nh;
+  const double fm2 = S.fm * S.fm;
+  double H[dmd::kNgHist][dmd::kNgHist], r[dmd::kNgHist];
+  double dmax = 0.0;
+  for (int i = 0; i < n; i++) {
+    r[i] = fm2 - S.fmf[i];
+    for (int j = 0; j < n; j++) H[i][j] = S.ff[i][j] - S.fmf[i] - S.fmf[j] + fm2;
+    dmax = std::max(dmax, H[i][i]);
+  }
+  for (int t = 0; t < dmd::kNgHist; t++) S.beta[t] = 0.0;
+  if (!(dmax > 0.0) || !std::isfinite(dmax)) return;
+  for (int i = 0; i < n; i++) H[i][i] += 1e-13 * dmax;
+  double L[dmd::kNgHist][dmd::kNgHist] = {};
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j <= i; j++) {
+      double v = H[i][j];
+      for (int k = 0; k < j; k++) v -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (!(v > 0.0)) return;  // not positive definite: x^A = x^M
+        L[i][i] = std::sqrt(v);
+      } else {
+        L[i][j] = v / L[j][j];
+      }
+    }
+  double y[dmd::kNgHist];
+  for (int i = 0; i < n; i++) {
+    double v = r[i];
+    for (int k = 0; k < i; k++) v -= L[i][k] * y[k];
+    y[i] = v / L[i][i];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double v = y[i];
+    for (int k = i + 1; k < n; k++) v -= L[k][i] * S.beta[k];
+    S.beta[i] = v / L[i][i];
+  }
+  for (int i = 0; i < n; i++)
+    if (!std::isfinite(S.beta[i])) {
+      for (int t = 0; t < dmd::kNgHist; t++) S.beta[t] = 0.0;
+      return;
+    }
+}
+
+// The first residual (at X): the history holds (X, F) only.
+void ng_start(Solve& S, dmd::SolveOp& nx, const double* d) {
+  S.rnorm = std::sqrt(d[0]);
+  S.energy = d[dmd::kDotEnergy];
+  if (S.newton == 0) S.rnorm0 = S.rnorm;
+  nx.upd |= dmd::kUpdR0FromRv;
+  if (!std::isfinite(S.rnorm)) {
+    finish(S, DMDGPU_RELAX_STALLED);
+    return;
+  }
+  if (S.rnorm <= S.o.atol) {
+    finish(S, DMDGPU_RELAX_CONVERGED);
+    return;
+  }
+  nx.upd |= dmd::kUpdNgPush;
+  nx.slot = 0;
+  S.nh = 1;
+  S.ng_next = 1 % S.o.ngmres_m;
+  S.ng_fail = 0;
+  S.ff[0][0] = d[0];
+  S.fmin = S.rnorm;
+  S.cap = ng_bound(d[3], d[4]);
+  S.lam = std::min(kNgLam0, S.cap);
+  ng_ls_begin(S, d[0]);
+}
+
+// One NGMRES stage's dots (ST_NG_LS, ST_NG_LSF, ST_NG_A).
+void ng_digest(Solve& S, dmd::SolveOp& nx, const double* d) {
+  const bool finite = std::isfinite(d[0]) && std::isfinite(d[1]) && d[2] == 0.0;
+  if (S.stage == ST_NG_LS || S.stage == ST_NG_LSF) {
+    if (!finite) {  // a trial left the residual's domain: shorter steps
+      S.lam *= 0.5;
+      if (++S.ls > 12) finish(S, DMDGPU_RELAX_STALLED);
+      return;
+    }
+  }
+  if (S.stage == ST_NG_LS) {
+    // secant step on dphi(lambda) = -F(X - lambda F) . F towards its zero
+    const double dphi = -d[1];
+    double ln = S.lam - dphi * (S.lam - S.lam_old) / (dphi - S.dphi_old);
+    if (!std::isfinite(ln) || !(ln > 0.0)) ln = dphi < 0.0 ? 2.0 * S.lam : 0.5 * S.lam;
+    ln = std::min(ln, S.cap);
+    const bool done = ++S.ls >= kNgLsIts || std::fabs(ln - S.lam) <= 1e-2 * ln;
+    S.lam_old = S.lam;
+    S.dphi_old = dphi;
+    S.lam = ln;
+    if (done) S.stage = ST_NG_LSF;
+    return;
+  }
+  if (S.stage == ST_NG_LSF) {
+    // x^M = X - lambda F and F^M: keep them (D, Rr), then the combination
+    nx.upd |= dmd::kUpdNgKeepM;
+    S.fm = std::sqrt(d[0]);
+    S.em = d[dmd::kDotEnergy];
+    for (int t = 0; t < S.nh; t++) S.fmf[t] = d[5 + t];
+    S.cap_m = ng_bound(d[3], d[4]);
+    ng_combine(S);
+    S.stage = ST_NG_A;
+    return;
+  }
+  // ST_NG_A: select x^A or x^M, push it into the history
+  const double fa = std::sqrt(d[0]);
+  double dmin = 1e300;
+  for (int t = 0; t < S.nh; t++) dmin = std::min(dmin, d[9 + t]);
+  dmin = std::sqrt(dmin);
+  const double dn = std::sqrt(d[8]);
+  const bool take_a = finite && fa < kNgGammaA * S.fmin &&
+                      (kNgEpsB * dn < dmin || fa * fa < kNgDeltaB * S.fmin * S.fmin);
+  double fn, cap, fd[dmd::kNgHist];
+  if (take_a) {
+    nx.upd |= dmd::kUpdAcceptY | dmd::kUpdR0FromRv;
+    fn = fa;
+    cap = ng_bound(d[3], d[4]);
+    for (int t = 0; t < S.nh; t++) fd[t] = d[5 + t];
+    S.energy = d[dmd::kDotEnergy];
+  } else {
+    nx.upd |= dmd::kUpdNgAcceptM;
+    fn = S.fm;
+    cap = S.cap_m;
+    for (int t = 0; t < S.nh; t++) fd[t] = S.fmf[t];
+    S.energy = S.em;
+  }
+  S.rnorm = fn;
+  S.newton++;
+  S.ng_fail = fn > kNgGammaC * S.fmin ? S.ng_fail + 1 : 0;
+  S.fmin = std::min(S.fmin, fn);
+  const int m = S.o.ngmres_m;
+  if (S.ng_fail > kNgRestart) {  // restart: the history keeps the new iterate only
+    S.nh = 0;
+    S.ng_next = 0;
+    S.ng_fail = 0;
+  }
+  const int slot = S.ng_next;
+  S.ng_next = (S.ng_next + 1) % m;
+  for (int t = 0; t < S.nh; t++)
+    if (t != slot) S.ff[slot][t] = S.ff[t][slot] = fd[t];
+  S.ff[slot][slot] = fn * fn;
+  S.nh = std::min(S.nh + 1, m);
+  nx.upd |= dmd::kUpdNgPush;
+  nx.slot = slot;
+  if (fn <= S.o.atol) {
+    finish(S, DMDGPU_RELAX_CONVERGED);
+  } else if (S.newton >= S.o.max_newton) {
+    finish(S, DMDGPU_RELAX_MAXITER);
+  } else {
+    S.cap = cap;
+    S.lam = std::min(2.0 * S.lam, cap);
+    ng_ls_begin(S, fn * fn);
+  }
+}
+
 // Consumes the tick's dots: advances each instance's state machine and queues
 // the vector updates for the start of the next tick.
 void digest_tick(dmdgpu_ctx* c) {
@@ -1200,7 +1420,16 @@ void digest_tick(dmdgpu_ctx* c) {
       continue;
     }
     switch (S.stage) {
+      case ST_NG_LS:
+      case ST_NG_LSF:
+      case ST_NG_A:
+        ng_digest(S, nx, d);
+        break;
       case ST_START: {
+        if (S.o.solver) {
+          ng_start(S, nx, d);
+          break;
+        }
         S.rnorm = std::sqrt(d[0]);
         S.energy = d[dmd::kDotEnergy];
         if (S.newton == 0) S.rnorm0 = S.rnorm;
@@ -1394,7 +1623,8 @@ dmdgpu_status tick_device(dmdgpu_ctx* c, int mask) {
   HIPCHK(dmd::launch_diff_flux(c->stream, s, c->d_ops.p, make_vecs(c), o, c->d_dnn.p,
                                c->d_dnbr.p));
   HIPCHK(dmd::launch_solver_dots(c->stream, c->extent, ni, c->d_inst.p, c->d_spc.p, c->d_ne.p,
-                                 c->d_ops.p, c->d_prec.p, make_vecs(c), o, c->d_xr.p, c->d_spart.p,
+                                 c->d_ops.p, c->d_prec.p, make_vecs(c), o, c->d_xr.p, c->d_qs.p,
+                                 c->d_spart.p,
                                  c->d_chunk_lo.p, c->d_chunk_hi.p, c->d_sdots.p));
   HIPCHK(dmd::launch_diff_apply(c->stream, c->extent, c->d_inst.p, c->d_spc.p, c->d_ops.p,
                                 make_vecs(c), c->d_xr.p, c->d_ne.p));
@@ -2129,9 +2359,12 @@ dmdgpu_status dmdgpu_relax_start(dmdgpu_ctx* c, int id, const dmdgpu_relax_opts*
   S.o = o ? *o : dmdgpu_relax_opts{1e-8, 50, 60, 0.05, 0, 100000, 0, 0.0, 0, 0};
   if (!(S.o.atol > 0) || S.o.max_newton <= 0 || S.o.max_cg <= 0 || !(S.o.eta > 0) ||
       S.o.dof_mask < 0 || S.o.dof_mask > 15 || !(S.o.probe >= 0) || S.o.precond < 0 ||
-      (S.o.merit != 0 && S.o.merit != 1))
+      (S.o.merit != 0 && S.o.merit != 1) || (S.o.solver != 0 && S.o.solver != 1) ||
+      S.o.ngmres_m < 0 || S.o.ngmres_m > dmd::kNgHist ||
+      (S.o.solver == 1 && (S.o.mixed || S.o.precond)))
     return fail(DMDGPU_ERR_ARG, "bad relaxation options");
   if (S.o.max_evals <= 0) S.o.max_evals = 100000;
+  if (S.o.ngmres_m == 0) S.o.ngmres_m = dmd::kNgHist;
   if (S.o.dof_mask == 0) S.o.dof_mask = DMDGPU_DOF_MG_Q | DMDGPU_DOF_MG_SIGMA;
   S.stage = ST_START;
   int nd = 0;

@@ -55,17 +55,21 @@ class RelaxOpts(C.Structure):
     _fields_ = [("atol", C.c_double), ("max_newton", C.c_int), ("max_cg", C.c_int),
                 ("eta", C.c_double), ("mixed", C.c_int), ("max_evals", C.c_int),
                 ("dof_mask", C.c_int), ("probe", C.c_double), ("precond", C.c_int),
-                ("merit", C.c_int)]
+                ("merit", C.c_int), ("solver", C.c_int), ("ngmres_m", C.c_int)]
 
 
 DOF_MG_Q, DOF_MG_SIGMA, DOF_H_Q, DOF_H_SIGMA = 1, 2, 4, 8
 
 
 def relax_opts(atol=1e-8, max_newton=50, max_cg=60, eta=0.05, mixed=False, max_evals=0,
-               dof_mask=DOF_MG_Q | DOF_MG_SIGMA, probe=0.0, precond=False, merit="residual"):
-    """merit: "residual" (line search on |R|_2) or "energy" (on sum_i S0_i)."""
+               dof_mask=DOF_MG_Q | DOF_MG_SIGMA, probe=0.0, precond=False, merit="residual",
+               solver="newton", ngmres_m=3):
+    """merit: "residual" (line search on |R|_2) or "energy" (on sum_i S0_i).
+    solver: "newton" (inexact Newton-CG) or "ngmres" (NGMRES(ngmres_m) with a
+    critical-point line search, the reference's SNES configuration)."""
     return RelaxOpts(atol, max_newton, max_cg, eta, 1 if mixed else 0, max_evals, dof_mask,
-                     probe, int(precond), {"residual": 0, "energy": 1}[merit])
+                     probe, int(precond), {"residual": 0, "energy": 1}[merit],
+                     {"newton": 0, "ngmres": 1}[solver], ngmres_m)
 
 
 class DiffuseOpts(C.Structure):

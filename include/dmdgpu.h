@@ -331,6 +331,18 @@ typedef struct {
                       rounding level and |R| decreases): monotone descent
                       where the landscape is not convex (free surfaces,
                       interstitial H positions) */
+  int solver;      /* 0: the inexact Newton-CG above; 1: NGMRES(m) with a
+                      critical-point line search, the reference's SNES
+                      configuration (main-BT-AM.cpp:63-66: snes_type ngmres,
+                      ngmres_m 3, linesearch cp): per iteration a step
+                      X - lambda R with lambda from secant iterations on
+                      R(X - lambda R) . R = 0 (at most 3, steps bounded as
+                      above), then the combination of the last m iterates
+                      that minimises the linearised residual, kept when it
+                      passes the Oosterlee-Washio selection tests (restart
+                      after 2 failures); FP64 evaluations only (mixed and
+                      precond must be 0); max_newton bounds its iterations */
+  int ngmres_m;    /* NGMRES history depth 1..3 (0 = 3) */
 } dmdgpu_relax_opts;
 /* Relaxed unknowns.  Interstitial H at fixed occupancy has a soft (and at low
  * occupancy width-unbounded: sigma_eq^2 ~ 1/(beta k x)) landscape, so the

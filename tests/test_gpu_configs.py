@@ -16,7 +16,8 @@
   request served alone.
 * C1: the 500-site pure-Mg crystal relaxed to |R|_2 <= 1e-8, confirmed by the
   oracle's residual of the final state, and to 1e-11, the FP64 resolution of
-  its residual.
+  its residual; also with the reference's SNES configuration (NGMRES(3) with
+  a critical-point line search).
 """
 import sys
 
@@ -256,3 +257,45 @@ def test_c1_crystal_tight_tolerance(gpu, oracle):
             say(f"    |R_gpu - R_oracle| {order:.2e} (summation order), oracle |R| {rn:.2e}")
             assert order < 1e-10 and rn < 1e-10
     assert out[1e-12]["state"] in ("converged", "stalled") and out[1e-12]["rnorm"] < 1e-11
+
+
+def test_c1_crystal_relaxation_ngmres(gpu, oracle):
+    """C1 with the reference's SNES configuration (main-BT-AM.cpp:63-66):
+    NGMRES(3) with a critical-point line search relaxes the crystal to
+    |R|_2 <= 1e-8, confirmed by the oracle; the same system batched with two
+    clusters gives bit for bit the solve alone; NGMRES takes FP64 evaluations
+    only (mixed or preconditioned options are argument errors)."""
+    say("[gpu] C1 crystal relaxation, NGMRES(3) + critical-point line search")
+    sd = lat.hcp_mgh(5, 5, 5, with_h=False, seed=1)
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=400, solver="ngmres", ngmres_m=3)
+    gpu.set_state(sd)
+    gpu.neighbors()
+    gpu.relax(opts)
+    st = gpu.relax_status(0)
+    say(f"    {st}")
+    assert st["state"] == "converged" and st["rnorm"] <= 1e-8, st
+    q, sg, _ = gpu.instance_state(0)
+    fin = dict(sd)
+    fin["q"], fin["sigma"] = q, sg
+    ref = oracle_of(oracle, fin)
+    rn = np.sqrt((ref["Rq"] ** 2).sum() + (ref["Rs"] ** 2).sum())
+    say(f"    oracle |R| of the final state {rn:.3e}")
+    assert rn < 1e-7, rn
+    ctx = dmdgpu.Context(0)
+    try:
+        ids = [ctx.add_instance(s) for s in
+               (lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=41), sd,
+                lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=42))]
+        ctx.relax(opts)
+        sb = ctx.relax_status(ids[1])
+        assert (sb["state"], sb["newton"], sb["evals"]) == (st["state"], st["newton"], st["evals"])
+        qb, sgb, _ = ctx.instance_state(ids[1])
+        assert np.array_equal(qb, q) and np.array_equal(sgb, sg)
+        for i in (ids[0], ids[2]):
+            assert ctx.relax_status(i)["state"] == "converged", ctx.relax_status(i)
+        for bad in (dict(mixed=True), dict(precond=10)):
+            with pytest.raises(dmdgpu.DmdError) as e:
+                ctx.relax_start(ids[0], dmdgpu.relax_opts(solver="ngmres", **bad))
+            assert e.value.code == 1
+    finally:
+        ctx.close()
