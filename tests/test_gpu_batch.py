@@ -6,8 +6,10 @@
   final state's residual.  The mixed mode (split-precision Hessian probes,
   FP64 residuals) reaches the same tolerance.
 * A relaxation inside a batch follows bitwise the same trajectory as alone.
-* Relaxation ticks skip the occupancy residual R_x; the residual read after a
-  relaxation still carries the full R_x of the final state.
+* Relaxation ticks skip the occupancy residual R_x (and, with the residual
+  merit, the energies); the residual read after a relaxation still carries the
+  full R_x and totals of the final state, under every pair rule and in both
+  precisions.
 """
 import sys
 
@@ -139,19 +141,37 @@ def test_relaxation_converges(ctx, oracle, mixed, tuning):
     ctx.set_tuning(4, 4, 1)
 
 
-@pytest.mark.parametrize("dof", [dmdgpu.DOF_MG_Q | dmdgpu.DOF_MG_SIGMA,
-                                 dmdgpu.DOF_MG_Q | dmdgpu.DOF_MG_SIGMA | dmdgpu.DOF_H_SIGMA])
-def test_relaxation_tick_residual_equals_full_evaluation(ctx, dof):
-    """The first tick of a relaxation evaluates the FP64 residual at the start
-    point the way relaxation ticks do: without R_x and, when only Mg unknowns
-    are relaxed, without the pairs with two H ends and the H sites' sums.  The
-    norm it reports over the relaxed unknowns equals the norm of the same rows
-    of a full evaluation (to the summation order's rounding), and reading the
-    residual afterwards gives the full evaluation bit for bit."""
-    say(f"[gpu] relaxation tick residual vs full evaluation, dof {dof}")
+MG_QS = dmdgpu.DOF_MG_Q | dmdgpu.DOF_MG_SIGMA
+
+
+@pytest.mark.parametrize("dof,rule,prec", [(MG_QS, 3, "fp64"),
+                                           (MG_QS | dmdgpu.DOF_H_SIGMA, 3, "fp64"),
+                                           (MG_QS, 3, "split32"), (MG_QS, 5, "fp64"),
+                                           (MG_QS, 44, "split32"), (MG_QS, 6, "fp64")])
+def test_relaxation_tick_residual_equals_full_evaluation(ctx, dof, rule, prec):
+    """The first tick of a relaxation evaluates the residual at the start
+    point the way relaxation ticks do: without R_x, without energies (the
+    pair-rule points give densities only) and, when only Mg unknowns are
+    relaxed, without the pairs with two H ends and the H sites' sums; FP64
+    probes start in FP64, split-precision probes in split precision.  The norm
+    it reports over the relaxed unknowns equals the norm of the same rows of a
+    full evaluation in that precision (to the summation order's rounding),
+    under every pair rule, and reading the residual afterwards gives the full
+    evaluation bit for bit."""
+    say(f"[gpu] relaxation tick residual vs full evaluation, dof {dof}, rule {rule}, {prec}")
     sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=7)
     ctx.set_state(sd)
     ctx.neighbors()
+    ctx.set_rule(rule)
+    ctx.set_precision(prec)
+    try:
+        _tick_vs_full(ctx, sd, dof, prec == "split32")
+    finally:
+        ctx.set_rule(3)
+        ctx.set_precision("fp64")
+
+
+def _tick_vs_full(ctx, sd, dof, mixed):
     full = ctx.residual()
     act = sd["x"] >= 5e-4
     mg = (sd["species"] == lat.MG) & act
@@ -160,7 +180,7 @@ def test_relaxation_tick_residual_equals_full_evaluation(ctx, dof):
     if dof & dmdgpu.DOF_H_SIGMA:
         n2 += (full["Rs"][h] ** 2).sum()
     ctx.relax_start(0, dmdgpu.relax_opts(atol=1e-8, max_newton=40, max_cg=80, eta=0.05,
-                                         mixed=False, dof_mask=dof))
+                                         mixed=mixed, dof_mask=dof))
     ctx.relax_tick()
     st = ctx.relax_status(0)
     say(f"    tick norm {st['rnorm']:.15e}, full {np.sqrt(n2):.15e}")
