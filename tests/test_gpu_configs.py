@@ -268,6 +268,8 @@ def test_c1_crystal_relaxation_ngmres(gpu, oracle):
     say("[gpu] C1 crystal relaxation, NGMRES(3) + critical-point line search")
     sd = lat.hcp_mgh(5, 5, 5, with_h=False, seed=1)
     opts = dmdgpu.relax_opts(atol=1e-8, max_newton=400, solver="ngmres", ngmres_m=3)
+    tun = (16, 8, 0)  # the same tuning (summation order) alone and batched
+    gpu.set_tuning(*tun)
     gpu.set_state(sd)
     gpu.neighbors()
     gpu.relax(opts)
@@ -283,6 +285,7 @@ def test_c1_crystal_relaxation_ngmres(gpu, oracle):
     assert rn < 1e-7, rn
     ctx = dmdgpu.Context(0)
     try:
+        ctx.set_tuning(*tun)
         ids = [ctx.add_instance(s) for s in
                (lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=41), sd,
                 lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=42))]
