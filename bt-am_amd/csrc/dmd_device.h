@@ -322,6 +322,29 @@ DEV void grad_psi(const P* L, const P* p, P ir, P u, P du, P w, P dw, P* g) {
   g[2] = c * p[2] + u * L[2] + w * g2;
 }
 
+// A point's product with itself is radial: with r = |p|,
+//   <psi(p), psi(p)> = 1/2 u^2 r^2 + 1/3 w^2 r^4
+// (the trace term removes a third of B:B = w^2 r^4), and the gradient of
+// L . psi(p) at the fixed adjoint L = M psi(p) is half the gradient of that
+// product: J(p)^T M psi(p) = h p with
+//   h = 1/2 u (u' r + u) + w r^2 (1/3 w' r + 2/3 w).
+// The coincident (pair-rule) angular terms and the j1 = j2 self terms of the
+// triplet sums take these forms instead of the 9-component algebra.
+// Written with explicit FMAs and no implicit contraction, so they compile to
+// the same operations wherever they are inlined.
+template <typename P>
+DEV P self_psi(P u, P w, P r2) {
+#pragma clang fp contract(off)
+  return r2 * fma(P(0.5) * u, u, (P(1.0 / 3.0) * w) * (w * r2));
+}
+template <typename P>
+DEV P self_hgrad(P u, P du, P w, P dw, P r, P r2) {
+#pragma clang fp contract(off)
+  const P a = fma(du, r, u);
+  const P b = fma(P(1.0 / 3.0) * dw, r, P(2.0 / 3.0) * w);
+  return fma(P(0.5) * u, a, (w * r2) * b);
+}
+
 DEV void min_image(const Box& b, const double4& qi, const double4& qj, double* r) {
   double v0 = qi.x - qj.x, v1 = qi.y - qj.y, v2 = qi.z - qj.z;
   if (b.per[0]) v0 -= b.L[0] * rint(v0 * b.invL[0]);

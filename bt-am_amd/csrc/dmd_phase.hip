@@ -121,13 +121,13 @@ DEV void phase1_pairs(const State& s, const Tables& T, const Outputs& o, int ins
     double C6 = 0, Phi = 0, Ru = 0, Rk = 0;  // per-pair sums in FP64
 #pragma unroll 1
     for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      P p[3], off, ir, dsi[3], dsj[3], cw[8], f[16], cr[4], ps[9];
+      P p[3], off, ir, dsi[3], dsj[3], cw[8], f[16], cr[4];
       int m;
       const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
       gather_cpt<P, CO, CO, VALUES>(uwt, t * nr + m, prt, t * nr + m, stage, cw, f);
       if constexpr (VALUES) {
-        psi_from(cval(cw, off), cval(cw + 4, off), p, ps);
-        C6 += wt * (double)dotM(ps, ps);
+        const P r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        C6 += wt * (double)self_psi(cval(cw, off), cval(cw + 4, off), r2);
         Phi += wt * (double)cval(f, off);
       }
       fused_rho(f, spk, cr);
@@ -227,10 +227,14 @@ DEV void phase1_site(const State& s, const Tables& T, const Outputs& o, int i, i
 #pragma unroll
         for (int b = 0; b < kShiftsPerWalk; b++) {
           P ps[9];
-          psi_from(cval(cw[b], off[b]), cval(cw[b] + 4, off[b]), p[b], ps);
+          const P uu = cval(cw[b], off[b]), ww = cval(cw[b] + 4, off[b]);
+          psi_from(uu, ww, p[b], ps);
 #pragma unroll
           for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
-          if constexpr (VALUES) selfA += nj * nj * (double)dotM(ps, ps);
+          if constexpr (VALUES) {
+            const P r2 = p[b][0] * p[b][0] + p[b][1] * p[b][1] + p[b][2] * p[b][2];
+            selfA += nj * nj * (double)self_psi(uu, ww, r2);
+          }
         }
       } else {
 #pragma unroll
@@ -241,10 +245,12 @@ DEV void phase1_site(const State& s, const Tables& T, const Outputs& o, int i, i
           int m;
           geo.point(T, ls >> 1, sh, p, m, off, ir);
           const P* c = uwt + ((size_t)(rb + m) << 3);
-          psi_from(cval(c, off), cval(c + 4, off), p, ps);
+          const P uu = cval(c, off), ww = cval(c + 4, off);
+          psi_from(uu, ww, p, ps);
 #pragma unroll
           for (int e = 0; e < 9; e++) G[b][e] += nj * (double)ps[e];
-          if constexpr (VALUES) selfA += nj * nj * (double)dotM(ps, ps);
+          if constexpr (VALUES)
+            selfA += nj * nj * (double)self_psi(uu, ww, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
         }
       }
     }
@@ -650,8 +656,12 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
       const P cra = (P)(bu * nu * FPu * nk);
       const P crb = (P)(bk * nk * FPk * nu);
       const P cang = (P)(2.0 * (wa * nk + wb * nu));
+      // Every term at a pair-rule point is radial (the angular one through
+      // self_hgrad), so its gradient is one scalar times p.
       auto cpt = [&](int pt) {
-        P p[3], ir, o2, ps[9], L[9], g3[3], dsu[3], dsk[3], c[8], f[16], ck[4], cu[4];
+        // explicit FMAs only: the same operations with and without XR
+#pragma clang fp contract(off)
+        P p[3], ir, o2, dsu[3], dsk[3], c[8], f[16], ck[4], cu[4];
         int mm;
         const double wt =
             PairRule<RULE>::point(geo, T, pt, suP, skP, p, mm, o2, ir, dsu, dsk);
@@ -661,18 +671,22 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
         fused_rho(f, spk, ck);
         fused_rho(f, spu, cu);
         const P uu = cval(c, o2), du = cder(c, o2), ww = cval(c + 4, o2), dw = cder(c + 4, o2);
-        psi_from(uu, ww, p, ps);
-        applyM(wtP * cang, ps, L);
-        grad_psi(L, p, ir, uu, du, ww, dw, g3);
-        const P cs = wtP * (cphi * cder(cp, o2) + cra * cder(ck, o2) + crb * cder(cu, o2)) * ir;
-        g3[0] += cs * p[0]; g3[1] += cs * p[1]; g3[2] += cs * p[2];
+        const P r2 = fma(p[2], p[2], fma(p[1], p[1], p[0] * p[0]));
+        const P rad = fma(cphi, cder(cp, o2), fma(cra, cder(ck, o2), crb * cder(cu, o2)));
+        const P cs = wtP * fma(cang, self_hgrad(uu, du, ww, dw, r2 * ir, r2), rad * ir);
+        const P g3[3] = {cs * p[0], cs * p[1], cs * p[2]};
         q0 += (double)g3[0]; q1 += (double)g3[1]; q2 += (double)g3[2];
-        sgu += (double)(g3[0] * dsu[0] + g3[1] * dsu[1] + g3[2] * dsu[2]);
-        sgk += (double)(g3[0] * dsk[0] + g3[1] * dsk[1] + g3[2] * dsk[2]);
+        sgu += (double)fma(g3[2], dsu[2], fma(g3[1], dsu[1], g3[0] * dsu[0]));
+        sgk += (double)fma(g3[2], dsk[2], fma(g3[1], dsk[1], g3[0] * dsk[0]));
         if (XR) {  // angular, density and pair terms of the other end's V0
-          const double pp = (double)dotM(ps, ps), hp = 0.5 * (double)cval(cp, o2);
-          xu += wt * (2.0 * xb * nu * pp + bk * nk * (FPk * (double)cval(cu, o2) + hp));
-          xk += wt * (2.0 * xbk * nk * pp + bu * nu * (FPu * (double)cval(ck, o2) + hp));
+          // from opaque copies of the inputs: the gradient above then compiles
+          // the same with and without these terms
+          const P ox = o2;
+          const P *cpx = cp, *cux = cu, *ckx = ck;
+          const double pp = (double)self_psi(uu, ww, r2),
+                       hp = 0.5 * (double)cval(cpx, ox);
+          xu += wt * (2.0 * xb * nu * pp + bk * nk * (FPk * (double)cval(cux, ox) + hp));
+          xk += wt * (2.0 * xbk * nk * pp + bu * nu * (FPu * (double)cval(ckx, ox) + hp));
         }
       };
       if (UNR && PairRule<RULE>::NP <= 27) {  // 44 points unrolled by three spill
