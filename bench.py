@@ -29,21 +29,22 @@ import numpy as np  # noqa: E402
 # (profiles/valu_mgh_20x15x15.md).  Phase 1: site sums plus the pair-rule pass
 # (once per unordered pair); phase 2: the pair kernel (each unordered pair
 # once) plus the slot sums.  The PQE count per residual stays the workload's
-# fixed 50 per active ordered pair.
-FLOP_PER_PAIR = {"phase1_site_sums": 1771, "phase2_gradient": 2941}
+# fixed 50 per active ordered pair.  (The pair-rule terms in radial form took
+# these from 1771 and 2941 with the same results.)
+FLOP_PER_PAIR = {"phase1_site_sums": 1480, "phase2_gradient": 2426}
 # The split-precision evaluation, counted the same way (SQ_INSTS_VALU_FLOPS_FP32
 # and _FP64 x 64 per active ordered pair, profiles/valu_mgh_20x15x15.md):
 # (FP32, FP64) FLOPs per pair of each phase.
-FLOP_PER_PAIR_SPLIT = {"phase1_site_sums": (821 + 532, 335 + 78),
-                       "phase2_gradient": (2644, 229 + 22)}
+FLOP_PER_PAIR_SPLIT = {"phase1_site_sums": (712 + 351, 335 + 78),
+                       "phase2_gradient": (2135, 229 + 22)}
 # Relaxation ticks (Mg positions and widths, residual-norm line search) form
 # less: the pair-rule pass the densities only, phase 1 no energy sums, phase 2
 # no occupancy terms and only the pairs with an Mg end.  Counted the same way
 # on ticks of this workload (profiles/valu_mgh_20x15x15.md, relaxation ticks):
 # per active ordered pair for phase 1, per pair with an Mg end for phase 2;
 # FP64 ticks, then split ticks as (FP32, FP64).
-FLOP_PER_PAIR_TICK = {"phase1_site_sums": 1203, "phase2_gradient": 2368}
-FLOP_PER_PAIR_TICK_SPLIT = {"phase1_site_sums": (839, 358), "phase2_gradient": (2186, 144)}
+FLOP_PER_PAIR_TICK = {"phase1_site_sums": 1203, "phase2_gradient": 1944}
+FLOP_PER_PAIR_TICK_SPLIT = {"phase1_site_sums": (840, 358), "phase2_gradient": (1768, 144)}
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
 HBM_PEAK = 8.0e12          # B/s
