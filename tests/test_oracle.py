@@ -484,3 +484,36 @@ def test_zero_width_limit_is_zero_temperature_density(oracle):
         f = oracle.fact_residual(s)
         errs.append(rel(f["mf_rho"], oracle.ref_rho0(s)))
     assert errs[1] < 2e-2 * errs[0] + 1e-13
+
+
+def test_radial_self_product_and_gradient():
+    """The identity the kernels use at pair-rule points and for the j1 = j2
+    self terms: for psi(p) = (u(r) p, w(r) p p^T), r = |p|, and the dipole +
+    quadrupole form <X,Y> = 1/2 a.a' + 1/2 B:B' - 1/6 trB trB'
+    (src/ADP/MgHx-ADP.cpp:1233-1281, 1781-1831),
+      <psi, psi> = 1/2 u^2 r^2 + 1/3 w^2 r^4,
+    and the gradient of L . psi(p) at fixed L = M psi(p) -- half the gradient
+    of <psi, psi> -- is h p with h = 1/2 u (u' r + u) + w r^2 (1/3 w' r + 2/3 w).
+    Checked against the bilinear form itself and central differences, with
+    polynomial u, w (a sign or factor slip in h fails it)."""
+    rng = np.random.default_rng(5)
+    cu, cw = rng.normal(size=4), rng.normal(size=4)
+    u = lambda r: np.polyval(cu, r)
+    w = lambda r: np.polyval(cw, r)
+    du = lambda r: np.polyval(np.polyder(cu), r)
+    dw = lambda r: np.polyval(np.polyder(cw), r)
+
+    def form(p):
+        r = np.linalg.norm(p)
+        a, B = u(r) * p, w(r) * np.outer(p, p)
+        return 0.5 * a @ a + 0.5 * np.sum(B * B) - np.trace(B) ** 2 / 6.0
+
+    for _ in range(20):
+        p = rng.normal(size=3) * 2.0
+        r = np.linalg.norm(p)
+        assert form(p) == pytest.approx(0.5 * u(r) ** 2 * r ** 2 + w(r) ** 2 * r ** 4 / 3.0,
+                                        rel=1e-12)
+        h = 0.5 * u(r) * (du(r) * r + u(r)) + w(r) * r ** 2 * (dw(r) * r / 3.0 + 2.0 * w(r) / 3.0)
+        e = 1e-5
+        g = np.array([(form(p + e * d) - form(p - e * d)) / (4 * e) for d in np.eye(3)])
+        assert np.allclose(g, h * p, rtol=1e-7, atol=1e-9 * np.abs(h * p).max())
