@@ -679,14 +679,9 @@ DEV void phase2_pairs(const State& s, const Tables& T, const Outputs& o, int ins
         sgu += (double)fma(g3[2], dsu[2], fma(g3[1], dsu[1], g3[0] * dsu[0]));
         sgk += (double)fma(g3[2], dsk[2], fma(g3[1], dsk[1], g3[0] * dsk[0]));
         if (XR) {  // angular, density and pair terms of the other end's V0
-          // from opaque copies of the inputs: the gradient above then compiles
-          // the same with and without these terms
-          const P ox = o2;
-          const P *cpx = cp, *cux = cu, *ckx = ck;
-          const double pp = (double)self_psi(uu, ww, r2),
-                       hp = 0.5 * (double)cval(cpx, ox);
-          xu += wt * (2.0 * xb * nu * pp + bk * nk * (FPk * (double)cval(cux, ox) + hp));
-          xk += wt * (2.0 * xbk * nk * pp + bu * nu * (FPu * (double)cval(ckx, ox) + hp));
+          const double pp = (double)self_psi(uu, ww, r2), hp = 0.5 * (double)cval(cp, o2);
+          xu += wt * (2.0 * xb * nu * pp + bk * nk * (FPk * (double)cval(cu, o2) + hp));
+          xk += wt * (2.0 * xbk * nk * pp + bu * nu * (FPu * (double)cval(ck, o2) + hp));
         }
       };
       if (UNR && PairRule<RULE>::NP <= 27) {  // 44 points unrolled by three spill
