@@ -125,3 +125,41 @@ def test_smallest_systems(gpu, oracle):
                         [lat.MG, lat.MG, lat.H], [1.0, 1.0, 0.0])
     out, _ = check(gpu, oracle, trio)
     assert not np.any(out["Rq"][2]) and out["Rs"][2] == 0.0
+
+
+def test_edge_systems_in_one_batch(oracle):
+    """The random boxes, the compressed lattice and the dimer resident together
+    and evaluated with one launch per kernel, in both precisions: each equals
+    its evaluation alone bit for bit and the oracle within the bars above."""
+    import dmdgpu
+    say("[gpu] edge systems in one batch")
+    syss = [random_system(s) for s in range(6)] + [
+        dense_rows(), small_system([[0.0, 0.0, 0.0], [3.1, 0.4, -0.2]], [lat.MG, lat.MG], [1.0, 1.0])]
+    refs = []
+    for sd in syss:
+        st, nnb, nbr = oracle.neighbors(sd, RC)
+        assert st == 0
+        refs.append(oracle.fact_residual(oracle.system(sd, nnb, nbr)))
+    ctx, alone = dmdgpu.Context(0), dmdgpu.Context(0)
+    try:
+        ids = [ctx.add_instance(sd) for sd in syss]
+        for prec in ("fp64", "split32"):
+            ctx.set_precision(prec)
+            alone.set_precision(prec)
+            ctx.evaluate()
+            tol = {k: TOL for k in ("Rq", "Rs", "Rx", "S0_total")} if prec == "fp64" else SPLIT_TOL
+            for iid, sd, ref in zip(ids, syss, refs):
+                out = ctx.instance_residual(iid)
+                alone.set_state(sd)
+                alone.neighbors()
+                a = alone.residual()
+                for key in ("Rq", "Rs", "Rx"):
+                    assert np.array_equal(out[key], a[key]), (prec, iid, key)
+                assert out["S0_total"] == a["S0_total"]
+                err = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
+                err["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / np.abs(ref["S0"]).sum()
+                for k, t in tol.items():
+                    assert err[k] < t, (prec, iid, k, err[k])
+    finally:
+        ctx.close()
+        alone.close()
