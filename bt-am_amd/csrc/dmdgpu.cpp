@@ -303,8 +303,13 @@ void pack4(const Spline& s, double* out) {
     for (int k = 0; k < 4; k++) out[(size_t)m * 4 + k] = s.c[(size_t)m * 9 + k];
 }
 
+void drop_graph(dmdgpu_ctx* c);
+
 dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   if (!c->tables_dirty) return DMDGPU_OK;
+  // Captured tick graphs hold the table pointers and grid in their kernel
+  // arguments; new tables may move or resize them.
+  drop_graph(c);
   if (!(c->have[0] && c->have[1] && c->have[2]))
     return fail(DMDGPU_ERR_STATE, "all three ADP table files (MgMg, HH, MgH) must be loaded");
   const int nr = c->rho[0].n;
@@ -569,8 +574,10 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   return DMDGPU_OK;
 }
 
-// Per-instance metadata (boxes, chunk ranges) to the device.
+// Per-instance metadata (boxes, chunk ranges) to the device, and tables
+// loaded since the last evaluation.
 dmdgpu_status upload_meta(dmdgpu_ctx* c) {
+  CHK(upload_tables(c));
   if (!c->meta_dirty) return DMDGPU_OK;
   const int ni = (int)c->inst.size();
   const size_t cap = std::max(ni, 1);

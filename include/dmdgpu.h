@@ -75,7 +75,9 @@ dmdgpu_status dmdgpu_sync(dmdgpu_ctx* ctx);
  * Replaces init_adp_MgHx(adpPotential*, species_comb_MgH, const char* folder)
  * (src/ADP/MgHx-ADP.cpp:366-501): reads adpmg.dat / adph.dat / adpmgh.dat in
  * the reference layout from `folder` and uploads them.  All three files must be
- * loaded before dmdgpu_set_state. */
+ * loaded before dmdgpu_set_state.  Tables loaded later (any grid) take effect
+ * at the next evaluation, relaxation tick or diffusion step of every resident
+ * instance. */
 dmdgpu_status dmdgpu_init_adp_MgHx(dmdgpu_ctx* ctx, int species_comb, const char* folder);
 
 /* Neighbour cut-off on mean distances; default r_cutoff_ADP_MgHx =

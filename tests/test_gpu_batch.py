@@ -411,3 +411,38 @@ def test_graph_ticks_equal_direct_launches(mixed):
         assert sa == sb and sa["state"] == "converged"
         for a, b in zip(xa, xb):
             assert np.array_equal(a, b)
+
+
+def test_tables_loaded_between_ticks(tmp_path):
+    """Tables loaded while instances are resident take effect at the next tick,
+    and the captured tick graphs, which carry the table pointers and grid in
+    their kernel arguments, are captured again: after a few ticks with the
+    reference tables (10,000 rows), a switch to a 3,000-row synthetic set and a
+    reset of the unknowns to the start, the relaxation equals bit for bit the
+    one of a context that had the synthetic tables from the start."""
+    import dmdgpu.tables as tabs
+    say("[gpu] tables loaded between ticks")
+    tabs.synthetic_mgh(str(tmp_path), seed=3, nr=3000, ne=3000)
+    sd = lat.cluster(3, 2, 2, x_h=(0.05, 0.3), seed=21)
+    opts = dmdgpu.relax_opts(atol=1e-8, max_newton=4, max_cg=80, eta=0.05)
+    a = dmdgpu.Context(0)
+    b = dmdgpu.Context(0, str(tmp_path))
+    try:
+        ia = a.add_instance(sd)
+        a.relax_start(ia, opts)
+        for _ in range(3):
+            a.relax_tick()  # graphs captured with the reference tables
+        for comb in range(3):
+            assert a.L.dmdgpu_init_adp_MgHx(a.h, comb, str(tmp_path).encode()) == 0
+        a.instance_set_unknowns(ia, q=sd["q"], sigma=sd["sigma"], x=sd["x"])
+        a.relax(opts)  # restarts the resident instance from its (reset) state
+        ib = b.add_instance(sd)
+        b.relax(opts)
+        sa, sb = a.relax_status(ia), b.relax_status(ib)
+        say(f"    {sa}")
+        assert sa == sb and sa["evals"] > 10, (sa, sb)
+        for u, v in zip(a.instance_state(ia), b.instance_state(ib)):
+            assert np.array_equal(u, v)
+    finally:
+        a.close()
+        b.close()
