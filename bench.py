@@ -509,7 +509,8 @@ def _relax_batched(ctx, insts, opts):
         d = [dt for lv, dt in tick_s if lo <= lv <= hi]
         if d:
             by_live[f"{lo}-{hi}"] = {"ticks": len(d), "ms_per_tick": 1e3 * sum(d) / len(d)}
-    flops = (flops, by_live)
+    ev = [ctx.relax_evals(i) for i in ids]
+    flops = (flops, by_live, (sum(e[0] for e in ev), sum(e[1] for e in ev)))
     for i in ids:
         ctx.remove_instance(i)
     return t, ticks, res, flops
@@ -588,7 +589,8 @@ def solves_leg(tuning, count, graph=True):
             "batched_split32_probes": {
                 "solves_per_hour": n * 3600.0 / t_batch, "seconds": t_batch,
                 "converged": sum(b[0]["state"] == "converged" for b in batch),
-                "residual_evals": sum(b[0]["evals"] for b in batch), "ticks": ticks,
+                "residual_evals": sum(b[0]["evals"] for b in batch),
+                "residual_evals_fp64_split": list(fl_batch[2]), "ticks": ticks,
                 "ms_per_tick": t_batch * 1e3 / max(ticks, 1),
                 "ms_per_tick_by_live_solves": fl_batch[1],
                 "roofline": roofline_of(fl_batch[0][0], fl_batch[0][1], t_batch)},
