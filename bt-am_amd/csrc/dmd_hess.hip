@@ -13,6 +13,9 @@
 // JG0^T M JU) with the Jacobian sums JG_ls = sum_k n_k J(r_uk + 3 s sigma_u
 // e_l), JU = sum_k n_k sum_ls J(r_uk + 3 s sigma_k e_l), JG0 = sum_k n_k J(r_uk),
 // less their j1 = j2 parts, plus 2 n_k^2 J^T M J on the coincident points.
+// Everything at a pair-rule point (coincident angular term, its J^T M J, pair,
+// densities) and every j1 = j2 part of one point is an isotropic tensor of the
+// point, a p p^T + b I, formed from u, w and their r-derivatives.
 // For a neighbour centre k only psi_u moves; the reference sums codes 4 and 8
 // of the coincident triplet (k, u, u) and not the mixed codes 5 and 7
 // (:1540-1570), so its 2 n_k^2 n_u^2 J^T M J is added only when `exact` is
@@ -129,6 +132,36 @@ DEV void adjoint(const double* Ru, int bu, double wa, const double* Rk, int bk, 
   L[8] += wa * Ru[kRecTail + bu] + wb * Rk[kRecTail + bk];
 }
 
+// At a pair-rule point the coincident angular term's adjoint is the point's own
+// L = k M psi(p), and both of its Hessian terms are isotropic tensors of p
+// (a p p^T + b I).  With u, w and their r-derivatives at r = |p|:
+//   J^T M J = gam p p^T + alp I,
+//     alp = u^2/2 + w^2 r^2,
+//     gam = u u'/r + u'^2/2 + (w'^2 r^2 + 4 w w' r + w^2)/3;
+//   sum_a (M psi)_a d2psi_a/dp2 = (1/2) d2<psi, M psi>/dp2 - J^T M J
+//                               = ang_a p p^T + ang_b I  (<psi, M psi> = u^2 r^2/2 + w^2 r^4/3),
+//     ang_a = u (u'' + u'/r)/2 + w (w + w' r + w'' r^2/3),
+//     ang_b = u u' r/2 + w r^2 (w' r - w)/3.
+struct SelfTerms {
+  double alp, gam, anga, angb;
+};
+DEV SelfTerms self_terms(const double* uv, double r, double ir) {
+  const double u = uv[0], du = uv[1], d2u = uv[2], w = uv[3], dw = uv[4], d2w = uv[5];
+  const double wr = w * r, dwr = dw * r;
+  SelfTerms o;
+  o.alp = 0.5 * u * u + wr * wr;
+  o.gam = u * du * ir + 0.5 * du * du + (dwr * dwr + 4.0 * w * dwr + w * w) * (1.0 / 3.0);
+  o.anga = 0.5 * u * (d2u + du * ir) + w * (w + dwr + (1.0 / 3.0) * d2w * r * r);
+  o.angb = 0.5 * u * du * r + (1.0 / 3.0) * wr * r * (dwr - w);
+  return o;
+}
+
+// H += a p p^T + b I.
+DEV void iso_add(const double* p, double a, double b, double* H) {
+  H[0] += a * p[0] * p[0] + b; H[1] += a * p[1] * p[1] + b; H[2] += a * p[2] * p[2] + b;
+  H[3] += a * p[0] * p[1]; H[4] += a * p[0] * p[2]; H[5] += a * p[1] * p[2];
+}
+
 DEV double emb_d2(const Tables& T, int sp, double rho) {
   double pe;
   const int me = seg_index(rho, T.dxe[sp], T.inv_dxe[sp], T.ne[sp], pe);
@@ -242,33 +275,28 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
     double gk[3] = {0, 0, 0};
 #pragma unroll 1
     for (int pt = 0; pt < PairRule<RULE>::NP; pt++) {
-      double p[3], ir, off, uv[6], ps[9], L[9], dsi[3], dsj[3], J[27];
+      double p[3], ir, off, uv[6], dsi[3], dsj[3];
       int m;
       const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
       uw_at(T, t, m, off, uv);
-      psi_from(uv[0], uv[3], p, ps);
-      applyM(wt * cang, ps, L);
-      hess_psi(L, p, ir, uv, 1.0, H);
-      jac_psi(p, ir, uv, J);
-      {
-        // J^T M J once: 2 wt n_k^2 of it to the centre's terms, wt cjj to the
-        // mixed codes 5 + 7 of (k, u, u) (cjj = 0 unless exact)
-        double X[6] = {0, 0, 0, 0, 0, 0};
-        jtmj_self(J, 1.0, X);
-        const double c1 = 2.0 * wt * nk * nk, c2 = wt * cjj;
-#pragma unroll
-        for (int t = 0; t < 6; t++) {
-          HJ[t] += c1 * X[t];
-          H[t] += c2 * X[t];
-        }
-      }
+      const double r = (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir;
+      const SelfTerms st = self_terms(uv, r, ir);
       const double* f = T.pr + ((size_t)t * T.nr + m) * 16;
       const double* fk = f + 4 + 4 * spk;
       const double* fu = f + 4 + 4 * spu;
-      rad_hess(cder(f, off), cder2(f, off), p, ir, wt * cphi, H);
-      rad_hess(cder(fk, off), cder2(fk, off), p, ir, wt * cra, H);
-      rad_hess(cder(fu, off), cder2(fu, off), p, ir, wt * crb, H);
-      const double a1 = wt * nk * cder(fk, off) * ir, b1 = wt * nu * cder(fu, off) * ir;
+      const double dk = cder(fk, off), du = cder(fu, off);
+      // pair and both densities: radial Hessians k (c'' - c'/r)/r^2 p p^T + k c'/r I
+      const double kp = wt * cphi, kk1 = wt * cra, ku = wt * crb;
+      const double s1 = (kp * cder(f, off) + kk1 * dk + ku * du) * ir;
+      const double s2 = kp * cder2(f, off) + kk1 * cder2(fk, off) + ku * cder2(fu, off);
+      // angular: wt cang (M psi)-contracted Hessian; J^T M J with wt cjj to the
+      // mixed codes 5 + 7 of (k, u, u) (cjj = 0 unless exact) and 2 wt n_k^2 to
+      // the centre's terms
+      const double kang = wt * cang, c2 = wt * cjj, c1 = 2.0 * wt * nk * nk;
+      iso_add(p, kang * st.anga + c2 * st.gam + (s2 - s1) * ir * ir,
+              kang * st.angb + c2 * st.alp + s1, H);
+      iso_add(p, c1 * st.gam, c1 * st.alp, HJ);
+      const double a1 = wt * nk * dk * ir, b1 = wt * nu * du * ir;
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         gu[a] += a1 * p[a];
@@ -308,7 +336,10 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
         jac_psi(p, ir, uv, J);
 #pragma unroll
         for (int c = 0; c < 27; c++) S[c] += nk * J[c];
-        jtmj_self(J, -nk * nk / 9.0, HJ);
+        // the j1 = j2 part: J^T M J of one point, isotropic
+        const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
+        const double cj = -nk * nk / 9.0;
+        iso_add(p, cj * st.gam, cj * st.alp, HJ);
       } else if (blk == 6) {
 #pragma unroll 1
         for (int ls = 0; ls < 6; ls++) {
