@@ -27,8 +27,8 @@
 // ... and every sum over k closes with a butterfly reduction (the same value
 // on every lane, independent of the order the lanes ran).  The work is split
 // into passes over the neighbours (adjoint points, pair-rule points, then the
-// Jacobian sums one 27-value block at a time) so that little state is live at
-// once: two wavefronts per SIMD fit with 36 B of spill.  (In one pass the
+// Jacobian sums one block at a time, each held in 20 values) so that little
+// state is live at once.  (In one pass the
 // kernel held one wavefront per SIMD and took 7.3 ms on 54,000 sites; split,
 // 5.3 ms.)
 #include "dmd_device.h"
@@ -71,24 +71,6 @@ DEV void hess_psi(const double* L, const double* p, double ir, const double* uv,
   H[3] += cu * (p[0] * L[1] + L[0] * p[1]) + cw * (p[0] * g1 + g0 * p[1]) + w * L[6];
   H[4] += cu * (p[0] * L[2] + L[0] * p[2]) + cw * (p[0] * g2 + g0 * p[2]) + w * L[7];
   H[5] += cu * (p[1] * L[2] + L[1] * p[2]) + cw * (p[1] * g2 + g1 * p[2]) + w * L[8];
-}
-
-// J[3 a + g] = dpsi_a/dp_g.
-DEV void jac_psi(const double* p, double ir, const double* uv, double* J) {
-  const double du = uv[1] * ir, u = uv[0], dw = uv[4] * ir, w = uv[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int g = 0; g < 3; g++) J[3 * a + g] = du * p[a] * p[g] + (a == g ? u : 0.0);
-  constexpr int qi[6] = {0, 1, 2, 0, 0, 1}, qj[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int m = 0; m < 6; m++) {
-    const double pij = p[qi[m]] * p[qj[m]];
-#pragma unroll
-    for (int g = 0; g < 3; g++)
-      J[3 * (3 + m) + g] = dw * p[g] * pij + w * ((qi[m] == g ? p[qj[m]] : 0.0) +
-                                                  (qj[m] == g ? p[qi[m]] : 0.0));
-  }
 }
 
 // A^T M B for 9 x 3 Jacobians, M the form above written out: weights 1/2 on
@@ -161,6 +143,50 @@ DEV void iso_add(const double* p, double a, double b, double* H) {
   H[0] += a * p[0] * p[0] + b; H[1] += a * p[1] * p[1] + b; H[2] += a * p[2] * p[2] + b;
   H[3] += a * p[0] * p[1]; H[4] += a * p[0] * p[2]; H[5] += a * p[1] * p[2];
 }
+
+// A weighted sum of Jacobians sum_k n_k J(p_k), held in 20 values instead of
+// 27: J(p) = [u I + U p p^T ; W p_g p_i p_j + w (d_ig p_j + d_jg p_i)] (U = u'/r,
+// W = w'/r), so the sum is fixed by sum n u, sum n U p p^T (6), the symmetric
+// third-order tensor sum n W p p p (10) and the vector sum n w p (3).
+struct JSum {
+  double a[20];  // u | U pp: xx yy zz xy xz yz | W ppp: xxx yyy zzz xxy xxz xyy xzz yyz yzz xyz | w p
+  DEV void zero() {
+#pragma unroll
+    for (int c = 0; c < 20; c++) a[c] = 0.0;
+  }
+  DEV void add(double n, const double* p, double ir, const double* uv) {
+    const double nu = n * uv[0], nU = n * uv[1] * ir, nW = n * uv[4] * ir, nw = n * uv[3];
+    const double xx = p[0] * p[0], yy = p[1] * p[1], zz = p[2] * p[2];
+    const double xy = p[0] * p[1], xz = p[0] * p[2], yz = p[1] * p[2];
+    a[0] += nu;
+    a[1] += nU * xx; a[2] += nU * yy; a[3] += nU * zz;
+    a[4] += nU * xy; a[5] += nU * xz; a[6] += nU * yz;
+    const double wx = nW * p[0], wy = nW * p[1], wz = nW * p[2];
+    a[7] += wx * xx; a[8] += wy * yy; a[9] += wz * zz;
+    a[10] += wy * xx; a[11] += wz * xx; a[12] += wx * yy;
+    a[13] += wx * zz; a[14] += wz * yy; a[15] += wy * zz; a[16] += wx * yz;
+    a[17] += nw * p[0]; a[18] += nw * p[1]; a[19] += nw * p[2];
+  }
+  // The 27 values S[3 a + g] = sum n dpsi_a/dp_g.
+  DEV void expand(double* S) const {
+    // index of U pp (i, j) and of W ppp (i, j, g) in a[]
+    constexpr int u2[3][3] = {{1, 4, 5}, {4, 2, 6}, {5, 6, 3}};
+    constexpr int t3[3][3][3] = {{{7, 10, 11}, {10, 12, 16}, {11, 16, 13}},
+                                 {{10, 12, 16}, {12, 8, 14}, {16, 14, 15}},
+                                 {{11, 16, 13}, {16, 14, 15}, {13, 15, 9}}};
+    constexpr int qi[6] = {0, 1, 2, 0, 0, 1}, qj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int g = 0; g < 3; g++) S[3 * r + g] = a[u2[r][g]] + (r == g ? a[0] : 0.0);
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+      for (int g = 0; g < 3; g++)
+        S[3 * (3 + m) + g] = a[t3[qi[m]][qj[m]][g]] + (qi[m] == g ? a[17 + qj[m]] : 0.0) +
+                             (qj[m] == g ? a[17 + qi[m]] : 0.0);
+  }
+};
 
 DEV double emb_d2(const Tables& T, int sp, double rho) {
   double pe;
@@ -311,13 +337,12 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
   // ---- Jacobian sums of centre u, one block at a time ----
   // blk 0..5: JG_ls (with their j1 = j2 parts); blk 6: JU; blk 7: JG0, closing
   // JU^T M JG0 + JG0^T M JU; blk 8: the j1 = j2 part of that product.
-  __shared__ double sJU[27];
+  __shared__ double sJU[20];
   double Hw[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform
 #pragma unroll 1
   for (int blk = 0; blk < 9; blk++) {
-    double S[27];
-#pragma unroll
-    for (int c = 0; c < 27; c++) S[c] = 0.0;
+    JSum Z;
+    Z.zero();
 #pragma unroll 1
     for (int kk = lane; kk < nn; kk += 64) {
       const int k = row[kk];
@@ -325,7 +350,7 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
       if (!(nk > 0.0)) continue;
       const double4 qk = s.qs[k];
       const int t = pair_slot(spu, s.spc[k]);
-      double rd[3], p[3], ir, off, uv[6], J[27];
+      double rd[3], p[3], ir, off, uv[6];
       int m;
       min_image(box, qu, qk, rd);
       PairGeo<double> geo;
@@ -333,9 +358,7 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
       if (blk < 6) {
         geo.point(T, blk >> 1, ((blk & 1) ? -3.0 : 3.0) * su, p, m, off, ir);
         uw_at(T, t, m, off, uv);
-        jac_psi(p, ir, uv, J);
-#pragma unroll
-        for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+        Z.add(nk, p, ir, uv);
         // the j1 = j2 part: J^T M J of one point, isotropic
         const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
         const double cj = -nk * nk / 9.0;
@@ -345,49 +368,69 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
         for (int ls = 0; ls < 6; ls++) {
           geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
           uw_at(T, t, m, off, uv);
-          jac_psi(p, ir, uv, J);
-#pragma unroll
-          for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+          Z.add(nk, p, ir, uv);
         }
       } else if (blk == 7) {
         geo.point(T, 0, 0.0, p, m, off, ir);
         uw_at(T, t, m, off, uv);
-        jac_psi(p, ir, uv, J);
-#pragma unroll
-        for (int c = 0; c < 27; c++) S[c] += nk * J[c];
+        Z.add(nk, p, ir, uv);
       } else {
-        // S holds this pair's sum over the six j-shift points, J then J0
+        // -n_k^2/9 (J1^T M J0 + J0^T M J1) summed over this pair's six j-shift
+        // points p1 against its unshifted point p0, in closed form: with
+        // c = p1.p0, U = u'/r, W = w'/r, t = W r^2 + 2 w (index 1 at p1, 0 at p0),
+        //   J1^T M J0 + J0^T M J1 = 2 a I + 2 b1 p1 p1^T + 2 b0 p0 p0^T + d (p1 p0^T + p0 p1^T),
+        //   a = u1 u0/2 + w1 w0 c,  b1 = U1 u0/2 + W1 w0 c,  b0 = u1 U0/2 + w1 W0 c,
+        //   d = U1 U0 c/2 + W1 W0 c^2/2 - t1 t0/6 + w1 w0.
+        double p0[3], ir0, off0, uv0[6];
+        geo.point(T, 0, 0.0, p0, m, off0, ir0);
+        uw_at(T, t, m, off0, uv0);
+        const double u0 = uv0[0], U0 = uv0[1] * ir0, w0 = uv0[3], W0 = uv0[4] * ir0;
+        const double t0 = W0 * (p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2]) + 2.0 * w0;
+        const double k2 = -2.0 * nk * nk / 9.0;
+        double b0s = 0.0, q[3] = {0, 0, 0};
 #pragma unroll 1
         for (int ls = 0; ls < 6; ls++) {
           geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
           uw_at(T, t, m, off, uv);
-          jac_psi(p, ir, uv, J);
-#pragma unroll
-          for (int c = 0; c < 27; c++) S[c] += J[c];
+          const double u1 = uv[0], U1 = uv[1] * ir, w1 = uv[3], W1 = uv[4] * ir;
+          const double c = p[0] * p0[0] + p[1] * p0[1] + p[2] * p0[2];
+          const double t1 = W1 * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2.0 * w1;
+          const double a = 0.5 * u1 * u0 + w1 * w0 * c;
+          const double b1 = 0.5 * U1 * u0 + W1 * w0 * c;
+          b0s += 0.5 * u1 * U0 + w1 * W0 * c;
+          const double d = (0.5 * U1 * U0 + 0.5 * W1 * W0 * c) * c - (1.0 / 6.0) * t1 * t0 + w1 * w0;
+          iso_add(p, k2 * b1, k2 * a, HJ);
+          q[0] += d * p[0]; q[1] += d * p[1]; q[2] += d * p[2];
         }
-        geo.point(T, 0, 0.0, p, m, off, ir);
-        uw_at(T, t, m, off, uv);
-        jac_psi(p, ir, uv, J);
-        jtmj_sym(S, J, -nk * nk / 9.0, HJ);
-#pragma unroll
-        for (int c = 0; c < 27; c++) S[c] = 0.0;
+        const double kh = 0.5 * k2;
+        iso_add(p0, k2 * b0s, 0.0, HJ);
+        HJ[0] += k2 * q[0] * p0[0]; HJ[1] += k2 * q[1] * p0[1]; HJ[2] += k2 * q[2] * p0[2];
+        HJ[3] += kh * (q[0] * p0[1] + p0[0] * q[1]);
+        HJ[4] += kh * (q[0] * p0[2] + p0[0] * q[2]);
+        HJ[5] += kh * (q[1] * p0[2] + p0[1] * q[2]);
       }
     }
     if (blk == 8) break;
 #pragma unroll
-    for (int c = 0; c < 27; c++) S[c] = group_sum<64>(S[c]);
-    if (blk < 6) {
-      jtmj_self(S, 1.0 / 9.0, Hw);
-    } else if (blk == 6) {
-      if (lane == 0)  // S is uniform after the reduction
+    for (int c = 0; c < 20; c++) Z.a[c] = group_sum<64>(Z.a[c]);
+    if (blk == 6) {
+      if (lane == 0)  // Z is uniform after the reduction
 #pragma unroll
-        for (int c = 0; c < 27; c++) sJU[c] = S[c];
+        for (int c = 0; c < 20; c++) sJU[c] = Z.a[c];
       __syncthreads();
     } else {
-      double JU[27];
+      double S[27];
+      Z.expand(S);
+      if (blk < 6) {
+        jtmj_self(S, 1.0 / 9.0, Hw);
+      } else {
+        JSum ZU;
 #pragma unroll
-      for (int c = 0; c < 27; c++) JU[c] = sJU[c];
-      jtmj_sym(JU, S, 1.0 / 9.0, Hw);
+        for (int c = 0; c < 20; c++) ZU.a[c] = sJU[c];
+        double JU[27];
+        ZU.expand(JU);
+        jtmj_sym(JU, S, 1.0 / 9.0, Hw);
+      }
     }
   }
 
