@@ -45,6 +45,10 @@ FLOP_PER_PAIR_SPLIT = {"phase1_site_sums": (712 + 351, 335 + 78),
 # FP64 ticks, then split ticks as (FP32, FP64).
 FLOP_PER_PAIR_TICK = {"phase1_site_sums": 1203, "phase2_gradient": 1944}
 FLOP_PER_PAIR_TICK_SPLIT = {"phase1_site_sums": (840, 358), "phase2_gradient": (1768, 144)}
+# k_diag (the diagonal Hessian blocks, 3rd-order rule, reference assembly):
+# SQ_INSTS_VALU_FLOPS_FP64 x 64 per launch on the same workload, per active
+# ordered pair
+FLOP_PER_PAIR_DIAG = 13486
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
 HBM_PEAK = 8.0e12          # B/s
@@ -352,10 +356,11 @@ def host_leg(ctx, sysd, pqe, reps=10):
                                                     "residual_evaluations": 3}}
 
 
-def diag_leg(ctx, reps=5):
+def diag_leg(ctx, pairs, reps=5):
     """dmdgpu_diag_blocks on the same workload (reference assembly, FP64): the
-    k_diag kernel time from HIP events, the FP64 phase 1 it reads, and the wall
-    time of the whole call (phase 1 + blocks + copy-out of n x 9 doubles)."""
+    k_diag kernel time from HIP events, the FP64 phase 1 it reads, the wall
+    time of the whole call (phase 1 + blocks + copy-out of n x 9 doubles), and
+    k_diag's FP64 rate against the vector peak."""
     ctx.set_precision("fp64")
     ctx.diag_blocks()
     ctx.sync()
@@ -369,8 +374,13 @@ def diag_leg(ctx, reps=5):
     kt = ctx.kernel_times()
     ctx.reset_kernel_times()
     per = lambda k: kt[k][1] / max(kt[k][0], 1)
-    return {"kernel_ms": per("diag_blocks"), "phase1_ms": per("phase1_site_sums"),
-            "call_ms": dt * 1e3}
+    ks = per("diag_blocks") * 1e-3
+    achieved = FLOP_PER_PAIR_DIAG * pairs / ks
+    return {"kernel_ms": ks * 1e3, "phase1_ms": per("phase1_site_sums"), "call_ms": dt * 1e3,
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12,
+                         "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP64_VALU,
+                         "flop_per_launch": FLOP_PER_PAIR_DIAG * pairs}}
 
 
 RULE_POINTS = {3: 12, 5: 27, 44: 44, 6: 6}
@@ -846,7 +856,7 @@ def main():
 
     diag = None
     if not args.no_diag:
-        diag = diag_leg(ctx)
+        diag = diag_leg(ctx, pairs)
         log(f"diag blocks: {diag}")
 
     serving = None
