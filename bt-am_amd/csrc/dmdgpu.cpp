@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -192,6 +193,7 @@ struct dmdgpu_ctx {
   dmd::Tables T{};
   // arena
   int extent = 0;     // sites in use, multiple of kChunk
+  size_t arena_cap = 0;  // sites every per-site array holds
   size_t budget = 0;  // device bytes allowed for the arena (0 = unlimited)
   std::vector<Instance> inst;
   int primary = -1;         // instance of the one-instance API
@@ -304,6 +306,7 @@ void pack4(const Spline& s, double* out) {
 }
 
 void drop_graph(dmdgpu_ctx* c);
+void recompute_extent(dmdgpu_ctx* c);
 
 dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   if (!c->tables_dirty) return DMDGPU_OK;
@@ -537,18 +540,30 @@ size_t device_bytes(dmdgpu_ctx* c) {
   return b;
 }
 
+// Number of the per-site array (1, 2, ... in for_site_bufs order) whose next
+// growth is refused as if the device were out of memory, from the environment
+// variable DMDGPU_TEST_FAIL_GROWTH; 0 (unset): none.  Lets tests drive the
+// recovery from a failed growth without exhausting the device.
+int test_fail_growth() {
+  const char* v = std::getenv("DMDGPU_TEST_FAIL_GROWTH");
+  return v ? std::atoi(v) : 0;
+}
+
 // Makes every per-site array hold at least `sites` sites (keeping contents).
 // The capacity grows by half again when it must grow at all, but never past
 // what the budget allows: the context's other allocations, the new arrays and
-// the one old array held during its copy must fit.
+// the one old array held during its copy must fit.  A growth that fails part
+// way leaves every array holding at least the old capacity (arena_cap), so the
+// resident instances stay intact; if even that cannot be restored, the
+// instances are dropped.
 dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   const size_t keep = c->extent;
-  if (sites <= c->d_qs.n && c->d_qs.p) return DMDGPU_OK;
+  if (sites <= c->arena_cap) return DMDGPU_OK;
   auto round_up = [](size_t v) { return (v + kChunk - 1) / kChunk * kChunk; };
-  size_t cap = round_up(std::max(sites, (size_t)(c->d_qs.n * 3 / 2)));
+  size_t cap = round_up(std::max(sites, c->arena_cap * 3 / 2));
   if (c->budget) {
     size_t kept_old = 0;
-    const size_t old = site_bytes(c, c->d_qs.p ? c->d_qs.n : 0, &kept_old);
+    const size_t old = site_bytes(c, c->arena_cap, &kept_old);
     const size_t other = device_bytes(c) - std::min(device_bytes(c), old);
     auto fits = [&](size_t cp) { return other + site_bytes(c, cp) + kept_old <= c->budget; };
     const size_t need = round_up(sites);
@@ -565,11 +580,41 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
   }
   drop_graph(c);
   hipError_t err = hipSuccess;
+  const int fail_at = test_fail_growth();
+  int idx = 0;
   for_site_bufs(c, cap, [&](auto& buf, size_t cnt, bool kp) {
     if (err != hipSuccess) return;
+    if (++idx == fail_at && cnt > buf.n) {
+      if (!kp) buf.release();  // as a failed allocation of an array without contents to keep
+      err = hipErrorOutOfMemory;
+      return;
+    }
     err = buf.ensure(cnt, kp ? cnt * keep / cap : 0);
   });
-  HIPCHK(err);
+  if (err != hipSuccess) {
+    // The arrays grown before the failure hold more than the old capacity and
+    // those after it the old capacity; an array whose contents are not kept
+    // was released before its allocation: back to the old capacity.
+    const size_t oc = c->arena_cap;
+    hipError_t e2 = hipSuccess;
+    for_site_bufs(c, oc, [&](auto& buf, size_t cnt, bool kp) {
+      if (e2 == hipSuccess && !kp && oc) e2 = buf.ensure(cnt, 0);
+    });
+    if (e2 != hipSuccess) {
+      for (Instance& I : c->inst) I = Instance();
+      std::fill(c->h_inst.begin(), c->h_inst.end(), -1);
+      std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
+      c->primary = -1;
+      c->arena_cap = 0;
+      recompute_extent(c);
+      c->meta_dirty = true;
+      return fail(DMDGPU_ERR_HIP, std::string("arena growth failed (") + hipGetErrorString(err) +
+                                      "), and the arena could not be restored: every instance "
+                                      "was removed");
+    }
+    return fail(DMDGPU_ERR_HIP, std::string("arena growth failed: ") + hipGetErrorString(err));
+  }
+  c->arena_cap = cap;
   HIPCHK(c->d_flags.ensure(3));
   return DMDGPU_OK;
 }
@@ -870,7 +915,7 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   // The arena in use after placement, with the context's other allocations,
   // must fit the budget (ensure_sites also bounds what a growth allocates).
   if (c->budget) {
-    const size_t held = site_bytes(c, c->d_qs.p ? c->d_qs.n : 0);
+    const size_t held = site_bytes(c, c->arena_cap);
     const size_t other = device_bytes(c) - std::min(device_bytes(c), held);
     if (other + site_bytes(c, need) > c->budget)
       return fail(DMDGPU_ERR_BUDGET, "instance does not fit the device-memory budget");

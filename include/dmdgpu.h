@@ -260,7 +260,10 @@ dmdgpu_status dmdgpu_set_rule(dmdgpu_ctx* ctx, int order);
 /* Copies an instance in (same fields as dmdgpu_set_state), builds its neighbour
  * lists and returns its id.  DMDGPU_ERR_BUDGET when the arena would exceed the
  * budget set with dmdgpu_set_budget; DMDGPU_ERR_ARG for more than 6,710,784
- * sites (as dmdgpu_set_state). */
+ * sites (as dmdgpu_set_state); DMDGPU_ERR_HIP when the device cannot grow the
+ * arena, which leaves the resident instances as they were (should even their
+ * arrays not be restorable, every instance is removed and the message says
+ * so). */
 dmdgpu_status dmdgpu_add_instance(dmdgpu_ctx* ctx, int n, const double* h_mean_q,
                                   const double* h_stdv_q, const double* h_xi,
                                   const int32_t* h_specie, const double* h_beta,

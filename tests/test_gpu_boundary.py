@@ -11,6 +11,8 @@
   site there; a relaxation that meets it ends in the "domain" state.
 * Non-finite positions, widths or occupancies are argument errors, and so is
   an instance above the 6,710,784-site limit.
+* An arena growth that fails part way (forced from the host) is an error of
+  the add that asked for it and leaves the resident instances intact.
 """
 import ctypes as C
 import sys
@@ -26,7 +28,7 @@ import dmdgpu.tables as tab
 
 pytestmark = pytest.mark.gpu
 
-ERR_ARG, ERR_MAXNEIGH, ERR_STATE, ERR_BUDGET, ERR_DOMAIN = 1, 5, 8, 10, 11
+ERR_ARG, ERR_HIP, ERR_MAXNEIGH, ERR_STATE, ERR_BUDGET, ERR_DOMAIN = 1, 2, 5, 8, 10, 11
 
 
 def say(msg):
@@ -94,6 +96,46 @@ def test_budget_bounds_device_memory():
             ctx.instance_residual(i)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("k", [1, 14, 39, 47])
+def test_failed_arena_growth_keeps_resident_instances(monkeypatch, k):
+    """DMDGPU_TEST_FAIL_GROWTH = k refuses the k-th per-site array's allocation
+    in the next arena growth, as an out-of-memory device would: k = 1 the
+    positions (contents kept, nothing grown yet), 14 the row slots, 39 the
+    diagonal blocks and 47 the last array (no contents kept: released before
+    their allocation, so they must be put back).  The add fails with
+    DMDGPU_ERR_HIP; the resident instance evaluates bit for bit as before; the
+    same add then succeeds and equals its evaluation in a context of its own."""
+    say(f"[gpu] failed arena growth at array {k}")
+    small, big = lat.box_of_size(3000, seed=1), lat.box_of_size(30000, seed=2)
+    ctx, alone = dmdgpu.Context(0), dmdgpu.Context(0)
+    try:
+        ia = ctx.add_instance(small)
+        ctx.evaluate()
+        ra = ctx.instance_residual(ia)
+        monkeypatch.setenv("DMDGPU_TEST_FAIL_GROWTH", str(k))
+        with pytest.raises(dmdgpu.DmdError) as e:
+            ctx.add_instance(big)
+        assert e.value.code == ERR_HIP
+        monkeypatch.delenv("DMDGPU_TEST_FAIL_GROWTH")
+        ctx.evaluate()
+        r = ctx.instance_residual(ia)
+        for key in ("Rq", "Rs", "Rx"):
+            assert np.array_equal(r[key], ra[key]), key
+        ib = ctx.add_instance(big)
+        ctx.evaluate()
+        rb = ctx.instance_residual(ib)
+        r = ctx.instance_residual(ia)
+        alone.add_instance(big)
+        alone.evaluate()
+        rb1 = alone.instance_residual(0)
+        for key in ("Rq", "Rs", "Rx"):
+            assert np.array_equal(r[key], ra[key]), key
+            assert np.array_equal(rb[key], rb1[key]), key
+    finally:
+        ctx.close()
+        alone.close()
 
 
 def test_table_domain_status(oracle, tmp_path):
