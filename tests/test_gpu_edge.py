@@ -1,7 +1,8 @@
 """GPU parity on the edges of the input domain: seeded random systems (box,
 periodicity, lattice constant, widths, temperatures, occupancies with masked
-sites), neighbour rows at the maximum length the reference allows, and the
-smallest systems (a dimer; a dimer with a masked H site beside it).
+sites), also under the 27-, 44- and 6-point pair rules and for the diagonal
+Hessian blocks, neighbour rows at the maximum length the reference allows,
+and the smallest systems (a dimer; a dimer with a masked H site beside it).
 
 Bars as in test_gpu_parity: FP64 within 1e-10 relative of the oracle, split
 precision within the stated per-quantity bars, except the free-energy total,
@@ -77,22 +78,26 @@ def small_system(q, species, x, sigma=0.12):
             "lo": lo, "hi": hi, "periodic": np.zeros(3, dtype=np.int32)}
 
 
-def check(gpu, oracle, sysd, precision="fp64"):
+def check(gpu, oracle, sysd, precision="fp64", rule=3):
     st, nnb, nbr = oracle.neighbors(sysd, RC)
     assert st == 0
-    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr))
+    ref = oracle.fact_residual(oracle.system(sysd, nnb, nbr, rule))
     gpu.set_state(sysd)
     gpu.set_precision(precision)
+    gpu.set_rule(rule)
     gpu.neighbors()
     gn, gl = gpu.get_neighbors()
     assert np.array_equal(gn, nnb)
     for i in range(sysd["n"]):
         assert np.array_equal(gl[i, :gn[i]], nbr[i, :nnb[i]])
-    out = gpu.residual()
-    gpu.set_precision("fp64")
+    try:
+        out = gpu.residual()
+    finally:
+        gpu.set_precision("fp64")
+        gpu.set_rule(3)
     err = {k: rel(out[k], ref[k]) for k in ("Rq", "Rs", "Rx")}
     err["S0_total"] = abs(out["S0_total"] - ref["S0_total"]) / np.abs(ref["S0"]).sum()
-    say(f"    {precision}: {sysd['n']} sites, max row {int(nnb.max())}, errors {err}")
+    say(f"    {precision}, rule {rule}: {sysd['n']} sites, max row {int(nnb.max())}, errors {err}")
     tol = {k: TOL for k in err} if precision == "fp64" else SPLIT_TOL
     for k, t in tol.items():
         assert err[k] < t, (k, err[k])
@@ -107,6 +112,46 @@ def test_random_systems(gpu, oracle, seed, precision):
     out, ref = check(gpu, oracle, sysd, precision)
     masked = sysd["x"] < 5e-4
     assert not np.any(out["Rq"][masked]) and not np.any(out["Rs"][masked])
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+@pytest.mark.parametrize("rule", [5, 44, 6])
+@pytest.mark.parametrize("precision", ["fp64", "split32"])
+def test_random_systems_other_rules(gpu, oracle, seed, rule, precision):
+    """The 27-point, 44-point and 6-point pair rules on random boxes with
+    masked sites, against the oracle under the same rule."""
+    say(f"[gpu] random system seed {seed}, rule {rule}, {precision}")
+    sysd = random_system(seed)
+    out, _ = check(gpu, oracle, sysd, precision, rule)
+    masked = sysd["x"] < 5e-4
+    assert not np.any(out["Rq"][masked]) and not np.any(out["Rs"][masked])
+
+
+@pytest.mark.parametrize("which", ["random1", "random4", "dense", "trio"])
+@pytest.mark.parametrize("exact", [False, True])
+def test_diag_blocks_on_edge_systems(gpu, oracle, which, exact):
+    """Diagonal Hessian blocks (FP64) of random boxes with masked and full H
+    sites, rows near the neighbour limit, and a dimer beside a masked H site,
+    against the oracle's factorised blocks within 1e-10; masked sites' blocks
+    are zero and every block is symmetric."""
+    say(f"[gpu] diag blocks on {which}, exact {exact}")
+    sysd = {"random1": lambda: random_system(1), "random4": lambda: random_system(4),
+            "dense": dense_rows,
+            "trio": lambda: small_system([[0.0, 0.0, 0.0], [3.1, 0.4, -0.2], [1.2, 1.9, 0.8]],
+                                         [lat.MG, lat.MG, lat.H], [1.0, 1.0, 0.0])}[which]()
+    st, nnb, nbr = oracle.neighbors(sysd, RC)
+    assert st == 0
+    gpu.set_state(sysd)
+    gpu.set_precision("fp64")
+    gpu.neighbors()
+    D = gpu.diag_blocks(exact=exact)
+    F = oracle.fact_diag_blocks(oracle.system(sysd, nnb, nbr), exact=exact)
+    err = rel(D, F)
+    say(f"    {sysd['n']} sites: max rel err {err:.3e}")
+    assert err < TOL
+    masked = sysd["x"] < 5e-4
+    assert not np.any(D[masked])
+    assert np.abs(D - D.transpose(0, 2, 1)).max() <= 1e-13 * np.abs(D).max()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split32"])
