@@ -1,7 +1,8 @@
 """GPU parity on the edges of the input domain: seeded random systems (box,
 periodicity, lattice constant, widths, temperatures, occupancies with masked
-sites), also under the 27-, 44- and 6-point pair rules and for the diagonal
-Hessian blocks, neighbour rows at the maximum length the reference allows,
+sites), also under the 27-, 44- and 6-point pair rules, for the diagonal
+Hessian blocks, the diffusion rates and the per-(u, i) hooks, neighbour rows
+at the maximum length the reference allows,
 and the smallest systems (a dimer; a dimer with a masked H site beside it).
 
 Bars as in test_gpu_parity: FP64 within 1e-10 relative of the oracle, split
@@ -152,6 +153,64 @@ def test_diag_blocks_on_edge_systems(gpu, oracle, which, exact):
     masked = sysd["x"] < 5e-4
     assert not np.any(D[masked])
     assert np.abs(D - D.transpose(0, 2, 1)).max() <= 1e-13 * np.abs(D).max()
+
+
+@pytest.mark.parametrize("seed", [2, 5])
+def test_diffusion_rates_on_random_systems(gpu, oracle, seed):
+    """Occupancy rates of the diffusion step on random boxes (masked, full and
+    partial H sites, random temperatures and chemical multipliers) against the
+    oracle within 1e-10; they conserve the hydrogen content."""
+    import dmdgpu
+    say(f"[gpu] diffusion rates on random system seed {seed}")
+    sd = random_system(seed)
+    gpu.set_state(sd)
+    gpu.set_precision("fp64")
+    gpu.neighbors()
+    D0, EB = 1e-7, 0.4
+    xd = gpu.diffusion_rates(0, dmdgpu.diffuse_opts(D0=D0, Eb=EB))
+    st, nnb, nbr = oracle.neighbors(sd, RC)
+    res = oracle.fact_residual(oracle.system(sd, nnb, nbr))
+    mu = oracle.diff_mu(sd, res["Rx"])
+    st, cnt, lst = oracle.diff_neighbors(sd)
+    assert st == 0
+    ref = oracle.diff_rates(sd, mu, cnt, lst, D0, EB)
+    err = np.abs(xd - ref).max() / np.abs(ref).max()
+    say(f"    {sd['n']} sites: max rel err {err:.2e}")
+    assert err < TOL
+    assert abs(xd.sum()) <= 1e-12 * np.abs(xd).sum()
+
+
+def test_hooks_on_a_random_system(gpu, oracle):
+    """The per-(u, i) hooks on a random box (masked and full H sites, random
+    temperatures) against the reference-order oracle, for u = i and every
+    neighbour of Mg and H centres, within 1e-10 of the largest entry."""
+    say("[gpu] per-(u, i) hooks on a random system")
+    sd = random_system(2)
+    st, nnb, nbr = oracle.neighbors(sd, RC)
+    s = oracle.system(sd, nnb, nbr)
+    rho = oracle.ref_mf_rho(s)
+    rng = np.random.default_rng(11)
+    mg = np.flatnonzero(sd["species"] == lat.MG)
+    h = np.flatnonzero((sd["species"] == lat.H) & (sd["x"] >= 5e-4))
+    full = np.flatnonzero((sd["species"] == lat.H) & (sd["x"] == 1.0))
+    centres = list(rng.choice(mg, 3, replace=False)) + list(rng.choice(h, 2, replace=False)) + \
+        [int(full[0])]
+    us, is_ = [], []
+    for i in centres:
+        for u in [i] + list(nbr[i, :nnb[i]]):
+            us.append(int(u))
+            is_.append(int(i))
+    us, is_ = np.array(us, dtype=np.int32), np.array(is_, dtype=np.int32)
+    gpu.set_state(sd)
+    gpu.set_precision("fp64")
+    gpu.neighbors()
+    dq = gpu.site_DV0_Dq(us, is_)
+    dx = gpu.site_DV0_Dxi(us, is_)
+    ref_q = np.array([oracle.ref_DV0_Dq(s, int(u), int(i), rho) for u, i in zip(us, is_)])
+    ref_x = np.array([oracle.ref_DV0_Dxi(s, int(u), int(i), rho) for u, i in zip(us, is_)])
+    err = {"q": rel(dq[:, :3], ref_q[:, :3]), "sigma": rel(dq[:, 3], ref_q[:, 3]), "x": rel(dx, ref_x)}
+    say(f"    {len(us)} queries: {err}")
+    assert max(err.values()) < TOL
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split32"])
