@@ -754,7 +754,7 @@ def main():
                          "calibration of the traffic profile) and print it")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+        print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:

@@ -807,7 +807,7 @@ void recompute_extent(dmdgpu_ctx* c) {
 }
 
 dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
-  // The lists are rebuilt in place: until this build succeeds they are not
+  // The lists are rebuilt in place: until the rebuild succeeds they are not
   // valid, and no evaluation may read them.
   I.have_neigh = false;
   I.pairs = I.pairs_mg = 0;

@@ -17,9 +17,10 @@ HCP_BASIS = np.array([[0.0, 0.0, 0.0],
                       [0.5, 5.0 / 6.0, 0.5],
                       [0.0, 1.0 / 3.0, 0.5]])
 
-# Width rule of this build: the reference stores sigma only in the dump.  The
-# harmonic estimate sigma^2 = k_B T / k with k ~ 1.8 eV/A^2 for Mg at 300 K
-# gives ~0.12 A; H sits in a softer, lighter well and starts wider.
+# Starting widths of the generated systems (the reference stores sigma only in
+# its dump files).  The harmonic estimate sigma^2 = k_B T / k with
+# k ~ 1.8 eV/A^2 for Mg at 300 K gives ~0.12 A; H sits in a softer, lighter
+# well and starts wider.
 SIGMA0 = {MG: 0.12, H: 0.15}
 
 

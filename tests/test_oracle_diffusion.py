@@ -3,7 +3,7 @@
 The reference declares the diffusive topology (src/Atoms/Neighbors.cpp:139-277,
 r_cutoff_Eb = 2.6 A and max_chemical_neighs = 50, src/Macros.hpp:860,863) and
 the diffusion parameters (src/Variables.hpp:66-90) but no rate law, so the
-rate law is a definition of this build and is pinned by closed forms only:
+rate law is a definition of this library and is pinned by closed forms only:
 
 * the diffusive lists equal a numpy enumeration of the H-H pairs within 2.6 A;
 * without interactions a dimer exchanges by Fick's law with the Arrhenius rate
