@@ -517,3 +517,89 @@ def test_radial_self_product_and_gradient():
         e = 1e-5
         g = np.array([(form(p + e * d) - form(p - e * d)) / (4 * e) for d in np.eye(3)])
         assert np.allclose(g, h * p, rtol=1e-7, atol=1e-9 * np.abs(h * p).max())
+
+
+def test_isotropic_hessian_terms_of_the_diagonal_blocks():
+    """The closed forms the diagonal-block kernel uses, for the same psi and
+    form as above with its 9 x 3 Jacobian J = dpsi/dp written out (rows: the
+    dipole, then the quadrupole entries xx yy zz xy xz yz):
+      J^T M J = gam p p^T + alp I, alp = u^2/2 + w^2 r^2,
+        gam = u u'/r + u'^2/2 + (w'^2 r^2 + 4 w w' r + w^2)/3;
+      sum_a (M psi)_a d2psi_a/dp2 = ang_a p p^T + ang_b I,
+        ang_a = u (u'' + u'/r)/2 + w (w + w' r + w'' r^2/3),
+        ang_b = u u' r/2 + w r^2 (w' r - w)/3;
+      J1^T M J0 + J0^T M J1 = 2 a I + 2 b1 p1 p1^T + 2 b0 p0 p0^T + d (p1 p0^T + p0 p1^T)
+        with c = p1.p0, U = u'/r, W = w'/r, t = W r^2 + 2 w,
+        a = u1 u0/2 + w1 w0 c, b1 = U1 u0/2 + W1 w0 c, b0 = u1 U0/2 + w1 W0 c,
+        d = U1 U0 c/2 + W1 W0 c^2/2 - t1 t0/6 + w1 w0;
+    and a weighted Jacobian sum sum_k n_k J(p_k) is fixed by sum n u,
+    sum n U p p^T, sum n W p p p and sum n w p.  Checked against the Jacobians
+    themselves and central differences of J^T M psi at fixed M psi."""
+    rng = np.random.default_rng(8)
+    cu, cw = rng.normal(size=4), rng.normal(size=4)
+    f = lambda c, k, r: np.polyval(np.polyder(c, k) if k else c, r)
+    qi, qj = [0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]
+
+    def psi(p):
+        r = np.linalg.norm(p)
+        return np.concatenate([f(cu, 0, r) * p, [f(cw, 0, r) * p[i] * p[j] for i, j in zip(qi, qj)]])
+
+    def jac(p):
+        r = np.linalg.norm(p)
+        u, U, w, W = f(cu, 0, r), f(cu, 1, r) / r, f(cw, 0, r), f(cw, 1, r) / r
+        J = np.zeros((9, 3))
+        J[:3] = u * np.eye(3) + U * np.outer(p, p)
+        for m, (i, j) in enumerate(zip(qi, qj)):
+            for g in range(3):
+                J[3 + m, g] = W * p[g] * p[i] * p[j] + w * ((i == g) * p[j] + (j == g) * p[i])
+        return J
+
+    def M(X):  # the form's matrix applied to the rows of X (9 x k or 9)
+        Y = np.array(X, dtype=float)
+        Y[:3] *= 0.5
+        Y[3:6] = 0.5 * X[3:6] - (X[3] + X[4] + X[5]) / 6.0
+        return Y
+
+    I3 = np.eye(3)
+    for _ in range(10):
+        p, p0 = rng.normal(size=3) * 2.0, rng.normal(size=3) * 2.0
+        r, r0 = np.linalg.norm(p), np.linalg.norm(p0)
+        u, du, d2u = (f(cu, k, r) for k in range(3))
+        w, dw, d2w = (f(cw, k, r) for k in range(3))
+        J = jac(p)
+        alp = 0.5 * u * u + w * w * r * r
+        gam = u * du / r + 0.5 * du * du + (dw * dw * r * r + 4 * w * dw * r + w * w) / 3.0
+        ref = J.T @ M(J)
+        assert np.allclose(gam * np.outer(p, p) + alp * I3, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        L = M(psi(p))
+        e = 1e-5
+        H = np.array([(jac(p + e * d).T @ L - jac(p - e * d).T @ L) / (2 * e) for d in I3])
+        anga = 0.5 * u * (d2u + du / r) + w * (w + dw * r + d2w * r * r / 3.0)
+        angb = 0.5 * u * du * r + w * r * r * (dw * r - w) / 3.0
+        assert np.allclose(anga * np.outer(p, p) + angb * I3, H, rtol=0, atol=1e-7 * np.abs(H).max())
+        u0, U0, w0, W0 = f(cu, 0, r0), f(cu, 1, r0) / r0, f(cw, 0, r0), f(cw, 1, r0) / r0
+        U1, W1 = du / r, dw / r
+        c = p @ p0
+        t1, t0 = W1 * r * r + 2 * w, W0 * r0 * r0 + 2 * w0
+        a = 0.5 * u * u0 + w * w0 * c
+        b1, b0 = 0.5 * U1 * u0 + W1 * w0 * c, 0.5 * u * U0 + w * W0 * c
+        dd = 0.5 * U1 * U0 * c + 0.5 * W1 * W0 * c * c - t1 * t0 / 6.0 + w * w0
+        J0 = jac(p0)
+        ref = J.T @ M(J0) + J0.T @ M(J)
+        got = 2 * a * I3 + 2 * b1 * np.outer(p, p) + 2 * b0 * np.outer(p0, p0) + \
+            dd * (np.outer(p, p0) + np.outer(p0, p))
+        assert np.allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    # a weighted sum of Jacobians from its 20 values
+    ps, ns = rng.normal(size=(7, 3)) * 2.0, rng.random(7)
+    S = sum(n * jac(p) for n, p in zip(ns, ps))
+    z_u = sum(n * f(cu, 0, np.linalg.norm(p)) for n, p in zip(ns, ps))
+    z_U = sum(n * f(cu, 1, np.linalg.norm(p)) / np.linalg.norm(p) * np.outer(p, p) for n, p in zip(ns, ps))
+    z_W = sum(n * f(cw, 1, np.linalg.norm(p)) / np.linalg.norm(p) * np.einsum("i,j,k->ijk", p, p, p)
+              for n, p in zip(ns, ps))
+    z_w = sum(n * f(cw, 0, np.linalg.norm(p)) * p for n, p in zip(ns, ps))
+    T = np.zeros((9, 3))
+    T[:3] = z_u * I3 + z_U
+    for m, (i, j) in enumerate(zip(qi, qj)):
+        for g in range(3):
+            T[3 + m, g] = z_W[i, j, g] + (i == g) * z_w[j] + (j == g) * z_w[i]
+    assert np.allclose(T, S, rtol=0, atol=1e-12 * np.abs(S).max())
