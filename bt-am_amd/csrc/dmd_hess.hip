@@ -334,13 +334,13 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
     H[3] += ck * gk[0] * gk[1]; H[4] += ck * gk[0] * gk[2]; H[5] += ck * gk[1] * gk[2];
   }
 
-  // ---- Jacobian sums of centre u, one block at a time ----
-  // blk 0..5: JG_ls (with their j1 = j2 parts); blk 6: JU; blk 7: JG0, closing
-  // JU^T M JG0 + JG0^T M JU; blk 8: the j1 = j2 part of that product.
-  __shared__ double sJU[20];
+  // ---- Jacobian sums of centre u ----
+  // JG_ls one block (one walk) at a time, with their j1 = j2 parts; then JU and
+  // JG0 in one walk, closing JU^T M JG0 + JG0^T M JU, with the j1 = j2 part of
+  // that product from the same points.
   double Hw[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform
 #pragma unroll 1
-  for (int blk = 0; blk < 9; blk++) {
+  for (int blk = 0; blk < 6; blk++) {
     JSum Z;
     Z.zero();
 #pragma unroll 1
@@ -355,83 +355,81 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
       min_image(box, qu, qk, rd);
       PairGeo<double> geo;
       geo.init(rd);
-      if (blk < 6) {
-        geo.point(T, blk >> 1, ((blk & 1) ? -3.0 : 3.0) * su, p, m, off, ir);
-        uw_at(T, t, m, off, uv);
-        Z.add(nk, p, ir, uv);
-        // the j1 = j2 part: J^T M J of one point, isotropic
-        const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
-        const double cj = -nk * nk / 9.0;
-        iso_add(p, cj * st.gam, cj * st.alp, HJ);
-      } else if (blk == 6) {
-#pragma unroll 1
-        for (int ls = 0; ls < 6; ls++) {
-          geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
-          uw_at(T, t, m, off, uv);
-          Z.add(nk, p, ir, uv);
-        }
-      } else if (blk == 7) {
-        geo.point(T, 0, 0.0, p, m, off, ir);
-        uw_at(T, t, m, off, uv);
-        Z.add(nk, p, ir, uv);
-      } else {
-        // -n_k^2/9 (J1^T M J0 + J0^T M J1) summed over this pair's six j-shift
-        // points p1 against its unshifted point p0, in closed form: with
-        // c = p1.p0, U = u'/r, W = w'/r, t = W r^2 + 2 w (index 1 at p1, 0 at p0),
-        //   J1^T M J0 + J0^T M J1 = 2 a I + 2 b1 p1 p1^T + 2 b0 p0 p0^T + d (p1 p0^T + p0 p1^T),
-        //   a = u1 u0/2 + w1 w0 c,  b1 = U1 u0/2 + W1 w0 c,  b0 = u1 U0/2 + w1 W0 c,
-        //   d = U1 U0 c/2 + W1 W0 c^2/2 - t1 t0/6 + w1 w0.
-        double p0[3], ir0, off0, uv0[6];
-        geo.point(T, 0, 0.0, p0, m, off0, ir0);
-        uw_at(T, t, m, off0, uv0);
-        const double u0 = uv0[0], U0 = uv0[1] * ir0, w0 = uv0[3], W0 = uv0[4] * ir0;
-        const double t0 = W0 * (p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2]) + 2.0 * w0;
-        const double k2 = -2.0 * nk * nk / 9.0;
-        double b0s = 0.0, q[3] = {0, 0, 0};
-#pragma unroll 1
-        for (int ls = 0; ls < 6; ls++) {
-          geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
-          uw_at(T, t, m, off, uv);
-          const double u1 = uv[0], U1 = uv[1] * ir, w1 = uv[3], W1 = uv[4] * ir;
-          const double c = p[0] * p0[0] + p[1] * p0[1] + p[2] * p0[2];
-          const double t1 = W1 * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2.0 * w1;
-          const double a = 0.5 * u1 * u0 + w1 * w0 * c;
-          const double b1 = 0.5 * U1 * u0 + W1 * w0 * c;
-          b0s += 0.5 * u1 * U0 + w1 * W0 * c;
-          const double d = (0.5 * U1 * U0 + 0.5 * W1 * W0 * c) * c - (1.0 / 6.0) * t1 * t0 + w1 * w0;
-          iso_add(p, k2 * b1, k2 * a, HJ);
-          q[0] += d * p[0]; q[1] += d * p[1]; q[2] += d * p[2];
-        }
-        const double kh = 0.5 * k2;
-        iso_add(p0, k2 * b0s, 0.0, HJ);
-        HJ[0] += k2 * q[0] * p0[0]; HJ[1] += k2 * q[1] * p0[1]; HJ[2] += k2 * q[2] * p0[2];
-        HJ[3] += kh * (q[0] * p0[1] + p0[0] * q[1]);
-        HJ[4] += kh * (q[0] * p0[2] + p0[0] * q[2]);
-        HJ[5] += kh * (q[1] * p0[2] + p0[1] * q[2]);
-      }
+      geo.point(T, blk >> 1, ((blk & 1) ? -3.0 : 3.0) * su, p, m, off, ir);
+      uw_at(T, t, m, off, uv);
+      Z.add(nk, p, ir, uv);
+      // the j1 = j2 part: J^T M J of one point, isotropic
+      const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
+      const double cj = -nk * nk / 9.0;
+      iso_add(p, cj * st.gam, cj * st.alp, HJ);
     }
-    if (blk == 8) break;
 #pragma unroll
     for (int c = 0; c < 20; c++) Z.a[c] = group_sum<64>(Z.a[c]);
-    if (blk == 6) {
-      if (lane == 0)  // Z is uniform after the reduction
-#pragma unroll
-        for (int c = 0; c < 20; c++) sJU[c] = Z.a[c];
-      __syncthreads();
-    } else {
-      double S[27];
-      Z.expand(S);
-      if (blk < 6) {
-        jtmj_self(S, 1.0 / 9.0, Hw);
-      } else {
-        JSum ZU;
-#pragma unroll
-        for (int c = 0; c < 20; c++) ZU.a[c] = sJU[c];
-        double JU[27];
-        ZU.expand(JU);
-        jtmj_sym(JU, S, 1.0 / 9.0, Hw);
+    double S[27];
+    Z.expand(S);
+    jtmj_self(S, 1.0 / 9.0, Hw);
+  }
+  {
+    JSum ZU, Z0;
+    ZU.zero();
+    Z0.zero();
+#pragma unroll 1
+    for (int kk = lane; kk < nn; kk += 64) {
+      const int k = row[kk];
+      const double nk = s.ne[k];
+      if (!(nk > 0.0)) continue;
+      const double4 qk = s.qs[k];
+      const int t = pair_slot(spu, s.spc[k]);
+      double rd[3], p[3], ir, off, uv[6];
+      int m;
+      min_image(box, qu, qk, rd);
+      PairGeo<double> geo;
+      geo.init(rd);
+      // -n_k^2/9 (J1^T M J0 + J0^T M J1) summed over this pair's six j-shift
+      // points p1 against its unshifted point p0, in closed form: with
+      // c = p1.p0, U = u'/r, W = w'/r, t = W r^2 + 2 w (index 1 at p1, 0 at p0),
+      //   J1^T M J0 + J0^T M J1 = 2 a I + 2 b1 p1 p1^T + 2 b0 p0 p0^T + d (p1 p0^T + p0 p1^T),
+      //   a = u1 u0/2 + w1 w0 c,  b1 = U1 u0/2 + W1 w0 c,  b0 = u1 U0/2 + w1 W0 c,
+      //   d = U1 U0 c/2 + W1 W0 c^2/2 - t1 t0/6 + w1 w0.
+      double p0[3], ir0, off0, uv0[6];
+      geo.point(T, 0, 0.0, p0, m, off0, ir0);
+      uw_at(T, t, m, off0, uv0);
+      Z0.add(nk, p0, ir0, uv0);
+      const double u0 = uv0[0], U0 = uv0[1] * ir0, w0 = uv0[3], W0 = uv0[4] * ir0;
+      const double t0 = W0 * (p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2]) + 2.0 * w0;
+      const double k2 = -2.0 * nk * nk / 9.0;
+      double b0s = 0.0, q[3] = {0, 0, 0};
+#pragma unroll 1
+      for (int ls = 0; ls < 6; ls++) {
+        geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * qk.w, p, m, off, ir);
+        uw_at(T, t, m, off, uv);
+        ZU.add(nk, p, ir, uv);
+        const double u1 = uv[0], U1 = uv[1] * ir, w1 = uv[3], W1 = uv[4] * ir;
+        const double c = p[0] * p0[0] + p[1] * p0[1] + p[2] * p0[2];
+        const double t1 = W1 * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2.0 * w1;
+        const double a = 0.5 * u1 * u0 + w1 * w0 * c;
+        const double b1 = 0.5 * U1 * u0 + W1 * w0 * c;
+        b0s += 0.5 * u1 * U0 + w1 * W0 * c;
+        const double d = (0.5 * U1 * U0 + 0.5 * W1 * W0 * c) * c - (1.0 / 6.0) * t1 * t0 + w1 * w0;
+        iso_add(p, k2 * b1, k2 * a, HJ);
+        q[0] += d * p[0]; q[1] += d * p[1]; q[2] += d * p[2];
       }
+      const double kh = 0.5 * k2;
+      iso_add(p0, k2 * b0s, 0.0, HJ);
+      HJ[0] += k2 * q[0] * p0[0]; HJ[1] += k2 * q[1] * p0[1]; HJ[2] += k2 * q[2] * p0[2];
+      HJ[3] += kh * (q[0] * p0[1] + p0[0] * q[1]);
+      HJ[4] += kh * (q[0] * p0[2] + p0[0] * q[2]);
+      HJ[5] += kh * (q[1] * p0[2] + p0[1] * q[2]);
     }
+#pragma unroll
+    for (int c = 0; c < 20; c++) {
+      ZU.a[c] = group_sum<64>(ZU.a[c]);
+      Z0.a[c] = group_sum<64>(Z0.a[c]);
+    }
+    double JU[27], J0[27];
+    ZU.expand(JU);
+    Z0.expand(J0);
+    jtmj_sym(JU, J0, 1.0 / 9.0, Hw);
   }
 
   // ---- close ----
