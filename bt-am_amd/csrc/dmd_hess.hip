@@ -236,9 +236,11 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
     double p[3], ir, off, uv[6], ps[9], psi0[9], L[9], v[9];
     double Tuk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, SA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int m;
-    geo.point(T, 0, 0.0, p, m, off, ir);
-    uw_at(T, t, m, off, uv);
-    psi_from(uv[0], uv[3], p, psi0);
+    // the unshifted point, kept for its own contraction at the end
+    double p0[3], ir0, uv0[6];
+    geo.point(T, 0, 0.0, p0, m, off, ir0);
+    uw_at(T, t, m, off, uv0);
+    psi_from(uv0[0], uv0[3], p0, psi0);
     // j-shift points of centre u (shift sigma_k) = i-shift points (l, -s) of k
 #pragma unroll 1
     for (int ls = 0; ls < 6; ls++) {
@@ -273,9 +275,7 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
 #pragma unroll
     for (int c = 0; c < 9; c++) v[c] = ca * Tuk[c] + cb * SA[c];
     adjoint(Ru, kBlkLU, wa, Rk, kBlkLU, wb, v, L);
-    geo.point(T, 0, 0.0, p, m, off, ir);
-    uw_at(T, t, m, off, uv);
-    hess_psi(L, p, ir, uv, 1.0, H);
+    hess_psi(L, p0, ir0, uv0, 1.0, H);
   }
 
   // ---- pass B: pair-rule points: coincident angular term, pair and density ----
@@ -335,14 +335,15 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
   }
 
   // ---- Jacobian sums of centre u ----
-  // JG_ls one block (one walk) at a time, with their j1 = j2 parts; then JU and
-  // JG0 in one walk, closing JU^T M JG0 + JG0^T M JU, with the j1 = j2 part of
-  // that product from the same points.
+  // JG_ls two blocks (the two shifts along one axis, one walk) at a time, with
+  // their j1 = j2 parts; then JU and JG0 in one walk, closing JU^T M JG0 +
+  // JG0^T M JU, with the j1 = j2 part of that product from the same points.
   double Hw[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform
 #pragma unroll 1
-  for (int blk = 0; blk < 6; blk++) {
-    JSum Z;
-    Z.zero();
+  for (int l = 0; l < 3; l++) {
+    JSum Z[2];
+    Z[0].zero();
+    Z[1].zero();
 #pragma unroll 1
     for (int kk = lane; kk < nn; kk += 64) {
       const int k = row[kk];
@@ -350,24 +351,31 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
       if (!(nk > 0.0)) continue;
       const double4 qk = s.qs[k];
       const int t = pair_slot(spu, s.spc[k]);
-      double rd[3], p[3], ir, off, uv[6];
-      int m;
+      double rd[3];
       min_image(box, qu, qk, rd);
       PairGeo<double> geo;
       geo.init(rd);
-      geo.point(T, blk >> 1, ((blk & 1) ? -3.0 : 3.0) * su, p, m, off, ir);
-      uw_at(T, t, m, off, uv);
-      Z.add(nk, p, ir, uv);
-      // the j1 = j2 part: J^T M J of one point, isotropic
-      const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
       const double cj = -nk * nk / 9.0;
-      iso_add(p, cj * st.gam, cj * st.alp, HJ);
+#pragma unroll
+      for (int sg = 0; sg < 2; sg++) {
+        double p[3], ir, off, uv[6];
+        int m;
+        geo.point(T, l, (sg ? -3.0 : 3.0) * su, p, m, off, ir);
+        uw_at(T, t, m, off, uv);
+        Z[sg].add(nk, p, ir, uv);
+        // the j1 = j2 part: J^T M J of one point, isotropic
+        const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
+        iso_add(p, cj * st.gam, cj * st.alp, HJ);
+      }
     }
 #pragma unroll
-    for (int c = 0; c < 20; c++) Z.a[c] = group_sum<64>(Z.a[c]);
-    double S[27];
-    Z.expand(S);
-    jtmj_self(S, 1.0 / 9.0, Hw);
+    for (int sg = 0; sg < 2; sg++) {
+#pragma unroll
+      for (int c = 0; c < 20; c++) Z[sg].a[c] = group_sum<64>(Z[sg].a[c]);
+      double S[27];
+      Z[sg].expand(S);
+      jtmj_self(S, 1.0 / 9.0, Hw);
+    }
   }
   {
     JSum ZU, Z0;
