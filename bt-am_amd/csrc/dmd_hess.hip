@@ -26,11 +26,11 @@
 // One wavefront per site u; the lanes take the neighbours k = lane, lane + 64,
 // ... and every sum over k closes with a butterfly reduction (the same value
 // on every lane, independent of the order the lanes ran).  The work is split
-// into passes over the neighbours (adjoint points, pair-rule points, then the
-// Jacobian sums one block at a time, each held in 20 values) so that little
-// state is live at once.  (In one pass the
-// kernel held one wavefront per SIMD and took 7.3 ms on 54,000 sites; split,
-// 5.3 ms.)
+// into walks over the neighbours so that little state is live at once: the
+// adjoint points; the pair-rule points; the i-shift Jacobian sums, the two
+// shifts along one axis per walk; JU and JG0 together.  Each Jacobian sum is
+// held in 20 values.  Two wavefronts per SIMD, no spills.  (In one pass the
+// kernel held one wavefront per SIMD and took 7.3 ms on 54,000 sites.)
 #include "dmd_device.h"
 
 namespace dmd {
