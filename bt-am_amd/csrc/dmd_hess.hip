@@ -27,8 +27,8 @@
 // ... and every sum over k closes with a butterfly reduction (the same value
 // on every lane, independent of the order the lanes ran).  The work is split
 // into walks over the neighbours so that little state is live at once: the
-// adjoint points; the pair-rule points; the i-shift Jacobian sums, the two
-// shifts along one axis per walk; JU and JG0 together.  Each Jacobian sum is
+// adjoint points; the pair-rule points; the i-shift Jacobian sums, three
+// shifts per walk; JU and JG0 together.  Each Jacobian sum is
 // held in 20 values.  Two wavefronts per SIMD, no spills.  (In one pass the
 // kernel held one wavefront per SIMD and took 7.3 ms on 54,000 sites.)
 #include "dmd_device.h"
@@ -335,15 +335,16 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
   }
 
   // ---- Jacobian sums of centre u ----
-  // JG_ls two blocks (the two shifts along one axis, one walk) at a time, with
+  // JG_ls three blocks (three of the six shifts, one walk) at a time, with
   // their j1 = j2 parts; then JU and JG0 in one walk, closing JU^T M JG0 +
   // JG0^T M JU, with the j1 = j2 part of that product from the same points.
   double Hw[6] = {0, 0, 0, 0, 0, 0};  // wave-uniform
 #pragma unroll 1
-  for (int l = 0; l < 3; l++) {
-    JSum Z[2];
+  for (int w3 = 0; w3 < 2; w3++) {
+    JSum Z[3];
     Z[0].zero();
     Z[1].zero();
+    Z[2].zero();
 #pragma unroll 1
     for (int kk = lane; kk < nn; kk += 64) {
       const int k = row[kk];
@@ -357,23 +358,24 @@ __global__ __launch_bounds__(64, 2) void k_diag(State s, Tables T, const double*
       geo.init(rd);
       const double cj = -nk * nk / 9.0;
 #pragma unroll
-      for (int sg = 0; sg < 2; sg++) {
+      for (int b = 0; b < 3; b++) {
+        const int ls = 3 * w3 + b;
         double p[3], ir, off, uv[6];
         int m;
-        geo.point(T, l, (sg ? -3.0 : 3.0) * su, p, m, off, ir);
+        geo.point(T, ls >> 1, ((ls & 1) ? -3.0 : 3.0) * su, p, m, off, ir);
         uw_at(T, t, m, off, uv);
-        Z[sg].add(nk, p, ir, uv);
+        Z[b].add(nk, p, ir, uv);
         // the j1 = j2 part: J^T M J of one point, isotropic
         const SelfTerms st = self_terms(uv, (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) * ir, ir);
         iso_add(p, cj * st.gam, cj * st.alp, HJ);
       }
     }
 #pragma unroll
-    for (int sg = 0; sg < 2; sg++) {
+    for (int b = 0; b < 3; b++) {
 #pragma unroll
-      for (int c = 0; c < 20; c++) Z[sg].a[c] = group_sum<64>(Z[sg].a[c]);
+      for (int c = 0; c < 20; c++) Z[b].a[c] = group_sum<64>(Z[b].a[c]);
       double S[27];
-      Z[sg].expand(S);
+      Z[b].expand(S);
       jtmj_self(S, 1.0 / 9.0, Hw);
     }
   }
