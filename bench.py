@@ -48,7 +48,7 @@ FLOP_PER_PAIR_TICK_SPLIT = {"phase1_site_sums": (840, 358), "phase2_gradient": (
 # k_diag (the diagonal Hessian blocks, 3rd-order rule, reference assembly):
 # SQ_INSTS_VALU_FLOPS_FP64 x 64 per launch on the same workload, per active
 # ordered pair
-FLOP_PER_PAIR_DIAG = 12811
+FLOP_PER_PAIR_DIAG = 12773
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
 HBM_PEAK = 8.0e12          # B/s
