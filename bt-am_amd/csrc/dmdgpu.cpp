@@ -252,6 +252,18 @@ struct dmdgpu_ctx {
 
 namespace {
 
+// Copies between host and device (or within the device) outside the kernel
+// chain go through the context's stream and wait for it: the kernels run on a
+// non-blocking stream, which a null-stream copy does not wait for, so a copy
+// issued while an evaluation is still in flight (dmdgpu_residual_resident
+// returns without waiting) would race with it.
+hipError_t copy_sync(dmdgpu_ctx* c, void* dst, const void* src, size_t bytes,
+                     hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+
 // ---- instrumentation --------------------------------------------------------
 dmdgpu_status begin_timed(dmdgpu_ctx* c, hipEvent_t* a) {
   if (!c->timing) return DMDGPU_OK;
@@ -311,8 +323,9 @@ void recompute_extent(dmdgpu_ctx* c);
 dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   if (!c->tables_dirty) return DMDGPU_OK;
   // Captured tick graphs hold the table pointers and grid in their kernel
-  // arguments; new tables may move or resize them.
+  // arguments; new tables may move or resize them (after the work in flight).
   drop_graph(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (!(c->have[0] && c->have[1] && c->have[2]))
     return fail(DMDGPU_ERR_STATE, "all three ADP table files (MgMg, HH, MgH) must be loaded");
   const int nr = c->rho[0].n;
@@ -353,23 +366,23 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   HIPCHK(c->d_ph.ensure(ph.size()));
   HIPCHK(c->d_rh.ensure(rh.size()));
   HIPCHK(c->d_em.ensure(em.size()));
-  HIPCHK(hipMemcpy(c->d_uw.p, uw.data(), uw.size() * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_ph.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_rh.p, rh.data(), rh.size() * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_em.p, em.data(), em.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_uw.p, uw.data(), uw.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_ph.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_rh.p, rh.data(), rh.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_em.p, em.data(), em.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c->d_pr.ensure(pr.size()));
-  HIPCHK(hipMemcpy(c->d_pr.p, pr.data(), pr.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_pr.p, pr.data(), pr.size() * 8, hipMemcpyHostToDevice));
   // FP32 copies for the split-precision evaluation
   std::vector<float> uw32(uw.begin(), uw.end()), ph32(ph.begin(), ph.end()),
       rh32(rh.begin(), rh.end()), pr32(pr.begin(), pr.end());
   HIPCHK(c->d_pr32.ensure(pr32.size()));
-  HIPCHK(hipMemcpy(c->d_pr32.p, pr32.data(), pr32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_pr32.p, pr32.data(), pr32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(c->d_uw32.ensure(uw32.size()));
   HIPCHK(c->d_ph32.ensure(ph32.size()));
   HIPCHK(c->d_rh32.ensure(rh32.size()));
-  HIPCHK(hipMemcpy(c->d_uw32.p, uw32.data(), uw32.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_ph32.p, ph32.data(), ph32.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_rh32.p, rh32.data(), rh32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_uw32.p, uw32.data(), uw32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_ph32.p, ph32.data(), ph32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_rh32.p, rh32.data(), rh32.size() * 4, hipMemcpyHostToDevice));
   dmd::Tables& T = c->T;
   T.uw = c->d_uw.p;
   T.ph = c->d_ph.p;
@@ -579,6 +592,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
     }
   }
   drop_graph(c);
+  HIPCHK(hipStreamSynchronize(c->stream));  // the arrays move: after the work in flight
   hipError_t err = hipSuccess;
   const int fail_at = test_fail_growth();
   int idx = 0;
@@ -624,6 +638,7 @@ dmdgpu_status ensure_sites(dmdgpu_ctx* c, size_t sites) {
 dmdgpu_status upload_meta(dmdgpu_ctx* c) {
   CHK(upload_tables(c));
   if (!c->meta_dirty) return DMDGPU_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));  // the metadata arrays may move
   const int ni = (int)c->inst.size();
   const size_t cap = std::max(ni, 1);
   if (c->d_boxes.n < cap) drop_graph(c);
@@ -646,9 +661,9 @@ dmdgpu_status upload_meta(dmdgpu_ctx* c) {
     lo[k] = I.base / kChunk;
     hi[k] = (I.base + I.cap) / kChunk;
   }
-  HIPCHK(hipMemcpy(c->d_boxes.p, bx.data(), sizeof(dmd::Box) * cap, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_chunk_lo.p, lo.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_chunk_hi.p, hi.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_boxes.p, bx.data(), sizeof(dmd::Box) * cap, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_chunk_lo.p, lo.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_chunk_hi.p, hi.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
   c->meta_dirty = false;
   return DMDGPU_OK;
 }
@@ -724,8 +739,17 @@ void make_order(const dmdgpu_ctx* c, Instance& I) {
     key[i] = ((long long)cc[2] * g.nc[1] + cc[1]) * g.nc[0] + cc[0];
   }
   I.perm.resize(n);
-  for (int i = 0; i < n; i++) I.perm[i] = i;
-  std::stable_sort(I.perm.begin(), I.perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  const size_t ncell = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
+  if (ncell <= 4 * (size_t)n + 4096) {
+    // counting sort by cell: stable, so the order of sorting by (cell, index)
+    std::vector<int> start(ncell + 1, 0);
+    for (int i = 0; i < n; i++) start[key[i] + 1]++;
+    for (size_t k = 0; k < ncell; k++) start[k + 1] += start[k];
+    for (int i = 0; i < n; i++) I.perm[start[key[i]]++] = i;
+  } else {  // sparse grid (sites spread over a large free box)
+    for (int i = 0; i < n; i++) I.perm[i] = i;
+    std::stable_sort(I.perm.begin(), I.perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
   I.inv.resize(n);
   for (int i = 0; i < n; i++) I.inv[I.perm[i]] = i;
 }
@@ -746,7 +770,7 @@ dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const
   if (q || sg) {
     std::vector<double4> qs(n);
     if (!first)
-      HIPCHK(hipMemcpy(qs.data(), c->d_qs.p + I.base, sizeof(double4) * n, hipMemcpyDeviceToHost));
+      HIPCHK(copy_sync(c, qs.data(), c->d_qs.p + I.base, sizeof(double4) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) {
       const int o = I.perm[i];
       if (q) {
@@ -760,8 +784,9 @@ dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const
         qs[i].w = sg[o];
       }
     }
-    HIPCHK(hipMemcpy(c->d_qs.p + I.base, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_X.p + I.base, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(c, c->d_qs.p + I.base, qs.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(c, c->d_X.p + I.base, c->d_qs.p + I.base, sizeof(double4) * n,
+                     hipMemcpyDeviceToDevice));
   }
   if (x) {
     std::vector<double> ne(n), xr(n);
@@ -778,8 +803,8 @@ dmdgpu_status upload_unknowns(dmdgpu_ctx* c, Instance& I, const double* q, const
     I.nactive = na;
     I.h_act.resize(n);
     for (int i = 0; i < n; i++) I.h_act[i] = ne[i] != 0.0;
-    HIPCHK(hipMemcpy(c->d_ne.p + I.base, ne.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_xr.p + I.base, xr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(c, c->d_ne.p + I.base, ne.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(c, c->d_xr.p + I.base, xr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   }
   return DMDGPU_OK;
 }
@@ -813,7 +838,7 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
   I.pairs = I.pairs_mg = 0;
   if (I.h_q_stale) {  // positions were last set on the device
     std::vector<double4> qs(I.n);
-    HIPCHK(hipMemcpy(qs.data(), c->d_qs.p + I.base, sizeof(double4) * I.n, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(c, qs.data(), c->d_qs.p + I.base, sizeof(double4) * I.n, hipMemcpyDeviceToHost));
     for (int i = 0; i < I.n; i++) {
       const int o = I.perm[i];
       I.h_q[3 * o] = qs[i].x;
@@ -831,6 +856,7 @@ dmdgpu_status build_neighbors(dmdgpu_ctx* c, Instance& I) {
     if (device_bytes(c) + grow > c->budget)
       return fail(DMDGPU_ERR_BUDGET, "the cell grid does not fit the device-memory budget");
   }
+  HIPCHK(hipStreamSynchronize(c->stream));  // the cell arrays may move
   HIPCHK(c->d_cell_count.ensure(ncell));
   HIPCHK(c->d_cell_start.ensure(ncell + 1));
   HIPCHK(c->d_cell_cursor.ensure(ncell));
@@ -870,7 +896,7 @@ dmdgpu_status remove_instance(dmdgpu_ctx* c, int id) {
     return fail(DMDGPU_ERR_ARG, "no such instance");
   Instance& I = c->inst[id];
   for (int i = 0; i < I.cap; i++) c->h_inst[I.base + i] = -1;
-  HIPCHK(hipMemcpy(c->d_inst.p + I.base, c->h_inst.data() + I.base, sizeof(int) * I.cap,
+  HIPCHK(copy_sync(c, c->d_inst.p + I.base, c->h_inst.data() + I.base, sizeof(int) * I.cap,
                    hipMemcpyHostToDevice));
   I = Instance();
   c->h_prec[id] = -1;
@@ -925,8 +951,8 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   I.h_q.assign(q, q + 3 * (size_t)n);
   make_order(c, I);
   I.h_slot.resize(n);
-  std::vector<double> bp(I.cap, 0.0), gp(I.cap, 0.0), z(I.cap, 0.0);
-  std::vector<int> sl(I.cap, 0), zi(I.cap, 0), pm(I.cap, -1);
+  std::vector<double> bp(I.cap, 0.0), gp(I.cap, 0.0);
+  std::vector<int> sl(I.cap, 0), pm(I.cap, -1);
   for (int i = 0; i < n; i++) {
     const int o = I.perm[i];
     I.h_slot[i] = spc[o] == 1 ? 1 : 0;
@@ -946,29 +972,35 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
     c->inst.emplace_back();
   }
   const int b = I.base, cp = I.cap;
-  HIPCHK(hipMemcpy(c->d_spc.p + b, sl.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_perm.p + b, pm.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_beta.p + b, bp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_gamma.p + b, gp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
-  // padding sites of the range: inactive (x = 0), no neighbours, unit width
-  HIPCHK(hipMemcpy(c->d_ne.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_xr.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_nnb.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_chunk_np.p + b / kChunk, zi.data(), sizeof(int) * (cp / kChunk),
-                   hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_chunk_npx.p + b / kChunk, zi.data(), sizeof(int) * (cp / kChunk),
-                   hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_dnn.p + b, zi.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_Xn.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_Xt.p + b, z.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_spc.p + b, sl.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_perm.p + b, pm.data(), sizeof(int) * cp, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_beta.p + b, bp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_gamma.p + b, gp.data(), sizeof(double) * cp, hipMemcpyHostToDevice));
+  // Zero on the device (no host copies): occupancies (the padding sites of the
+  // range stay inactive, x = 0), neighbour counts and the solver vectors; done
+  // before the sites' own values are written below.
   {
-    std::vector<double4> q0(cp, make_double4(0.0, 0.0, 0.0, 1.0));
-    std::vector<double4> z4(cp, make_double4(0.0, 0.0, 0.0, 0.0));
-    HIPCHK(hipMemcpy(c->d_qs.p + b, q0.data(), sizeof(double4) * cp, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_X.p + b, q0.data(), sizeof(double4) * cp, hipMemcpyHostToDevice));
+    auto zero = [&](auto* p, size_t cnt) {
+      return hipMemsetAsync(p, 0, sizeof(*p) * cnt, c->stream);
+    };
+    HIPCHK(zero(c->d_ne.p + b, cp));
+    HIPCHK(zero(c->d_xr.p + b, cp));
+    HIPCHK(zero(c->d_nnb.p + b, cp));
+    HIPCHK(zero(c->d_chunk_np.p + b / kChunk, cp / kChunk));
+    HIPCHK(zero(c->d_chunk_npx.p + b / kChunk, cp / kChunk));
+    HIPCHK(zero(c->d_dnn.p + b, cp));
+    HIPCHK(zero(c->d_Xn.p + b, cp));
+    HIPCHK(zero(c->d_Xt.p + b, cp));
     double4* vz[] = {c->d_R0.p, c->d_R0f.p, c->d_D.p, c->d_Rr.p, c->d_Pp.p};
-    for (double4* v : vz)
-      HIPCHK(hipMemcpy(v + b, z4.data(), sizeof(double4) * cp, hipMemcpyHostToDevice));
+    for (double4* v : vz) HIPCHK(zero(v + b, cp));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (cp > n) {  // padding sites: unit width at the origin (the sites' own are uploaded below)
+    std::vector<double4> q0(cp - n, make_double4(0.0, 0.0, 0.0, 1.0));
+    HIPCHK(copy_sync(c, c->d_qs.p + b + n, q0.data(), sizeof(double4) * (cp - n),
+                     hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(c, c->d_X.p + b + n, q0.data(), sizeof(double4) * (cp - n),
+                     hipMemcpyHostToDevice));
   }
   c->inst[id] = std::move(I);
   Instance& J = c->inst[id];
@@ -979,7 +1011,7 @@ dmdgpu_status add_instance(dmdgpu_ctx* c, int n, const double* q, const double* 
   }
   if ((int)c->h_inst.size() < J.base + J.cap) c->h_inst.resize(J.base + J.cap, -1);
   for (int i = 0; i < J.cap; i++) c->h_inst[J.base + i] = id;
-  HIPCHK(hipMemcpy(c->d_inst.p + J.base, c->h_inst.data() + J.base, sizeof(int) * J.cap,
+  HIPCHK(copy_sync(c, c->d_inst.p + J.base, c->h_inst.data() + J.base, sizeof(int) * J.cap,
                    hipMemcpyHostToDevice));
   recompute_extent(c);
   c->meta_dirty = true;
@@ -1719,7 +1751,7 @@ dmdgpu_status end_diffusion(dmdgpu_ctx* c, Instance& I) {
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int i = 0; i < I.n; i++) ne[i] = (I.h_slot[i] == 0 || x[i] >= kMinOcc) ? x[i] : 0.0;
-  HIPCHK(hipMemcpy(c->d_ne.p + I.base, ne.data(), sizeof(double) * I.n, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_ne.p + I.base, ne.data(), sizeof(double) * I.n, hipMemcpyHostToDevice));
   return refresh_activity(c, I);
 }
 
@@ -2284,6 +2316,9 @@ dmdgpu_status dmdgpu_diag_blocks_noalloc(dmdgpu_ctx* c, int exact, double* d_D) 
   // and per-site arrays exist since dmdgpu_neighbors, and no timing events
   // are created.
   if (c->meta_dirty) return fail(DMDGPU_ERR_STATE, "dmdgpu_neighbors has not been called");
+  if (c->tables_dirty)  // their upload may allocate
+    return fail(DMDGPU_ERR_STATE, "tables loaded since the last evaluation: evaluate once with "
+                                  "an allocating entry point first");
   const bool timing = c->timing;
   c->timing = false;
   std::fill(c->h_prec.begin(), c->h_prec.end(), -1);
@@ -2571,7 +2606,7 @@ dmdgpu_status dmdgpu_diffusion_rates(dmdgpu_ctx* c, int id, const dmdgpu_diffuse
   const int ni = (int)c->inst.size();
   std::vector<dmd::SolveOp> ops(ni);
   set_diff_op(ops[id], *o, dmd::kDiffRates);
-  HIPCHK(hipMemcpy(c->d_ops.p, ops.data(), sizeof(dmd::SolveOp) * ni, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, c->d_ops.p, ops.data(), sizeof(dmd::SolveOp) * ni, hipMemcpyHostToDevice));
   HIPCHK(dmd::launch_diff_flux(c->stream, make_state(c), c->d_ops.p, make_vecs(c),
                                make_outputs(c), c->d_dnn.p, c->d_dnbr.p));
   return d2h_sites(c, *I, xdot, c->d_Xt.p, 1);
@@ -2586,8 +2621,8 @@ dmdgpu_status dmdgpu_diffusion_neighbors(dmdgpu_ctx* c, int id, int* counts, int
   CHK(build_diff_neighbors(c, *I));
   const int n = I->n;
   std::vector<int> cnt(n), lst((size_t)n * dmd::kDCap);
-  HIPCHK(hipMemcpy(cnt.data(), c->d_dnn.p + I->base, sizeof(int) * n, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(lst.data(), c->d_dnbr.p + (size_t)I->base * dmd::kDCap,
+  HIPCHK(copy_sync(c, cnt.data(), c->d_dnn.p + I->base, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c, lst.data(), c->d_dnbr.p + (size_t)I->base * dmd::kDCap,
                    sizeof(int) * n * dmd::kDCap, hipMemcpyDeviceToHost));
   for (int i = 0; i < n; i++) {
     const int o = I->perm[i];

@@ -212,7 +212,9 @@ dmdgpu_status dmdgpu_instance_diag_blocks(dmdgpu_ctx* ctx, int id, int exact, do
  * process ends); an instance with more sites is refused with DMDGPU_ERR_ARG.
  * The array is shared by every call on that device and keeps the last call's
  * contents: calls that overlap on one device (from different contexts or
- * threads) race.  Needs dmdgpu_neighbors first; returns after the stream has
+ * threads) race.  Needs dmdgpu_neighbors first, and tables loaded since the
+ * last evaluation to have been used by an allocating call (DMDGPU_ERR_STATE
+ * otherwise: their upload may allocate); returns after the stream has
  * finished (with DMDGPU_ERR_DOMAIN as dmdgpu_diag_blocks). */
 dmdgpu_status dmdgpu_diag_blocks_noalloc(dmdgpu_ctx* ctx, int exact, double* d_D);
 

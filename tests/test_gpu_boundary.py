@@ -13,6 +13,8 @@
   an instance above the 6,710,784-site limit.
 * An arena growth that fails part way (forced from the host) is an error of
   the add that asked for it and leaves the resident instances intact.
+* Host writes (state updates, added instances) wait for an evaluation still
+  in flight on the context's stream.
 """
 import ctypes as C
 import sys
@@ -270,3 +272,37 @@ def test_residual_packed_host_and_device(gpu, oracle):
     with pytest.raises(dmdgpu.DmdError) as e:
         gpu.residual_packed(bad)
     assert e.value.code == ERR_ARG
+
+
+def test_host_writes_wait_for_evaluations_in_flight():
+    """dmdgpu_residual_resident returns with its kernels still running; a state
+    update and an added instance issued right after it must wait for them (the
+    library copies through the context's stream).  Each result equals the one
+    of a context that synchronised in between."""
+    say("[gpu] host writes after an evaluation in flight")
+    a = lat.box_of_size(20000, seed=3)
+    q2 = a["q"] + 0.01 * np.random.default_rng(4).standard_normal(a["q"].shape)
+    b = lat.box_of_size(6000, seed=5)
+    results = []
+    for wait in (False, True):
+        ctx = dmdgpu.Context(0)
+        try:
+            ctx.set_state(a)
+            ctx.neighbors()
+            for _ in range(3):
+                ctx.residual_resident()
+            if wait:
+                ctx.sync()
+            ctx.update_state(q=q2)  # overwrites positions the evaluation reads
+            r = ctx.residual()
+            ctx.residual_resident()
+            if wait:
+                ctx.sync()
+            ib = ctx.add_instance(b)  # grows the arena the evaluation uses
+            ctx.evaluate()
+            results.append((r, ctx.instance_residual(ib)))
+        finally:
+            ctx.close()
+    for k in ("Rq", "Rs", "Rx"):
+        assert np.array_equal(results[0][0][k], results[1][0][k]), k
+        assert np.array_equal(results[0][1][k], results[1][1][k]), k
