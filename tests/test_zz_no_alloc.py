@@ -6,7 +6,9 @@ calls in a row -- the second with a smaller system -- in both assembly modes.
 The call allocates nothing: the context's device allocations are the same
 before and after it, and so is the device's free memory (after the first call
 of the process, which may load kernel code).  An instance larger than the
-scratch array is refused.  Device memory for the output comes from the same
+scratch array is refused, and so are tables loaded since the last evaluation
+(their upload may allocate) until an allocating call has used them.  Device
+memory for the output comes from the same
 HIP runtime the library uses (hipMalloc through ctypes), not from the library.
 """
 import ctypes as C
@@ -19,7 +21,7 @@ import dmdgpu
 import dmdgpu.lattice as lat
 
 pytestmark = pytest.mark.gpu
-ERR_ARG = 1
+ERR_ARG, ERR_STATE = 1, 8
 SCRATCH_SITES = 65536
 
 
@@ -80,6 +82,21 @@ def test_diag_blocks_noalloc_matches_allocating_entry_point():
                     assert np.array_equal(got, ref)
             finally:
                 hip.free(buf)
+        # tables loaded again: refused until an allocating call uploads them
+        n = int(systems[-1]["n"])
+        buf = hip.malloc(8 * 9 * n)
+        try:
+            say("[gpu] noalloc diag blocks after a table reload")
+            for comb in (0, 1, 2):
+                assert ctx.L.dmdgpu_init_adp_MgHx(ctx.h, comb, dmdgpu.TABLE_DIR.encode()) == 0
+            with pytest.raises(dmdgpu.DmdError) as e:
+                ctx.diag_blocks_noalloc(False, buf)
+            assert e.value.code == ERR_STATE
+            ref = ctx.diag_blocks(False).reshape(-1)
+            ctx.diag_blocks_noalloc(False, buf)
+            assert np.array_equal(hip.to_host(buf, 9 * n), ref)
+        finally:
+            hip.free(buf)
         big = lat.hcp_mgh(24, 18, 18, x_h=(0.0, 0.1), seed=5)
         assert big["n"] > SCRATCH_SITES
         ctx.set_state(big)
