@@ -48,6 +48,9 @@ template <> DEV const float* tab_rh<float>(const Tables& T) { return T.rh32; }
 template <typename P> DEV const P* tab_pr(const Tables& T);
 template <> DEV const double* tab_pr<double>(const Tables& T) { return T.pr; }
 template <> DEV const float* tab_pr<float>(const Tables& T) { return T.pr32; }
+template <typename P> DEV const P* tab_rr(const Tables& T);
+template <> DEV const double* tab_rr<double>(const Tables& T) { return T.rr; }
+template <> DEV const float* tab_rr<float>(const Tables& T) { return T.rr32; }
 
 // ---------------------------------------------------------------------------
 // Quadrature-point geometry.  A point is p = r + d e_l (one axis shifted).

@@ -44,6 +44,8 @@ struct Tables {
   const float* rh32;
   const double* pr;    // fused pair-rule rows [t][m]: phi_t | rho_Mg | rho_H | 0 (16 values)
   const float* pr32;
+  const double* rr;    // density rows [m]: rho_Mg | rho_H (8 values; relaxation ticks)
+  const float* rr32;
   int nr;
   double dxr, inv_dxr;
   float dxr32, inv_dxr32;

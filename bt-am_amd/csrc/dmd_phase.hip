@@ -92,7 +92,8 @@ constexpr int kPairGroup = 8;
 // slots of both ends (the slots are free until phase 2); k_phase1 adds them up
 // in row order.  Work layout as in phase 2 (per-chunk owned-pair lists).
 // VALUES = false (relaxation ticks with the residual-norm line search): the
-// densities only -- the fused row of each point, no u/w row, no C6 or Phi.
+// densities only -- one density row (rho_Mg | rho_H, the same coefficients as
+// the fused row's) per point, no u/w or phi row, no C6 or Phi.
 template <typename P, int B, int RULE, bool VALUES>
 DEV void phase1_pairs(const State& s, const Tables& T, const Outputs& o, int inst, int ch,
                       int part, u32x4* stage) {
@@ -100,6 +101,7 @@ DEV void phase1_pairs(const State& s, const Tables& T, const Outputs& o, int ins
   const Box box = s.boxes[inst];
   const P* uwt = tab_uw<P>(T);
   const P* prt = tab_pr<P>(T);
+  const P* rrt = tab_rr<P>(T);
   const int nr = T.nr;
   const int c0 = ch * kChunk;
   const int np = s.chunk_np[ch];
@@ -124,16 +126,26 @@ DEV void phase1_pairs(const State& s, const Tables& T, const Outputs& o, int ins
       P p[3], off, ir, dsi[3], dsj[3], cw[8], f[16], cr[4];
       int m;
       const double wt = PairRule<RULE>::point(geo, T, pt, su, sk, p, m, off, ir, dsi, dsj);
-      gather_cpt<P, CO, CO, VALUES>(uwt, t * nr + m, prt, t * nr + m, stage, cw, f);
       if constexpr (VALUES) {
+        gather_cpt<P, CO, CO, true>(uwt, t * nr + m, prt, t * nr + m, stage, cw, f);
         const P r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
         C6 += wt * (double)self_psi(cval(cw, off), cval(cw + 4, off), r2);
         Phi += wt * (double)cval(f, off);
+        fused_rho(f, spk, cr);
+        Ru += wt * (double)cval(cr, off);
+        fused_rho(f, spu, cr);
+        Rk += wt * (double)cval(cr, off);
+      } else {
+        gather_row8<P, CO>(rrt, m, stage, cw);
+        P ck[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          ck[e] = spk ? cw[4 + e] : cw[e];
+          cr[e] = spu ? cw[4 + e] : cw[e];
+        }
+        Ru += wt * (double)cval(ck, off);
+        Rk += wt * (double)cval(cr, off);
       }
-      fused_rho(f, spk, cr);
-      Ru += wt * (double)cval(cr, off);
-      fused_rho(f, spu, cr);
-      Rk += wt * (double)cval(cr, off);
     }
     if (inb) {
       double* eu = o.slot + ((size_t)u * kCap + kk) * kSlotW;

@@ -188,8 +188,8 @@ struct dmdgpu_ctx {
   Spline rho[2], embed[2], pair[3], u[3], w[3];
   double mass[2] = {0.0, 0.0};
   bool tables_dirty = true;
-  DevBuf<double> d_uw, d_ph, d_rh, d_em, d_pr;
-  DevBuf<float> d_uw32, d_ph32, d_rh32, d_pr32, d_rec32;
+  DevBuf<double> d_uw, d_ph, d_rh, d_em, d_pr, d_rr;
+  DevBuf<float> d_uw32, d_ph32, d_rh32, d_pr32, d_rr32, d_rec32;
   dmd::Tables T{};
   // arena
   int extent = 0;     // sites in use, multiple of kChunk
@@ -359,6 +359,13 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
         row[8 + k] = rh[((size_t)1 * nr + m) * 4 + k];
       }
     }
+  // density rows for the pair-rule points of relaxation ticks: rho_Mg | rho_H
+  std::vector<double> rr((size_t)nr * 8, 0.0);
+  for (int m = 0; m < nr; m++)
+    for (int k = 0; k < 4; k++) {
+      rr[(size_t)m * 8 + k] = rh[((size_t)0 * nr + m) * 4 + k];
+      rr[(size_t)m * 8 + 4 + k] = rh[((size_t)1 * nr + m) * 4 + k];
+    }
   const int ne_stride = std::max(c->embed[0].n, c->embed[1].n);
   std::vector<double> em((size_t)2 * ne_stride * 4, 0.0);
   for (int s = 0; s < 2; s++) pack4(c->embed[s], &em[(size_t)s * ne_stride * 4]);
@@ -370,11 +377,15 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   HIPCHK(copy_sync(c, c->d_ph.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(copy_sync(c, c->d_rh.p, rh.data(), rh.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(copy_sync(c, c->d_em.p, em.data(), em.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c->d_rr.ensure(rr.size()));
+  HIPCHK(copy_sync(c, c->d_rr.p, rr.data(), rr.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c->d_pr.ensure(pr.size()));
   HIPCHK(copy_sync(c, c->d_pr.p, pr.data(), pr.size() * 8, hipMemcpyHostToDevice));
   // FP32 copies for the split-precision evaluation
   std::vector<float> uw32(uw.begin(), uw.end()), ph32(ph.begin(), ph.end()),
-      rh32(rh.begin(), rh.end()), pr32(pr.begin(), pr.end());
+      rh32(rh.begin(), rh.end()), pr32(pr.begin(), pr.end()), rr32(rr.begin(), rr.end());
+  HIPCHK(c->d_rr32.ensure(rr32.size()));
+  HIPCHK(copy_sync(c, c->d_rr32.p, rr32.data(), rr32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(c->d_pr32.ensure(pr32.size()));
   HIPCHK(copy_sync(c, c->d_pr32.p, pr32.data(), pr32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(c->d_uw32.ensure(uw32.size()));
@@ -393,6 +404,8 @@ dmdgpu_status upload_tables(dmdgpu_ctx* c) {
   T.rh32 = c->d_rh32.p;
   T.pr = c->d_pr.p;
   T.pr32 = c->d_pr32.p;
+  T.rr = c->d_rr.p;
+  T.rr32 = c->d_rr32.p;
   T.nr = nr;
   T.dxr = dx;
   T.inv_dxr = 1.0 / dx;
@@ -538,8 +551,8 @@ size_t site_bytes(dmdgpu_ctx* c, size_t cap, size_t* max_kept = nullptr) {
 // per-instance metadata and solver descriptors.
 template <typename F>
 void for_other_bufs(dmdgpu_ctx* c, F&& f) {
-  f(c->d_uw); f(c->d_ph); f(c->d_rh); f(c->d_em); f(c->d_pr);
-  f(c->d_uw32); f(c->d_ph32); f(c->d_rh32); f(c->d_pr32);
+  f(c->d_uw); f(c->d_ph); f(c->d_rh); f(c->d_em); f(c->d_pr); f(c->d_rr);
+  f(c->d_uw32); f(c->d_ph32); f(c->d_rh32); f(c->d_pr32); f(c->d_rr32);
   f(c->d_cell_count); f(c->d_cell_start); f(c->d_cell_cursor); f(c->d_flags);
   f(c->d_boxes); f(c->d_prec); f(c->d_noxr); f(c->d_chunk_lo); f(c->d_chunk_hi);
   f(c->d_totals); f(c->d_ops); f(c->d_sdots); f(c->d_hookq); f(c->d_hookv); f(c->d_jscal);
